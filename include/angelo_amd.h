@@ -1,0 +1,248 @@
+/*
+ * angelo_amd -- C-ABI of the MI355X (gfx950) NeuS / Neuralangelo hot path.
+ *
+ * Conventions (all entry points):
+ *   - return int status: AM_OK (0) or a negative error class; am_last_error()
+ *     gives the thread-local message of the last failure.  No exception ever
+ *     crosses this boundary.
+ *   - every device buffer is allocated by the caller (PyTorch's caching
+ *     allocator) and passed as a raw pointer plus sizes; the library never
+ *     allocates device memory.  Scratch comes from a caller workspace whose
+ *     size the matching *_workspace_size query returns.
+ *   - launches are asynchronous on the caller's HIP stream (`stream`, a
+ *     hipStream_t passed as void*); nothing synchronises the device.
+ *   - arguments are validated on the host before any launch.
+ *   - fp32 everywhere unless the name says otherwise; row-major arrays.
+ *
+ * Reference interfaces replaced are cited as path:line into hugoycj/
+ * Instant-angelo (tiny-cuda-nn and nerfacc 0.3.3 are not vendored there;
+ * their call sites are cited).
+ */
+#ifndef ANGELO_AMD_H
+#define ANGELO_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AM_OK 0
+#define AM_ERR_INVALID -1     /* bad argument / shape / size */
+#define AM_ERR_HIP -2         /* HIP runtime error */
+#define AM_ERR_WORKSPACE -3   /* workspace too small */
+#define AM_ERR_UNSUPPORTED -4 /* configuration not implemented */
+
+#define AM_MAX_LEVELS 32
+
+const char* am_last_error(void);
+int am_version(void);
+
+/* ------------------------------------------------------------------------ */
+/* Multiresolution hash grid (tcnn HashGrid through ProgressiveBandHashGrid,  */
+/* models/network_utils.py:40-66, and get_encoding, :83-93).                 */
+/* ------------------------------------------------------------------------ */
+typedef struct am_grid_desc {
+    int32_t n_levels;           /* L <= 32 */
+    int32_t n_features;         /* F in {1, 2, 4, 8} */
+    int32_t log2_hashmap_size;  /* T = 2^log2 entries per hashed level */
+    int32_t base_resolution;
+    float per_level_scale;
+    uint32_t n_entries;         /* total table rows (sum of level sizes) */
+    uint32_t hashed_mask;       /* bit l set: level l uses the prime hash */
+    float scale[AM_MAX_LEVELS]; /* fmaf(scale, x, 0.5) per level */
+    uint32_t res[AM_MAX_LEVELS];
+    uint32_t offset[AM_MAX_LEVELS]; /* first table row of level l */
+    uint32_t size[AM_MAX_LEVELS];   /* rows of level l (the index modulus) */
+} am_grid_desc;
+
+/* Fills the level table by tcnn's published rule (scale_l = 2^(l*log2(s))*base
+ * - 1 in fp32, res_l = ceil(scale_l)+1, rows = min(next_mult8(res^3), T)).
+ * res_override (nullable, L entries) replaces the computed resolutions (the
+ * scale is then res-1), for matching another implementation's table.
+ * Replaces tcnn.Encoding construction at models/network_utils.py:45-47. */
+int am_grid_init(am_grid_desc* g, int32_t n_levels, int32_t n_features,
+                 int32_t log2_hashmap_size, int32_t base_resolution,
+                 float per_level_scale, const uint32_t* res_override);
+
+/* Encode x[n,3] in [0,1] -> out[n, L*F] level-major (column l*F+f); levels
+ * >= active_levels give 0 (the ProgressiveBandHashGrid mask,
+ * models/network_utils.py:53-58).  table: [n_entries, F] fp32.
+ * Replaces ProgressiveBandHashGrid.forward (models/network_utils.py:56-59). */
+int am_hashgrid_fwd(const am_grid_desc* g, const float* x, int64_t n, const float* table,
+                    int32_t active_levels, float* out, void* stream);
+
+/* Deterministic backward: dtable[n_entries,F] (fully overwritten) by
+ * sort-by-row + segmented reduce; dx[n,3] (nullable) = dL/dx.
+ * Replaces tcnn's encoding backward (atomic scatter) behind the same call. */
+size_t am_hashgrid_bwd_workspace_size(const am_grid_desc* g, int64_t n, int32_t active_levels);
+int am_hashgrid_bwd(const am_grid_desc* g, const float* x, int64_t n, const float* table,
+                    const float* dy, int32_t active_levels, float* dtable, float* dx,
+                    void* workspace, size_t workspace_bytes, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* VanillaMLP (models/network_utils.py:96-140, get_mlp :177-185).            */
+/* Parameters: one flat fp32 buffer [W0,b0,W1,b1,...,Wlast,blast], each W    */
+/* row-major [out,in] as nn.Linear.weight (weight-norm already applied by    */
+/* the host: W = g*v/||v|| per output row, :132-133).                        */
+/* ------------------------------------------------------------------------ */
+#define AM_ACT_NONE 0
+#define AM_ACT_RELU 1
+#define AM_ACT_SOFTPLUS 2 /* Softplus(beta), threshold 20 (torch default) */
+#define AM_ACT_SIGMOID 3
+
+typedef struct am_mlp_desc {
+    int32_t n_in;            /* <= 64 */
+    int32_t n_out;           /* <= 64 */
+    int32_t n_hidden_layers; /* 1..4 */
+    int32_t n_neurons;       /* 64 */
+    int32_t hidden_act;      /* AM_ACT_RELU or AM_ACT_SOFTPLUS */
+    int32_t output_act;      /* AM_ACT_NONE / RELU / SIGMOID */
+    float softplus_beta;
+} am_mlp_desc;
+
+int64_t am_mlp_n_params(const am_mlp_desc* m);
+int am_mlp_fwd(const am_mlp_desc* m, const float* x, int64_t n, const float* params, float* y,
+               void* stream);
+size_t am_mlp_bwd_workspace_size(const am_mlp_desc* m, int64_t n);
+/* dx (nullable) [n,n_in]; dparams [n_params] fully overwritten (deterministic). */
+int am_mlp_bwd(const am_mlp_desc* m, const float* x, int64_t n, const float* params,
+               const float* dy, float* dx, float* dparams, void* workspace,
+               size_t workspace_bytes, void* stream);
+
+/* Real spherical harmonics of a direction given in [0,1]^3 (tcnn
+ * SphericalHarmonics, models/texture.py:17,24-25): out[n, degree^2],
+ * degree 1..4.  Replaces the tcnn encoding call at models/texture.py:25. */
+int am_sh_fwd(const float* dirs01, int64_t n, int32_t degree, float* out, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Ray marching (nerfacc 0.3.3 ray_aabb_intersect + ray_marching as called at */
+/* models/neus.py:209-220).  rays: [R,6] = origin, unit direction.           */
+/* ------------------------------------------------------------------------ */
+int am_ray_aabb(const float* rays, int64_t n_rays, const float* aabb6_host, float* t_near,
+                float* t_far, void* stream);
+
+/* Pass 1: packed[R,2] = (start, count) and total[1] (device int32).
+ * jitter (nullable [R]): stratified start t_near + u*step.
+ * grid (nullable, uint8 [res^3], x-major): occupancy; NULL = all occupied.
+ * cap: per-ray sample cap (0 = none).  roi6_host: grid AABB. */
+size_t am_march_workspace_size(int64_t n_rays);
+int am_march_count(const float* rays, int64_t n_rays, const float* t_near, const float* t_far,
+                   const float* jitter, const float* roi6_host, const uint8_t* grid,
+                   int32_t grid_res, float step, int32_t cap, int32_t* packed,
+                   int32_t* total, void* workspace, size_t workspace_bytes, void* stream);
+/* Pass 2: ray-sorted samples; ray_indices[N] int32, t_starts/t_ends[N]. */
+int am_march_fill(const float* rays, int64_t n_rays, const float* t_near, const float* t_far,
+                  const float* jitter, const float* roi6_host, const uint8_t* grid,
+                  int32_t grid_res, float step, int32_t cap, const int32_t* packed,
+                  int32_t* ray_indices, float* t_starts, float* t_ends, void* stream);
+
+/* Sample positions for models/neus.py:222-227: pts = o[idx] + d[idx]*mid,
+ * mid = (t0+t1)/2, dists = t1-t0 (each a separate fp32 op, as torch). */
+int am_sample_points(const float* rays, const int32_t* ray_indices, const float* t_starts,
+                     const float* t_ends, int64_t n, float* pts, float* mid, float* dists,
+                     void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Fused VolumeSDF.forward (models/geometry.py:195-287): contraction,         */
+/* encode + SDF MLP of the base point, 6 central-difference points (grad)    */
+/* and, with laplace, the 6+ curvature points around the tangent-shifted     */
+/* point, all in one launch.  params: flat SDF-MLP buffer (am_mlp_desc with   */
+/* n_in = 3 + L*F, n_out = feature_dim).                                      */
+/* ------------------------------------------------------------------------ */
+typedef struct am_sdf_desc {
+    float radius;         /* scene radius r: AABB [-r,r]^3 */
+    float fd_eps;         /* finite-difference eps (VolumeSDF._finite_difference_eps) */
+    int32_t with_grad;    /* 6 central-difference evaluations */
+    int32_t with_laplace; /* 6 more around the shifted point (needs with_grad) */
+    int32_t recontract;   /* 1: shifted point re-contracted as the reference does
+                             (models/geometry.py:246,264-265); 0: shifted in world space */
+    int32_t active_levels;
+} am_sdf_desc;
+
+/* Outputs (each nullable except sdf): sdf[n], grad[n,3], normal[n,3] =
+ * F.normalize(grad), feature[n, n_out+3] = cat(mlp_out, x01*2-1),
+ * laplace[n].  randn[n,3]: the curvature directions (torch.randn_like). */
+int am_sdf_fwd(const am_grid_desc* g, const am_mlp_desc* m, const am_sdf_desc* s,
+               const float* pts, const float* randn, int64_t n, const float* table,
+               const float* params, float* sdf, float* grad, float* normal, float* feature,
+               float* laplace, void* stream);
+size_t am_sdf_bwd_workspace_size(const am_grid_desc* g, const am_mlp_desc* m,
+                                 const am_sdf_desc* s, int64_t n);
+/* Upstream gradients are nullable (= 0).  dtable [n_entries,F] and
+ * dparams are fully overwritten; both reductions are deterministic. */
+int am_sdf_bwd(const am_grid_desc* g, const am_mlp_desc* m, const am_sdf_desc* s,
+               const float* pts, const float* randn, int64_t n, const float* table,
+               const float* params, const float* g_sdf, const float* g_grad,
+               const float* g_normal, const float* g_feature, const float* g_laplace,
+               float* dtable, float* dparams, void* workspace, size_t workspace_bytes,
+               void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Fused VolumeRadiance.forward (models/texture.py:23-30): SH(deg) of        */
+/* (d+1)/2, cat(feature, sh, normal) -> MLP -> sigmoid.  dirs are gathered    */
+/* from rays[ray_indices] (models/neus.py:224).                               */
+/* ------------------------------------------------------------------------ */
+int am_radiance_fwd(const am_mlp_desc* m, int32_t sh_degree, const float* feature,
+                    int32_t feature_dim, const float* rays, const int32_t* ray_indices,
+                    const float* normal, int64_t n, const float* params, float* rgb,
+                    void* stream);
+size_t am_radiance_bwd_workspace_size(const am_mlp_desc* m, int64_t n);
+int am_radiance_bwd(const am_mlp_desc* m, int32_t sh_degree, const float* feature,
+                    int32_t feature_dim, const float* rays, const int32_t* ray_indices,
+                    const float* normal, int64_t n, const float* params, const float* g_rgb,
+                    float* g_feature, float* g_normal, float* dparams, void* workspace,
+                    size_t workspace_bytes, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* NeuS alpha + transmittance + accumulation (models/neus.py:117-139,231-241 */
+/* with nerfacc render_weight_from_alpha / accumulate_along_rays).           */
+/* packed[R,2] (start,count) over ray-sorted samples.  inv_s: device scalar. */
+/* ------------------------------------------------------------------------ */
+typedef struct am_composite_desc {
+    float cos_anneal_ratio;
+    int32_t kahan; /* 1: compensated transmittance/accumulation sums */
+} am_composite_desc;
+
+/* Outputs: weights[n] (w_i = T_i*alpha_i), trans[n] (T_i, nullable, kept for
+ * backward), opacity[R], depth[R], comp_rgb[R,3], comp_normal[R,3]
+ * (normalised, 0 where opacity <= 0.1), comp_normal_raw[R,3] (the plain sum,
+ * nullable, kept for backward).  Rays without samples give zeros. */
+int am_composite_fwd(const am_composite_desc* c, const float* sdf, const float* normal,
+                     const float* rays, const int32_t* ray_indices, const float* t_starts,
+                     const float* t_ends, const float* rgb, const float* inv_s,
+                     const int32_t* packed, int64_t n_rays, int64_t n, float* weights,
+                     float* trans, float* opacity, float* depth, float* comp_rgb,
+                     float* comp_normal, float* comp_normal_raw, void* stream);
+size_t am_composite_bwd_workspace_size(int64_t n_rays, int64_t n);
+/* Upstream grads nullable (= 0).  trans, opacity, comp_normal_raw: from the
+ * forward.  g_inv_s: device scalar, a deterministic reduction. */
+int am_composite_bwd(const am_composite_desc* c, const float* sdf, const float* normal,
+                     const float* rays, const int32_t* ray_indices, const float* t_starts,
+                     const float* t_ends, const float* rgb, const float* inv_s,
+                     const int32_t* packed, int64_t n_rays, int64_t n, const float* trans,
+                     const float* opacity, const float* comp_normal_raw,
+                     const float* g_weights, const float* g_opacity, const float* g_depth,
+                     const float* g_comp_rgb, const float* g_comp_normal, float* g_sdf,
+                     float* g_normal, float* g_rgb, float* g_inv_s, void* workspace,
+                     size_t workspace_bytes, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Occupancy grid (nerfacc 0.3.3 OccupancyGrid, models/neus.py:63-68,90-111). */
+/* ------------------------------------------------------------------------ */
+/* EMA update of occs[cells] with the NeuS alpha estimate of the SDF at the   */
+/* jittered cell points (occ_eval_fn, models/neus.py:90-101), then the      */
+/* binary threshold min(mean(occs), thre).  cells (nullable = all cells in  */
+/* order) [m] int32; jitter [m,3] in [0,1).                                   */
+size_t am_occ_update_workspace_size(int32_t grid_res, int64_t m);
+int am_occ_update(const am_grid_desc* g, const am_mlp_desc* mlp, const am_sdf_desc* s,
+                  const float* table, const float* params, const float* inv_s,
+                  float step, const int32_t* cells, const float* jitter, int64_t m,
+                  int32_t grid_res, float ema_decay, float occ_thre, float* occs,
+                  uint8_t* binary, void* workspace, size_t workspace_bytes, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ANGELO_AMD_H */

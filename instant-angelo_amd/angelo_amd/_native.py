@@ -1,0 +1,161 @@
+"""ctypes binding of libangelo_amd.so (the C-ABI declared in include/angelo_amd.h).
+
+The library is loaded from angelo_amd/lib/.  There is no fallback: if it is
+missing, every GPU entry point raises.  Build it with
+`python instant-angelo_amd/build.py` (or `__graft_entry__.build()`).
+"""
+import ctypes
+import os
+
+import torch
+
+_LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libangelo_amd.so")
+_lib = None
+
+AM_MAX_LEVELS = 32
+ACT = {"none": 0, "relu": 1, "softplus": 2, "sigmoid": 3}
+
+
+class GridDesc(ctypes.Structure):
+    _fields_ = [
+        ("n_levels", ctypes.c_int32),
+        ("n_features", ctypes.c_int32),
+        ("log2_hashmap_size", ctypes.c_int32),
+        ("base_resolution", ctypes.c_int32),
+        ("per_level_scale", ctypes.c_float),
+        ("n_entries", ctypes.c_uint32),
+        ("hashed_mask", ctypes.c_uint32),
+        ("scale", ctypes.c_float * AM_MAX_LEVELS),
+        ("res", ctypes.c_uint32 * AM_MAX_LEVELS),
+        ("offset", ctypes.c_uint32 * AM_MAX_LEVELS),
+        ("size", ctypes.c_uint32 * AM_MAX_LEVELS),
+    ]
+
+
+class MlpDesc(ctypes.Structure):
+    _fields_ = [
+        ("n_in", ctypes.c_int32),
+        ("n_out", ctypes.c_int32),
+        ("n_hidden_layers", ctypes.c_int32),
+        ("n_neurons", ctypes.c_int32),
+        ("hidden_act", ctypes.c_int32),
+        ("output_act", ctypes.c_int32),
+        ("softplus_beta", ctypes.c_float),
+    ]
+
+
+class SdfDesc(ctypes.Structure):
+    _fields_ = [
+        ("radius", ctypes.c_float),
+        ("fd_eps", ctypes.c_float),
+        ("with_grad", ctypes.c_int32),
+        ("with_laplace", ctypes.c_int32),
+        ("recontract", ctypes.c_int32),
+        ("active_levels", ctypes.c_int32),
+    ]
+
+
+class CompositeDesc(ctypes.Structure):
+    _fields_ = [("cos_anneal_ratio", ctypes.c_float), ("kahan", ctypes.c_int32)]
+
+
+_P = ctypes.c_void_p
+_I64 = ctypes.c_int64
+_I32 = ctypes.c_int32
+_F = ctypes.c_float
+_SZ = ctypes.c_size_t
+
+# name -> (restype, argtypes)
+_SIGS = {
+    "am_last_error": (ctypes.c_char_p, []),
+    "am_version": (ctypes.c_int, []),
+    "am_grid_init": (ctypes.c_int, [_P, _I32, _I32, _I32, _I32, _F, _P]),
+    "am_hashgrid_fwd": (ctypes.c_int, [_P, _P, _I64, _P, _I32, _P, _P]),
+    "am_hashgrid_bwd_workspace_size": (_SZ, [_P, _I64, _I32]),
+    "am_hashgrid_bwd": (ctypes.c_int, [_P, _P, _I64, _P, _P, _I32, _P, _P, _P, _SZ, _P]),
+    "am_mlp_n_params": (_I64, [_P]),
+    "am_mlp_fwd": (ctypes.c_int, [_P, _P, _I64, _P, _P, _P]),
+    "am_mlp_bwd_workspace_size": (_SZ, [_P, _I64]),
+    "am_mlp_bwd": (ctypes.c_int, [_P, _P, _I64, _P, _P, _P, _P, _P, _SZ, _P]),
+    "am_sh_fwd": (ctypes.c_int, [_P, _I64, _I32, _P, _P]),
+    "am_ray_aabb": (ctypes.c_int, [_P, _I64, _P, _P, _P, _P]),
+    "am_march_workspace_size": (_SZ, [_I64]),
+    "am_march_count": (ctypes.c_int, [_P, _I64, _P, _P, _P, _P, _P, _I32, _F, _I32, _P, _P, _P, _SZ, _P]),
+    "am_march_fill": (ctypes.c_int, [_P, _I64, _P, _P, _P, _P, _P, _I32, _F, _I32, _P, _P, _P, _P, _P]),
+    "am_sample_points": (ctypes.c_int, [_P, _P, _P, _P, _I64, _P, _P, _P, _P]),
+    "am_sdf_fwd": (ctypes.c_int, [_P, _P, _P, _P, _P, _I64, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "am_sdf_bwd_workspace_size": (_SZ, [_P, _P, _P, _I64]),
+    "am_sdf_bwd": (ctypes.c_int, [_P, _P, _P, _P, _P, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _SZ, _P]),
+    "am_radiance_fwd": (ctypes.c_int, [_P, _I32, _P, _I32, _P, _P, _P, _I64, _P, _P, _P]),
+    "am_radiance_bwd_workspace_size": (_SZ, [_P, _I64]),
+    "am_radiance_bwd": (ctypes.c_int, [_P, _I32, _P, _I32, _P, _P, _P, _I64, _P, _P, _P, _P, _P, _P, _SZ, _P]),
+    "am_composite_fwd": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I64, _I64, _P, _P, _P, _P, _P, _P]),
+    "am_composite_bwd_workspace_size": (_SZ, [_I64, _I64]),
+    "am_composite_bwd": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I64, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _SZ, _P]),
+    "am_occ_update_workspace_size": (_SZ, [_I32, _I64]),
+    "am_occ_update": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _F, _P, _P, _I64, _I32, _F, _F, _P, _P, _P, _SZ, _P]),
+}
+
+
+class NativeError(RuntimeError):
+    pass
+
+
+def lib_path():
+    return _LIB_PATH
+
+
+def load():
+    """Loads (once) and returns the library; raises if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB_PATH):
+        raise NativeError(
+            f"angelo_amd native library not built ({_LIB_PATH}); run "
+            "`python instant-angelo_amd/build.py`")
+    lib = ctypes.CDLL(_LIB_PATH)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name, None)
+        if fn is None:
+            continue
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def exported_symbols():
+    return list(_SIGS.keys())
+
+
+def check(rc, what):
+    if rc != 0:
+        msg = load().am_last_error().decode(errors="replace")
+        raise NativeError(f"{what} failed (status {rc}): {msg}")
+
+
+def ptr(t):
+    """Device/host pointer of a tensor (None -> NULL)."""
+    if t is None:
+        return None
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def sptr(s):
+    return ctypes.byref(s)
+
+
+def stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def require_cuda(*tensors):
+    for t in tensors:
+        if t is not None and not t.is_cuda:
+            raise NativeError("angelo_amd kernels need device tensors (got a CPU tensor); "
+                              "the HIP path has no CPU fallback")
+
+
+def workspace(nbytes, device):
+    return torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device=device)

@@ -1,0 +1,322 @@
+"""torch.autograd.Function wrappers over the HIP kernels (C-ABI in
+include/angelo_amd.h).  Each forward and backward is one or a few native
+launches on the current stream; there is no CPU or eager-torch fallback.
+"""
+import torch
+
+from . import _native as nat
+
+
+def _f32c(t):
+    return t.contiguous() if t.dtype == torch.float32 else t.float().contiguous()
+
+
+class HashGridFn(torch.autograd.Function):
+    """tcnn HashGrid encode behind ProgressiveBandHashGrid
+    (models/network_utils.py:45-47,56-59): x[n,3] in [0,1] -> [n, L*F]."""
+
+    @staticmethod
+    def forward(ctx, x, table, grid, active_levels):
+        nat.require_cuda(x, table)
+        lib = nat.load()
+        x = _f32c(x)
+        n = x.shape[0]
+        out = torch.empty(n, grid.n_levels * grid.n_features, device=x.device, dtype=torch.float32)
+        nat.check(lib.am_hashgrid_fwd(nat.sptr(grid), nat.ptr(x), n, nat.ptr(table),
+                                      int(active_levels), nat.ptr(out), nat.stream()),
+                  "am_hashgrid_fwd")
+        ctx.save_for_backward(x, table)
+        ctx.grid, ctx.active = grid, int(active_levels)
+        return out
+
+    @staticmethod
+    def backward(ctx, g):
+        x, table = ctx.saved_tensors
+        lib = nat.load()
+        g = _f32c(g)
+        n = x.shape[0]
+        grid = ctx.grid
+        dx = torch.empty_like(x) if ctx.needs_input_grad[0] else None
+        dtable = torch.empty_like(table) if ctx.needs_input_grad[1] else None
+        ws = None
+        if dtable is not None:
+            ws = nat.workspace(lib.am_hashgrid_bwd_workspace_size(nat.sptr(grid), n, ctx.active),
+                               x.device)
+        nat.check(lib.am_hashgrid_bwd(nat.sptr(grid), nat.ptr(x), n, nat.ptr(table), nat.ptr(g),
+                                      ctx.active, nat.ptr(dtable), nat.ptr(dx), nat.ptr(ws),
+                                      ws.numel() if ws is not None else 0, nat.stream()),
+                  "am_hashgrid_bwd")
+        return dx, dtable, None, None
+
+
+class MLPFn(torch.autograd.Function):
+    """VanillaMLP forward/backward (models/network_utils.py:96-140) on a flat
+    parameter vector [W0,b0,...] (weight-norm applied by the caller)."""
+
+    @staticmethod
+    def forward(ctx, x, params, desc):
+        nat.require_cuda(x, params)
+        lib = nat.load()
+        x = _f32c(x)
+        params = _f32c(params)
+        n = x.shape[0]
+        y = torch.empty(n, desc.n_out, device=x.device, dtype=torch.float32)
+        nat.check(lib.am_mlp_fwd(nat.sptr(desc), nat.ptr(x), n, nat.ptr(params), nat.ptr(y),
+                                 nat.stream()), "am_mlp_fwd")
+        ctx.save_for_backward(x, params)
+        ctx.desc = desc
+        return y
+
+    @staticmethod
+    def backward(ctx, g):
+        x, params = ctx.saved_tensors
+        lib = nat.load()
+        g = _f32c(g)
+        n = x.shape[0]
+        dx = torch.empty_like(x) if ctx.needs_input_grad[0] else None
+        dparams = torch.empty_like(params)
+        ws = nat.workspace(lib.am_mlp_bwd_workspace_size(nat.sptr(ctx.desc), n), x.device)
+        nat.check(lib.am_mlp_bwd(nat.sptr(ctx.desc), nat.ptr(x), n, nat.ptr(params), nat.ptr(g),
+                                 nat.ptr(dx), nat.ptr(dparams), nat.ptr(ws), ws.numel(),
+                                 nat.stream()), "am_mlp_bwd")
+        return dx, dparams, None
+
+
+def hashgrid_encode(x, table, grid, active_levels):
+    return HashGridFn.apply(x, table, grid, active_levels)
+
+
+def mlp_apply(x, params, desc):
+    return MLPFn.apply(x, params, desc)
+
+
+def sh_encode(dirs01, degree):
+    """tcnn SphericalHarmonics forward (models/texture.py:25); directions carry
+    no gradient on the NeuS path."""
+    nat.require_cuda(dirs01)
+    d = _f32c(dirs01)
+    out = torch.empty(d.shape[0], degree * degree, device=d.device, dtype=torch.float32)
+    nat.check(nat.load().am_sh_fwd(nat.ptr(d), d.shape[0], int(degree), nat.ptr(out),
+                                   nat.stream()), "am_sh_fwd")
+    return out
+
+
+class SDFFieldFn(torch.autograd.Function):
+    """Fused VolumeSDF.forward (models/geometry.py:195-287): outputs sdf [n],
+    grad [n,3], normal [n,3] = F.normalize(grad), feature [n, n_out+3],
+    laplace [n] (grad/normal/laplace empty when not requested)."""
+
+    @staticmethod
+    def forward(ctx, pts, randn, table, params, grid, mdesc, sdesc):
+        nat.require_cuda(pts, table, params)
+        lib = nat.load()
+        pts = _f32c(pts)
+        n = pts.shape[0]
+        dev = pts.device
+        f32 = dict(device=dev, dtype=torch.float32)
+        sdf = torch.empty(n, **f32)
+        feature = torch.empty(n, mdesc.n_out + 3, **f32)
+        grad = torch.empty(n, 3, **f32) if sdesc.with_grad else torch.empty(0, 3, **f32)
+        normal = torch.empty(n, 3, **f32) if sdesc.with_grad else torch.empty(0, 3, **f32)
+        lap = torch.empty(n, **f32) if sdesc.with_laplace else torch.empty(0, **f32)
+        rnd = _f32c(randn) if (sdesc.with_laplace and randn is not None) else None
+        nat.check(lib.am_sdf_fwd(nat.sptr(grid), nat.sptr(mdesc), nat.sptr(sdesc), nat.ptr(pts),
+                                 nat.ptr(rnd), n, nat.ptr(table), nat.ptr(params), nat.ptr(sdf),
+                                 nat.ptr(grad) if sdesc.with_grad else None,
+                                 nat.ptr(normal) if sdesc.with_grad else None, nat.ptr(feature),
+                                 nat.ptr(lap) if sdesc.with_laplace else None, nat.stream()),
+                  "am_sdf_fwd")
+        ctx.save_for_backward(pts, rnd if rnd is not None else pts.new_empty(0), table, params)
+        ctx.descs = (grid, mdesc, sdesc)
+        return sdf, grad, normal, feature, lap
+
+    @staticmethod
+    def backward(ctx, g_sdf, g_grad, g_normal, g_feature, g_lap):
+        pts, rnd, table, params = ctx.saved_tensors
+        grid, mdesc, sdesc = ctx.descs
+        lib = nat.load()
+        n = pts.shape[0]
+
+        def opt(t, numel):
+            return _f32c(t) if (t is not None and t.numel() == numel and numel > 0) else None
+
+        gs = opt(g_sdf, n)
+        gg = opt(g_grad, 3 * n) if sdesc.with_grad else None
+        gn = opt(g_normal, 3 * n) if sdesc.with_grad else None
+        gf = opt(g_feature, n * (mdesc.n_out + 3))
+        gl = opt(g_lap, n) if sdesc.with_laplace else None
+        dtable = torch.empty_like(table)
+        dparams = torch.empty_like(params)
+        ws = nat.workspace(lib.am_sdf_bwd_workspace_size(nat.sptr(grid), nat.sptr(mdesc),
+                                                         nat.sptr(sdesc), n), pts.device)
+        nat.check(lib.am_sdf_bwd(nat.sptr(grid), nat.sptr(mdesc), nat.sptr(sdesc), nat.ptr(pts),
+                                 nat.ptr(rnd) if rnd.numel() else None, n, nat.ptr(table),
+                                 nat.ptr(params), nat.ptr(gs), nat.ptr(gg), nat.ptr(gn),
+                                 nat.ptr(gf), nat.ptr(gl), nat.ptr(dtable), nat.ptr(dparams),
+                                 nat.ptr(ws), ws.numel(), nat.stream()), "am_sdf_bwd")
+        return None, None, dtable, dparams, None, None, None
+
+
+class RadianceFn(torch.autograd.Function):
+    """Fused VolumeRadiance.forward (models/texture.py:23-30) with sigmoid colour:
+    rgb = sigmoid(MLP(cat(feature, SH((d+1)/2), normal))), d = rays[ridx, 3:6]."""
+
+    @staticmethod
+    def forward(ctx, feature, normal, params, rays, ray_indices, mdesc, sh_degree):
+        nat.require_cuda(feature, normal, params, rays, ray_indices)
+        lib = nat.load()
+        feature = _f32c(feature)
+        normal = _f32c(normal)
+        n = feature.shape[0]
+        rgb = torch.empty(n, mdesc.n_out, device=feature.device, dtype=torch.float32)
+        nat.check(lib.am_radiance_fwd(nat.sptr(mdesc), int(sh_degree), nat.ptr(feature),
+                                      feature.shape[1], nat.ptr(rays), nat.ptr(ray_indices),
+                                      nat.ptr(normal), n, nat.ptr(params), nat.ptr(rgb),
+                                      nat.stream()), "am_radiance_fwd")
+        ctx.save_for_backward(feature, normal, params, rays, ray_indices)
+        ctx.mdesc, ctx.deg = mdesc, int(sh_degree)
+        return rgb
+
+    @staticmethod
+    def backward(ctx, g_rgb):
+        feature, normal, params, rays, ridx = ctx.saved_tensors
+        lib = nat.load()
+        n = feature.shape[0]
+        g_rgb = _f32c(g_rgb)
+        gf = torch.zeros_like(feature) if ctx.needs_input_grad[0] else None
+        gn = torch.zeros_like(normal) if ctx.needs_input_grad[1] else None
+        dparams = torch.empty_like(params)
+        ws = nat.workspace(lib.am_radiance_bwd_workspace_size(nat.sptr(ctx.mdesc), n),
+                           feature.device)
+        nat.check(lib.am_radiance_bwd(nat.sptr(ctx.mdesc), ctx.deg, nat.ptr(feature),
+                                      feature.shape[1], nat.ptr(rays), nat.ptr(ridx),
+                                      nat.ptr(normal), n, nat.ptr(params), nat.ptr(g_rgb),
+                                      nat.ptr(gf), nat.ptr(gn), nat.ptr(dparams), nat.ptr(ws),
+                                      ws.numel(), nat.stream()), "am_radiance_bwd")
+        return gf, gn, dparams, None, None, None, None
+
+
+class CompositeFn(torch.autograd.Function):
+    """NeuS alpha (models/neus.py:117-139) + render_weight_from_alpha +
+    accumulate_along_rays (:234-241).  Outputs weights [n], opacity [R],
+    depth [R], comp_rgb [R,3], comp_normal [R,3]."""
+
+    @staticmethod
+    def forward(ctx, sdf, normal, rgb, inv_s, rays, ray_indices, t_starts, t_ends, packed, cdesc):
+        nat.require_cuda(sdf, normal, rgb, inv_s, rays, packed)
+        lib = nat.load()
+        n = sdf.shape[0]
+        R = packed.shape[0]
+        dev = sdf.device
+        f32 = dict(device=dev, dtype=torch.float32)
+        weights = torch.empty(n, **f32)
+        trans = torch.empty(n, **f32)
+        opacity = torch.empty(R, **f32)
+        depth = torch.empty(R, **f32)
+        comp_rgb = torch.empty(R, 3, **f32)
+        comp_normal = torch.empty(R, 3, **f32)
+        cn_raw = torch.empty(R, 3, **f32)
+        sdf, normal, rgb = _f32c(sdf), _f32c(normal), _f32c(rgb)
+        inv_s = _f32c(inv_s.reshape(1))
+        nat.check(lib.am_composite_fwd(nat.sptr(cdesc), nat.ptr(sdf), nat.ptr(normal), nat.ptr(rays),
+                                       nat.ptr(ray_indices), nat.ptr(t_starts), nat.ptr(t_ends),
+                                       nat.ptr(rgb), nat.ptr(inv_s), nat.ptr(packed), R, n,
+                                       nat.ptr(weights), nat.ptr(trans), nat.ptr(opacity),
+                                       nat.ptr(depth), nat.ptr(comp_rgb), nat.ptr(comp_normal),
+                                       nat.ptr(cn_raw), nat.stream()), "am_composite_fwd")
+        ctx.save_for_backward(sdf, normal, rgb, inv_s, rays, ray_indices, t_starts, t_ends,
+                              packed, trans, opacity, cn_raw)
+        ctx.cdesc = cdesc
+        return weights, opacity, depth, comp_rgb, comp_normal
+
+    @staticmethod
+    def backward(ctx, g_w, g_op, g_dep, g_rgb, g_cn):
+        (sdf, normal, rgb, inv_s, rays, ridx, t0, t1, packed, trans, opacity,
+         cn_raw) = ctx.saved_tensors
+        lib = nat.load()
+        n, R = sdf.shape[0], packed.shape[0]
+
+        def opt(t):
+            return _f32c(t) if t is not None else None
+
+        gs = torch.empty_like(sdf)
+        gn = torch.empty_like(normal)
+        gr = torch.empty_like(rgb)
+        gi = torch.empty(1, device=sdf.device, dtype=torch.float32)
+        ws = nat.workspace(lib.am_composite_bwd_workspace_size(R, n), sdf.device)
+        nat.check(lib.am_composite_bwd(nat.sptr(ctx.cdesc), nat.ptr(sdf), nat.ptr(normal),
+                                       nat.ptr(rays), nat.ptr(ridx), nat.ptr(t0), nat.ptr(t1),
+                                       nat.ptr(rgb), nat.ptr(inv_s), nat.ptr(packed), R, n,
+                                       nat.ptr(trans), nat.ptr(opacity), nat.ptr(cn_raw),
+                                       nat.ptr(opt(g_w)), nat.ptr(opt(g_op)), nat.ptr(opt(g_dep)),
+                                       nat.ptr(opt(g_rgb)), nat.ptr(opt(g_cn)), nat.ptr(gs),
+                                       nat.ptr(gn), nat.ptr(gr), nat.ptr(gi), nat.ptr(ws),
+                                       ws.numel(), nat.stream()), "am_composite_bwd")
+        return gs, gn, gr, gi, None, None, None, None, None, None
+
+
+def composite(sdf, normal, rgb, inv_s, rays, ray_indices, t_starts, t_ends, packed, cdesc):
+    return CompositeFn.apply(sdf, normal, rgb, inv_s, rays, ray_indices, t_starts, t_ends, packed,
+                             cdesc)
+
+
+# ---- non-differentiable launches ------------------------------------------
+def ray_aabb(rays, aabb):
+    """nerfacc ray_aabb_intersect (t_min clipped to >= 0, 1e10 on a miss)."""
+    import ctypes
+    nat.require_cuda(rays)
+    R = rays.shape[0]
+    t_near = torch.empty(R, device=rays.device, dtype=torch.float32)
+    t_far = torch.empty_like(t_near)
+    box = (ctypes.c_float * 6)(*[float(v) for v in aabb])
+    nat.check(nat.load().am_ray_aabb(nat.ptr(rays), R, ctypes.cast(box, ctypes.c_void_p),
+                                     nat.ptr(t_near), nat.ptr(t_far), nat.stream()), "am_ray_aabb")
+    return t_near, t_far
+
+
+def ray_marching(rays, t_near, t_far, roi, grid_binary, step, jitter=None, cap=0):
+    """nerfacc ray_marching (models/neus.py:209-220): count -> scan -> fill.
+    Returns (ray_indices int32 [N], t_starts [N], t_ends [N], packed [R,2]);
+    one host read of N, as in the reference."""
+    import ctypes
+    lib = nat.load()
+    nat.require_cuda(rays, t_near, t_far)
+    R = rays.shape[0]
+    dev = rays.device
+    roi_h = (ctypes.c_float * 6)(*[float(v) for v in roi])
+    roi_p = ctypes.cast(roi_h, ctypes.c_void_p)
+    res = 0
+    gptr = None
+    if grid_binary is not None:
+        res = int(grid_binary.shape[-1])
+        gptr = nat.ptr(grid_binary)
+    packed = torch.empty(R, 2, device=dev, dtype=torch.int32)
+    total = torch.empty(1, device=dev, dtype=torch.int32)
+    ws = nat.workspace(lib.am_march_workspace_size(R), dev)
+    jit = _f32c(jitter) if jitter is not None else None
+    nat.check(lib.am_march_count(nat.ptr(rays), R, nat.ptr(t_near), nat.ptr(t_far), nat.ptr(jit),
+                                 roi_p, gptr, res, float(step), int(cap), nat.ptr(packed),
+                                 nat.ptr(total), nat.ptr(ws), ws.numel(), nat.stream()),
+              "am_march_count")
+    N = int(total.item())
+    ridx = torch.empty(N, device=dev, dtype=torch.int32)
+    t0 = torch.empty(N, device=dev, dtype=torch.float32)
+    t1 = torch.empty(N, device=dev, dtype=torch.float32)
+    nat.check(lib.am_march_fill(nat.ptr(rays), R, nat.ptr(t_near), nat.ptr(t_far), nat.ptr(jit),
+                                roi_p, gptr, res, float(step), int(cap), nat.ptr(packed),
+                                nat.ptr(ridx), nat.ptr(t0), nat.ptr(t1), nat.stream()),
+              "am_march_fill")
+    return ridx, t0, t1, packed
+
+
+def sample_points(rays, ray_indices, t_starts, t_ends):
+    """models/neus.py:222-227: positions, midpoints, dists."""
+    n = ray_indices.shape[0]
+    dev = rays.device
+    pts = torch.empty(n, 3, device=dev, dtype=torch.float32)
+    mid = torch.empty(n, device=dev, dtype=torch.float32)
+    dists = torch.empty(n, device=dev, dtype=torch.float32)
+    nat.check(nat.load().am_sample_points(nat.ptr(rays), nat.ptr(ray_indices), nat.ptr(t_starts),
+                                          nat.ptr(t_ends), n, nat.ptr(pts), nat.ptr(mid),
+                                          nat.ptr(dists), nat.stream()), "am_sample_points")
+    return pts, mid, dists

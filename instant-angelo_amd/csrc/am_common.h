@@ -1,0 +1,106 @@
+// Shared device/host helpers for the MI355X (gfx950) NeuS hot-path library.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+#include "../../include/angelo_amd.h"
+
+namespace am {
+
+// ---- error reporting (thread-local message, int status through the C-ABI) ----
+void set_error(const char* fmt, ...);
+const char* last_error();
+
+#define AM_CHECK_ARG(cond, ...)                 \
+    do {                                        \
+        if (!(cond)) {                          \
+            ::am::set_error(__VA_ARGS__);       \
+            return AM_ERR_INVALID;              \
+        }                                       \
+    } while (0)
+
+#define AM_HIP(call)                                                              \
+    do {                                                                          \
+        hipError_t e_ = (call);                                                   \
+        if (e_ != hipSuccess) {                                                   \
+            ::am::set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                            __FILE__, __LINE__);                                  \
+            return AM_ERR_HIP;                                                    \
+        }                                                                         \
+    } while (0)
+
+#define AM_LAUNCH_CHECK()                                                               \
+    do {                                                                                \
+        hipError_t e_ = hipGetLastError();                                              \
+        if (e_ != hipSuccess) {                                                         \
+            ::am::set_error("kernel launch failed: %s (%s:%d)", hipGetErrorString(e_), \
+                            __FILE__, __LINE__);                                        \
+            return AM_ERR_HIP;                                                          \
+        }                                                                               \
+    } while (0)
+
+static inline hipStream_t to_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+static inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+static inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+// Carves aligned sub-buffers out of one caller-provided workspace.
+struct WsCarver {
+    char* base;
+    size_t cap, used;
+    WsCarver(void* p, size_t c) : base(static_cast<char*>(p)), cap(c), used(0) {}
+    template <typename T>
+    T* take(size_t count) {
+        used = align_up(used, 256);
+        T* r = base ? reinterpret_cast<T*>(base + used) : nullptr;
+        used += count * sizeof(T);
+        return r;
+    }
+    bool ok() const { return used <= cap; }
+};
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+// Deterministic column sum out[c] = sum_r in[r*cols + c] (r in increasing
+// order within fixed slices, slices summed in order).  tmp: workspace of
+// colsum_tmp_floats(rows, cols) floats.
+int64_t colsum_tmp_floats(int64_t rows, int64_t cols);
+int colsum_f32(const float* in, int64_t rows, int64_t cols, float* out, float* tmp,
+               hipStream_t stream);
+
+// Persistent-grid block count: enough blocks for `tiles` work items of
+// `per_block` each, capped at `max_blocks`.
+static inline int persistent_blocks(int64_t tiles, int per_block, int max_blocks) {
+    int64_t b = (tiles + per_block - 1) / per_block;
+    if (b > max_blocks) b = max_blocks;
+    if (b < 1) b = 1;
+    return (int)b;
+}
+
+// ---- wave helpers (wave64) ----
+__device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63); }
+
+__device__ __forceinline__ float shfl(float v, int src_lane) {
+    return __shfl(v, src_lane, 64);
+}
+
+// Sum over the four lane groups (lanes l, l^16, l^32, l^48).
+__device__ __forceinline__ float sum_groups(float v) {
+    v += __shfl_xor(v, 16, 64);
+    v += __shfl_xor(v, 32, 64);
+    return v;
+}
+
+// Sum over the 16 lanes of a lane group (lanes sharing l/16).
+__device__ __forceinline__ float sum_in_group(float v) {
+    v += __shfl_xor(v, 1, 64);
+    v += __shfl_xor(v, 2, 64);
+    v += __shfl_xor(v, 4, 64);
+    v += __shfl_xor(v, 8, 64);
+    return v;
+}
+
+__device__ __forceinline__ v4f mfma4(float a, float b, v4f c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+}  // namespace am

@@ -1,0 +1,40 @@
+// Helpers shared by the fused geometry and radiance kernels.
+#pragma once
+#include "am_common.h"
+
+namespace am {
+
+static constexpr int kFieldBlocksMax = 1024;
+
+static inline int field_blocks(int64_t n) {
+    return persistent_blocks((n + 15) / 16, 4, kFieldBlocksMax);
+}
+
+static inline int check_mlp_common(const am_mlp_desc* m) {
+    AM_CHECK_ARG(m != nullptr, "mlp descriptor is NULL");
+    AM_CHECK_ARG(m->n_neurons == 64, "n_neurons %d: only 64 is implemented", m->n_neurons);
+    AM_CHECK_ARG(m->n_hidden_layers == 1 || m->n_hidden_layers == 2,
+                 "fused field kernels take 1 or 2 hidden layers (got %d)", m->n_hidden_layers);
+    AM_CHECK_ARG(m->hidden_act == AM_ACT_RELU || m->hidden_act == AM_ACT_SOFTPLUS ||
+                     m->hidden_act == AM_ACT_NONE,
+                 "hidden_act %d", m->hidden_act);
+    AM_CHECK_ARG(m->output_act == AM_ACT_NONE, "fused field kernels need output_act none");
+    AM_CHECK_ARG(m->n_out >= 1 && m->n_out <= 16, "n_out %d out of [1,16]", m->n_out);
+    return AM_OK;
+}
+
+// Flat VanillaMLP parameter layout [W1,b1,(W2,b2),Wo,bo].
+static inline void layout_params(int n_in, int n_out, int nh, int64_t* pw1, int64_t* pb1,
+                                 int64_t* pw2, int64_t* pb2, int64_t* pwo, int64_t* pbo,
+                                 int64_t* n_params) {
+    int64_t o = 0;
+    *pw1 = o; o += 64LL * n_in;
+    *pb1 = o; o += 64;
+    *pw2 = o; *pb2 = o;
+    if (nh == 2) { *pw2 = o; o += 64 * 64; *pb2 = o; o += 64; }
+    *pwo = o; o += 64LL * n_out;
+    *pbo = o; o += n_out;
+    *n_params = o;
+}
+
+}  // namespace am

@@ -1,0 +1,125 @@
+// Device-side multiresolution hash-grid arithmetic (gfx950).
+//
+// Index and position arithmetic restates tiny-cuda-nn's HashGrid (the
+// encoding behind models/network_utils.py:45-47): pos = fmaf(scale, x, 0.5),
+// cell = floor(pos), frac = pos - cell; corner c (bit0 x, bit1 y, bit2 z)
+// takes cell + bit; dense index x + y*res + z*res^2 or prime hash
+// x ^ y*2654435761 ^ z*805459861 (uint32 wrap), then mod the level's rows;
+// trilinear weight = product over dims in x,y,z order, blend by fma in corner
+// order.  Every step is a single rounded fp32 op (-ffp-contract=off), so the
+// indices and positions are bit-exact with the C restatement in the oracle.
+#pragma once
+#include "am_common.h"
+
+namespace am {
+
+// One level's parameters (16 bytes: kept in LDS by the fused kernels, where
+// lane groups address different levels).
+struct LevelParams {
+    float scale;
+    uint32_t res;
+    uint32_t offset;
+    uint32_t size_flags;  // rows | bit31 hashed | bit30 rows is a power of two
+};
+
+constexpr uint32_t kHashedBit = 1u << 31;
+constexpr uint32_t kPow2Bit = 1u << 30;
+constexpr uint32_t kSizeMask = kPow2Bit - 1u;
+
+struct GridArgs {
+    LevelParams lv[AM_MAX_LEVELS];
+    int32_t L;
+    int32_t F;
+};
+
+GridArgs make_grid_args(const am_grid_desc* g);
+
+__device__ __forceinline__ uint32_t grid_row(const LevelParams& p, uint32_t cx, uint32_t cy,
+                                             uint32_t cz) {
+    const uint32_t sz = p.size_flags & kSizeMask;
+    uint32_t idx;
+    if (p.size_flags & kHashedBit) {
+        idx = cx ^ (cy * 2654435761u) ^ (cz * 805459861u);
+    } else {
+        idx = cx + cy * p.res + cz * (p.res * p.res);
+    }
+    if (p.size_flags & kPow2Bit) return idx & (sz - 1u);
+    return idx < sz ? idx : idx % sz;
+}
+
+// Cell and fractional position of x at one level.
+struct LevelPos {
+    uint32_t c[3];
+    float f[3];
+};
+
+__device__ __forceinline__ LevelPos level_pos(const LevelParams& lp, const float x[3]) {
+    LevelPos p;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const float v = __fmaf_rn(lp.scale, x[d], 0.5f);
+        const float fl = floorf(v);
+        p.c[d] = (uint32_t)(int)fl;
+        p.f[d] = v - fl;
+    }
+    return p;
+}
+
+// Trilinear weight of corner c (tcnn order: start at 1, multiply x, y, z).
+__device__ __forceinline__ float corner_weight(const LevelPos& p, int c) {
+    float w = 1.0f;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) w *= ((c >> d) & 1) ? p.f[d] : (1.0f - p.f[d]);
+    return w;
+}
+
+// d(weight_c)/d(frac_d).
+__device__ __forceinline__ float corner_dweight(const LevelPos& p, int c, int d) {
+    float w = ((c >> d) & 1) ? 1.0f : -1.0f;
+#pragma unroll
+    for (int e = 0; e < 3; ++e)
+        if (e != d) w *= ((c >> e) & 1) ? p.f[e] : (1.0f - p.f[e]);
+    return w;
+}
+
+__device__ __forceinline__ uint32_t corner_row(const LevelParams& lp, const LevelPos& p, int c) {
+    return lp.offset + grid_row(lp, p.c[0] + (c & 1), p.c[1] + ((c >> 1) & 1),
+                                p.c[2] + ((c >> 2) & 1));
+}
+
+// Two-feature level encode (the configuration on the NeuS path).  Optionally
+// returns d enc_f / d x_d (including the level scale) for input gradients.
+template <bool WITH_DX>
+__device__ __forceinline__ void encode_level_f2(const LevelParams& lp, const float x[3],
+                                                const float2* __restrict__ table, float* out2,
+                                                float (*dx)[3]) {
+    const LevelPos p = level_pos(lp, x);
+    float2 v[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) v[c] = table[corner_row(lp, p, c)];
+    float a0 = 0.0f, a1 = 0.0f;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        const float w = corner_weight(p, c);
+        a0 = __fmaf_rn(w, v[c].x, a0);
+        a1 = __fmaf_rn(w, v[c].y, a1);
+    }
+    out2[0] = a0;
+    out2[1] = a1;
+    if constexpr (WITH_DX) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            float s0 = 0.0f, s1 = 0.0f;
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                const float dw = corner_dweight(p, c, d);
+                s0 = __fmaf_rn(dw, v[c].x, s0);
+                s1 = __fmaf_rn(dw, v[c].y, s1);
+            }
+            dx[0][d] = s0 * lp.scale;
+            dx[1][d] = s1 * lp.scale;
+        }
+    }
+}
+
+}  // namespace am

@@ -1,0 +1,290 @@
+// Ray-AABB intersection and occupancy-grid ray marching (nerfacc 0.3.3
+// ray_aabb_intersect + ray_marching with AABB contraction and cone angle 0,
+// as called at models/neus.py:209-220): count pass -> exclusive scan ->
+// fill pass, giving packed ray-sorted (ray_indices, t_starts, t_ends).
+//
+// The arithmetic is the same IEEE sequence as the C restatement in the
+// oracle (built with -ffp-contract=off; `origin + t_mid*dir` fused), so
+// sample intervals and ray indices are bit-exact with it.  Build-side
+// additions shared with the oracle: an optional per-ray sample cap and the
+// end of a ray when a step no longer advances t.
+#include <rocprim/device/device_scan.hpp>
+#include "am_common.h"
+
+namespace am {
+
+struct MarchArgs {
+    float roi[6];
+    int res;
+    float step;
+    int cap;
+    int has_grid;
+};
+
+__device__ __forceinline__ void aabb_one(const float* o, const float* d, const float* aabb,
+                                         float* t_near, float* t_far) {
+    float tmin = (aabb[0] - o[0]) / d[0];
+    float tmax = (aabb[3] - o[0]) / d[0];
+    if (tmin > tmax) { const float t = tmin; tmin = tmax; tmax = t; }
+    float tymin = (aabb[1] - o[1]) / d[1];
+    float tymax = (aabb[4] - o[1]) / d[1];
+    if (tymin > tymax) { const float t = tymin; tymin = tymax; tymax = t; }
+    if (tmin > tymax || tymin > tmax) { *t_near = 1e10f; *t_far = 1e10f; return; }
+    if (tymin > tmin) tmin = tymin;
+    if (tymax < tmax) tmax = tymax;
+    float tzmin = (aabb[2] - o[2]) / d[2];
+    float tzmax = (aabb[5] - o[2]) / d[2];
+    if (tzmin > tzmax) { const float t = tzmin; tzmin = tzmax; tzmax = t; }
+    if (tmin > tzmax || tzmin > tmax) { *t_near = 1e10f; *t_far = 1e10f; return; }
+    if (tzmin > tmin) tmin = tzmin;
+    if (tzmax < tmax) tmax = tzmax;
+    *t_near = tmin > 0.0f ? tmin : 0.0f;
+    *t_far = tmax;
+}
+
+__device__ __forceinline__ bool occupied(const MarchArgs& a, const float xyz[3],
+                                         const uint8_t* __restrict__ grid) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+        if (xyz[d] < a.roi[d] || xyz[d] > a.roi[3 + d]) return false;
+    if (!a.has_grid) return true;
+    int ix[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const float u = (xyz[d] - a.roi[d]) / (a.roi[3 + d] - a.roi[d]);
+        int v = (int)(u * (float)a.res);
+        v = v < 0 ? 0 : (v > a.res - 1 ? a.res - 1 : v);
+        ix[d] = v;
+    }
+    return grid[((int64_t)ix[0] * a.res + ix[1]) * a.res + ix[2]] != 0;
+}
+
+__device__ __forceinline__ float dist_next_voxel(const MarchArgs& a, const float xyz[3],
+                                                 const float dir[3], const float inv[3]) {
+    float t = INFINITY;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const float range = a.roi[3 + d] - a.roi[d];
+        const float u = ((xyz[d] - a.roi[d]) / range) * (float)a.res;
+        const float sg = copysignf(1.0f, dir[d]);
+        const float tt = ((floorf(u + 0.5f + 0.5f * sg) - u) * inv[d]) / (float)a.res * range;
+        t = fminf(t, tt);
+    }
+    return fmaxf(t, 0.0f);
+}
+
+// Marches one ray; writes samples when out pointers are given.  Returns count.
+__device__ int march_one(const MarchArgs& a, const float* o, const float* d, float near,
+                         float far, const uint8_t* __restrict__ grid, float* ts, float* te,
+                         int32_t* ri, int32_t ray) {
+    const float inv[3] = {1.0f / d[0], 1.0f / d[1], 1.0f / d[2]};
+    const float step = a.step;
+    int j = 0;
+    float t0 = near;
+    float t1 = t0 + step;
+    float t_mid = (t0 + t1) * 0.5f;
+    while (t_mid < far) {
+        if (a.cap > 0 && j >= a.cap) break;
+        float xyz[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) xyz[k] = __fmaf_rn(t_mid, d[k], o[k]);
+        if (occupied(a, xyz, grid)) {
+            if (ts) {
+                ts[j] = t0;
+                te[j] = t1;
+                ri[j] = ray;
+            }
+            ++j;
+            const float prev = t_mid;
+            t0 = t1;
+            t1 = t0 + step;
+            t_mid = (t0 + t1) * 0.5f;
+            if (!(t_mid > prev)) break;
+        } else {
+            const float target = t_mid + dist_next_voxel(a, xyz, d, inv);
+            float tt = t_mid;
+            bool stuck = false;
+            do {
+                const float nt = tt + step;
+                if (!(nt > tt)) { stuck = true; break; }
+                tt = nt;
+            } while (tt < target);
+            if (stuck) break;
+            t_mid = tt;
+            t0 = t_mid - step * 0.5f;
+            t1 = t_mid + step * 0.5f;
+        }
+    }
+    return j;
+}
+
+__device__ __forceinline__ float ray_near(const float* t_near, const float* jitter, int64_t i,
+                                          float step) {
+    float nr = t_near[i];
+    if (jitter) nr = nr + jitter[i] * step;
+    return nr;
+}
+
+__global__ void k_ray_aabb(const float* __restrict__ rays, int64_t n, float aabb0, float aabb1,
+                           float aabb2, float aabb3, float aabb4, float aabb5,
+                           float* __restrict__ t_near, float* __restrict__ t_far) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float aabb[6] = {aabb0, aabb1, aabb2, aabb3, aabb4, aabb5};
+    float o[3] = {rays[6 * i], rays[6 * i + 1], rays[6 * i + 2]};
+    float d[3] = {rays[6 * i + 3], rays[6 * i + 4], rays[6 * i + 5]};
+    aabb_one(o, d, aabb, t_near + i, t_far + i);
+}
+
+__global__ void k_march_count(MarchArgs a, const float* __restrict__ rays, int64_t n,
+                              const float* __restrict__ t_near, const float* __restrict__ t_far,
+                              const float* __restrict__ jitter, const uint8_t* __restrict__ grid,
+                              int32_t* __restrict__ counts) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float o[3] = {rays[6 * i], rays[6 * i + 1], rays[6 * i + 2]};
+    const float d[3] = {rays[6 * i + 3], rays[6 * i + 4], rays[6 * i + 5]};
+    counts[i] = march_one(a, o, d, ray_near(t_near, jitter, i, a.step), t_far[i], grid, nullptr,
+                          nullptr, nullptr, (int32_t)i);
+}
+
+// packed[i] = (start, count) from counts and their exclusive scan; total.
+__global__ void k_march_pack(const int32_t* __restrict__ counts, const int32_t* __restrict__ starts,
+                             int64_t n, int32_t* __restrict__ packed, int32_t* __restrict__ total) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n == 0 && i == 0) total[0] = 0;
+    if (i >= n) return;
+    packed[2 * i] = starts[i];
+    packed[2 * i + 1] = counts[i];
+    if (i == n - 1) total[0] = starts[i] + counts[i];
+}
+
+__global__ void k_march_fill(MarchArgs a, const float* __restrict__ rays, int64_t n,
+                             const float* __restrict__ t_near, const float* __restrict__ t_far,
+                             const float* __restrict__ jitter, const uint8_t* __restrict__ grid,
+                             const int32_t* __restrict__ packed, int32_t* __restrict__ ridx,
+                             float* __restrict__ ts, float* __restrict__ te) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t base = packed[2 * i];
+    const float o[3] = {rays[6 * i], rays[6 * i + 1], rays[6 * i + 2]};
+    const float d[3] = {rays[6 * i + 3], rays[6 * i + 4], rays[6 * i + 5]};
+    MarchArgs b = a;
+    b.cap = packed[2 * i + 1];  // never write past the counted samples
+    if (b.cap == 0) return;
+    march_one(b, o, d, ray_near(t_near, jitter, i, a.step), t_far[i], grid, ts + base, te + base,
+              ridx + base, (int32_t)i);
+}
+
+static MarchArgs march_args(const float* roi6, const uint8_t* grid, int res, float step, int cap) {
+    MarchArgs a;
+    for (int d = 0; d < 6; ++d) a.roi[d] = roi6[d];
+    a.res = res;
+    a.step = step;
+    a.cap = cap;
+    a.has_grid = grid != nullptr;
+    return a;
+}
+
+static size_t scan_tmp_bytes(int64_t n) {
+    size_t bytes = 0;
+    rocprim::exclusive_scan(nullptr, bytes, (const int32_t*)nullptr, (int32_t*)nullptr, 0,
+                            (size_t)n, rocprim::plus<int32_t>(), 0, false);
+    return bytes;
+}
+
+}  // namespace am
+
+using namespace am;
+
+extern "C" int am_ray_aabb(const float* rays, int64_t n_rays, const float* aabb6_host,
+                           float* t_near, float* t_far, void* stream) {
+    AM_CHECK_ARG(n_rays >= 0, "n_rays < 0");
+    if (n_rays == 0) return AM_OK;
+    AM_CHECK_ARG(rays && aabb6_host && t_near && t_far, "NULL buffer");
+    const float* b = aabb6_host;
+    hipLaunchKernelGGL(k_ray_aabb, dim3((unsigned)ceil_div(n_rays, 256)), dim3(256), 0,
+                       to_stream(stream), rays, n_rays, b[0], b[1], b[2], b[3], b[4], b[5], t_near,
+                       t_far);
+    AM_LAUNCH_CHECK();
+    return AM_OK;
+}
+
+extern "C" size_t am_march_workspace_size(int64_t n_rays) {
+    if (n_rays < 0) return 0;
+    WsCarver c(nullptr, 0);
+    c.take<int32_t>((size_t)n_rays + 1);
+    c.take<int32_t>((size_t)n_rays + 1);
+    c.take<char>(scan_tmp_bytes(n_rays > 0 ? n_rays : 1) + 256);
+    return c.used + 256;
+}
+
+static int check_march(int64_t n_rays, const float* rays, const float* t_near, const float* t_far,
+                       const float* roi6, const uint8_t* grid, int32_t res, float step,
+                       int32_t cap) {
+    AM_CHECK_ARG(n_rays >= 0 && n_rays < (1ll << 31), "n_rays %lld", (long long)n_rays);
+    AM_CHECK_ARG(rays && t_near && t_far && roi6, "NULL buffer");
+    AM_CHECK_ARG(!grid || (res >= 1 && res <= 4096), "grid_res %d", res);
+    AM_CHECK_ARG(step > 0.0f, "step %g <= 0", (double)step);
+    AM_CHECK_ARG(cap >= 0, "cap < 0");
+    for (int d = 0; d < 3; ++d) AM_CHECK_ARG(roi6[3 + d] > roi6[d], "empty roi");
+    return AM_OK;
+}
+
+extern "C" int am_march_count(const float* rays, int64_t n_rays, const float* t_near,
+                              const float* t_far, const float* jitter, const float* roi6_host,
+                              const uint8_t* grid, int32_t grid_res, float step, int32_t cap,
+                              int32_t* packed, int32_t* total, void* workspace,
+                              size_t workspace_bytes, void* stream) {
+    if (int rc = check_march(n_rays, rays, t_near, t_far, roi6_host, grid, grid_res, step, cap))
+        return rc;
+    AM_CHECK_ARG(packed && total && workspace, "NULL packed/total/workspace");
+    hipStream_t s = to_stream(stream);
+    if (n_rays == 0) {
+        hipLaunchKernelGGL(k_march_pack, dim3(1), dim3(1), 0, s, (const int32_t*)nullptr,
+                           (const int32_t*)nullptr, (int64_t)0, packed, total);
+        AM_LAUNCH_CHECK();
+        return AM_OK;
+    }
+    WsCarver c(workspace, workspace_bytes);
+    int32_t* counts = c.take<int32_t>((size_t)n_rays + 1);
+    int32_t* starts = c.take<int32_t>((size_t)n_rays + 1);
+    size_t sb = scan_tmp_bytes(n_rays);
+    char* stmp = c.take<char>(sb + 256);
+    if (!c.ok()) {
+        set_error("workspace too small: need %zu, got %zu", c.used, workspace_bytes);
+        return AM_ERR_WORKSPACE;
+    }
+    const MarchArgs a = march_args(roi6_host, grid, grid_res, step, cap);
+    const unsigned nb = (unsigned)ceil_div(n_rays, 128);
+    hipLaunchKernelGGL(k_march_count, dim3(nb), dim3(128), 0, s, a, rays, n_rays, t_near, t_far,
+                       jitter, grid, counts);
+    AM_LAUNCH_CHECK();
+    hipError_t e = rocprim::exclusive_scan(stmp, sb, counts, starts, 0, (size_t)n_rays,
+                                           rocprim::plus<int32_t>(), s, false);
+    if (e != hipSuccess) {
+        set_error("exclusive scan failed: %s", hipGetErrorString(e));
+        return AM_ERR_HIP;
+    }
+    hipLaunchKernelGGL(k_march_pack, dim3((unsigned)ceil_div(n_rays, 256)), dim3(256), 0, s, counts,
+                       starts, n_rays, packed, total);
+    AM_LAUNCH_CHECK();
+    return AM_OK;
+}
+
+extern "C" int am_march_fill(const float* rays, int64_t n_rays, const float* t_near,
+                             const float* t_far, const float* jitter, const float* roi6_host,
+                             const uint8_t* grid, int32_t grid_res, float step, int32_t cap,
+                             const int32_t* packed, int32_t* ray_indices, float* t_starts,
+                             float* t_ends, void* stream) {
+    if (int rc = check_march(n_rays, rays, t_near, t_far, roi6_host, grid, grid_res, step, cap))
+        return rc;
+    if (n_rays == 0) return AM_OK;
+    AM_CHECK_ARG(packed && ray_indices && t_starts && t_ends, "NULL output");
+    const MarchArgs a = march_args(roi6_host, grid, grid_res, step, cap);
+    hipLaunchKernelGGL(k_march_fill, dim3((unsigned)ceil_div(n_rays, 128)), dim3(128), 0,
+                       to_stream(stream), a, rays, n_rays, t_near, t_far, jitter, grid, packed,
+                       ray_indices, t_starts, t_ends);
+    AM_LAUNCH_CHECK();
+    return AM_OK;
+}

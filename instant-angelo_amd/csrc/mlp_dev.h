@@ -1,0 +1,153 @@
+// Wavefront-resident MLP engine on CDNA4 fp32 MFMA (v_mfma_f32_16x16x4f32).
+//
+// A wave evaluates a tile of 16 points.  Lane l serves point p = l % 16 and
+// lane group k = l / 16.  For D = A(16x4) * B(4x16) the operand layouts are
+// A[l%16][l/16], B[l/16][l%16], D[4*(l/16)+r][l%16] (r = 0..3); a layer
+// computes D[out][p] = W[out][in] * X[in][p], so points are the N dimension.
+//
+// Row space: the B fragment of K-step s in lane group k holds input row
+// krow(s,k) = 16*(s/4) + 4*k + s%4.  That is exactly the row a D tile t,
+// register r puts in lane group k (t = s/4, r = s%4), so a layer's output
+// registers are the next layer's B fragments as they stand, and an input
+// gradient W^T*G comes out in the same lanes that hold the inputs.  Weight
+// matrices sit in LDS as [out rows][ld] with columns in row space.
+#pragma once
+#include "am_common.h"
+
+namespace am {
+namespace mlp {
+
+__device__ __forceinline__ int krow(int s, int k) { return ((s >> 2) << 4) + (k << 2) + (s & 3); }
+
+// Orders LDS traffic of one wave (lanes exchange data through LDS).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// ---- activations (torch semantics) ----
+__device__ __forceinline__ float act_fwd(int act, float z, float beta) {
+    if (act == AM_ACT_RELU) return z > 0.0f ? z : 0.0f;
+    if (act == AM_ACT_SOFTPLUS) {
+        const float bz = z * beta;
+        return bz > 20.0f ? z : log1pf(expf(bz)) / beta;
+    }
+    if (act == AM_ACT_SIGMOID) return 1.0f / (1.0f + expf(-z));
+    return z;
+}
+
+// d act / d z evaluated at pre-activation z.
+__device__ __forceinline__ float act_bwd(int act, float z, float beta) {
+    if (act == AM_ACT_RELU) return z > 0.0f ? 1.0f : 0.0f;
+    if (act == AM_ACT_SOFTPLUS) {
+        const float bz = z * beta;
+        if (bz > 20.0f) return 1.0f;
+        const float e = expf(bz);
+        return e / (e + 1.0f);
+    }
+    if (act == AM_ACT_SIGMOID) {
+        const float y = 1.0f / (1.0f + expf(-z));
+        return (1.0f - y) * y;
+    }
+    return 1.0f;
+}
+
+__device__ __forceinline__ v4f act4(int act, v4f z, float beta) {
+    v4f h;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) h[r] = act_fwd(act, z[r], beta);
+    return h;
+}
+
+// Bias into a D tile: lane group k, register r -> row 16t + 4k + r.
+__device__ __forceinline__ v4f bias_tile(const float* __restrict__ b, int t, int nrows) {
+    const int k = lane_id() >> 4;
+    v4f v;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int row = 16 * t + 4 * k + r;
+        v[r] = row < nrows ? b[row] : 0.0f;
+    }
+    return v;
+}
+
+// acc[ot] += W[16ot + i][krow(s,k)] * b[s]
+template <int KS, int OT>
+__device__ __forceinline__ void mm_fwd(const float* __restrict__ W, int ld, const float* b,
+                                       v4f* acc) {
+    const int l = lane_id();
+    const int i = l & 15, k = l >> 4;
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+#pragma unroll
+        for (int ot = 0; ot < OT; ++ot) {
+            const float a = W[(16 * ot + i) * ld + krow(s, k)];
+            acc[ot] = mfma4(a, b[s], acc[ot]);
+        }
+    }
+}
+
+// gin[it] += W[krow(s,k)][16it + i] * g[s]   (W^T * G)
+template <int KS, int IT>
+__device__ __forceinline__ void mm_bwd_in(const float* __restrict__ W, int ld, const float* g,
+                                          v4f* gin) {
+    const int l = lane_id();
+    const int i = l & 15, k = l >> 4;
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+#pragma unroll
+        for (int it = 0; it < IT; ++it) {
+            const float a = W[krow(s, k) * ld + 16 * it + i];
+            gin[it] = mfma4(a, g[s], gin[it]);
+        }
+    }
+}
+
+constexpr int kScratchLd = 17;
+
+// dW[ot*IT + it] += G (OT D tiles, rows = out) x X^T (KS B fragments, rows =
+// input row space) summed over the 16 points of the tile.  Transposes go
+// through the wave-private LDS scratch (16*(OT+IT) rows x 17).
+// Result layout: lane (i,k), register r -> dW[16ot + 4k + r][16it + i].
+template <int OT, int IT, int KS>
+__device__ __forceinline__ void mm_dw(const v4f* G, const float* X, float* scratch, v4f* dW) {
+    const int l = lane_id();
+    const int i = l & 15, k = l >> 4;
+    float* Gs = scratch;
+    float* Xs = scratch + 16 * OT * kScratchLd;
+#pragma unroll
+    for (int t = 0; t < OT; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) Gs[(16 * t + 4 * k + r) * kScratchLd + i] = G[t][r];
+#pragma unroll
+    for (int s = 0; s < 4 * IT; ++s) Xs[krow(s, k) * kScratchLd + i] = s < KS ? X[s] : 0.0f;
+    wave_sync();
+#pragma unroll
+    for (int ot = 0; ot < OT; ++ot) {
+#pragma unroll
+        for (int it = 0; it < IT; ++it) {
+            v4f acc = dW[ot * IT + it];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const float a = Gs[(16 * ot + i) * kScratchLd + 4 * q + k];
+                const float b = Xs[(16 * it + i) * kScratchLd + 4 * q + k];
+                acc = mfma4(a, b, acc);
+            }
+            dW[ot * IT + it] = acc;
+        }
+    }
+    wave_sync();
+}
+
+// Sum of a D tile's rows over the 16 points (bias gradient), all lanes of the
+// group receive the sums of their 4 rows.
+__device__ __forceinline__ v4f rowsum_tile(v4f g) {
+    v4f s;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) s[r] = sum_in_group(g[r]);
+    return s;
+}
+
+}  // namespace mlp
+}  // namespace am

@@ -1,0 +1,49 @@
+// Deterministic table-gradient reduction: (row key, value) pairs emitted by a
+// backward kernel -> stable radix sort by key -> segmented reduce in a fixed
+// order -> one write per touched row; untouched rows are written 0.
+//
+// The emitters write pairs into carve.keys[0] / carve.vals[0].  Keys equal to
+// sentinel() mark empty slots; they sort last and are skipped.
+#pragma once
+#include "am_common.h"
+
+namespace am {
+
+template <int F>
+struct Vals {
+    float v[F];
+};
+
+constexpr int kSegChunk = 32;  // items reduced sequentially per thread
+
+struct TableReduceWs {
+    uint32_t* keys[2];
+    void* vals[2];
+    void* sort_tmp;
+    size_t sort_bytes;
+    float* lead;
+    float* tail;
+    uint8_t* lead_full;
+    uint8_t* tail_open;
+    int64_t m;
+    int end_bit;
+};
+
+// Number of key bits: smallest b with 2^b > n_entries (so the sentinel
+// (2^b - 1) is >= n_entries and never a valid row).
+int table_key_bits(uint32_t n_entries);
+static inline uint32_t table_sentinel(int end_bit) { return (uint32_t)((1ull << end_bit) - 1ull); }
+
+// Carves the reduction workspace for m pairs of F features out of `c`.
+// Returns the sort's temp byte count through TableReduceWs (0 when c has no
+// base pointer: size query; the sort size query needs a device).
+int table_reduce_carve(WsCarver& c, int F, int64_t m, uint32_t n_entries, TableReduceWs* out);
+
+// Writes v to p[0..n) on `stream` (a fill kernel, safe under graph capture).
+int fill_f32(float* p, int64_t n, float v, hipStream_t stream);
+
+// Runs sort + segmented reduce; dtable [n_entries, F] fully overwritten.
+int table_reduce_run(TableReduceWs& w, int F, uint32_t n_entries, float* dtable,
+                     hipStream_t stream);
+
+}  // namespace am

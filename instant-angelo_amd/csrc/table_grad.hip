@@ -1,0 +1,189 @@
+// Deterministic table-gradient reduction: stable radix sort of (row, value)
+// pairs, then a fixed-order segmented reduce (see table_grad.h).
+#include <rocprim/device/device_radix_sort.hpp>
+#include "table_grad.h"
+
+namespace am {
+
+int table_key_bits(uint32_t n_entries) {
+    int b = 1;
+    while (b < 32 && (1ull << b) <= (uint64_t)n_entries) ++b;
+    return b;
+}
+
+template <int F>
+static hipError_t sort_pairs(void* tmp, size_t& bytes, TableReduceWs& w, uint32_t** kout,
+                             Vals<F>** vout, hipStream_t s) {
+    rocprim::double_buffer<uint32_t> kb(w.keys[0], w.keys[1]);
+    rocprim::double_buffer<Vals<F>> vb(reinterpret_cast<Vals<F>*>(w.vals[0]),
+                                       reinterpret_cast<Vals<F>*>(w.vals[1]));
+    hipError_t e = rocprim::radix_sort_pairs(tmp, bytes, kb, vb, (size_t)w.m, 0u,
+                                             (unsigned)w.end_bit, s, false);
+    if (kout) *kout = kb.current();
+    if (vout) *vout = vb.current();
+    return e;
+}
+
+int table_reduce_carve(WsCarver& c, int F, int64_t m, uint32_t n_entries, TableReduceWs* out) {
+    TableReduceWs w{};
+    w.m = m;
+    w.end_bit = table_key_bits(n_entries);
+    const size_t vbytes = (size_t)F * sizeof(float);
+    w.keys[0] = c.take<uint32_t>((size_t)m);
+    w.keys[1] = c.take<uint32_t>((size_t)m);
+    w.vals[0] = c.take<char>((size_t)m * vbytes);
+    w.vals[1] = c.take<char>((size_t)m * vbytes);
+    const int64_t nc = ceil_div(m, kSegChunk);
+    w.lead = c.take<float>((size_t)nc * F);
+    w.tail = c.take<float>((size_t)nc * F);
+    w.lead_full = c.take<uint8_t>((size_t)nc);
+    w.tail_open = c.take<uint8_t>((size_t)nc);
+    size_t sb = 0;
+    hipError_t e = hipSuccess;
+    switch (F) {
+        case 1: e = sort_pairs<1>(nullptr, sb, w, nullptr, nullptr, 0); break;
+        case 2: e = sort_pairs<2>(nullptr, sb, w, nullptr, nullptr, 0); break;
+        case 4: e = sort_pairs<4>(nullptr, sb, w, nullptr, nullptr, 0); break;
+        case 8: e = sort_pairs<8>(nullptr, sb, w, nullptr, nullptr, 0); break;
+        default: set_error("unsupported n_features %d", F); return AM_ERR_UNSUPPORTED;
+    }
+    if (e != hipSuccess) {
+        set_error("radix sort size query failed: %s", hipGetErrorString(e));
+        return AM_ERR_HIP;
+    }
+    w.sort_bytes = sb;
+    w.sort_tmp = c.take<char>(sb + 256);
+    *out = w;
+    return AM_OK;
+}
+
+__global__ void k_fill_f32(float* __restrict__ p, int64_t n, float v) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
+}
+
+// Pass A: each thread reduces kSegChunk consecutive sorted items.  Segments
+// wholly inside the chunk are written; the piece continuing a segment from the
+// previous chunk goes to lead[t] (lead_full: the chunk is entirely that piece
+// and the segment goes on); a segment starting here and continuing goes to
+// tail[t] (tail_open).
+template <int F>
+__global__ void k_seg_a(const uint32_t* __restrict__ keys, const Vals<F>* __restrict__ vals,
+                        int64_t m, uint32_t n_entries, float* __restrict__ out,
+                        float* __restrict__ lead, float* __restrict__ tail,
+                        uint8_t* __restrict__ lead_full, uint8_t* __restrict__ tail_open) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nc = (m + kSegChunk - 1) / kSegChunk;
+    if (t >= nc) return;
+    const int64_t b = t * kSegChunk;
+    const int64_t e = b + kSegChunk < m ? b + kSegChunk : m;
+    bool is_lead = b > 0 && keys[b] == keys[b - 1];
+    uint32_t cur = keys[b];
+    float acc[F];
+#pragma unroll
+    for (int f = 0; f < F; ++f) acc[f] = 0.0f;
+    for (int64_t j = b; j < e; ++j) {
+        const uint32_t k = keys[j];
+        if (k != cur) {
+            if (is_lead) {
+#pragma unroll
+                for (int f = 0; f < F; ++f) lead[t * F + f] = acc[f];
+                is_lead = false;
+            } else if (cur < n_entries) {
+#pragma unroll
+                for (int f = 0; f < F; ++f) out[(int64_t)cur * F + f] = acc[f];
+            }
+            cur = k;
+#pragma unroll
+            for (int f = 0; f < F; ++f) acc[f] = 0.0f;
+        }
+        const Vals<F> v = vals[j];
+#pragma unroll
+        for (int f = 0; f < F; ++f) acc[f] += v.v[f];
+    }
+    const bool cont_out = e < m && keys[e] == cur;
+    uint8_t lf = 0, to = 0;
+    if (is_lead) {
+#pragma unroll
+        for (int f = 0; f < F; ++f) lead[t * F + f] = acc[f];
+        lf = cont_out ? 1 : 0;
+    } else if (cont_out) {
+#pragma unroll
+        for (int f = 0; f < F; ++f) tail[t * F + f] = acc[f];
+        to = 1;
+    } else if (cur < n_entries) {
+#pragma unroll
+        for (int f = 0; f < F; ++f) out[(int64_t)cur * F + f] = acc[f];
+    }
+    lead_full[t] = lf;
+    tail_open[t] = to;
+}
+
+// Pass B: a segment spanning chunks is summed by the chunk it starts in:
+// its tail piece, then each following chunk's lead piece, in chunk order.
+template <int F>
+__global__ void k_seg_b(const uint32_t* __restrict__ keys, int64_t m, uint32_t n_entries,
+                        float* __restrict__ out, const float* __restrict__ lead,
+                        const float* __restrict__ tail, const uint8_t* __restrict__ lead_full,
+                        const uint8_t* __restrict__ tail_open) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nc = (m + kSegChunk - 1) / kSegChunk;
+    if (t >= nc || !tail_open[t]) return;
+    const int64_t last = (t + 1) * kSegChunk - 1;
+    const uint32_t key = keys[last];
+    if (key >= n_entries) return;
+    float acc[F];
+#pragma unroll
+    for (int f = 0; f < F; ++f) acc[f] = tail[t * F + f];
+    for (int64_t u = t + 1; u < nc; ++u) {
+#pragma unroll
+        for (int f = 0; f < F; ++f) acc[f] += lead[u * F + f];
+        if (!lead_full[u]) break;
+    }
+#pragma unroll
+    for (int f = 0; f < F; ++f) out[(int64_t)key * F + f] = acc[f];
+}
+
+int fill_f32(float* p, int64_t n, float v, hipStream_t s) {
+    if (n <= 0) return AM_OK;
+    int blocks = (int)(ceil_div(n, 256) < 8192 ? ceil_div(n, 256) : 8192);
+    hipLaunchKernelGGL(k_fill_f32, dim3(blocks), dim3(256), 0, s, p, n, v);
+    AM_LAUNCH_CHECK();
+    return AM_OK;
+}
+
+template <int F>
+static int run_f(TableReduceWs& w, uint32_t n_entries, float* dtable, hipStream_t s) {
+    if (int rc = fill_f32(dtable, (int64_t)n_entries * F, 0.0f, s)) return rc;
+    if (w.m == 0) return AM_OK;
+    uint32_t* ks = nullptr;
+    Vals<F>* vs = nullptr;
+    size_t sb = w.sort_bytes;
+    hipError_t e = sort_pairs<F>(w.sort_tmp, sb, w, &ks, &vs, s);
+    if (e != hipSuccess) {
+        set_error("radix sort failed: %s", hipGetErrorString(e));
+        return AM_ERR_HIP;
+    }
+    const int64_t nc = ceil_div(w.m, kSegChunk);
+    const int gb = (int)ceil_div(nc, 256);
+    hipLaunchKernelGGL(k_seg_a<F>, dim3(gb), dim3(256), 0, s, ks, vs, w.m, n_entries, dtable,
+                       w.lead, w.tail, w.lead_full, w.tail_open);
+    AM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_seg_b<F>, dim3(gb), dim3(256), 0, s, ks, w.m, n_entries, dtable, w.lead,
+                       w.tail, w.lead_full, w.tail_open);
+    AM_LAUNCH_CHECK();
+    return AM_OK;
+}
+
+int table_reduce_run(TableReduceWs& w, int F, uint32_t n_entries, float* dtable,
+                     hipStream_t stream) {
+    switch (F) {
+        case 1: return run_f<1>(w, n_entries, dtable, stream);
+        case 2: return run_f<2>(w, n_entries, dtable, stream);
+        case 4: return run_f<4>(w, n_entries, dtable, stream);
+        case 8: return run_f<8>(w, n_entries, dtable, stream);
+        default: set_error("unsupported n_features %d", F); return AM_ERR_UNSUPPORTED;
+    }
+}
+
+}  // namespace am
