@@ -1,0 +1,114 @@
+"""Training-step glue of NeuSSystem (systems/neus.py:24-206) without
+PyTorch-Lightning: the loss terms of training_step (:120-206) with the
+scheduled weights of BaseSystem.C (systems/base.py:28-45), the dynamic ray
+count (:126-128), and a dynamic loss scaler for the reduced-precision form.
+"""
+import torch
+import torch.nn.functional as F
+
+from .utils import scheduled_value
+
+
+def binary_cross_entropy(input, target):
+    """systems/criterions.py:155-159 (stable BCE)."""
+    return -(target * torch.log(input) + (1 - target) * torch.log(1 - input)).mean()
+
+
+class NeuSTrainer:
+    """Holds the per-step state of NeuSSystem that the hot path needs."""
+
+    def __init__(self, model, system_config, model_config):
+        self.model = model
+        self.cfg = system_config
+        self.mcfg = model_config
+        self.train_num_rays = int(model_config.train_num_rays)
+        self.train_num_samples = self.train_num_rays * (
+            model_config.num_samples_per_ray + model_config.get("num_samples_per_ray_bg", 0))
+        self.global_step = 0
+        self.current_epoch = 0
+
+    def C(self, value):
+        return scheduled_value(value, self.global_step, self.current_epoch)
+
+    def update_num_rays(self, out):
+        """systems/neus.py:126-128 (one host read, as in the reference)."""
+        if self.mcfg.get("dynamic_ray_sampling", False):
+            n = int(out["num_samples_full"].sum().item())
+            train_num_rays = int(self.train_num_rays * (self.train_num_samples / max(n, 1)))
+            self.train_num_rays = min(int(self.train_num_rays * 0.9 + train_num_rays * 0.1),
+                                      self.mcfg.max_train_num_rays)
+
+    def losses(self, out, batch):
+        """systems/neus.py:130-194; returns (loss, dict of terms)."""
+        L = self.cfg.loss
+        terms = {}
+        loss = 0.0
+        valid = out["rays_valid_full"][..., 0]
+        terms["rgb_mse"] = F.mse_loss(out["comp_rgb_full"][valid], batch["rgb"][valid])
+        loss = loss + terms["rgb_mse"] * self.C(L.lambda_rgb_mse)
+        if self.C(L.get("lambda_rgb_l1", 0.0)) > 0:
+            terms["rgb_l1"] = F.l1_loss(out["comp_rgb_full"][valid], batch["rgb"][valid])
+            loss = loss + terms["rgb_l1"] * self.C(L.lambda_rgb_l1)
+        terms["eikonal"] = ((torch.linalg.norm(out["sdf_grad_samples"], ord=2, dim=-1) - 1.0) ** 2).mean()
+        loss = loss + terms["eikonal"] * self.C(L.lambda_eikonal)
+        opacity = torch.clamp(out["opacity"].squeeze(-1), 1e-3, 1.0 - 1e-3)
+        if self.C(L.get("lambda_mask", 0.0)) > 0 and "fg_mask" in batch:
+            terms["mask"] = binary_cross_entropy(opacity, batch["fg_mask"].float())
+            loss = loss + terms["mask"] * self.C(L.lambda_mask)
+        if self.C(L.get("lambda_opaque", 0.0)) > 0:
+            terms["opaque"] = binary_cross_entropy(opacity, opacity)
+            loss = loss + terms["opaque"] * self.C(L.lambda_opaque)
+        if self.C(L.get("lambda_sparsity", 0.0)) > 0:
+            terms["sparsity"] = torch.exp(-L.sparsity_scale * out["sdf_samples"].abs()).mean()
+            loss = loss + terms["sparsity"] * self.C(L.lambda_sparsity)
+        if self.C(L.get("lambda_curvature", 0.0)) > 0:
+            terms["curvature"] = out["sdf_laplace_samples"].abs().mean()
+            loss = loss + terms["curvature"] * self.C(L.lambda_curvature)
+        for k in ("lambda_distortion", "lambda_sdf_l1"):
+            if self.C(L.get(k, 0.0)) > 0:
+                raise NotImplementedError(f"{k} > 0 is not part of this build")
+        return loss, terms
+
+
+class DynamicLossScaler:
+    """Dynamic loss scaling (the GradScaler semantics behind precision: 16):
+    scale the loss, skip the step and halve the scale on inf/nan gradients,
+    double it after `growth_interval` clean steps."""
+
+    def __init__(self, init_scale=2.0 ** 16, growth_factor=2.0, backoff_factor=0.5,
+                 growth_interval=2000):
+        self.scale = float(init_scale)
+        self.growth_factor, self.backoff_factor = growth_factor, backoff_factor
+        self.growth_interval = growth_interval
+        self._good = 0
+
+    def scale_loss(self, loss):
+        return loss * self.scale
+
+    def unscale_and_check(self, params):
+        """Unscales grads in place; returns True if all are finite."""
+        grads = [p.grad for p in params if p.grad is not None]
+        if not grads:
+            return True
+        inv = 1.0 / self.scale
+        for g in grads:
+            g.mul_(inv)
+        finite = torch.stack([torch.isfinite(g).all() for g in grads]).all()
+        return bool(finite.item())
+
+    def update(self, found_finite):
+        if found_finite:
+            self._good += 1
+            if self._good >= self.growth_interval:
+                self.scale *= self.growth_factor
+                self._good = 0
+        else:
+            self.scale *= self.backoff_factor
+            self._good = 0
+
+    def step(self, optimizer, params):
+        ok = self.unscale_and_check(params)
+        if ok:
+            optimizer.step()
+        self.update(ok)
+        return ok

@@ -90,19 +90,46 @@ def encode_exact(x, params, lt, active_levels=None):
     return out
 
 
-def encode_torch(x, params, lt, active_levels=None):
+def _rows_torch(cell, lt):
+    """Level-local rows of the 8 corners from integer cells [n,L,3] (int64),
+    uint32 wrap emulated with masks."""
+    m = 0xFFFFFFFF
+    out = torch.zeros(cell.shape[0], lt.n_levels, 8, dtype=torch.int64)
+    for l in range(lt.n_levels):
+        res, size = int(lt.res[l]), int(lt.size[l])
+        for c in range(8):
+            cx = (cell[:, l, 0] + (c & 1)) & m
+            cy = (cell[:, l, 1] + ((c >> 1) & 1)) & m
+            cz = (cell[:, l, 2] + ((c >> 2) & 1)) & m
+            if lt.hashed[l]:
+                idx = cx ^ ((cy * 2654435761) & m) ^ ((cz * 805459861) & m)
+            else:
+                idx = (cx + cy * res + cz * res * res) & m
+            out[:, l, c] = idx % size
+    return out
+
+
+def encode_torch(x, params, lt, active_levels=None, mode="f32"):
     """Differentiable encode (autograd w.r.t. params [n_entries, F] and x).
 
-    Rows and fractional positions come from the exact C restatement; the
-    fraction carries d frac / d x = scale_l through autograd.  Levels at or
-    above active_levels are multiplied by zero (the progressive mask)."""
+    mode 'f32': rows and fractional positions from the exact C restatement
+    (fp32 fmaf), the fraction carrying d frac / d x = scale_l.  mode 'f64':
+    positions, cells and fractions computed in float64 (error budgets).
+    Levels at or above active_levels are multiplied by zero (the mask)."""
     xs = x.reshape(-1, 3)
     n, L, F = xs.shape[0], lt.n_levels, lt.n_features
-    _, frac, idx, _ = corners(xs.detach().cpu().numpy(), lt)
-    frac_t = torch.from_numpy(frac).to(xs.dtype)                         # [n,L,3]
     scale = torch.from_numpy(lt.scale.astype(np.float64)).to(xs.dtype)   # [L]
-    frac_t = frac_t + scale[None, :, None] * (xs - xs.detach())[:, None, :]
-    rows = torch.from_numpy(idx.astype(np.int64) + lt.offset.astype(np.int64)[None, :, None])
+    if mode == "f32":
+        _, frac, idx, _ = corners(xs.detach().cpu().numpy(), lt)
+        frac_t = torch.from_numpy(frac).to(xs.dtype)                     # [n,L,3]
+        frac_t = frac_t + scale[None, :, None] * (xs - xs.detach())[:, None, :]
+        idx = torch.from_numpy(idx.astype(np.int64))
+    else:
+        pos = scale[None, :, None] * xs[:, None, :] + 0.5
+        cell = torch.floor(pos.detach())
+        frac_t = pos - cell
+        idx = _rows_torch(cell.to(torch.int64), lt)
+    rows = idx + torch.from_numpy(lt.offset.astype(np.int64))[None, :, None]
     one = torch.ones((), dtype=xs.dtype)
     outs = []
     for l in range(L):

@@ -1,0 +1,137 @@
+"""End-to-end NeuS training step (NeuSModel.forward + losses + backward) on
+the HIP path vs the oracle with shared parameters, and the occupancy update."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import render as orender
+
+import helpers
+
+pytestmark = pytest.mark.gpu
+
+
+def _system_cfg():
+    from angelo_amd.utils import to_config
+    return to_config({"loss": {"lambda_rgb_mse": 10.0, "lambda_eikonal": 0.1,
+                               "lambda_curvature": [0, 0, 0.5, 5000]}})
+
+
+def _leaf_params(model):
+    """Replace the weight-norm products by leaves so d loss / d [W,b] can be
+    compared with the oracle's flat parameter gradients."""
+    leaves = {}
+    for name, net in (("geo", model.geometry.network), ("rad", model.texture.network)):
+        leaf = net.flat_params().detach().clone().requires_grad_(True)
+        net.flat_params = (lambda l=leaf: l)
+        leaves[name] = leaf
+    return leaves
+
+
+@pytest.mark.parametrize("n_rays,geo_hidden,L", [(64, 1, 8), (96, 2, 16)])
+def test_neus_step_matches_oracle(dev, n_rays, geo_hidden, L):
+    import angelo_amd
+    from angelo_amd import scenes
+    from angelo_amd.system import NeuSTrainer
+    kw = dict(L=L, geo_hidden=geo_hidden)
+    if L == 16:
+        kw.update(log2T=16, base=32, scale=1.3195079107728942)
+    cfg = helpers.model_config(**kw)
+    cfg.train_num_rays = n_rays
+    model = helpers.randomize_(angelo_amd.make("neus", cfg), seed=1)
+    model.train()
+    model.update_step(0, 30000)
+    model = model.to(dev)
+    rays, gt = scenes.config0_batch(n_rays=n_rays, n_views=4, seed=2)
+    ridx, t0, t1, packed = scenes.uniform_samples(rays, 64)
+    n = ridx.shape[0]
+    g = torch.Generator().manual_seed(3)
+    randn = torch.randn(n, 3, generator=g)
+
+    # oracle, float64
+    field, radiance, ol = helpers.oracle_fns(model, torch.float64, "f64")
+    var = model.variance.variance.detach().cpu().double().clone().requires_grad_(True)
+    inv_s = torch.exp(var * 10.0).clip(1e-6, 1e6)
+    eps = float(model.geometry._finite_difference_eps)
+    o = orender.neus_forward(rays.double(), ridx, t0.double(), t1.double(), n_rays, field,
+                             radiance, inv_s, 1.0, eps, model.cos_anneal_ratio, randn.double())
+    o_loss, o_terms = orender.losses(o, gt.double(), 10.0, 0.1, 0.5)
+    o_loss.backward()
+
+    # HIP path
+    leaves = _leaf_params(model)
+    trainer = NeuSTrainer(model, _system_cfg(), cfg)
+    trainer.global_step = 30000
+    samples = (ridx.to(dev), t0.to(dev), t1.to(dev), packed.to(dev))
+    out = model(rays.to(dev), samples=samples, randn=randn.to(dev))
+    loss, terms = trainer.losses(out, {"rgb": gt.to(dev)})
+    loss.backward()
+
+    def chk(a, b, tol, name):
+        e = helpers.rel_err(a, b)
+        assert e < tol, (name, e)
+
+    chk(out["comp_rgb"], o["comp_rgb"], 1e-4, "comp_rgb")
+    chk(out["opacity"], o["opacity"], 1e-4, "opacity")
+    chk(out["depth"], o["depth"], 1e-4, "depth")
+    chk(out["comp_normal"], o["comp_normal"], 1e-3, "comp_normal")
+    chk(out["sdf_samples"], o["sdf"], 1e-5, "sdf")
+    chk(out["sdf_grad_samples"], o["grad"], 1e-3, "grad")
+    chk(out["sdf_laplace_samples"], o["laplace"], 5e-3, "laplace")
+    chk(out["weights"], o["weights"], 1e-4, "weights")
+    assert abs(loss.item() - o_loss.item()) / abs(o_loss.item()) < 1e-4
+    assert set(["comp_rgb", "comp_normal", "opacity", "depth", "rays_valid", "num_samples",
+                "sdf_samples", "sdf_grad_samples", "weights", "points", "intervals",
+                "ray_indices", "sdf_laplace_samples", "comp_rgb_bg", "num_samples_bg",
+                "rays_valid_bg", "comp_rgb_full", "num_samples_full", "rays_valid_full",
+                "inv_s"]) <= set(out.keys())
+    assert int(out["num_samples"].item()) == n
+    # gradients of every parameter group
+    chk(model.geometry.grid_module.params.grad.view(-1, 2), ol["table"].grad, 2e-3, "d table")
+    chk(leaves["geo"].grad, ol["geo"].grad, 2e-3, "d sdf mlp")
+    chk(leaves["rad"].grad, ol["rad"].grad, 1e-3, "d radiance mlp")
+    chk(model.variance.variance.grad.reshape(1), var.grad.reshape(1), 1e-3, "d variance")
+
+
+def test_occupancy_update_matches_oracle(dev):
+    import angelo_amd
+    from oracle import field as ofield
+    cfg = helpers.model_config(grid_prune=True, step=2.0 / 256)
+    cfg.grid_resolution = 32
+    model = helpers.randomize_(angelo_amd.make("neus", cfg), seed=4)
+    model.train()
+    model.update_step(1, 30000 + 1)  # not a multiple of 16: no update yet
+    model = model.to(dev)
+    og = model.occupancy_grid
+    g = torch.Generator().manual_seed(5)
+    occs0 = torch.rand(og.num_cells, generator=g) * 0.01
+    og.occs.copy_(occs0.to(dev))
+    m = 3000
+    cells = torch.randint(0, og.num_cells, (m,), generator=g).to(torch.int32)
+    cells[:10] = cells[10]                        # duplicates
+    jitter = torch.rand(m, 3, generator=g)
+    model._occ_update(cells.to(dev), jitter.to(dev), m, 0.001, 0.95)
+    # oracle
+    field, _, _ = helpers.oracle_fns(model, torch.float64, "f64")
+    res = og.resolution
+    c = cells.long()
+    coords = torch.stack([c // (res * res), (c // res) % res, c % res], -1).double()
+    pts = ((coords + jitter.double()) / res) * 2.0 - 1.0
+    with torch.no_grad():
+        sdf = ofield.volume_sdf(pts, field, 1.0, 1e-3, False, mode="f64")["sdf"]
+        inv_s = torch.exp(model.variance.variance.detach().cpu().double() * 10.0).clip(1e-6, 1e6)
+        h = model.render_step_size * 0.5
+        pc, nc = torch.sigmoid((sdf + h) * inv_s), torch.sigmoid((sdf - h) * inv_s)
+        alpha = ((pc - nc + 1e-5) / (pc + 1e-5)).clip(0, 1)
+    best = torch.full((og.num_cells,), -1.0, dtype=torch.float64)
+    best = best.scatter_reduce(0, c, alpha, reduce="amax")
+    occ = occs0.double().clone()
+    touched = best >= 0
+    occ[touched] = torch.maximum(occ[touched] * 0.95, best[touched])
+    thre = min(occ.mean().item(), 0.001)
+    got = og.occs.cpu().double()
+    assert helpers.rel_err(got, occ) < 1e-4
+    ref_bin = occ > thre
+    near = (occ - thre).abs() < 1e-6
+    gb = og.binary.cpu().flatten().bool()
+    assert torch.equal(gb[~near], ref_bin[~near])
