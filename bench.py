@@ -1,0 +1,208 @@
+#!/usr/bin/env python
+"""Benchmark: NeuS / Neuralangelo training step on one MI355X.
+
+A step is one pass of the hot path over one batch: occupancy update (every
+16th step), ray-AABB + occupancy marching, fused VolumeSDF (base + 6
+central-difference + 6 curvature evaluations per sample), fused radiance,
+NeuS alpha / transmittance / accumulation, rgb-MSE + eikonal + curvature
+losses, backward to every parameter, AdamW.  Inputs are device-resident
+before timing; the timed region is bracketed by device synchronisation.
+
+Prints one JSON line (metric, value, ms_per_step, roofline of the dominant
+kernel from live HIP-event timers, and the CPU oracle baseline).
+
+    python bench.py [--steps K] [--warmup W] [--workload neuralangelo-fp32|sphere-fp32]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "instant-angelo_amd"))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FP32_MFMA_PEAK_TFLOPS = 157.3
+TIMED_KERNELS = ["geo_fwd", "geo_bwd", "radix_sort", "seg_reduce", "rad_fwd", "rad_bwd",
+                 "composite_fwd", "composite_bwd", "march_count", "march_fill", "occ_sdf"]
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(kernel, spec, n_samples, n_rays):
+    """Algorithmic HBM bytes of one launch.  Per encode evaluation of L levels,
+    F features, fp32 table: 12 B position + 8*L*F*4 B corner reads (the fused
+    kernels keep the encoding on chip, so the L*F output term is dropped);
+    13 evaluations per sample (base + 6 FD + 6 curvature)."""
+    L, F = spec["levels"], 2
+    per_eval = 12 + 8 * L * F * 4
+    if kernel in ("geo_fwd", "geo_bwd"):
+        # + per-sample outputs (fwd) or upstream gradients (bwd): sdf, grad,
+        # normal, feature (19), laplace = 27 floats
+        return n_samples * (13 * per_eval + 27 * 4)
+    if kernel == "radix_sort":
+        # pairs (4 B key + 8 B value) read and written once
+        return n_samples * 13 * L * 8 * 12 * 2
+    if kernel in ("composite_fwd", "composite_bwd"):
+        return n_samples * 40 + n_rays * 32
+    return None
+
+
+def cpu_baseline(spec_name, workload, seconds=15.0):
+    """The oracle (torch CPU, fp32, CUDA-semantics restatement) on a bounded
+    sample of the same workload: fwd + losses + bwd, samples/s."""
+    import numpy as np
+    import torch
+    from oracle import march as omarch, render as orender
+    from oracle import field as ofield, hashgrid as ohg, nets as onets
+    cores = len(os.sched_getaffinity(0))
+    torch.set_num_threads(cores)
+    model = workload.model
+    spec = workload.spec
+    b = workload.batches[0]
+    R = 128 if not workload.uniform else 64
+    rays = b["rays"][:R].detach().cpu()
+    gt = b["rgb"][:R].detach().cpu()
+    if workload.uniform:
+        from angelo_amd import scenes
+        ridx, t0, t1, _ = scenes.uniform_samples(rays, spec["samples_per_ray"])
+    else:
+        og = model.occupancy_grid
+        tn, tf = omarch.ray_aabb(rays.numpy(), [-1, -1, -1, 1, 1, 1])
+        ri, a0, a1, _ = omarch.ray_marching(rays.numpy(), tn, tf, [-1, -1, -1, 1, 1, 1],
+                                            og.binary.cpu().numpy(), spec["step"], spec["cap"])
+        ridx, t0, t1 = torch.from_numpy(ri), torch.from_numpy(a0), torch.from_numpy(a1)
+    n = int(ridx.shape[0])
+    geo, tex = model.geometry, model.texture
+    g = geo.grid_module
+    lt = ohg.level_table(g.n_levels, g.base_resolution, g.per_level_scale, g.log2_hashmap_size)
+    table = g.table.detach().cpu().clone().requires_grad_(True)
+    gflat = geo.network.flat_params().detach().cpu().clone().requires_grad_(True)
+    rflat = tex.network.flat_params().detach().cpu().clone().requires_grad_(True)
+    gd = onets.mlp_dims(geo.network.dim_in, geo.network.dim_out, geo.network.n_hidden_layers)
+    rd = onets.mlp_dims(tex.network.dim_in, tex.network.dim_out, tex.network.n_hidden_layers)
+    field = ofield.field_from(lambda x: ohg.encode_torch(x, table, lt, g.active_levels, "f32"),
+                              lambda h: onets.vanilla_mlp(h, onets.unflatten(gflat, gd), "softplus"))
+
+    def radiance(feat, dirs, nrm):
+        sh = onets.sh_encode((dirs + 1.0) / 2.0, 4)
+        return torch.sigmoid(onets.vanilla_mlp(torch.cat([feat, sh, nrm], -1), onets.unflatten(rflat, rd)))
+
+    var = model.variance.variance.detach().cpu().clone().requires_grad_(True)
+    randn = torch.randn(n, 3)
+    eps = float(geo._finite_difference_eps)
+    times = []
+    t_start = time.time()
+    while len(times) < 3 or (time.time() - t_start < seconds and len(times) < 10):
+        t = time.perf_counter()
+        inv_s = torch.exp(var * 10.0).clip(1e-6, 1e6)
+        o = orender.neus_forward(rays, ridx, t0, t1, R, field, radiance, inv_s, 1.0, eps,
+                                 model.cos_anneal_ratio, randn)
+        loss, _ = orender.losses(o, gt)
+        loss.backward()
+        times.append(time.perf_counter() - t)
+        if time.time() - t_start > 4 * seconds:
+            break
+    med = sorted(times)[len(times) // 2]
+    return {"value": n / med, "unit": "samples/s", "cores": cores, "kind": "port",
+            "sample": f"{R} rays of the same workload, {n} samples, oracle fwd+loss+bwd in "
+                      f"torch CPU fp32, median of {len(times)} steps ({med:.2f} s/step)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="neuralangelo-fp32")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--rays", type=int, default=None)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
+              "path is a design note, not a measured configuration.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from angelo_amd import _native as nat
+    from angelo_amd.workloads import Workload
+    if not torch.cuda.is_available():
+        print("no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    dev = torch.device("cuda:0")
+    nat.load()
+    log(f"[bench] building workload {args.workload}")
+    wl = Workload(args.workload, dev, rays=args.rays)
+    spec = wl.spec
+    for i in range(args.warmup):
+        wl.step(i)
+    torch.cuda.synchronize()
+    log(f"[bench] warmup done ({args.warmup} steps); timing {args.steps} steps")
+    nat.timer_enable(True)
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    total_samples = 0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    start.record()
+    for i in range(args.steps):
+        total_samples += wl.step(args.warmup + i)
+    stop.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    gpu_ms = start.elapsed_time(stop)
+    kern = {}
+    for k in TIMED_KERNELS:
+        ms, cnt = nat.timer_read(k)
+        if cnt:
+            kern[k] = {"total_ms": ms, "launches": cnt, "avg_ms": ms / cnt}
+    nat.timer_enable(False)
+    ms_per_step = gpu_ms / args.steps
+    samples_per_step = total_samples / args.steps
+    rays = spec["rays"]
+    value = total_samples / (gpu_ms / 1e3)
+    log(f"[bench] {ms_per_step:.3f} ms/step, {samples_per_step:.0f} samples/step, wall {wall:.2f}s")
+    for k, v in sorted(kern.items(), key=lambda kv: -kv[1]["total_ms"]):
+        log(f"[bench]   {k:14s} {v['avg_ms']:.3f} ms x {v['launches']}  "
+            f"({100 * v['total_ms'] / gpu_ms:.1f}% of step time)")
+    # roofline of the dominant kernel
+    roof = None
+    if kern:
+        dom = max(kern, key=lambda k: kern[k]["total_ms"])
+        nbytes = algorithmic_bytes(dom, spec, samples_per_step, rays)
+        if nbytes is not None:
+            ach = nbytes / (kern[dom]["avg_ms"] / 1e3) / 1e9
+            roof = {"kernel": dom, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                    "algorithmic_bytes": int(nbytes), "avg_ms": round(kern[dom]["avg_ms"], 4)}
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("[bench] CPU baseline (oracle) ...")
+        try:
+            cpu = cpu_baseline(args.workload, wl)
+        except Exception as e:  # noqa: BLE001 - reported, not fatal for the GPU number
+            cpu = {"error": repr(e)}
+    line = {
+        "metric": "train samples/s (fwd+bwd)", "value": round(value, 1), "unit": "samples/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic (analytic scene, seeded cameras)",
+        "config": {"workload": args.workload, "rays_per_step": rays,
+                   "samples_per_step": round(samples_per_step, 1), "levels": spec["levels"],
+                   "log2_hashmap_size": spec["log2T"], "evals_per_sample": 13,
+                   "optimizer": "AdamW (in the timed step)"},
+        "rays_per_s": round(rays * args.steps / (gpu_ms / 1e3), 1),
+        "kernels_ms": {k: round(v["avg_ms"], 4) for k, v in kern.items()},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

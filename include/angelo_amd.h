@@ -39,6 +39,14 @@ extern "C" {
 const char* am_last_error(void);
 int am_version(void);
 
+/* Kernel timers for measurement: when enabled, the named hot kernels are
+ * bracketed by HIP events on their launch stream.  am_timer_read
+ * synchronises on the recorded events and returns the summed duration and
+ * launch count of one kernel name (e.g. "geo_bwd").  Not thread-safe;
+ * meant for benchmarks.  Enabling resets all counters. */
+int am_timer_enable(int32_t on);
+int am_timer_read(const char* name, double* total_ms, int64_t* count);
+
 /* ------------------------------------------------------------------------ */
 /* Multiresolution hash grid (tcnn HashGrid through ProgressiveBandHashGrid,  */
 /* models/network_utils.py:40-66, and get_encoding, :83-93).                 */

@@ -69,6 +69,8 @@ _SZ = ctypes.c_size_t
 _SIGS = {
     "am_last_error": (ctypes.c_char_p, []),
     "am_version": (ctypes.c_int, []),
+    "am_timer_enable": (ctypes.c_int, [_I32]),
+    "am_timer_read": (ctypes.c_int, [ctypes.c_char_p, _P, _P]),
     "am_grid_init": (ctypes.c_int, [_P, _I32, _I32, _I32, _I32, _F, _P]),
     "am_hashgrid_fwd": (ctypes.c_int, [_P, _P, _I64, _P, _I32, _P, _P]),
     "am_hashgrid_bwd_workspace_size": (_SZ, [_P, _I64, _I32]),
@@ -159,3 +161,15 @@ def require_cuda(*tensors):
 
 def workspace(nbytes, device):
     return torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device=device)
+
+
+def timer_enable(on=True):
+    check(load().am_timer_enable(1 if on else 0), "am_timer_enable")
+
+
+def timer_read(name):
+    """(total ms, launch count) of a timed kernel since timer_enable."""
+    ms = ctypes.c_double(0.0)
+    cnt = ctypes.c_int64(0)
+    check(load().am_timer_read(name.encode(), ctypes.byref(ms), ctypes.byref(cnt)), "am_timer_read")
+    return ms.value, cnt.value

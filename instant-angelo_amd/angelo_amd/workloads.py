@@ -1,0 +1,136 @@
+"""Benchmark workloads: model / loss configurations with the reference's
+config keys (configs/neuralangelo-colmap_sparse.yaml:16-134 layout) and the
+synthetic data that feeds them.
+
+  sphere-fp32   plain NeuS step: hash grid 8 levels x 2 features, 2^15 rows
+                per level, base 16, scale 1.5; SDF MLP 1x64 -> 1+15; radiance
+                2x64 sigmoid; 1024 rays x 64 uniform samples inside the unit
+                sphere; Lambertian sphere of radius 0.5 as target.
+  neuralangelo-fp32
+                16 levels x 2 features, 2^21 rows, base 32, finest 2049;
+                SDF MLP 1x64, radiance 2x64; 8192 rays per step marched
+                through a 128^3 occupancy grid with step 2/1024 (scene
+                diameter 2), at most 512 samples per ray; finite-difference
+                normals, eikonal and curvature losses; sphere + torus scene.
+"""
+import torch
+
+from . import make, scenes
+from .system import NeuSTrainer
+from .utils import to_config
+
+NEURALANGELO_SCALE = 1.3195079107728942
+
+
+def model_config(levels, log2T, base, scale, rays, samples_per_ray, step=None, cap=0,
+                 grid_prune=False, geo_hidden=1, feature_dim=16, radius=1.0, start_level=None,
+                 kahan=0, seed=0):
+    return to_config({
+        "name": "neus", "radius": radius, "num_samples_per_ray": samples_per_ray,
+        "train_num_rays": rays, "max_train_num_rays": rays, "dynamic_ray_sampling": False,
+        "ray_chunk": 2048, "randomized": True, "grid_prune": grid_prune,
+        "grid_prune_occ_thre": 0.001, "cos_anneal_end": 20000, "learned_background": False,
+        "background_color": "white", "render_step_size": step, "max_samples_per_ray": cap,
+        "kahan": kahan, "variance": {"init_val": 0.3, "modulate": False},
+        "geometry": {
+            "name": "volume-sdf", "radius": radius, "feature_dim": feature_dim,
+            "grad_type": "finite_difference", "finite_difference_eps": "progressive",
+            "seed": seed, "isosurface": None,
+            "xyz_encoding_config": {
+                "otype": "ProgressiveBandHashGrid", "n_levels": levels, "n_features_per_level": 2,
+                "log2_hashmap_size": log2T, "base_resolution": base, "per_level_scale": scale,
+                "include_xyz": True, "start_level": start_level if start_level else levels,
+                "start_step": 0, "update_steps": 1000},
+            "mlp_network_config": {"otype": "VanillaMLP", "activation": "ReLU",
+                                   "output_activation": "none", "n_neurons": 64,
+                                   "n_hidden_layers": geo_hidden, "sphere_init": True,
+                                   "sphere_init_radius": 0.5, "weight_norm": True}},
+        "texture": {
+            "name": "volume-radiance", "input_feature_dim": feature_dim + 6,
+            "dir_encoding_config": {"otype": "SphericalHarmonics", "degree": 4},
+            "mlp_network_config": {"otype": "VanillaMLP", "activation": "ReLU",
+                                   "output_activation": "none", "n_neurons": 64,
+                                   "n_hidden_layers": 2},
+            "color_activation": "sigmoid"},
+    })
+
+
+def system_config():
+    """Loss weights of configs/neuralangelo-colmap_sparse.yaml:121-133 (the
+    zero-weighted terms omitted) and the AdamW settings of :134-150."""
+    return to_config({
+        "loss": {"lambda_rgb_mse": 10.0, "lambda_eikonal": 0.1,
+                 "lambda_curvature": [0, 0, 0.5, 5000]},
+        "optimizer": {"lr": 0.01, "betas": (0.9, 0.99), "eps": 1e-15, "variance_lr": 0.001},
+    })
+
+
+def make_optimizer(model, scfg):
+    o = scfg.optimizer
+    groups = [{"params": list(model.geometry.parameters()), "lr": o.lr},
+              {"params": list(model.texture.parameters()), "lr": o.lr},
+              {"params": list(model.variance.parameters()), "lr": o.variance_lr}]
+    try:
+        return torch.optim.AdamW(groups, lr=o.lr, betas=tuple(o.betas), eps=o.eps, fused=True)
+    except (RuntimeError, TypeError):
+        return torch.optim.AdamW(groups, lr=o.lr, betas=tuple(o.betas), eps=o.eps, foreach=True)
+
+
+WORKLOADS = {
+    "neuralangelo-fp32": dict(levels=16, log2T=21, base=32, scale=NEURALANGELO_SCALE, rays=8192,
+                              samples_per_ray=512, step=2.0 / 1024, cap=512, grid_prune=True),
+    "sphere-fp32": dict(levels=8, log2T=15, base=16, scale=1.5, rays=1024, samples_per_ray=64,
+                        step=None, cap=0, grid_prune=False),
+}
+
+
+class Workload:
+    """Model, trainer, optimizer and a pool of device-resident batches."""
+
+    def __init__(self, name, device, pool=8, seed=0, rays=None, global_step=20000, band_cells=1.5):
+        spec = dict(WORKLOADS[name])
+        if rays is not None:
+            spec["rays"] = rays
+        self.name, self.spec, self.device = name, spec, device
+        self.uniform = not spec["grid_prune"]
+        mcfg = model_config(spec["levels"], spec["log2T"], spec["base"], spec["scale"], spec["rays"],
+                            spec["samples_per_ray"], spec["step"], spec["cap"], spec["grid_prune"],
+                            seed=seed)
+        torch.manual_seed(seed)
+        self.mcfg, self.scfg = mcfg, system_config()
+        self.model = make("neus", mcfg).to(device)
+        self.model.train()
+        self.trainer = NeuSTrainer(self.model, self.scfg, mcfg)
+        self.optimizer = make_optimizer(self.model, self.scfg)
+        self.global_step = global_step
+        self.batches = []
+        gen = torch.Generator().manual_seed(seed + 17)
+        if self.uniform:
+            for b in range(pool):
+                rays, rgb = scenes.config0_batch(spec["rays"], n_views=16, seed=seed + b)
+                rays, rgb = rays.to(device), rgb.to(device)
+                samples = scenes.uniform_samples(rays, spec["samples_per_ray"])
+                self.batches.append(dict(rays=rays, rgb=rgb, samples=samples))
+        else:
+            cams, whf = scenes.config1_pool(100, 512, 512, 40.0, seed)
+            for b in range(pool):
+                rays, rgb = scenes.config1_batch(cams, whf, spec["rays"], gen, device)
+                self.batches.append(dict(rays=rays, rgb=rgb, samples=None))
+            og = self.model.occupancy_grid
+            binary = scenes.prewarm_occupancy(scenes.sdf_union, og.resolution, 1.0, band_cells, device)
+            og.binary.copy_(binary)
+            og.occs.copy_(binary.flatten().float())
+        self.model.update_step(0, global_step - 1)
+
+    def step(self, i):
+        """One training step; returns the number of samples processed."""
+        b = self.batches[i % len(self.batches)]
+        gs = self.global_step + i
+        self.trainer.global_step = gs
+        self.model.update_step(0, gs)
+        out = self.model(b["rays"], samples=b["samples"])
+        loss, _ = self.trainer.losses(out, b)
+        self.optimizer.zero_grad(set_to_none=True)
+        loss.backward()
+        self.optimizer.step()
+        return out["ray_indices"].shape[0]

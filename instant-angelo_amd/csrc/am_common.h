@@ -40,6 +40,41 @@ const char* last_error();
     } while (0)
 
 static inline hipStream_t to_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+// Kernel timers (am_timer_enable / am_timer_read): a scope records a HIP
+// event pair around the launches it encloses when timing is on.
+bool timer_on();
+void timer_push(const char* name, hipEvent_t a, hipEvent_t b);
+hipEvent_t timer_event();
+// While set, timed launches are booked under this name instead of their own
+// (an entry point reusing another's kernels, e.g. the occupancy update).
+const char* timer_alias();
+void set_timer_alias(const char* alias);
+
+struct TimerAlias {
+    const char* prev;
+    explicit TimerAlias(const char* a) : prev(timer_alias()) { set_timer_alias(a); }
+    ~TimerAlias() { set_timer_alias(prev); }
+};
+
+struct KernelTimer {
+    const char* name;
+    hipStream_t s;
+    hipEvent_t a = nullptr, b = nullptr;
+    KernelTimer(const char* n, hipStream_t st) : name(timer_alias() ? timer_alias() : n), s(st) {
+        if (timer_on()) {
+            a = timer_event();
+            b = timer_event();
+            if (a && b) hipEventRecord(a, s);
+        }
+    }
+    ~KernelTimer() {
+        if (a && b) {
+            hipEventRecord(b, s);
+            timer_push(name, a, b);
+        }
+    }
+};
 static inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 static inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 

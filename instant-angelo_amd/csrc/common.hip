@@ -1,9 +1,49 @@
-// Error reporting and version for the C-ABI.
+// Error reporting, version, kernel timers and the deterministic column sum.
 #include <cstdarg>
 #include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
 #include "am_common.h"
 
 namespace am {
+
+struct TimedRecord {
+    std::string name;
+    hipEvent_t a, b;
+};
+static bool g_timer_on = false;
+static std::vector<TimedRecord> g_records;
+static std::vector<hipEvent_t> g_event_pool;
+
+bool timer_on() { return g_timer_on; }
+
+static const char* g_timer_alias = nullptr;
+const char* timer_alias() { return g_timer_alias; }
+void set_timer_alias(const char* a) { g_timer_alias = a; }
+
+hipEvent_t timer_event() {
+    if (!g_event_pool.empty()) {
+        hipEvent_t e = g_event_pool.back();
+        g_event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+void timer_push(const char* name, hipEvent_t a, hipEvent_t b) {
+    g_records.push_back(TimedRecord{name, a, b});
+}
+
+static void timer_reset() {
+    for (auto& r : g_records) {
+        g_event_pool.push_back(r.a);
+        g_event_pool.push_back(r.b);
+    }
+    g_records.clear();
+}
 
 static thread_local char g_err[1024] = "";
 
@@ -59,3 +99,26 @@ int colsum_f32(const float* in, int64_t rows, int64_t cols, float* out, float* t
 extern "C" const char* am_last_error(void) { return am::last_error(); }
 
 extern "C" int am_version(void) { return 10000; /* 1.0.0 */ }
+
+extern "C" int am_timer_enable(int32_t on) {
+    am::timer_reset();
+    am::g_timer_on = on != 0;
+    return AM_OK;
+}
+
+extern "C" int am_timer_read(const char* name, double* total_ms, int64_t* count) {
+    AM_CHECK_ARG(name && total_ms && count, "NULL argument");
+    double tot = 0.0;
+    int64_t c = 0;
+    for (auto& r : am::g_records) {
+        if (r.name != name) continue;
+        AM_HIP(hipEventSynchronize(r.b));
+        float ms = 0.0f;
+        AM_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+        tot += ms;
+        ++c;
+    }
+    *total_ms = tot;
+    *count = c;
+    return AM_OK;
+}

@@ -658,8 +658,11 @@ static int geo_fwd_launch(const GeoArgs& a, const float* pts, const float* randn
     const size_t lds = (size_t)a.lds_floats * sizeof(float);
     AM_HIP(hipFuncSetAttribute((const void*)k_geo_fwd<LJ, NH>,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL((k_geo_fwd<LJ, NH>), dim3(field_blocks(n)), dim3(256), lds, s, a, pts, randn,
-                       n, table, params, sdf, grad, normal, feature, laplace);
+    {
+        KernelTimer kt_("geo_fwd", s);
+        hipLaunchKernelGGL((k_geo_fwd<LJ, NH>), dim3(field_blocks(n)), dim3(256), lds, s, a, pts, randn,
+                           n, table, params, sdf, grad, normal, feature, laplace);
+    }
     AM_LAUNCH_CHECK();
     return AM_OK;
 }
@@ -690,8 +693,11 @@ static int geo_bwd_launch(const GeoArgs& a, const am_grid_desc* g, const float* 
     const size_t lds = (size_t)(a.lds_floats + 4 * 128 * mlp::kScratchLd) * sizeof(float);
     AM_HIP(hipFuncSetAttribute((const void*)k_geo_bwd<LJ, NH>,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL((k_geo_bwd<LJ, NH>), dim3(blocks), dim3(256), lds, s, a, pts, randn, n, table,
-                       params, g_sdf, g_grad, g_normal, g_feature, g_laplace, em, partial);
+    {
+        KernelTimer kt_("geo_bwd", s);
+        hipLaunchKernelGGL((k_geo_bwd<LJ, NH>), dim3(blocks), dim3(256), lds, s, a, pts, randn, n, table,
+                           params, g_sdf, g_grad, g_normal, g_feature, g_laplace, em, partial);
+    }
     AM_LAUNCH_CHECK();
     if (int rc = colsum_f32(partial, waves, a.n_params, dparams, ctmp, s)) return rc;
     return table_reduce_run(tw, 2, g->n_entries, dtable, s);

@@ -158,8 +158,11 @@ __global__ void __launch_bounds__(256) k_hg_dx(GridArgs g, const float* __restri
 template <int F>
 static int hg_fwd(const GridArgs& ga, const float* x, int64_t n, const float* table, int active,
                   float* out, hipStream_t s) {
-    hipLaunchKernelGGL(k_hg_fwd<F>, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, ga, x, n,
-                       table, active, out);
+    {
+        KernelTimer kt_("hg_fwd", s);
+        hipLaunchKernelGGL(k_hg_fwd<F>, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, ga, x, n,
+                           table, active, out);
+    }
     AM_LAUNCH_CHECK();
     return AM_OK;
 }
@@ -184,9 +187,12 @@ static int hg_bwd(const GridArgs& ga, uint32_t n_entries, const float* x, int64_
             set_error("workspace too small: need %zu bytes, got %zu", c.used, ws_bytes);
             return AM_ERR_WORKSPACE;
         }
-        hipLaunchKernelGGL(k_hg_emit<F>, dim3((unsigned)ceil_div(n * active, 256)), dim3(256), 0,
-                           s, ga, x, n, dy, active, w.keys[0],
-                           reinterpret_cast<Vals<F>*>(w.vals[0]));
+        {
+            KernelTimer kt_("hg_emit", s);
+            hipLaunchKernelGGL(k_hg_emit<F>, dim3((unsigned)ceil_div(n * active, 256)), dim3(256), 0,
+                               s, ga, x, n, dy, active, w.keys[0],
+                               reinterpret_cast<Vals<F>*>(w.vals[0]));
+        }
         AM_LAUNCH_CHECK();
         return table_reduce_run(w, F, n_entries, dtable, s);
     }

@@ -257,8 +257,11 @@ extern "C" int am_march_count(const float* rays, int64_t n_rays, const float* t_
     }
     const MarchArgs a = march_args(roi6_host, grid, grid_res, step, cap);
     const unsigned nb = (unsigned)ceil_div(n_rays, 128);
-    hipLaunchKernelGGL(k_march_count, dim3(nb), dim3(128), 0, s, a, rays, n_rays, t_near, t_far,
-                       jitter, grid, counts);
+    {
+        KernelTimer kt_("march_count", s);
+        hipLaunchKernelGGL(k_march_count, dim3(nb), dim3(128), 0, s, a, rays, n_rays, t_near, t_far,
+                           jitter, grid, counts);
+    }
     AM_LAUNCH_CHECK();
     hipError_t e = rocprim::exclusive_scan(stmp, sb, counts, starts, 0, (size_t)n_rays,
                                            rocprim::plus<int32_t>(), s, false);
@@ -282,9 +285,12 @@ extern "C" int am_march_fill(const float* rays, int64_t n_rays, const float* t_n
     if (n_rays == 0) return AM_OK;
     AM_CHECK_ARG(packed && ray_indices && t_starts && t_ends, "NULL output");
     const MarchArgs a = march_args(roi6_host, grid, grid_res, step, cap);
-    hipLaunchKernelGGL(k_march_fill, dim3((unsigned)ceil_div(n_rays, 128)), dim3(128), 0,
-                       to_stream(stream), a, rays, n_rays, t_near, t_far, jitter, grid, packed,
-                       ray_indices, t_starts, t_ends);
+    {
+        KernelTimer kt_("march_fill", to_stream(stream));
+        hipLaunchKernelGGL(k_march_fill, dim3((unsigned)ceil_div(n_rays, 128)), dim3(128), 0,
+                           to_stream(stream), a, rays, n_rays, t_near, t_far, jitter, grid, packed,
+                           ray_indices, t_starts, t_ends);
+    }
     AM_LAUNCH_CHECK();
     return AM_OK;
 }

@@ -141,6 +141,7 @@ extern "C" int am_occ_update(const am_grid_desc* g, const am_mlp_desc* mlp, cons
         hipLaunchKernelGGL(k_occ_points, dim3((unsigned)ceil_div(m, 256)), dim3(256), 0, st, cells,
                            jitter, m, grid_res, s->radius, pts);
         AM_LAUNCH_CHECK();
+        TimerAlias alias("occ_sdf");
         if (int rc = am_sdf_fwd(g, mlp, &sd, pts, nullptr, m, table, params, sdf, nullptr, nullptr,
                                 nullptr, nullptr, stream))
             return rc;

@@ -378,8 +378,11 @@ static int rad_fwd_launch(const RadArgs& a, const float* feature, const float* r
     const size_t lds = (size_t)a.lds_floats * sizeof(float);
     AM_HIP(hipFuncSetAttribute((const void*)k_rad_fwd<KS, NH>,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL((k_rad_fwd<KS, NH>), dim3(field_blocks(n)), dim3(256), lds, s, a, feature,
-                       rays, ridx, normal, n, params, rgb);
+    {
+        KernelTimer kt_("rad_fwd", s);
+        hipLaunchKernelGGL((k_rad_fwd<KS, NH>), dim3(field_blocks(n)), dim3(256), lds, s, a, feature,
+                           rays, ridx, normal, n, params, rgb);
+    }
     AM_LAUNCH_CHECK();
     return AM_OK;
 }
@@ -402,8 +405,11 @@ static int rad_bwd_launch(const RadArgs& a, const float* feature, const float* r
     const size_t lds = (size_t)(a.lds_floats + 4 * 128 * mlp::kScratchLd) * sizeof(float);
     AM_HIP(hipFuncSetAttribute((const void*)k_rad_bwd<KS, NH>,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL((k_rad_bwd<KS, NH>), dim3(blocks), dim3(256), lds, s, a, feature, rays, ridx,
-                       normal, n, params, g_rgb, g_feature, g_normal, partial);
+    {
+        KernelTimer kt_("rad_bwd", s);
+        hipLaunchKernelGGL((k_rad_bwd<KS, NH>), dim3(blocks), dim3(256), lds, s, a, feature, rays, ridx,
+                           normal, n, params, g_rgb, g_feature, g_normal, partial);
+    }
     AM_LAUNCH_CHECK();
     return colsum_f32(partial, waves, a.n_params, dparams, ctmp, s);
 }

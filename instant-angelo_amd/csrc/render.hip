@@ -349,10 +349,13 @@ extern "C" int am_composite_fwd(const am_composite_desc* c, const float* sdf, co
                  "NULL ray buffer");
     AM_CHECK_ARG(n == 0 || (sdf && normal && t_starts && t_ends && rgb && weights),
                  "NULL sample buffer");
-    hipLaunchKernelGGL(k_composite_fwd, dim3((unsigned)ceil_div(n_rays, 4)), dim3(256), 0,
-                       to_stream(stream), comp_args(c), sdf, normal, rays, t_starts, t_ends, rgb,
-                       inv_s, packed, n_rays, weights, trans, opacity, depth, comp_rgb,
-                       comp_normal, comp_normal_raw);
+    {
+        KernelTimer kt_("composite_fwd", to_stream(stream));
+        hipLaunchKernelGGL(k_composite_fwd, dim3((unsigned)ceil_div(n_rays, 4)), dim3(256), 0,
+                           to_stream(stream), comp_args(c), sdf, normal, rays, t_starts, t_ends, rgb,
+                           inv_s, packed, n_rays, weights, trans, opacity, depth, comp_rgb,
+                           comp_normal, comp_normal_raw);
+    }
     AM_LAUNCH_CHECK();
     return AM_OK;
 }
@@ -393,10 +396,13 @@ extern "C" int am_composite_bwd(const am_composite_desc* c, const float* sdf, co
         set_error("workspace too small: need %zu, got %zu", wc.used, workspace_bytes);
         return AM_ERR_WORKSPACE;
     }
-    hipLaunchKernelGGL(k_composite_bwd, dim3((unsigned)ceil_div(n_rays, 4)), dim3(256), 0, s,
-                       comp_args(c), sdf, normal, rays, t_starts, t_ends, rgb, inv_s, packed,
-                       n_rays, trans, opacity, comp_normal_raw, g_weights, g_opacity, g_depth,
-                       g_comp_rgb, g_comp_normal, g_sdf, g_normal, g_rgb, per_ray);
+    {
+        KernelTimer kt_("composite_bwd", s);
+        hipLaunchKernelGGL(k_composite_bwd, dim3((unsigned)ceil_div(n_rays, 4)), dim3(256), 0, s,
+                           comp_args(c), sdf, normal, rays, t_starts, t_ends, rgb, inv_s, packed,
+                           n_rays, trans, opacity, comp_normal_raw, g_weights, g_opacity, g_depth,
+                           g_comp_rgb, g_comp_normal, g_sdf, g_normal, g_rgb, per_ray);
+    }
     AM_LAUNCH_CHECK();
     return colsum_f32(per_ray, n_rays, 1, g_inv_s, tmp, s);
 }

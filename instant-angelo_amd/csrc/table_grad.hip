@@ -159,15 +159,22 @@ static int run_f(TableReduceWs& w, uint32_t n_entries, float* dtable, hipStream_
     uint32_t* ks = nullptr;
     Vals<F>* vs = nullptr;
     size_t sb = w.sort_bytes;
-    hipError_t e = sort_pairs<F>(w.sort_tmp, sb, w, &ks, &vs, s);
+    hipError_t e;
+    {
+        KernelTimer kt_("radix_sort", s);
+        e = sort_pairs<F>(w.sort_tmp, sb, w, &ks, &vs, s);
+    }
     if (e != hipSuccess) {
         set_error("radix sort failed: %s", hipGetErrorString(e));
         return AM_ERR_HIP;
     }
     const int64_t nc = ceil_div(w.m, kSegChunk);
     const int gb = (int)ceil_div(nc, 256);
-    hipLaunchKernelGGL(k_seg_a<F>, dim3(gb), dim3(256), 0, s, ks, vs, w.m, n_entries, dtable,
-                       w.lead, w.tail, w.lead_full, w.tail_open);
+    {
+        KernelTimer kt_("seg_reduce", s);
+        hipLaunchKernelGGL(k_seg_a<F>, dim3(gb), dim3(256), 0, s, ks, vs, w.m, n_entries, dtable,
+                           w.lead, w.tail, w.lead_full, w.tail_open);
+    }
     AM_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_seg_b<F>, dim3(gb), dim3(256), 0, s, ks, w.m, n_entries, dtable, w.lead,
                        w.tail, w.lead_full, w.tail_open);

@@ -1,0 +1,169 @@
+"""CPU tests of the C-ABI boundary (load, exports, host-only entry points,
+argument validation) and of the host-side Python mirror of the reference
+API (config access, schedules, update_step protocol, state-dict names,
+loss scaling, loud failure without the device path)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+import torch
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "angelo_amd.h")
+
+
+def _declared():
+    src = open(HEADER).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(am_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol():
+    from angelo_amd import _native as nat
+    lib = ctypes.CDLL(nat.lib_path())
+    declared = _declared()
+    assert len(declared) >= 25
+    missing = [s for s in declared if not hasattr(lib, s)]
+    assert not missing, missing
+    # the Python binding knows every declared entry point
+    assert set(declared) <= set(nat.exported_symbols())
+
+
+def test_grid_init_matches_oracle_level_table():
+    from angelo_amd import descs
+    from oracle import hashgrid as ohg
+    for cfg in [(8, 2, 15, 16, 1.5), (16, 2, 21, 32, 1.3195079107728942), (16, 2, 19, 32, 1.3195079107728942),
+                (12, 4, 14, 8, 2.0)]:
+        L, F, log2T, base, s = cfg
+        g = descs.grid_arrays(descs.grid_desc(L, F, log2T, base, s))
+        lt = ohg.level_table(L, base, s, log2T, F)
+        assert np.array_equal(g["scale"], lt.scale)
+        assert np.array_equal(g["res"], lt.res)
+        assert np.array_equal(g["offset"], lt.offset)
+        assert np.array_equal(g["size"], lt.size)
+        assert np.array_equal(g["hashed"], lt.hashed)
+        assert g["n_entries"] == lt.n_entries
+
+
+def test_grid_init_override_and_validation():
+    from angelo_amd import _native as nat, descs
+    g = descs.grid_desc(3, 2, 12, 16, 1.5, res_override=[16, 25, 40])
+    assert list(g.res[:3]) == [16, 25, 40] and list(g.scale[:3]) == [15.0, 24.0, 39.0]
+    with pytest.raises(nat.NativeError, match="n_levels"):
+        descs.grid_desc(0, 2, 12, 16, 1.5)
+    with pytest.raises(nat.NativeError, match="n_features"):
+        descs.grid_desc(4, 3, 12, 16, 1.5)
+
+
+def test_mlp_param_count_and_validation():
+    from angelo_amd import _native as nat, descs
+    assert descs.mlp_n_params(descs.mlp_desc(35, 16, 1)) == 64 * 35 + 64 + 16 * 64 + 16
+    assert descs.mlp_n_params(descs.mlp_desc(38, 3, 2)) == 64 * 38 + 64 + 64 * 64 + 64 + 3 * 64 + 3
+    with pytest.raises(nat.NativeError):
+        descs.mlp_n_params(descs.mlp_desc(35, 16, 1, n_neurons=32))
+
+
+def test_device_entry_points_reject_bad_arguments_without_launching():
+    """Validation happens on the host before any launch (no GPU needed)."""
+    from angelo_amd import _native as nat, descs
+    lib = nat.load()
+    g = descs.grid_desc(8, 2, 15, 16, 1.5)
+    rc = lib.am_hashgrid_fwd(ctypes.byref(g), None, 10, None, 8, None, None)
+    assert rc == -1 and b"NULL" in lib.am_last_error()
+    rc = lib.am_hashgrid_fwd(ctypes.byref(g), None, 10, None, 99, None, None)
+    assert rc == -1
+    assert lib.am_hashgrid_fwd(ctypes.byref(g), None, 0, None, 8, None, None) == 0  # empty batch
+
+
+def test_product_has_no_cpu_fallback():
+    import angelo_amd
+    from angelo_amd import _native as nat
+    from angelo_amd.workloads import model_config
+    model = angelo_amd.make("neus", model_config(8, 15, 16, 1.5, rays=8, samples_per_ray=8))
+    model.update_step(0, 100)
+    with pytest.raises(nat.NativeError, match="no CPU fallback"):
+        model.geometry.forward_level(torch.zeros(4, 3))
+
+
+def test_state_dict_names_mirror_reference():
+    import angelo_amd
+    from angelo_amd.workloads import model_config
+    model = angelo_amd.make("neus", model_config(8, 15, 16, 1.5, rays=8, samples_per_ray=8,
+                                                 geo_hidden=2))
+    keys = set(model.state_dict().keys())
+    for k in ["geometry.encoding.encoding.encoding.params", "geometry.network.layers.0.weight_g",
+              "geometry.network.layers.0.weight_v", "geometry.network.layers.0.bias",
+              "geometry.network.layers.2.weight_g", "geometry.network.layers.4.bias",
+              "texture.network.layers.0.weight", "texture.network.layers.4.bias",
+              "variance.variance", "scene_aabb"]:
+        assert k in keys, k
+    assert model.geometry.grid_module.params.numel() == 214528 * 2
+    p = model.geometry.grid_module.params
+    assert p.abs().max() <= 1e-4  # tcnn's uniform +-1e-4 init
+
+
+def test_update_step_protocol():
+    import angelo_amd
+    from angelo_amd.workloads import model_config
+    cfg = model_config(16, 19, 32, 1.3195079107728942, rays=8, samples_per_ray=8, start_level=4)
+    cfg.geometry.xyz_encoding_config.start_step = 5000
+    model = angelo_amd.make("neus", cfg)
+    geo = model.geometry
+    model.update_step(0, 0)
+    assert geo.active_levels() == 4
+    # progressive eps = 2 r / (base * scale^(level-1)) in float64 (geometry.py:304-309)
+    assert geo._finite_difference_eps == 2 * 1.0 / (32 * 1.3195079107728942 ** 3)
+    model.update_step(0, 7500)
+    assert geo.active_levels() == 6
+    model.update_step(0, 100000)
+    assert geo.active_levels() == 16
+    assert model.cos_anneal_ratio == 1.0
+    model.update_step(0, 5000)          # the mask only gains levels (network_utils.py:66)
+    assert geo.active_levels() == 16
+    assert abs(model.cos_anneal_ratio - 0.25) < 1e-12
+
+
+def test_scheduled_values_and_config():
+    from angelo_amd.utils import scheduled_value, to_config, chunk_batch, get_activation
+    assert scheduled_value([0, 0, 0.5, 5000], 2500) == 0.25
+    assert scheduled_value([0, 0, 0.5, 5000], 9000) == 0.5
+    assert scheduled_value([1, 0.0, 1.0, 3.0], 0, current_epoch=2) == 0.5
+    assert scheduled_value(0.1, 123) == 0.1
+    c = to_config({"a": {"b": 2}, "l": [{"x": 1}]})
+    assert c.a.b == 2 and c.get("z", 5) == 5 and c.l[0].x == 1
+    out = chunk_batch(lambda t: {"y": t * 2}, 3, False, torch.arange(10))
+    assert torch.equal(out["y"], torch.arange(10) * 2)
+    assert torch.equal(get_activation("scale2")(torch.tensor([3.0])), torch.tensor([1.0]))
+
+
+def test_dynamic_loss_scaler():
+    from angelo_amd.system import DynamicLossScaler
+    p = torch.nn.Parameter(torch.ones(3))
+    opt = torch.optim.SGD([p], lr=0.1)
+    s = DynamicLossScaler(init_scale=1024.0, growth_interval=2)
+    p.grad = torch.tensor([1.0, float("inf"), 0.0])
+    assert not s.step(opt, [p]) and s.scale == 512.0 and torch.equal(p.data, torch.ones(3))
+    p.grad = torch.tensor([512.0, 0.0, 0.0])
+    assert s.step(opt, [p]) and abs(p.data[0].item() - 0.9) < 1e-7
+    p.grad = torch.tensor([0.0, 0.0, 0.0])
+    s.step(opt, [p])
+    assert s.scale == 1024.0
+
+
+def test_losses_on_cpu_tensors():
+    from angelo_amd.system import NeuSTrainer
+    from angelo_amd.workloads import system_config
+    from angelo_amd.utils import to_config
+    tr = NeuSTrainer(None, system_config(), to_config({"train_num_rays": 4, "num_samples_per_ray": 8}))
+    tr.global_step = 10000
+    out = {"rays_valid_full": torch.tensor([[True], [False], [True], [True]]),
+           "comp_rgb_full": torch.full((4, 3), 0.5), "sdf_grad_samples": torch.tensor([[0.0, 2.0, 0.0]]),
+           "opacity": torch.ones(4, 1), "sdf_samples": torch.zeros(1),
+           "sdf_laplace_samples": torch.tensor([[0.2], [-0.4]])}
+    loss, terms = tr.losses(out, {"rgb": torch.zeros(4, 3)})
+    assert abs(terms["rgb_mse"].item() - 0.25) < 1e-7
+    assert abs(terms["eikonal"].item() - 1.0) < 1e-7
+    assert abs(terms["curvature"].item() - 0.3) < 1e-7
+    assert abs(loss.item() - (2.5 + 0.1 + 0.15)) < 1e-6
