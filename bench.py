@@ -25,8 +25,9 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 FP32_MFMA_PEAK_TFLOPS = 157.3
-TIMED_KERNELS = ["geo_fwd", "geo_bwd", "radix_sort", "seg_reduce", "rad_fwd", "rad_bwd",
-                 "composite_fwd", "composite_bwd", "march_count", "march_fill", "occ_sdf"]
+TIMED_KERNELS = ["geo_fwd", "geo_bwd", "pair_compact", "radix_sort", "seg_reduce", "rad_fwd",
+                 "rad_bwd", "composite_fwd", "composite_bwd", "march_count", "march_fill",
+                 "occ_sdf"]
 
 
 def log(*a):

@@ -171,21 +171,25 @@ typedef struct am_sdf_desc {
 
 /* Outputs (each nullable except sdf): sdf[n], grad[n,3], normal[n,3] =
  * F.normalize(grad), feature[n, n_out+3] = cat(mlp_out, x01*2-1),
- * laplace[n].  randn[n,3]: the curvature directions (torch.randn_like). */
+ * laplace[n], fd_values[n,12] (the 6 central-difference and 6 curvature SDF
+ * values, kept for the backward).  randn[n,3]: the curvature directions
+ * (torch.randn_like). */
 int am_sdf_fwd(const am_grid_desc* g, const am_mlp_desc* m, const am_sdf_desc* s,
                const float* pts, const float* randn, int64_t n, const float* table,
                const float* params, float* sdf, float* grad, float* normal, float* feature,
-               float* laplace, void* stream);
+               float* laplace, float* fd_values, void* stream);
 size_t am_sdf_bwd_workspace_size(const am_grid_desc* g, const am_mlp_desc* m,
                                  const am_sdf_desc* s, int64_t n);
-/* Upstream gradients are nullable (= 0).  dtable [n_entries,F] and
- * dparams are fully overwritten; both reductions are deterministic. */
+/* Upstream gradients are nullable (= 0).  fd_values: from the forward
+ * (nullable: recomputed).  dtable [n_entries,F] and dparams are fully
+ * overwritten; both reductions are deterministic.  The call reads the
+ * number of table-gradient pairs back to the host once (a stream sync). */
 int am_sdf_bwd(const am_grid_desc* g, const am_mlp_desc* m, const am_sdf_desc* s,
                const float* pts, const float* randn, int64_t n, const float* table,
-               const float* params, const float* g_sdf, const float* g_grad,
-               const float* g_normal, const float* g_feature, const float* g_laplace,
-               float* dtable, float* dparams, void* workspace, size_t workspace_bytes,
-               void* stream);
+               const float* params, const float* fd_values, const float* g_sdf,
+               const float* g_grad, const float* g_normal, const float* g_feature,
+               const float* g_laplace, float* dtable, float* dparams, void* workspace,
+               size_t workspace_bytes, void* stream);
 
 /* ------------------------------------------------------------------------ */
 /* Fused VolumeRadiance.forward (models/texture.py:23-30): SH(deg) of        */

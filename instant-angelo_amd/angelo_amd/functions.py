@@ -120,19 +120,23 @@ class SDFFieldFn(torch.autograd.Function):
         normal = torch.empty(n, 3, **f32) if sdesc.with_grad else torch.empty(0, 3, **f32)
         lap = torch.empty(n, **f32) if sdesc.with_laplace else torch.empty(0, **f32)
         rnd = _f32c(randn) if (sdesc.with_laplace and randn is not None) else None
+        keep = torch.is_grad_enabled() or any(ctx.needs_input_grad)
+        fdv = torch.empty(n, 12, **f32) if (sdesc.with_grad and keep) else None
         nat.check(lib.am_sdf_fwd(nat.sptr(grid), nat.sptr(mdesc), nat.sptr(sdesc), nat.ptr(pts),
                                  nat.ptr(rnd), n, nat.ptr(table), nat.ptr(params), nat.ptr(sdf),
                                  nat.ptr(grad) if sdesc.with_grad else None,
                                  nat.ptr(normal) if sdesc.with_grad else None, nat.ptr(feature),
-                                 nat.ptr(lap) if sdesc.with_laplace else None, nat.stream()),
-                  "am_sdf_fwd")
-        ctx.save_for_backward(pts, rnd if rnd is not None else pts.new_empty(0), table, params)
+                                 nat.ptr(lap) if sdesc.with_laplace else None, nat.ptr(fdv),
+                                 nat.stream()), "am_sdf_fwd")
+        empty = pts.new_empty(0)
+        ctx.save_for_backward(pts, rnd if rnd is not None else empty, table, params,
+                              fdv if fdv is not None else empty)
         ctx.descs = (grid, mdesc, sdesc)
         return sdf, grad, normal, feature, lap
 
     @staticmethod
     def backward(ctx, g_sdf, g_grad, g_normal, g_feature, g_lap):
-        pts, rnd, table, params = ctx.saved_tensors
+        pts, rnd, table, params, fdv = ctx.saved_tensors
         grid, mdesc, sdesc = ctx.descs
         lib = nat.load()
         n = pts.shape[0]
@@ -151,7 +155,8 @@ class SDFFieldFn(torch.autograd.Function):
                                                          nat.sptr(sdesc), n), pts.device)
         nat.check(lib.am_sdf_bwd(nat.sptr(grid), nat.sptr(mdesc), nat.sptr(sdesc), nat.ptr(pts),
                                  nat.ptr(rnd) if rnd.numel() else None, n, nat.ptr(table),
-                                 nat.ptr(params), nat.ptr(gs), nat.ptr(gg), nat.ptr(gn),
+                                 nat.ptr(params), nat.ptr(fdv) if fdv.numel() else None,
+                                 nat.ptr(gs), nat.ptr(gg), nat.ptr(gn),
                                  nat.ptr(gf), nat.ptr(gl), nat.ptr(dtable), nat.ptr(dparams),
                                  nat.ptr(ws), ws.numel(), nat.stream()), "am_sdf_bwd")
         return None, None, dtable, dparams, None, None, None

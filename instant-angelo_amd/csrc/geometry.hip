@@ -1,29 +1,39 @@
-// Fused VolumeSDF.forward (models/geometry.py:195-287) and
-// VolumeRadiance.forward (models/texture.py:23-30) on fp32 MFMA.
+// Fused VolumeSDF.forward (models/geometry.py:195-287) on fp32 MFMA.
 //
-// Geometry: one wave evaluates 16 samples.  Per sample: contraction
-// (scale_anything, models/utils.py:109-114, CUDA torch semantics: divide by
-// a scalar = multiply by its fp32 reciprocal), the base evaluation, 6
-// central-difference evaluations (:219-234) and, with laplace, 6 around the
-// tangent-shifted point (:236-275).  Each evaluation = hash-grid encode (lane
-// group k gathers levels k, k+4, k+8, k+12 of its sample, producing exactly
-// the MFMA B fragments of the first layer) + the SDF MLP on MFMA.  The
-// backward recomputes the evaluations, back-propagates through MLP and
-// encode, and emits (table row, value) pairs for the deterministic
-// sort + segmented-reduce table gradient; MLP weight gradients are
-// per-wave register accumulators reduced in a fixed order.
+// One wave evaluates 16 samples.  Per sample: contraction (scale_anything,
+// models/utils.py:109-114, with torch's CUDA scalar division = multiply by
+// the fp32 reciprocal), the base evaluation, 6 central-difference
+// evaluations (:219-234) and, with laplace, 6 around the tangent-shifted
+// point (:236-275).  One evaluation = hash-grid encode (lane group k gathers
+// levels k, k+4, k+8, k+12 of its sample, which are exactly its MFMA B
+// fragments of the first layer) + the SDF MLP on MFMA.  The evaluations run
+// in runtime loops so one copy of the encode + MLP body is compiled.
+//
+// The forward stores the 12 FD / curvature SDF values per sample; the
+// backward rebuilds grad, normal, tangent and the curvature terms from them,
+// back-propagates each evaluation through MLP and encode, and reduces the
+// table gradient deterministically:
+//   1. per lane, consecutive evaluations of a cluster (base + FD around the
+//      sample, or the 6 curvature points) that fall in the same cell at a
+//      level are summed into one set of 8 corner values;
+//   2. the last run of each lane is merged with equal-cell runs of the
+//      neighbouring samples (lanes) of the same lane group, a segmented
+//      reduction in a fixed shuffle tree;
+//   3. the surviving (row, value) pairs of a tile are written to its region
+//      in a fixed order, regions are compacted by an integer prefix sum, and
+//      a stable radix sort + fixed-order segmented reduce gives dtable.
+// MLP weight gradients are per-wave register accumulators summed in a fixed
+// order.
 #include "hashgrid_dev.h"
 #include "mlp_dev.h"
 #include "table_grad.h"
 #include "field_common.h"
+#include <rocprim/device/device_scan.hpp>
 
 namespace am {
 
 using mlp::krow;
 
-// ------------------------------------------------------------------------
-// geometry
-// ------------------------------------------------------------------------
 struct GeoArgs {
     GridArgs g;
     float r, inv2r, eps, inv_eps, inv_pi;
@@ -55,8 +65,8 @@ struct GeoDims {
 };
 
 template <int LJ, int NH>
-__device__ void geo_load_lds(const GeoArgs& a, const float* __restrict__ params, float* sm,
-                             LevelParams* lv) {
+__device__ __forceinline__ void geo_load_lds(const GeoArgs& a, const float* __restrict__ params,
+                                             float* sm, LevelParams* lv) {
     constexpr int RS1 = GeoDims<LJ>::RS1;
     for (int q = threadIdx.x; q < 64 * RS1; q += blockDim.x) {
         const int o = q / RS1, rs = q - o * RS1;
@@ -76,7 +86,11 @@ __device__ void geo_load_lds(const GeoArgs& a, const float* __restrict__ params,
         sm[a.wo + o * 65 + c] = o < a.n_out ? params[a.pwo + (int64_t)o * 64 + c] : 0.0f;
     }
     for (int q = threadIdx.x; q < 16; q += blockDim.x) sm[a.bo + q] = q < a.n_out ? params[a.pbo + q] : 0.0f;
-    for (int q = threadIdx.x; q < a.g.L; q += blockDim.x) lv[q] = a.g.lv[q];
+    // unrolled: every kernel-argument access has a constant offset (a
+    // per-thread index into the by-value struct would copy it to scratch)
+#pragma unroll
+    for (int q = 0; q < AM_MAX_LEVELS; ++q)
+        if ((int)threadIdx.x == q && q < a.g.L) lv[q] = a.g.lv[q];
 }
 
 template <int LJ, int NH, bool DX>
@@ -120,44 +134,45 @@ __device__ __forceinline__ void geo_eval(const GeoArgs& a, const float* sm,
     e.xb[2 * LJ] = k < 3 ? pick3(x, k) * 2.0f - 1.0f : 0.0f;
 #pragma unroll
     for (int t = 0; t < 4; ++t) e.z1[t] = mlp::bias_tile(sm + a.b1, t, 64);
-    mlp::mm_fwd<KS1, 4>(sm + a.w1, a.ld1, e.xb, e.z1);
+    mlp::mm_fwd<KS1, 4>(sm + a.w1, a.ld1, mlp::FragF{e.xb}, e.z1);
     v4f h[4];
 #pragma unroll
     for (int t = 0; t < 4; ++t) h[t] = mlp::act4(a.act, e.z1[t], a.beta);
     if constexpr (NH == 2) {
 #pragma unroll
         for (int t = 0; t < 4; ++t) e.z2[t] = mlp::bias_tile(sm + a.b2, t, 64);
-        mlp::mm_fwd<16, 4>(sm + a.w2, 65, reinterpret_cast<const float*>(h), e.z2);
+        mlp::mm_fwd<16, 4>(sm + a.w2, 65, mlp::FragV{h}, e.z2);
 #pragma unroll
         for (int t = 0; t < 4; ++t) h[t] = mlp::act4(a.act, e.z2[t], a.beta);
     }
     e.out = mlp::bias_tile(sm + a.bo, 0, a.n_out);
-    mlp::mm_fwd<16, 1>(sm + a.wo, 65, reinterpret_cast<const float*>(h), &e.out);
+    mlp::mm_fwd<16, 1>(sm + a.wo, 65, mlp::FragV{h}, &e.out);
 }
 
 __device__ __forceinline__ float contract1(float v, const GeoArgs& a) { return (v + a.r) * a.inv2r; }
 
 __device__ __forceinline__ float clampr(float v, float r) { return fminf(fmaxf(v, -r), r); }
 
-// FD point j (axis j/2, sign + for even j) around world point P, contracted.
-__device__ __forceinline__ void fd_point(const GeoArgs& a, const float P[3], int j, float x[3],
-                                         bool mask[3]) {
+// Offset point j (axis j/2, sign + for even j) around `base` (world units,
+// as the reference adds the offsets to it), clamped and contracted; mask =
+// the clamp passed the coordinate through.
+__device__ __forceinline__ void offset_point(const GeoArgs& a, const float base[3], int j,
+                                             float x[3], bool mask[3]) {
     const int ax = j >> 1;
     const float off = (j & 1) ? -a.eps : a.eps;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-        const float v = P[d] + (d == ax ? off : 0.0f);
+        const float v = base[d] + (d == ax ? off : 0.0f);
         mask[d] = v >= -a.r && v <= a.r;
         x[d] = contract1(clampr(v, a.r), a);
     }
 }
 
-__device__ __forceinline__ void normalize3(const float v[3], float out[3], float* nrm) {
+__device__ __forceinline__ void normalize3(const float v[3], float out[3]) {
     const float n = sqrtf((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]);
     const float d = fmaxf(n, 1e-12f);
 #pragma unroll
     for (int q = 0; q < 3; ++q) out[q] = v[q] / d;
-    if (nrm) *nrm = n;
 }
 
 // Backward of y = v / max(|v|, 1e-12) (F.normalize).
@@ -179,65 +194,34 @@ __device__ __forceinline__ void cross3(const float a[3], const float b[3], float
     c[2] = a[0] * b[1] - a[1] * b[0];
 }
 
-// Everything the forward computes per sample (all lanes of the sample).
+// Per-sample state rebuilt from the 12 stored SDF values (all lanes of the
+// sample hold it).
 struct GeoSample {
     float P[3], x0[3];
-    float f[6], grad[3], n[3];
-    float rn[3], ps[3], fs[6], grad_s[3], ns[3], dot;
+    float grad[3], n[3];
+    float rn[3], ps[3], grad_s[3], ns[3], dot;
 };
 
-template <int LJ, int NH>
-__device__ __forceinline__ void geo_shifted_point(const GeoArgs& a, const GeoSample& s, int j,
-                                                  float x[3], bool mask[3]) {
-    // recontract: the shifted point is in [0,1] coordinates and the FD
-    // offsets / clamp / contraction are applied to it again (reference).
-    // Otherwise the shift is applied in world space.
+__device__ __forceinline__ void diff_pair(float* acc3, int j, float v) {
+    // acc[j/2] = f+ then f+ - f- : bit-identical to the single subtraction
     const int ax = j >> 1;
-    const float off = (j & 1) ? -a.eps : a.eps;
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        const float base = s.ps[d];
-        const float v = base + (d == ax ? off : 0.0f);
-        mask[d] = v >= -a.r && v <= a.r;
-        x[d] = contract1(clampr(v, a.r), a);
-    }
+    for (int d = 0; d < 3; ++d)
+        if (d == ax) acc3[d] = (j & 1) ? acc3[d] - v : v;
 }
 
-// Forward part shared by fwd and bwd: base output, FD, tangent shift, laplace.
-template <int LJ, int NH>
-__device__ __forceinline__ void geo_sample_fwd(const GeoArgs& a, const float* sm,
-                                               const LevelParams* lv, const float2* table,
-                                               const float* __restrict__ pts,
-                                               const float* __restrict__ randn, int64_t n,
-                                               bool valid, GeoSample& s, v4f* base_out) {
-    const int p = lane_id() & 15;
+__device__ __forceinline__ void finish_grad(const GeoArgs& a, float g[3]) {
 #pragma unroll
-    for (int d = 0; d < 3; ++d) s.P[d] = valid ? pts[3 * n + d] : 0.0f;
-#pragma unroll
-    for (int d = 0; d < 3; ++d) s.x0[d] = contract1(s.P[d], a);
-    {
-        GeoEval<LJ, NH, false> e;
-        geo_eval<LJ, NH, false>(a, sm, lv, table, s.x0, e);
-        if (base_out) *base_out = e.out;
-    }
-    if (!a.with_grad) return;
-#pragma unroll
-    for (int j = 0; j < 6; ++j) {
-        float x[3];
-        bool m[3];
-        fd_point(a, s.P, j, x, m);
-        GeoEval<LJ, NH, false> e;
-        geo_eval<LJ, NH, false>(a, sm, lv, table, x, e);
-        s.f[j] = shfl(e.out[0], p);
-    }
-#pragma unroll
-    for (int d = 0; d < 3; ++d) s.grad[d] = (0.5f * (s.f[2 * d] - s.f[2 * d + 1])) * a.inv_eps;
-    normalize3(s.grad, s.n, nullptr);
-    if (!a.with_laplace) return;
+    for (int d = 0; d < 3; ++d) g[d] = (0.5f * g[d]) * a.inv_eps;
+}
+
+// grad -> normal -> tangent -> shifted point.
+__device__ __forceinline__ void geo_tangent(const GeoArgs& a, const float* __restrict__ randn,
+                                            int64_t n, bool valid, GeoSample& s) {
     float rd[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) rd[d] = valid ? randn[3 * n + d] : 1.0f;
-    normalize3(rd, s.rn, nullptr);
+    normalize3(rd, s.rn);
     float t[3];
     cross3(s.n, s.rn, t);
 #pragma unroll
@@ -245,19 +229,6 @@ __device__ __forceinline__ void geo_sample_fwd(const GeoArgs& a, const float* sm
         const float base = a.recontract ? s.x0[d] : s.P[d];
         s.ps[d] = base + t[d] * a.eps;
     }
-#pragma unroll
-    for (int j = 0; j < 6; ++j) {
-        float x[3];
-        bool m[3];
-        geo_shifted_point<LJ, NH>(a, s, j, x, m);
-        GeoEval<LJ, NH, false> e;
-        geo_eval<LJ, NH, false>(a, sm, lv, table, x, e);
-        s.fs[j] = shfl(e.out[0], p);
-    }
-#pragma unroll
-    for (int d = 0; d < 3; ++d) s.grad_s[d] = (0.5f * (s.fs[2 * d] - s.fs[2 * d + 1])) * a.inv_eps;
-    normalize3(s.grad_s, s.ns, nullptr);
-    s.dot = (s.n[0] * s.ns[0] + s.n[1] * s.ns[1]) + s.n[2] * s.ns[2];
 }
 
 template <int LJ, int NH>
@@ -268,7 +239,8 @@ __global__ void __launch_bounds__(256) k_geo_fwd(GeoArgs a, const float* __restr
                                                  float* __restrict__ sdf, float* __restrict__ grad,
                                                  float* __restrict__ normal,
                                                  float* __restrict__ feature,
-                                                 float* __restrict__ laplace) {
+                                                 float* __restrict__ laplace,
+                                                 float* __restrict__ fdv) {
     extern __shared__ float sm[];
     __shared__ LevelParams lv[AM_MAX_LEVELS];
     geo_load_lds<LJ, NH>(a, params, sm, lv);
@@ -283,30 +255,71 @@ __global__ void __launch_bounds__(256) k_geo_fwd(GeoArgs a, const float* __restr
         const int64_t n = tile * 16 + p;
         const bool valid = n < N;
         GeoSample s;
-        v4f out;
-        geo_sample_fwd<LJ, NH>(a, sm, lv, table, pts, randn, n, valid, s, &out);
-        if (!valid) continue;
-        if (k == 0) sdf[n] = out[0];
-        if (feature) {
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int row = 4 * k + r;
-                if (row < a.n_out) feature[n * fdim + row] = out[r];
+        for (int d = 0; d < 3; ++d) s.P[d] = valid ? pts[3 * n + d] : 0.0f;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) s.x0[d] = contract1(s.P[d], a);
+        {
+            GeoEval<LJ, NH, false> e;
+            geo_eval<LJ, NH, false>(a, sm, lv, table, s.x0, e);
+            if (valid) {
+                if (k == 0) sdf[n] = e.out[0];
+                if (feature) {
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int row = 4 * k + r;
+                        if (row < a.n_out) feature[n * fdim + row] = e.out[r];
+                    }
+                    if (k < 3) feature[n * fdim + a.n_out + k] = pick3(s.x0, k) * 2.0f - 1.0f;
+                }
             }
-            if (k < 3) feature[n * fdim + a.n_out + k] = pick3(s.x0, k) * 2.0f - 1.0f;
         }
-        if (k == 0) {
-            if (grad && a.with_grad)
+        if (!a.with_grad) continue;
+        float g[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll 1
+        for (int j = 0; j < 6; ++j) {
+            float x[3];
+            bool m[3];
+            offset_point(a, s.P, j, x, m);
+            GeoEval<LJ, NH, false> e;
+            geo_eval<LJ, NH, false>(a, sm, lv, table, x, e);
+            const float v = shfl(e.out[0], p);
+            diff_pair(g, j, v);
+            if (fdv && valid && k == 0) fdv[n * 12 + j] = v;
+        }
+        finish_grad(a, g);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) s.grad[d] = g[d];
+        normalize3(s.grad, s.n);
+        if (valid && k == 0) {
+            if (grad)
 #pragma unroll
                 for (int d = 0; d < 3; ++d) grad[3 * n + d] = s.grad[d];
-            if (normal && a.with_grad)
+            if (normal)
 #pragma unroll
                 for (int d = 0; d < 3; ++d) normal[3 * n + d] = s.n[d];
-            if (laplace && a.with_laplace) {
-                const float lo = -1.0f + 1e-6f, hi = 1.0f - 1e-6f;
-                const float dc = fminf(fmaxf(s.dot, lo), hi);
-                laplace[n] = acosf(dc) * a.inv_pi;
-            }
+        }
+        if (!a.with_laplace) continue;
+        geo_tangent(a, randn, n, valid, s);
+        float gs[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll 1
+        for (int j = 0; j < 6; ++j) {
+            float x[3];
+            bool m[3];
+            offset_point(a, s.ps, j, x, m);
+            GeoEval<LJ, NH, false> e;
+            geo_eval<LJ, NH, false>(a, sm, lv, table, x, e);
+            const float v = shfl(e.out[0], p);
+            diff_pair(gs, j, v);
+            if (fdv && valid && k == 0) fdv[n * 12 + 6 + j] = v;
+        }
+        finish_grad(a, gs);
+        float ns[3];
+        normalize3(gs, ns);
+        const float dot = (s.n[0] * ns[0] + s.n[1] * ns[1]) + s.n[2] * ns[2];
+        if (valid && k == 0 && laplace) {
+            const float lo = -1.0f + 1e-6f, hi = 1.0f - 1e-6f;
+            laplace[n] = acosf(fminf(fmaxf(dot, lo), hi)) * a.inv_pi;
         }
     }
 }
@@ -322,21 +335,20 @@ struct GeoGrad {
     v4f dbo;
 };
 
-struct EmitCtx {
-    uint32_t* keys;
-    Vals<2>* vals;
-    uint32_t sentinel;
-};
+// Output-gradient tile with only row 0 (the SDF) set.
+__device__ __forceinline__ v4f go_row0(float g) {
+    const int k = lane_id() >> 4;
+    return v4f{k == 0 ? g : 0.0f, 0.0f, 0.0f, 0.0f};
+}
 
 // MLP backward of one evaluation with output gradient `go` (D tile rows =
-// outputs); accumulates weight gradients, emits table pairs and, with DX,
+// outputs): accumulates weight gradients, stores the encode gradient of the
+// lane's level slots to genc (LDS, slot-major [LJ][2][64]) and, with DX,
 // returns the contracted-position gradient (summed over the 4 lane groups).
 template <int LJ, int NH, bool DX>
 __device__ __forceinline__ void geo_eval_bwd(const GeoArgs& a, const float* sm,
-                                             const LevelParams* lv, const float x[3],
                                              const GeoEval<LJ, NH, DX>& e, v4f go,
-                                             float* scratch, GeoGrad<LJ, NH>& G,
-                                             const EmitCtx& em, int64_t pair_base, bool valid,
+                                             float* scratch, GeoGrad<LJ, NH>& G, float* genc,
                                              float gpos[3]) {
     constexpr int KS1 = GeoDims<LJ>::KS1, IT1 = GeoDims<LJ>::IT1;
     const int l = lane_id(), k = l >> 4;
@@ -344,12 +356,12 @@ __device__ __forceinline__ void geo_eval_bwd(const GeoArgs& a, const float* sm,
     const v4f* zl = NH == 2 ? e.z2 : e.z1;
 #pragma unroll
     for (int t = 0; t < 4; ++t) h[t] = mlp::act4(a.act, zl[t], a.beta);
-    mlp::mm_dw<1, 4, 16>(&go, reinterpret_cast<const float*>(h), scratch, G.dWo);
+    mlp::mm_dw<1, 4, 16>(&go, mlp::FragV{h}, scratch, G.dWo);
     G.dbo += go;
     v4f gh[4];
 #pragma unroll
     for (int t = 0; t < 4; ++t) gh[t] = v4f{0, 0, 0, 0};
-    mlp::mm_bwd_in<4, 4>(sm + a.wo, 65, reinterpret_cast<const float*>(&go), gh);
+    mlp::mm_bwd_in<4, 4>(sm + a.wo, 65, mlp::FragV{&go}, gh);
     if constexpr (NH == 2) {
         v4f gz2[4], h1[4];
 #pragma unroll
@@ -359,10 +371,10 @@ __device__ __forceinline__ void geo_eval_bwd(const GeoArgs& a, const float* sm,
             h1[t] = mlp::act4(a.act, e.z1[t], a.beta);
             G.db2[t] += gz2[t];
         }
-        mlp::mm_dw<4, 4, 16>(gz2, reinterpret_cast<const float*>(h1), scratch, G.dW2);
+        mlp::mm_dw<4, 4, 16>(gz2, mlp::FragV{h1}, scratch, G.dW2);
 #pragma unroll
         for (int t = 0; t < 4; ++t) gh[t] = v4f{0, 0, 0, 0};
-        mlp::mm_bwd_in<16, 4>(sm + a.w2, 65, reinterpret_cast<const float*>(gz2), gh);
+        mlp::mm_bwd_in<16, 4>(sm + a.w2, 65, mlp::FragV{gz2}, gh);
     }
     v4f gz1[4];
 #pragma unroll
@@ -371,38 +383,16 @@ __device__ __forceinline__ void geo_eval_bwd(const GeoArgs& a, const float* sm,
         for (int r = 0; r < 4; ++r) gz1[t][r] = gh[t][r] * mlp::act_bwd(a.act, e.z1[t][r], a.beta);
         G.db1[t] += gz1[t];
     }
-    mlp::mm_dw<4, IT1, KS1>(gz1, e.xb, scratch, G.dW1);
+    mlp::mm_dw<4, IT1, KS1>(gz1, mlp::FragF{e.xb}, scratch, G.dW1);
     v4f gin[IT1];
 #pragma unroll
     for (int t = 0; t < IT1; ++t) gin[t] = v4f{0, 0, 0, 0};
-    mlp::mm_bwd_in<16, IT1>(sm + a.w1, a.ld1, reinterpret_cast<const float*>(gz1), gin);
-    const float* gb = reinterpret_cast<const float*>(gin);  // gb[s] = grad of B fragment s
-    // table pairs: slot ((j*8 + c) * 64 + lane) after pair_base
+    mlp::mm_bwd_in<16, IT1>(sm + a.w1, a.ld1, mlp::FragV{gz1}, gin);
+    const mlp::FragV gb{gin};  // gb[s] = gradient of B fragment s
 #pragma unroll
     for (int j = 0; j < LJ; ++j) {
-        const int level = k + 4 * j;
-        const bool on = valid && level < a.active;
-        LevelPos pp{};
-        LevelParams lp{};
-        if (on) {
-            lp = lv[level];
-            pp = level_pos(lp, x);
-        }
-        const float g0 = gb[2 * j], g1 = gb[2 * j + 1];
-#pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            const int64_t slot = pair_base + (int64_t)(j * 8 + c) * 64 + l;
-            if (on) {
-                const float w = corner_weight(pp, c);
-                em.keys[slot] = corner_row(lp, pp, c);
-                Vals<2> v;
-                v.v[0] = w * g0;
-                v.v[1] = w * g1;
-                em.vals[slot] = v;
-            } else {
-                em.keys[slot] = em.sentinel;
-            }
-        }
+        genc[(2 * j) * 64 + l] = gb[2 * j];
+        genc[(2 * j + 1) * 64 + l] = gb[2 * j + 1];
     }
     if constexpr (DX) {
         float gp[3] = {0.0f, 0.0f, 0.0f};
@@ -421,19 +411,135 @@ __device__ __forceinline__ void geo_eval_bwd(const GeoArgs& a, const float* sm,
     }
 }
 
-// Row-0-only output gradient tile (FD / curvature evaluations).
-__device__ __forceinline__ v4f go_row0(float g) {
-    const int k = lane_id() >> 4;
-    return v4f{k == 0 ? g : 0.0f, 0.0f, 0.0f, 0.0f};
+// ---- pair emission with merging ----
+struct EmitTile {
+    uint32_t* keys;   // tile region (keys)
+    Vals<2>* vals;    // tile region (values)
+    int cursor;       // wave-uniform write position
+};
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+    const int l = lane_id();
+    return l == 0 ? 0ull : (~0ull >> (64 - l));
+}
+
+// Lanes with `on` write 8 pairs (corner rows of `cell` at level `lp`,
+// values acc) at consecutive positions in lane order.
+__device__ __forceinline__ void emit_corners(EmitTile& et, bool on, const LevelParams& lp,
+                                             const uint32_t cell[3], const float acc[16]) {
+    const uint64_t mask = __ballot(on);
+    const int off = et.cursor + 8 * __popcll(mask & lanemask_lt());
+    if (on) {
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            et.keys[off + c] = lp.offset + grid_row(lp, cell[0] + (c & 1), cell[1] + ((c >> 1) & 1),
+                                                    cell[2] + ((c >> 2) & 1));
+            Vals<2> v;
+            v.v[0] = acc[2 * c];
+            v.v[1] = acc[2 * c + 1];
+            et.vals[off + c] = v;
+        }
+    }
+    et.cursor += 8 * __popcll(mask);
+}
+
+// Position of cluster evaluation q (0..n-1) for this lane.
+template <int CL>
+__device__ __forceinline__ void cluster_point(const GeoArgs& a, const GeoSample& s, int q,
+                                              float x[3]) {
+    bool m[3];
+    if (CL == 0) {  // base, then FD 0..5
+        if (q == 0) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) x[d] = s.x0[d];
+        } else {
+            offset_point(a, s.P, q - 1, x, m);
+        }
+    } else {        // curvature 0..5
+        offset_point(a, s.ps, q, x, m);
+    }
+}
+
+// Reduces the stored encode gradients of one cluster of evaluations into
+// merged corner pairs.  genc_base: LDS [eval][LJ][2][64] of this wave;
+// eslot(q) maps the cluster's q-th evaluation to its genc slot.
+template <int LJ, int CL>
+__device__ __forceinline__ void flush_cluster(const GeoArgs& a, const LevelParams* lv,
+                                              const GeoSample& s, bool valid, const float* genc_base,
+                                              int n_q, int slot0, EmitTile& et) {
+    const int l = lane_id(), p = l & 15, k = l >> 4;
+#pragma unroll 1
+    for (int j = 0; j < LJ; ++j) {
+        const int level = k + 4 * j;
+        const bool on = valid && level < a.active;
+        LevelParams lp{};
+        if (level < a.g.L) lp = lv[level];
+        uint32_t cell[3] = {0u, 0u, 0u};
+        bool have = false;
+        float acc[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) acc[q] = 0.0f;
+#pragma unroll 1
+        for (int q = 0; q < n_q; ++q) {
+            float x[3];
+            cluster_point<CL>(a, s, q, x);
+            LevelPos pp = level_pos(lp, x);
+            const bool same = have && pp.c[0] == cell[0] && pp.c[1] == cell[1] && pp.c[2] == cell[2];
+            // a run ends: emit it (no cross-lane merge for interior runs)
+            emit_corners(et, on && have && !same, lp, cell, acc);
+            if (!same) {
+#pragma unroll
+                for (int d = 0; d < 3; ++d) cell[d] = pp.c[d];
+#pragma unroll
+                for (int c = 0; c < 16; ++c) acc[c] = 0.0f;
+                have = true;
+            }
+            const int slot = slot0 + q;
+            const float g0 = genc_base[(slot * LJ + j) * 128 + l];
+            const float g1 = genc_base[(slot * LJ + j) * 128 + 64 + l];
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                const float w = corner_weight(pp, c);
+                acc[2 * c] = __fmaf_rn(w, g0, acc[2 * c]);
+                acc[2 * c + 1] = __fmaf_rn(w, g1, acc[2 * c + 1]);
+            }
+        }
+        // last run: merge equal cells of neighbouring lanes (same lane group),
+        // suffix segmented sum in a fixed tree, segment head emits.
+        const uint32_t INV = 0xFFFFFFFFu;
+        uint32_t key0 = on ? cell[0] : INV, key1 = on ? cell[1] : INV, key2 = on ? cell[2] : INV;
+        const uint32_t n0 = __shfl_down(key0, 1, 16), n1 = __shfl_down(key1, 1, 16),
+                       n2 = __shfl_down(key2, 1, 16);
+        bool end = (p == 15) || !(n0 == key0 && n1 == key1 && n2 == key2) || key0 == INV;
+#pragma unroll
+        for (int d = 1; d < 16; d <<= 1) {
+            float o[16];
+#pragma unroll
+            for (int c = 0; c < 16; ++c) o[c] = __shfl_down(acc[c], d, 16);
+            const bool oend = __shfl_down((int)end, d, 16) != 0;
+            if (!end && p + d < 16) {
+#pragma unroll
+                for (int c = 0; c < 16; ++c) acc[c] += o[c];
+                end = oend;
+            }
+        }
+        const uint32_t u0 = __shfl_up(key0, 1, 16), u1 = __shfl_up(key1, 1, 16),
+                       u2 = __shfl_up(key2, 1, 16);
+        const bool head = on && (p == 0 || !(u0 == key0 && u1 == key1 && u2 == key2));
+        emit_corners(et, head, lp, cell, acc);
+    }
 }
 
 template <int LJ, int NH>
 __global__ void __launch_bounds__(256) k_geo_bwd(
-    GeoArgs a, const float* __restrict__ pts, const float* __restrict__ randn, int64_t N,
-    const float* __restrict__ table_f, const float* __restrict__ params,
+    GeoArgs a, const float* __restrict__ pts,
+    const float* __restrict__ randn, int64_t N, const float* __restrict__ table_f,
+    const float* __restrict__ params, const float* __restrict__ fdv,
     const float* __restrict__ g_sdf, const float* __restrict__ g_grad,
     const float* __restrict__ g_normal, const float* __restrict__ g_feature,
-    const float* __restrict__ g_laplace, EmitCtx em, float* __restrict__ partial) {
+    const float* __restrict__ g_laplace, uint32_t* __restrict__ rkeys,
+    Vals<2>* __restrict__ rvals, int64_t region, int32_t* __restrict__ counts,
+    float* __restrict__ partial) {
     extern __shared__ float sm[];
     __shared__ LevelParams lv[AM_MAX_LEVELS];
     geo_load_lds<LJ, NH>(a, params, sm, lv);
@@ -442,9 +548,10 @@ __global__ void __launch_bounds__(256) k_geo_bwd(
     const int l = lane_id(), p = l & 15, k = l >> 4;
     const int nw = blockDim.x >> 6, wave = threadIdx.x >> 6;
     float* scratch = sm + a.lds_floats + wave * (128 * mlp::kScratchLd);
+    // genc storage: 7 evaluation slots x LJ x 2 x 64 lanes per wave
+    float* genc = sm + a.lds_floats + nw * (128 * mlp::kScratchLd) + wave * (7 * LJ * 128);
     const int64_t n_tiles = (N + 15) / 16;
     const int fdim = a.n_out + 3;
-    const int64_t pairs_per_eval = (int64_t)LJ * 8 * 64;
     GeoGrad<LJ, NH> G;
 #pragma unroll
     for (int q = 0; q < 4 * GeoDims<LJ>::IT1; ++q) G.dW1[q] = v4f{0, 0, 0, 0};
@@ -463,9 +570,33 @@ __global__ void __launch_bounds__(256) k_geo_bwd(
          tile += (int64_t)gridDim.x * nw) {
         const int64_t n = tile * 16 + p;
         const bool valid = n < N;
-        const int64_t pair_tile = tile * a.n_evals * pairs_per_eval;
+        EmitTile et{rkeys + tile * region, rvals + tile * region, 0};
         GeoSample s;
-        geo_sample_fwd<LJ, NH>(a, sm, lv, table, pts, randn, n, valid, s, nullptr);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) s.P[d] = valid ? pts[3 * n + d] : 0.0f;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) s.x0[d] = contract1(s.P[d], a);
+        // rebuild grad / normal / tangent / curvature terms from the stored values
+        if (a.with_grad) {
+            float g[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int j = 0; j < 6; ++j) diff_pair(g, j, valid ? fdv[n * 12 + j] : 0.0f);
+            finish_grad(a, g);
+#pragma unroll
+            for (int d = 0; d < 3; ++d) s.grad[d] = g[d];
+            normalize3(s.grad, s.n);
+        }
+        if (a.with_laplace) {
+            geo_tangent(a, randn, n, valid, s);
+            float gs[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int j = 0; j < 6; ++j) diff_pair(gs, j, valid ? fdv[n * 12 + 6 + j] : 0.0f);
+            finish_grad(a, gs);
+#pragma unroll
+            for (int d = 0; d < 3; ++d) s.grad_s[d] = gs[d];
+            normalize3(s.grad_s, s.ns);
+            s.dot = (s.n[0] * s.ns[0] + s.n[1] * s.ns[1]) + s.n[2] * s.ns[2];
+        }
         // upstream gradients of this sample
         float gg[3] = {0, 0, 0}, gnrm[3] = {0, 0, 0};
         if (valid && g_grad)
@@ -474,7 +605,23 @@ __global__ void __launch_bounds__(256) k_geo_bwd(
         if (valid && g_normal)
 #pragma unroll
             for (int d = 0; d < 3; ++d) gnrm[d] = g_normal[3 * n + d];
-        int ev = 0;
+        // base evaluation (genc slot 0)
+        {
+            v4f go;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = 4 * k + r;
+                float v = 0.0f;
+                if (valid && row < a.n_out) {
+                    if (g_feature) v = g_feature[n * fdim + row];
+                    if (row == 0 && g_sdf) v += g_sdf[n];
+                }
+                go[r] = v;
+            }
+            GeoEval<LJ, NH, false> e;
+            geo_eval<LJ, NH, false>(a, sm, lv, table, s.x0, e);
+            geo_eval_bwd<LJ, NH, false>(a, sm, e, go, scratch, G, genc, nullptr);
+        }
         if (a.with_laplace) {
             const float gl = (valid && g_laplace) ? g_laplace[n] : 0.0f;
             const float lo = -1.0f + 1e-6f, hi = 1.0f - 1e-6f;
@@ -490,22 +637,23 @@ __global__ void __launch_bounds__(256) k_geo_bwd(
             float g_gs[3];
             normalize3_bwd(s.grad_s, g_ns, g_gs);
             float g_ps[3] = {0.0f, 0.0f, 0.0f};
-#pragma unroll
+#pragma unroll 1
             for (int j = 0; j < 6; ++j) {
-                const float gf = (j & 1 ? -0.5f : 0.5f) * a.inv_eps * g_gs[j >> 1];
+                const float ggs = pick3(g_gs, j >> 1);
+                const float gf = (j & 1 ? -0.5f : 0.5f) * a.inv_eps * ggs;
                 float x[3];
                 bool m[3];
-                geo_shifted_point<LJ, NH>(a, s, j, x, m);
+                offset_point(a, s.ps, j, x, m);
                 GeoEval<LJ, NH, true> e;
                 geo_eval<LJ, NH, true>(a, sm, lv, table, x, e);
                 float gpos[3];
-                geo_eval_bwd<LJ, NH, true>(a, sm, lv, x, e, go_row0(gf), scratch, G, em,
-                                           pair_tile + (int64_t)(ev++) * pairs_per_eval, valid,
-                                           gpos);
+                geo_eval_bwd<LJ, NH, true>(a, sm, e, go_row0(gf), scratch, G,
+                                           genc + (1 + j) * LJ * 128, gpos);
 #pragma unroll
                 for (int d = 0; d < 3; ++d)
                     if (m[d]) g_ps[d] += gpos[d] * a.inv2r;
             }
+            flush_cluster<LJ, 1>(a, lv, s, valid, genc, 6, 1, et);
             float g_t[3], gcr[3];
 #pragma unroll
             for (int d = 0; d < 3; ++d) g_t[d] = g_ps[d] * a.eps;
@@ -518,37 +666,20 @@ __global__ void __launch_bounds__(256) k_geo_bwd(
             normalize3_bwd(s.grad, gnrm, gn2g);
 #pragma unroll
             for (int d = 0; d < 3; ++d) gg[d] += gn2g[d];
-#pragma unroll
+#pragma unroll 1
             for (int j = 0; j < 6; ++j) {
-                const float gf = (j & 1 ? -0.5f : 0.5f) * a.inv_eps * gg[j >> 1];
+                const float gf = (j & 1 ? -0.5f : 0.5f) * a.inv_eps * pick3(gg, j >> 1);
                 float x[3];
                 bool m[3];
-                fd_point(a, s.P, j, x, m);
+                offset_point(a, s.P, j, x, m);
                 GeoEval<LJ, NH, false> e;
                 geo_eval<LJ, NH, false>(a, sm, lv, table, x, e);
-                geo_eval_bwd<LJ, NH, false>(a, sm, lv, x, e, go_row0(gf), scratch, G, em,
-                                            pair_tile + (int64_t)(ev++) * pairs_per_eval, valid,
-                                            nullptr);
+                geo_eval_bwd<LJ, NH, false>(a, sm, e, go_row0(gf), scratch, G,
+                                            genc + (1 + j) * LJ * 128, nullptr);
             }
         }
-        {
-            v4f go;
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int row = 4 * k + r;
-                float v = 0.0f;
-                if (valid && row < a.n_out) {
-                    if (g_feature) v = g_feature[n * fdim + row];
-                    if (row == 0 && g_sdf) v += g_sdf[n];
-                }
-                go[r] = v;
-            }
-            GeoEval<LJ, NH, false> e;
-            geo_eval<LJ, NH, false>(a, sm, lv, table, s.x0, e);
-            geo_eval_bwd<LJ, NH, false>(a, sm, lv, s.x0, e, go, scratch, G, em,
-                                        pair_tile + (int64_t)(ev++) * pairs_per_eval, valid,
-                                        nullptr);
-        }
+        flush_cluster<LJ, 0>(a, lv, s, valid, genc, a.with_grad ? 7 : 1, 0, et);
+        if (l == 0) counts[tile] = et.cursor;
     }
     // per-wave partial gradient row in parameter layout
     float* out = partial + ((int64_t)blockIdx.x * nw + wave) * a.n_params;
@@ -598,6 +729,25 @@ __global__ void __launch_bounds__(256) k_geo_bwd(
 #pragma unroll
         for (int r = 0; r < 4; ++r)
             if (4 * k + r < a.n_out) out[a.pbo + 4 * k + r] = so[r];
+}
+
+// Compaction: tile regions -> contiguous pairs at the tiles' prefix offsets.
+__global__ void k_compact_pairs(const uint32_t* __restrict__ rkeys, const Vals<2>* __restrict__ rvals,
+                                int64_t region, const int32_t* __restrict__ counts,
+                                const int64_t* __restrict__ offsets, uint32_t* __restrict__ keys,
+                                Vals<2>* __restrict__ vals) {
+    const int64_t tile = blockIdx.x;
+    const int c = counts[tile];
+    const int64_t o = offsets[tile];
+    for (int q = threadIdx.x; q < c; q += blockDim.x) {
+        keys[o + q] = rkeys[tile * region + q];
+        vals[o + q] = rvals[tile * region + q];
+    }
+}
+
+__global__ void k_counts_to_i64(const int32_t* __restrict__ c, int64_t n, int64_t* __restrict__ o) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) o[i] = c[i];
 }
 
 // ------------------------------------------------------------------------
@@ -652,55 +802,127 @@ static int make_geo_args(const am_grid_desc* g, const am_mlp_desc* m, const am_s
 }
 
 template <int LJ, int NH>
-static int geo_fwd_launch(const GeoArgs& a, const float* pts, const float* randn, int64_t n,
-                          const float* table, const float* params, float* sdf, float* grad,
-                          float* normal, float* feature, float* laplace, hipStream_t s) {
+static int geo_fwd_launch(const GeoArgs& a, const float* pts,
+                          const float* randn, int64_t n, const float* table, const float* params,
+                          float* sdf, float* grad, float* normal, float* feature, float* laplace,
+                          float* fdv, hipStream_t s) {
     const size_t lds = (size_t)a.lds_floats * sizeof(float);
     AM_HIP(hipFuncSetAttribute((const void*)k_geo_fwd<LJ, NH>,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     {
         KernelTimer kt_("geo_fwd", s);
-        hipLaunchKernelGGL((k_geo_fwd<LJ, NH>), dim3(field_blocks(n)), dim3(256), lds, s, a, pts, randn,
-                           n, table, params, sdf, grad, normal, feature, laplace);
+        hipLaunchKernelGGL((k_geo_fwd<LJ, NH>), dim3(field_blocks(n)), dim3(256), lds, s, a, pts,
+                           randn, n, table, params, sdf, grad, normal, feature, laplace, fdv);
     }
     AM_LAUNCH_CHECK();
     return AM_OK;
 }
 
-static int64_t geo_pairs(const GeoArgs& a, int LJ, int64_t n) {
-    return ((n + 15) / 16) * (int64_t)a.n_evals * LJ * 8 * 64;
+// Upper bound of the pairs one tile can emit (every evaluation, every level
+// slot, every lane unmerged).
+static int64_t geo_region(const GeoArgs& a, int LJ) { return (int64_t)a.n_evals * LJ * 8 * 64; }
+
+struct GeoBwdWs {
+    float* fdv;
+    float* sdf_tmp;
+    float* partial;
+    float* ctmp;
+    int32_t* counts;
+    int64_t* offsets64;
+    int64_t* counts64;
+    void* scan_tmp;
+    size_t scan_bytes;
+    TableReduceWs tw;
+};
+
+static int carve_geo_bwd(WsCarver& c, const GeoArgs& a, int LJ, const am_grid_desc* g, int64_t n,
+                         bool need_fdv, GeoBwdWs* w) {
+    const int waves = field_blocks(n) * 4;
+    const int64_t tiles = (n + 15) / 16;
+    w->fdv = need_fdv ? c.take<float>((size_t)n * 12 + 1) : nullptr;
+    w->sdf_tmp = need_fdv ? c.take<float>((size_t)n + 1) : nullptr;
+    w->partial = c.take<float>((size_t)waves * a.n_params);
+    w->ctmp = c.take<float>((size_t)colsum_tmp_floats(waves, a.n_params) + 1);
+    w->counts = c.take<int32_t>((size_t)tiles + 1);
+    w->counts64 = c.take<int64_t>((size_t)tiles + 1);
+    w->offsets64 = c.take<int64_t>((size_t)tiles + 1);
+    size_t sb = 0;
+    rocprim::exclusive_scan(nullptr, sb, (const int64_t*)nullptr, (int64_t*)nullptr, (int64_t)0,
+                            (size_t)(tiles > 0 ? tiles : 1), rocprim::plus<int64_t>(), 0, false);
+    w->scan_bytes = sb;
+    w->scan_tmp = c.take<char>(sb + 256);
+    return table_reduce_carve(c, 2, tiles * geo_region(a, LJ), g->n_entries, &w->tw);
 }
 
 template <int LJ, int NH>
-static int geo_bwd_launch(const GeoArgs& a, const am_grid_desc* g, const float* pts,
-                          const float* randn, int64_t n, const float* table, const float* params,
+static int geo_bwd_launch(const GeoArgs& a, const am_grid_desc* g, const am_mlp_desc* m,
+                          const am_sdf_desc* sd, const float* pts, const float* randn, int64_t n,
+                          const float* table, const float* params, const float* fdv_in,
                           const float* g_sdf, const float* g_grad, const float* g_normal,
                           const float* g_feature, const float* g_laplace, float* dtable,
                           float* dparams, void* ws, size_t ws_bytes, hipStream_t s) {
     const int blocks = field_blocks(n);
     const int waves = blocks * 4;
+    const int64_t tiles = (n + 15) / 16;
+    const bool need_fdv = a.with_grad && fdv_in == nullptr;
     WsCarver c(ws, ws_bytes);
-    float* partial = c.take<float>((size_t)waves * a.n_params);
-    float* ctmp = c.take<float>((size_t)colsum_tmp_floats(waves, a.n_params) + 1);
-    TableReduceWs tw;
-    const int64_t m = geo_pairs(a, LJ, n);
-    if (int rc = table_reduce_carve(c, 2, m, g->n_entries, &tw)) return rc;
+    GeoBwdWs w;
+    if (int rc = carve_geo_bwd(c, a, LJ, g, n, need_fdv, &w)) return rc;
     if (!c.ok()) {
         set_error("workspace too small: need %zu, got %zu", c.used, ws_bytes);
         return AM_ERR_WORKSPACE;
     }
-    EmitCtx em{tw.keys[0], reinterpret_cast<Vals<2>*>(tw.vals[0]), table_sentinel(tw.end_bit)};
-    const size_t lds = (size_t)(a.lds_floats + 4 * 128 * mlp::kScratchLd) * sizeof(float);
+    const float* fdv = fdv_in;
+    if (need_fdv) {  // recompute the stored SDF values with a forward pass
+        if (int rc = geo_fwd_launch<LJ, NH>(a, pts, randn, n, table, params, w.sdf_tmp, nullptr,
+                                            nullptr, nullptr, nullptr, w.fdv, s))
+            return rc;
+        fdv = w.fdv;
+    }
+    const int64_t region = geo_region(a, LJ);
+    // tile regions live in the sort's alternate buffers; compacted pairs go to
+    // the primary ones
+    uint32_t* rkeys = w.tw.keys[1];
+    Vals<2>* rvals = reinterpret_cast<Vals<2>*>(w.tw.vals[1]);
+    const size_t lds = (size_t)(a.lds_floats + 4 * 128 * mlp::kScratchLd + 4 * 7 * LJ * 128) *
+                       sizeof(float);
+    AM_CHECK_ARG(lds <= 160 * 1024, "geometry backward needs %zu bytes of LDS", lds);
     AM_HIP(hipFuncSetAttribute((const void*)k_geo_bwd<LJ, NH>,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     {
         KernelTimer kt_("geo_bwd", s);
-        hipLaunchKernelGGL((k_geo_bwd<LJ, NH>), dim3(blocks), dim3(256), lds, s, a, pts, randn, n, table,
-                           params, g_sdf, g_grad, g_normal, g_feature, g_laplace, em, partial);
+        hipLaunchKernelGGL((k_geo_bwd<LJ, NH>), dim3(blocks), dim3(256), lds, s, a, pts, randn, n,
+                           table, params, fdv, g_sdf, g_grad, g_normal, g_feature, g_laplace,
+                           rkeys, rvals, region, w.counts, w.partial);
     }
     AM_LAUNCH_CHECK();
-    if (int rc = colsum_f32(partial, waves, a.n_params, dparams, ctmp, s)) return rc;
-    return table_reduce_run(tw, 2, g->n_entries, dtable, s);
+    if (int rc = colsum_f32(w.partial, waves, a.n_params, dparams, w.ctmp, s)) return rc;
+    // deterministic compaction: integer prefix sum of the per-tile counts
+    hipLaunchKernelGGL(k_counts_to_i64, dim3((unsigned)ceil_div(tiles, 256)), dim3(256), 0, s,
+                       w.counts, tiles, w.counts64);
+    AM_LAUNCH_CHECK();
+    size_t sb = w.scan_bytes;
+    hipError_t e = rocprim::exclusive_scan(w.scan_tmp, sb, w.counts64, w.offsets64, (int64_t)0,
+                                           (size_t)tiles, rocprim::plus<int64_t>(), s, false);
+    if (e != hipSuccess) {
+        set_error("tile-count scan failed: %s", hipGetErrorString(e));
+        return AM_ERR_HIP;
+    }
+    int64_t last_off = 0;
+    int32_t last_cnt = 0;
+    AM_HIP(hipMemcpyAsync(&last_off, w.offsets64 + tiles - 1, sizeof(int64_t),
+                          hipMemcpyDeviceToHost, s));
+    AM_HIP(hipMemcpyAsync(&last_cnt, w.counts + tiles - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    {
+        KernelTimer kt_("pair_compact", s);
+        hipLaunchKernelGGL(k_compact_pairs, dim3((unsigned)tiles), dim3(256), 0, s, rkeys, rvals,
+                           region, w.counts, w.offsets64, w.tw.keys[0],
+                           reinterpret_cast<Vals<2>*>(w.tw.vals[0]));
+    }
+    AM_LAUNCH_CHECK();
+    AM_HIP(hipStreamSynchronize(s));  // one host read: the number of pairs
+    w.tw.m = last_off + last_cnt;
+    return table_reduce_run(w.tw, 2, g->n_entries, dtable, s);
 }
 
 #define AM_GEO_DISPATCH(FN, ...)                                                   \
@@ -719,7 +941,7 @@ using namespace am;
 extern "C" int am_sdf_fwd(const am_grid_desc* g, const am_mlp_desc* m, const am_sdf_desc* sd,
                           const float* pts, const float* randn, int64_t n, const float* table,
                           const float* params, float* sdf, float* grad, float* normal,
-                          float* feature, float* laplace, void* stream) {
+                          float* feature, float* laplace, float* fd_values, void* stream) {
     GeoArgs a;
     int LJ;
     if (int rc = make_geo_args(g, m, sd, &a, &LJ)) return rc;
@@ -729,7 +951,7 @@ extern "C" int am_sdf_fwd(const am_grid_desc* g, const am_mlp_desc* m, const am_
     AM_CHECK_ARG(!sd->with_laplace || randn, "laplace needs randn");
     hipStream_t s = to_stream(stream);
     AM_GEO_DISPATCH(geo_fwd_launch, a, pts, randn, n, table, params, sdf, grad, normal, feature,
-                    laplace, s);
+                    laplace, fd_values, s);
 }
 
 extern "C" size_t am_sdf_bwd_workspace_size(const am_grid_desc* g, const am_mlp_desc* m,
@@ -737,30 +959,31 @@ extern "C" size_t am_sdf_bwd_workspace_size(const am_grid_desc* g, const am_mlp_
     GeoArgs a;
     int LJ;
     if (make_geo_args(g, m, sd, &a, &LJ) || n < 0) return 0;
-    const int waves = field_blocks(n) * 4;
     WsCarver c(nullptr, 0);
-    c.take<float>((size_t)waves * a.n_params);
-    c.take<float>((size_t)colsum_tmp_floats(waves, a.n_params) + 1);
-    TableReduceWs tw;
-    if (table_reduce_carve(c, 2, geo_pairs(a, LJ, n), g->n_entries, &tw)) return 0;
+    GeoBwdWs w;
+    if (carve_geo_bwd(c, a, LJ, g, n, true, &w)) return 0;
     return c.used + 256;
 }
 
 extern "C" int am_sdf_bwd(const am_grid_desc* g, const am_mlp_desc* m, const am_sdf_desc* sd,
                           const float* pts, const float* randn, int64_t n, const float* table,
-                          const float* params, const float* g_sdf, const float* g_grad,
-                          const float* g_normal, const float* g_feature, const float* g_laplace,
-                          float* dtable, float* dparams, void* workspace, size_t workspace_bytes,
-                          void* stream) {
+                          const float* params, const float* fd_values, const float* g_sdf,
+                          const float* g_grad, const float* g_normal, const float* g_feature,
+                          const float* g_laplace, float* dtable, float* dparams, void* workspace,
+                          size_t workspace_bytes, void* stream) {
     GeoArgs a;
     int LJ;
     if (int rc = make_geo_args(g, m, sd, &a, &LJ)) return rc;
     AM_CHECK_ARG(n >= 0, "n < 0");
     AM_CHECK_ARG(table && params && dtable && dparams && workspace, "NULL buffer");
-    AM_CHECK_ARG(n == 0 || pts, "NULL pts");
-    AM_CHECK_ARG(!sd->with_laplace || n == 0 || randn, "laplace needs randn");
     hipStream_t s = to_stream(stream);
-    AM_GEO_DISPATCH(geo_bwd_launch, a, g, pts, randn, n, table, params, g_sdf, g_grad, g_normal,
-                    g_feature, g_laplace, dtable, dparams, workspace, workspace_bytes, s);
+    if (n == 0) {
+        if (int rc = fill_f32(dtable, (int64_t)g->n_entries * 2, 0.0f, s)) return rc;
+        return fill_f32(dparams, a.n_params, 0.0f, s);
+    }
+    AM_CHECK_ARG(pts, "NULL pts");
+    AM_CHECK_ARG(!sd->with_laplace || randn, "laplace needs randn");
+    AM_GEO_DISPATCH(geo_bwd_launch, a, g, m, sd, pts, randn, n, table, params, fd_values, g_sdf,
+                    g_grad, g_normal, g_feature, g_laplace, dtable, dparams, workspace,
+                    workspace_bytes, s);
 }
-

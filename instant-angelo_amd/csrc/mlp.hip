@@ -73,9 +73,8 @@ __device__ void load_mlp_lds(const MlpArgs& a, const float* __restrict__ params,
     }
 }
 
-template <int OT>
-__device__ __forceinline__ void out_layer(const MlpArgs& a, const float* sm, const float* h,
-                                          v4f* o) {
+template <int OT, class Frag>
+__device__ __forceinline__ void out_layer(const MlpArgs& a, const float* sm, Frag h, v4f* o) {
     const int L = a.n_layers - 1;
 #pragma unroll
     for (int t = 0; t < OT; ++t) o[t] = mlp::bias_tile(sm + a.b_lds[L], t, a.rows_out[L]);
@@ -90,7 +89,7 @@ __device__ __forceinline__ void tile_fwd(const MlpArgs& a, const float* sm, cons
         v4f acc[4];
 #pragma unroll
         for (int t = 0; t < 4; ++t) acc[t] = mlp::bias_tile(sm + a.b_lds[0], t, 64);
-        mlp::mm_fwd<KS, 4>(sm + a.w_lds[0], a.ld[0], xb, acc);
+        mlp::mm_fwd<KS, 4>(sm + a.w_lds[0], a.ld[0], mlp::FragF{xb}, acc);
 #pragma unroll
         for (int t = 0; t < 4; ++t) z[0][t] = acc[t];
     }
@@ -103,7 +102,7 @@ __device__ __forceinline__ void tile_fwd(const MlpArgs& a, const float* sm, cons
                 h[t] = mlp::act4(a.act, z[L - 1][t], a.beta);
                 acc[t] = mlp::bias_tile(sm + a.b_lds[L], t, 64);
             }
-            mlp::mm_fwd<16, 4>(sm + a.w_lds[L], a.ld[L], reinterpret_cast<const float*>(h), acc);
+            mlp::mm_fwd<16, 4>(sm + a.w_lds[L], a.ld[L], mlp::FragV{h}, acc);
 #pragma unroll
             for (int t = 0; t < 4; ++t) z[L][t] = acc[t];
         }
@@ -118,7 +117,7 @@ __device__ __forceinline__ void tile_fwd(const MlpArgs& a, const float* sm, cons
             if (L == Lh) zz = z[L][t];
         h[t] = mlp::act4(a.act, zz, a.beta);
     }
-    out_layer<OT>(a, sm, reinterpret_cast<const float*>(h), zo);
+    out_layer<OT>(a, sm, mlp::FragV{h}, zo);
 }
 
 template <int KS>
@@ -234,12 +233,12 @@ __global__ void __launch_bounds__(64) k_mlp_bwd(MlpArgs a, const float* __restri
             v4f dW[OT * 4];
 #pragma unroll
             for (int q = 0; q < OT * 4; ++q) dW[q] = v4f{0, 0, 0, 0};
-            mlp::mm_dw<OT, 4, 16>(go, reinterpret_cast<const float*>(hin), scratch, dW);
+            mlp::mm_dw<OT, 4, 16>(go, mlp::FragV{hin}, scratch, dW);
             add_dw_lds<OT, 4>(gacc + a.w_lds[Lo], a.ld[Lo], dW);
             add_db_lds<OT>(gacc + a.b_lds[Lo], go);
 #pragma unroll
             for (int t = 0; t < 4; ++t) gh[t] = v4f{0, 0, 0, 0};
-            mlp::mm_bwd_in<4 * OT, 4>(sm + a.w_lds[Lo], a.ld[Lo], reinterpret_cast<const float*>(go), gh);
+            mlp::mm_bwd_in<4 * OT, 4>(sm + a.w_lds[Lo], a.ld[Lo], mlp::FragV{go}, gh);
         }
         // hidden layers, last to first
 #pragma unroll
@@ -255,13 +254,13 @@ __global__ void __launch_bounds__(64) k_mlp_bwd(MlpArgs a, const float* __restri
                 v4f dW[4 * (KS / 4)];
 #pragma unroll
                 for (int q = 0; q < 4 * (KS / 4); ++q) dW[q] = v4f{0, 0, 0, 0};
-                mlp::mm_dw<4, KS / 4, KS>(gz, xb, scratch, dW);
+                mlp::mm_dw<4, KS / 4, KS>(gz, mlp::FragF{xb}, scratch, dW);
                 add_dw_lds<4, KS / 4>(gacc + a.w_lds[0], a.ld[0], dW);
                 if (dx) {
                     v4f gx[KS / 4];
 #pragma unroll
                     for (int t = 0; t < KS / 4; ++t) gx[t] = v4f{0, 0, 0, 0};
-                    mlp::mm_bwd_in<16, KS / 4>(sm + a.w_lds[0], a.ld[0], reinterpret_cast<const float*>(gz), gx);
+                    mlp::mm_bwd_in<16, KS / 4>(sm + a.w_lds[0], a.ld[0], mlp::FragV{gz}, gx);
                     if (valid)
 #pragma unroll
                         for (int t = 0; t < KS / 4; ++t)
@@ -278,11 +277,11 @@ __global__ void __launch_bounds__(64) k_mlp_bwd(MlpArgs a, const float* __restri
                 v4f dW[16];
 #pragma unroll
                 for (int q = 0; q < 16; ++q) dW[q] = v4f{0, 0, 0, 0};
-                mlp::mm_dw<4, 4, 16>(gz, reinterpret_cast<const float*>(hin), scratch, dW);
+                mlp::mm_dw<4, 4, 16>(gz, mlp::FragV{hin}, scratch, dW);
                 add_dw_lds<4, 4>(gacc + a.w_lds[L], a.ld[L], dW);
 #pragma unroll
                 for (int t = 0; t < 4; ++t) gh[t] = v4f{0, 0, 0, 0};
-                mlp::mm_bwd_in<16, 4>(sm + a.w_lds[L], a.ld[L], reinterpret_cast<const float*>(gz), gh);
+                mlp::mm_bwd_in<16, 4>(sm + a.w_lds[L], a.ld[L], mlp::FragV{gz}, gh);
             }
         }
     }

@@ -11,6 +11,10 @@
 // registers are the next layer's B fragments as they stand, and an input
 // gradient W^T*G comes out in the same lanes that hold the inputs.  Weight
 // matrices sit in LDS as [out rows][ld] with columns in row space.
+//
+// Fragments are read through FragF (plain float arrays) or FragV (D tiles,
+// fragment s = tile s/4, register s%4) so register arrays are never
+// reinterpreted through pointers (which would push them to scratch).
 #pragma once
 #include "am_common.h"
 
@@ -18,6 +22,15 @@ namespace am {
 namespace mlp {
 
 __device__ __forceinline__ int krow(int s, int k) { return ((s >> 2) << 4) + (k << 2) + (s & 3); }
+
+struct FragF {
+    const float* p;
+    __device__ __forceinline__ float operator[](int s) const { return p[s]; }
+};
+struct FragV {
+    const v4f* p;
+    __device__ __forceinline__ float operator[](int s) const { return p[s >> 2][s & 3]; }
+};
 
 // Orders LDS traffic of one wave (lanes exchange data through LDS).
 __device__ __forceinline__ void wave_sync() {
@@ -73,33 +86,33 @@ __device__ __forceinline__ v4f bias_tile(const float* __restrict__ b, int t, int
 }
 
 // acc[ot] += W[16ot + i][krow(s,k)] * b[s]
-template <int KS, int OT>
-__device__ __forceinline__ void mm_fwd(const float* __restrict__ W, int ld, const float* b,
-                                       v4f* acc) {
+template <int KS, int OT, class Frag>
+__device__ __forceinline__ void mm_fwd(const float* __restrict__ W, int ld, Frag b, v4f* acc) {
     const int l = lane_id();
     const int i = l & 15, k = l >> 4;
 #pragma unroll
     for (int s = 0; s < KS; ++s) {
+        const float bs = b[s];
 #pragma unroll
         for (int ot = 0; ot < OT; ++ot) {
             const float a = W[(16 * ot + i) * ld + krow(s, k)];
-            acc[ot] = mfma4(a, b[s], acc[ot]);
+            acc[ot] = mfma4(a, bs, acc[ot]);
         }
     }
 }
 
 // gin[it] += W[krow(s,k)][16it + i] * g[s]   (W^T * G)
-template <int KS, int IT>
-__device__ __forceinline__ void mm_bwd_in(const float* __restrict__ W, int ld, const float* g,
-                                          v4f* gin) {
+template <int KS, int IT, class Frag>
+__device__ __forceinline__ void mm_bwd_in(const float* __restrict__ W, int ld, Frag g, v4f* gin) {
     const int l = lane_id();
     const int i = l & 15, k = l >> 4;
 #pragma unroll
     for (int s = 0; s < KS; ++s) {
+        const float gs = g[s];
 #pragma unroll
         for (int it = 0; it < IT; ++it) {
             const float a = W[krow(s, k) * ld + 16 * it + i];
-            gin[it] = mfma4(a, g[s], gin[it]);
+            gin[it] = mfma4(a, gs, gin[it]);
         }
     }
 }
@@ -110,8 +123,8 @@ constexpr int kScratchLd = 17;
 // input row space) summed over the 16 points of the tile.  Transposes go
 // through the wave-private LDS scratch (16*(OT+IT) rows x 17).
 // Result layout: lane (i,k), register r -> dW[16ot + 4k + r][16it + i].
-template <int OT, int IT, int KS>
-__device__ __forceinline__ void mm_dw(const v4f* G, const float* X, float* scratch, v4f* dW) {
+template <int OT, int IT, int KS, class Frag>
+__device__ __forceinline__ void mm_dw(const v4f* G, Frag X, float* scratch, v4f* dW) {
     const int l = lane_id();
     const int i = l & 15, k = l >> 4;
     float* Gs = scratch;

@@ -12,7 +12,7 @@
 extern "C" int am_sdf_fwd(const am_grid_desc* g, const am_mlp_desc* m, const am_sdf_desc* sd,
                           const float* pts, const float* randn, int64_t n, const float* table,
                           const float* params, float* sdf, float* grad, float* normal,
-                          float* feature, float* laplace, void* stream);
+                          float* feature, float* laplace, float* fd_values, void* stream);
 
 namespace am {
 
@@ -143,7 +143,7 @@ extern "C" int am_occ_update(const am_grid_desc* g, const am_mlp_desc* mlp, cons
         AM_LAUNCH_CHECK();
         TimerAlias alias("occ_sdf");
         if (int rc = am_sdf_fwd(g, mlp, &sd, pts, nullptr, m, table, params, sdf, nullptr, nullptr,
-                                nullptr, nullptr, stream))
+                                nullptr, nullptr, nullptr, stream))
             return rc;
         hipLaunchKernelGGL(k_occ_alpha, dim3((unsigned)ceil_div(m, 256)), dim3(256), 0, st, cells,
                            sdf, m, inv_s, step * 0.5f, best);

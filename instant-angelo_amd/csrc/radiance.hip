@@ -139,19 +139,19 @@ __device__ __forceinline__ void rad_eval(const RadArgs& a, const float* sm,
     }
 #pragma unroll
     for (int t = 0; t < 4; ++t) e.z1[t] = mlp::bias_tile(sm + a.b1, t, 64);
-    mlp::mm_fwd<KS, 4>(sm + a.w1, a.ld1, e.xb, e.z1);
+    mlp::mm_fwd<KS, 4>(sm + a.w1, a.ld1, mlp::FragF{e.xb}, e.z1);
     v4f h[4];
 #pragma unroll
     for (int t = 0; t < 4; ++t) h[t] = mlp::act4(a.act, e.z1[t], a.beta);
     if constexpr (NH == 2) {
 #pragma unroll
         for (int t = 0; t < 4; ++t) e.z2[t] = mlp::bias_tile(sm + a.b2, t, 64);
-        mlp::mm_fwd<16, 4>(sm + a.w2, 65, reinterpret_cast<const float*>(h), e.z2);
+        mlp::mm_fwd<16, 4>(sm + a.w2, 65, mlp::FragV{h}, e.z2);
 #pragma unroll
         for (int t = 0; t < 4; ++t) h[t] = mlp::act4(a.act, e.z2[t], a.beta);
     }
     e.out = mlp::bias_tile(sm + a.bo, 0, a.n_out);
-    mlp::mm_fwd<16, 1>(sm + a.wo, 65, reinterpret_cast<const float*>(h), &e.out);
+    mlp::mm_fwd<16, 1>(sm + a.wo, 65, mlp::FragV{h}, &e.out);
 }
 
 template <int KS, int NH>
@@ -229,12 +229,12 @@ __global__ void __launch_bounds__(256) k_rad_bwd(RadArgs a, const float* __restr
         const v4f* zl = NH == 2 ? e.z2 : e.z1;
 #pragma unroll
         for (int t = 0; t < 4; ++t) h[t] = mlp::act4(a.act, zl[t], a.beta);
-        mlp::mm_dw<1, 4, 16>(&go, reinterpret_cast<const float*>(h), scratch, dWo);
+        mlp::mm_dw<1, 4, 16>(&go, mlp::FragV{h}, scratch, dWo);
         dbo += go;
         v4f gh[4];
 #pragma unroll
         for (int t = 0; t < 4; ++t) gh[t] = v4f{0, 0, 0, 0};
-        mlp::mm_bwd_in<4, 4>(sm + a.wo, 65, reinterpret_cast<const float*>(&go), gh);
+        mlp::mm_bwd_in<4, 4>(sm + a.wo, 65, mlp::FragV{&go}, gh);
         if constexpr (NH == 2) {
             v4f gz2[4], h1[4];
 #pragma unroll
@@ -244,10 +244,10 @@ __global__ void __launch_bounds__(256) k_rad_bwd(RadArgs a, const float* __restr
                 h1[t] = mlp::act4(a.act, e.z1[t], a.beta);
                 db2[t] += gz2[t];
             }
-            mlp::mm_dw<4, 4, 16>(gz2, reinterpret_cast<const float*>(h1), scratch, dW2);
+            mlp::mm_dw<4, 4, 16>(gz2, mlp::FragV{h1}, scratch, dW2);
 #pragma unroll
             for (int t = 0; t < 4; ++t) gh[t] = v4f{0, 0, 0, 0};
-            mlp::mm_bwd_in<16, 4>(sm + a.w2, 65, reinterpret_cast<const float*>(gz2), gh);
+            mlp::mm_bwd_in<16, 4>(sm + a.w2, 65, mlp::FragV{gz2}, gh);
         }
         v4f gz1[4];
 #pragma unroll
@@ -256,11 +256,11 @@ __global__ void __launch_bounds__(256) k_rad_bwd(RadArgs a, const float* __restr
             for (int r = 0; r < 4; ++r) gz1[t][r] = gh[t][r] * mlp::act_bwd(a.act, e.z1[t][r], a.beta);
             db1[t] += gz1[t];
         }
-        mlp::mm_dw<4, IT1, KS>(gz1, e.xb, scratch, dW1);
+        mlp::mm_dw<4, IT1, KS>(gz1, mlp::FragF{e.xb}, scratch, dW1);
         v4f gin[IT1];
 #pragma unroll
         for (int t = 0; t < IT1; ++t) gin[t] = v4f{0, 0, 0, 0};
-        mlp::mm_bwd_in<16, IT1>(sm + a.w1, a.ld1, reinterpret_cast<const float*>(gz1), gin);
+        mlp::mm_bwd_in<16, IT1>(sm + a.w1, a.ld1, mlp::FragV{gz1}, gin);
         if (valid) {
 #pragma unroll
             for (int t = 0; t < IT1; ++t)

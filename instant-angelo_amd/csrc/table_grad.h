@@ -14,7 +14,7 @@ struct Vals {
     float v[F];
 };
 
-constexpr int kSegChunk = 32;  // items reduced sequentially per thread
+constexpr int kSegChunk = 17;  // items reduced sequentially per thread (odd: LDS banks)
 
 struct TableReduceWs {
     uint32_t* keys[2];

@@ -66,24 +66,43 @@ __global__ void k_fill_f32(float* __restrict__ p, int64_t n, float v) {
 // wholly inside the chunk are written; the piece continuing a segment from the
 // previous chunk goes to lead[t] (lead_full: the chunk is entirely that piece
 // and the segment goes on); a segment starting here and continuing goes to
-// tail[t] (tail_open).
+// tail[t] (tail_open).  The block first stages its 256 chunks through LDS
+// with coalesced loads (kSegChunk is odd, so per-thread LDS strides spread
+// over the banks).
 template <int F>
-__global__ void k_seg_a(const uint32_t* __restrict__ keys, const Vals<F>* __restrict__ vals,
-                        int64_t m, uint32_t n_entries, float* __restrict__ out,
-                        float* __restrict__ lead, float* __restrict__ tail,
-                        uint8_t* __restrict__ lead_full, uint8_t* __restrict__ tail_open) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+struct SegCfg {
+    static constexpr int threads = F <= 2 ? 256 : 64;
+};
+
+template <int F>
+__global__ void __launch_bounds__(SegCfg<F>::threads) k_seg_a(
+    const uint32_t* __restrict__ keys, const Vals<F>* __restrict__ vals, int64_t m,
+    uint32_t n_entries, float* __restrict__ out, float* __restrict__ lead,
+    float* __restrict__ tail, uint8_t* __restrict__ lead_full, uint8_t* __restrict__ tail_open) {
+    constexpr int kSegThreads = SegCfg<F>::threads;
+    __shared__ uint32_t sk[kSegThreads * kSegChunk + 1];
+    __shared__ float sv[kSegThreads * kSegChunk * F];
+    const int64_t blk0 = (int64_t)blockIdx.x * kSegThreads * kSegChunk;
+    const int64_t blk_n = m - blk0 < (int64_t)kSegThreads * kSegChunk ? m - blk0
+                                                                     : (int64_t)kSegThreads * kSegChunk;
+    for (int q = threadIdx.x; q < blk_n; q += kSegThreads) sk[q] = keys[blk0 + q];
+    const float* vf = reinterpret_cast<const float*>(vals) + blk0 * F;
+    for (int q = threadIdx.x; q < blk_n * F; q += kSegThreads) sv[q] = vf[q];
+    __syncthreads();
+    const int64_t t = (int64_t)blockIdx.x * kSegThreads + threadIdx.x;
     const int64_t nc = (m + kSegChunk - 1) / kSegChunk;
     if (t >= nc) return;
+    const int lb = threadIdx.x * kSegChunk;                 // local begin
     const int64_t b = t * kSegChunk;
     const int64_t e = b + kSegChunk < m ? b + kSegChunk : m;
-    bool is_lead = b > 0 && keys[b] == keys[b - 1];
-    uint32_t cur = keys[b];
+    const int le = lb + (int)(e - b);
+    bool is_lead = b > 0 && sk[lb] == (lb > 0 ? sk[lb - 1] : keys[b - 1]);
+    uint32_t cur = sk[lb];
     float acc[F];
 #pragma unroll
     for (int f = 0; f < F; ++f) acc[f] = 0.0f;
-    for (int64_t j = b; j < e; ++j) {
-        const uint32_t k = keys[j];
+    for (int j = lb; j < le; ++j) {
+        const uint32_t k = sk[j];
         if (k != cur) {
             if (is_lead) {
 #pragma unroll
@@ -97,9 +116,8 @@ __global__ void k_seg_a(const uint32_t* __restrict__ keys, const Vals<F>* __rest
 #pragma unroll
             for (int f = 0; f < F; ++f) acc[f] = 0.0f;
         }
-        const Vals<F> v = vals[j];
 #pragma unroll
-        for (int f = 0; f < F; ++f) acc[f] += v.v[f];
+        for (int f = 0; f < F; ++f) acc[f] += sv[j * F + f];
     }
     const bool cont_out = e < m && keys[e] == cur;
     uint8_t lf = 0, to = 0;
@@ -172,8 +190,9 @@ static int run_f(TableReduceWs& w, uint32_t n_entries, float* dtable, hipStream_
     const int gb = (int)ceil_div(nc, 256);
     {
         KernelTimer kt_("seg_reduce", s);
-        hipLaunchKernelGGL(k_seg_a<F>, dim3(gb), dim3(256), 0, s, ks, vs, w.m, n_entries, dtable,
-                           w.lead, w.tail, w.lead_full, w.tail_open);
+        constexpr int T = SegCfg<F>::threads;
+        hipLaunchKernelGGL(k_seg_a<F>, dim3((unsigned)ceil_div(nc, T)), dim3(T), 0, s, ks, vs, w.m,
+                           n_entries, dtable, w.lead, w.tail, w.lead_full, w.tail_open);
     }
     AM_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_seg_b<F>, dim3(gb), dim3(256), 0, s, ks, w.m, n_entries, dtable, w.lead,

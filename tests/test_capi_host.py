@@ -31,6 +31,41 @@ def test_library_exports_every_declared_symbol():
     assert set(declared) <= set(nat.exported_symbols())
 
 
+def _declared_arities():
+    """name -> (parameter count, list of 'ptr'|'int64'|'int32'|'float'|'size')
+    parsed from the prototypes in the header."""
+    src = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    out = {}
+    for m in re.finditer(r"\b(am_[a-z0-9_]+)\s*\(([^)]*)\)\s*;", src):
+        params = [p.strip() for p in m.group(2).split(",") if p.strip() and p.strip() != "void"]
+        kinds = []
+        for p in params:
+            if "*" in p:
+                kinds.append("ptr")
+            elif p.startswith("int64_t"):
+                kinds.append("int64")
+            elif p.startswith("size_t"):
+                kinds.append("size")
+            elif p.startswith("float"):
+                kinds.append("float")
+            else:
+                kinds.append("int32")
+        out[m.group(1)] = kinds
+    return out
+
+
+def test_ctypes_signatures_match_header_prototypes():
+    from angelo_amd import _native as nat
+    decl = _declared_arities()
+    assert len(decl) >= 25
+    kind_of = {ctypes.c_void_p: "ptr", ctypes.c_char_p: "ptr", ctypes.c_int64: "int64",
+               ctypes.c_size_t: "size", ctypes.c_float: "float", ctypes.c_int32: "int32"}
+    for name, kinds in decl.items():
+        _, args = nat._SIGS[name]
+        got = [kind_of[a] for a in args]
+        assert got == kinds, (name, got, kinds)
+
+
 def test_grid_init_matches_oracle_level_table():
     from angelo_amd import descs
     from oracle import hashgrid as ohg
