@@ -25,7 +25,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 FP32_MFMA_PEAK_TFLOPS = 157.3
-TIMED_KERNELS = ["geo_fwd", "geo_bwd", "pair_compact", "radix_sort", "seg_reduce", "rad_fwd",
+TIMED_KERNELS = ["geo_fwd", "geo_bwd", "pair_count", "radix_sort", "seg_reduce", "rad_fwd",
                  "rad_bwd", "composite_fwd", "composite_bwd", "march_count", "march_fill",
                  "occ_sdf"]
 
@@ -34,20 +34,30 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(kernel, spec, n_samples, n_rays):
-    """Algorithmic HBM bytes of one launch.  Per encode evaluation of L levels,
-    F features, fp32 table: 12 B position + 8*L*F*4 B corner reads (the fused
-    kernels keep the encoding on chip, so the L*F output term is dropped);
-    13 evaluations per sample (base + 6 FD + 6 curvature)."""
+def algorithmic_bytes(kernel, spec, n_samples, n_rays, n_pairs):
+    """Algorithmic HBM bytes of one launch (per-step averages).
+
+    geo_fwd / geo_bwd: per encode evaluation of L levels, F=2 features, fp32
+      table: 12 B position + 8*L*F*4 B corner reads (the fused kernels keep
+      the encoding on chip, so the L*F output term is dropped); 13
+      evaluations per sample (base + 6 FD + 6 curvature) + per-sample I/O:
+      forward writes sdf, grad, normal, feature (19), laplace and the 12
+      stored FD values (39 floats); backward reads the same upstream
+      gradients + FD values and writes 12 B per (row, value) pair.
+    radix_sort: each pair (4 B row + 8 B value) read and written once.
+    seg_reduce: each pair read once + one 8 B row written per pair (bound).
+    composite: 40 B per sample + 32 B per ray.
+    """
     L, F = spec["levels"], 2
     per_eval = 12 + 8 * L * F * 4
-    if kernel in ("geo_fwd", "geo_bwd"):
-        # + per-sample outputs (fwd) or upstream gradients (bwd): sdf, grad,
-        # normal, feature (19), laplace = 27 floats
-        return n_samples * (13 * per_eval + 27 * 4)
+    if kernel == "geo_fwd":
+        return n_samples * (13 * per_eval + 39 * 4)
+    if kernel == "geo_bwd":
+        return n_samples * (13 * per_eval + 39 * 4) + n_pairs * 12
     if kernel == "radix_sort":
-        # pairs (4 B key + 8 B value) read and written once
-        return n_samples * 13 * L * 8 * 12 * 2
+        return n_pairs * 12 * 2
+    if kernel == "seg_reduce":
+        return n_pairs * (12 + 8)
     if kernel in ("composite_fwd", "composite_bwd"):
         return n_samples * 40 + n_rays * 32
     return None
@@ -161,6 +171,7 @@ def main():
         ms, cnt = nat.timer_read(k)
         if cnt:
             kern[k] = {"total_ms": ms, "launches": cnt, "avg_ms": ms / cnt}
+    pairs_per_step = nat.timer_units("table_pairs") / args.steps
     nat.timer_enable(False)
     ms_per_step = gpu_ms / args.steps
     samples_per_step = total_samples / args.steps
@@ -170,16 +181,23 @@ def main():
     for k, v in sorted(kern.items(), key=lambda kv: -kv[1]["total_ms"]):
         log(f"[bench]   {k:14s} {v['avg_ms']:.3f} ms x {v['launches']}  "
             f"({100 * v['total_ms'] / gpu_ms:.1f}% of step time)")
-    # roofline of the dominant kernel
-    roof = None
+    # roofline of the dominant kernel, and of every kernel with a byte model
+    # (launch averages: one launch per step for each of these kernels)
+    roof, per_kernel = None, {}
+    for k, v in kern.items():
+        nbytes = algorithmic_bytes(k, spec, samples_per_step, rays, pairs_per_step)
+        if nbytes is None:
+            continue
+        ach = nbytes / (v["avg_ms"] / 1e3) / 1e9
+        per_kernel[k] = {"achieved": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4),
+                         "bytes": int(nbytes), "avg_ms": round(v["avg_ms"], 4)}
     if kern:
         dom = max(kern, key=lambda k: kern[k]["total_ms"])
-        nbytes = algorithmic_bytes(dom, spec, samples_per_step, rays)
-        if nbytes is not None:
-            ach = nbytes / (kern[dom]["avg_ms"] / 1e3) / 1e9
-            roof = {"kernel": dom, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
-                    "algorithmic_bytes": int(nbytes), "avg_ms": round(kern[dom]["avg_ms"], 4)}
+        if dom in per_kernel:
+            pk = per_kernel[dom]
+            roof = {"kernel": dom, "bound": "hbm", "achieved": pk["achieved"], "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": pk["frac"], "traffic": None,
+                    "algorithmic_bytes": pk["bytes"], "avg_ms": pk["avg_ms"]}
     cpu = None
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline (oracle) ...")
@@ -199,6 +217,8 @@ def main():
                    "optimizer": "AdamW (in the timed step)"},
         "rays_per_s": round(rays * args.steps / (gpu_ms / 1e3), 1),
         "kernels_ms": {k: round(v["avg_ms"], 4) for k, v in kern.items()},
+        "kernels_roofline": per_kernel,
+        "table_pairs_per_step": round(pairs_per_step, 1),
         "roofline": roof,
         "cpu_baseline": cpu,
     }

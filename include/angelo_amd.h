@@ -33,6 +33,7 @@ extern "C" {
 #define AM_ERR_HIP -2         /* HIP runtime error */
 #define AM_ERR_WORKSPACE -3   /* workspace too small */
 #define AM_ERR_UNSUPPORTED -4 /* configuration not implemented */
+#define AM_ERR_INTERNAL -5    /* internal consistency check failed */
 
 #define AM_MAX_LEVELS 32
 
@@ -46,6 +47,9 @@ int am_version(void);
  * meant for benchmarks.  Enabling resets all counters. */
 int am_timer_enable(int32_t on);
 int am_timer_read(const char* name, double* total_ms, int64_t* count);
+/* Work units recorded under `name` since am_timer_enable (e.g.
+ * "table_pairs": the (row, value) pairs the table-gradient sorts received). */
+int am_timer_units(const char* name, int64_t* units);
 
 /* ------------------------------------------------------------------------ */
 /* Multiresolution hash grid (tcnn HashGrid through ProgressiveBandHashGrid,  */

@@ -71,6 +71,7 @@ _SIGS = {
     "am_version": (ctypes.c_int, []),
     "am_timer_enable": (ctypes.c_int, [_I32]),
     "am_timer_read": (ctypes.c_int, [ctypes.c_char_p, _P, _P]),
+    "am_timer_units": (ctypes.c_int, [ctypes.c_char_p, _P]),
     "am_grid_init": (ctypes.c_int, [_P, _I32, _I32, _I32, _I32, _F, _P]),
     "am_hashgrid_fwd": (ctypes.c_int, [_P, _P, _I64, _P, _I32, _P, _P]),
     "am_hashgrid_bwd_workspace_size": (_SZ, [_P, _I64, _I32]),
@@ -173,3 +174,10 @@ def timer_read(name):
     cnt = ctypes.c_int64(0)
     check(load().am_timer_read(name.encode(), ctypes.byref(ms), ctypes.byref(cnt)), "am_timer_read")
     return ms.value, cnt.value
+
+
+def timer_units(name):
+    """Work units (e.g. "table_pairs") recorded since timer_enable."""
+    u = ctypes.c_int64(0)
+    check(load().am_timer_units(name.encode(), ctypes.byref(u)), "am_timer_units")
+    return u.value

@@ -45,6 +45,8 @@ static inline hipStream_t to_stream(void* s) { return reinterpret_cast<hipStream
 // event pair around the launches it encloses when timing is on.
 bool timer_on();
 void timer_push(const char* name, hipEvent_t a, hipEvent_t b);
+// Adds work units (e.g. table-gradient pairs) under `name` while timers are on.
+void timer_units(const char* name, int64_t units);
 hipEvent_t timer_event();
 // While set, timed launches are booked under this name instead of their own
 // (an entry point reusing another's kernels, e.g. the occupancy update).

@@ -14,6 +14,11 @@ struct TimedRecord {
 };
 static bool g_timer_on = false;
 static std::vector<TimedRecord> g_records;
+struct UnitRecord {
+    std::string name;
+    int64_t units;
+};
+static std::vector<UnitRecord> g_units;
 static std::vector<hipEvent_t> g_event_pool;
 
 bool timer_on() { return g_timer_on; }
@@ -37,7 +42,18 @@ void timer_push(const char* name, hipEvent_t a, hipEvent_t b) {
     g_records.push_back(TimedRecord{name, a, b});
 }
 
+void timer_units(const char* name, int64_t units) {
+    if (!g_timer_on) return;
+    for (auto& u : g_units)
+        if (u.name == name) {
+            u.units += units;
+            return;
+        }
+    g_units.push_back(UnitRecord{name, units});
+}
+
 static void timer_reset() {
+    g_units.clear();
     for (auto& r : g_records) {
         g_event_pool.push_back(r.a);
         g_event_pool.push_back(r.b);
@@ -120,5 +136,13 @@ extern "C" int am_timer_read(const char* name, double* total_ms, int64_t* count)
     }
     *total_ms = tot;
     *count = c;
+    return AM_OK;
+}
+
+extern "C" int am_timer_units(const char* name, int64_t* units) {
+    AM_CHECK_ARG(name && units, "NULL argument");
+    *units = 0;
+    for (auto& u : am::g_units)
+        if (u.name == name) *units = u.units;
     return AM_OK;
 }

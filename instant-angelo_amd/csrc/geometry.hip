@@ -19,9 +19,11 @@
 //   2. the last run of each lane is merged with equal-cell runs of the
 //      neighbouring samples (lanes) of the same lane group, a segmented
 //      reduction in a fixed shuffle tree;
-//   3. the surviving (row, value) pairs of a tile are written to its region
-//      in a fixed order, regions are compacted by an integer prefix sum, and
-//      a stable radix sort + fixed-order segmented reduce gives dtable.
+//   3. a count pass makes the same merge decisions from positions alone;
+//      an integer prefix sum of the per-tile counts gives each tile its
+//      first pair, the backward writes the surviving (row, value) pairs
+//      there in a fixed order, and a stable radix sort + fixed-order
+//      segmented reduce gives dtable.
 // MLP weight gradients are per-wave register accumulators summed in a fixed
 // order.
 #include "hashgrid_dev.h"
@@ -412,10 +414,14 @@ __device__ __forceinline__ void geo_eval_bwd(const GeoArgs& a, const float* sm,
 }
 
 // ---- pair emission with merging ----
+// A tile's pairs go to [keys, keys + limit): limit is the count the count
+// pass found for the tile; the backward emits the same sequence, and never
+// writes past limit.
 struct EmitTile {
-    uint32_t* keys;   // tile region (keys)
-    Vals<2>* vals;    // tile region (values)
+    uint32_t* keys;   // tile's first pair (keys)
+    Vals<2>* vals;    // tile's first pair (values)
     int cursor;       // wave-uniform write position
+    int limit;        // pairs reserved for the tile
 };
 
 __device__ __forceinline__ uint64_t lanemask_lt() {
@@ -424,12 +430,14 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
 }
 
 // Lanes with `on` write 8 pairs (corner rows of `cell` at level `lp`,
-// values acc) at consecutive positions in lane order.
+// values acc) at consecutive positions in lane order; COUNT only advances
+// the cursor.
+template <bool COUNT>
 __device__ __forceinline__ void emit_corners(EmitTile& et, bool on, const LevelParams& lp,
                                              const uint32_t cell[3], const float acc[16]) {
     const uint64_t mask = __ballot(on);
     const int off = et.cursor + 8 * __popcll(mask & lanemask_lt());
-    if (on) {
+    if (!COUNT && on && off + 8 <= et.limit) {
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
             et.keys[off + c] = lp.offset + grid_row(lp, cell[0] + (c & 1), cell[1] + ((c >> 1) & 1),
@@ -461,9 +469,10 @@ __device__ __forceinline__ void cluster_point(const GeoArgs& a, const GeoSample&
 }
 
 // Reduces the stored encode gradients of one cluster of evaluations into
-// merged corner pairs.  genc_base: LDS [eval][LJ][2][64] of this wave;
-// eslot(q) maps the cluster's q-th evaluation to its genc slot.
-template <int LJ, int CL>
+// merged corner pairs.  genc_base: LDS [eval][LJ][2][64] of this wave; the
+// cluster's q-th evaluation is genc slot slot0 + q.  COUNT: positions only
+// (no gradients), the cursor ends at the tile's pair count.
+template <int LJ, int CL, bool COUNT>
 __device__ __forceinline__ void flush_cluster(const GeoArgs& a, const LevelParams* lv,
                                               const GeoSample& s, bool valid, const float* genc_base,
                                               int n_q, int slot0, EmitTile& et) {
@@ -486,7 +495,7 @@ __device__ __forceinline__ void flush_cluster(const GeoArgs& a, const LevelParam
             LevelPos pp = level_pos(lp, x);
             const bool same = have && pp.c[0] == cell[0] && pp.c[1] == cell[1] && pp.c[2] == cell[2];
             // a run ends: emit it (no cross-lane merge for interior runs)
-            emit_corners(et, on && have && !same, lp, cell, acc);
+            emit_corners<COUNT>(et, on && have && !same, lp, cell, acc);
             if (!same) {
 #pragma unroll
                 for (int d = 0; d < 3; ++d) cell[d] = pp.c[d];
@@ -494,14 +503,16 @@ __device__ __forceinline__ void flush_cluster(const GeoArgs& a, const LevelParam
                 for (int c = 0; c < 16; ++c) acc[c] = 0.0f;
                 have = true;
             }
-            const int slot = slot0 + q;
-            const float g0 = genc_base[(slot * LJ + j) * 128 + l];
-            const float g1 = genc_base[(slot * LJ + j) * 128 + 64 + l];
+            if constexpr (!COUNT) {
+                const int slot = slot0 + q;
+                const float g0 = genc_base[(slot * LJ + j) * 128 + l];
+                const float g1 = genc_base[(slot * LJ + j) * 128 + 64 + l];
 #pragma unroll
-            for (int c = 0; c < 8; ++c) {
-                const float w = corner_weight(pp, c);
-                acc[2 * c] = __fmaf_rn(w, g0, acc[2 * c]);
-                acc[2 * c + 1] = __fmaf_rn(w, g1, acc[2 * c + 1]);
+                for (int c = 0; c < 8; ++c) {
+                    const float w = corner_weight(pp, c);
+                    acc[2 * c] = __fmaf_rn(w, g0, acc[2 * c]);
+                    acc[2 * c + 1] = __fmaf_rn(w, g1, acc[2 * c + 1]);
+                }
             }
         }
         // last run: merge equal cells of neighbouring lanes (same lane group),
@@ -511,22 +522,84 @@ __device__ __forceinline__ void flush_cluster(const GeoArgs& a, const LevelParam
         const uint32_t n0 = __shfl_down(key0, 1, 16), n1 = __shfl_down(key1, 1, 16),
                        n2 = __shfl_down(key2, 1, 16);
         bool end = (p == 15) || !(n0 == key0 && n1 == key1 && n2 == key2) || key0 == INV;
+        if constexpr (!COUNT) {
 #pragma unroll
-        for (int d = 1; d < 16; d <<= 1) {
-            float o[16];
+            for (int d = 1; d < 16; d <<= 1) {
+                float o[16];
 #pragma unroll
-            for (int c = 0; c < 16; ++c) o[c] = __shfl_down(acc[c], d, 16);
-            const bool oend = __shfl_down((int)end, d, 16) != 0;
-            if (!end && p + d < 16) {
+                for (int c = 0; c < 16; ++c) o[c] = __shfl_down(acc[c], d, 16);
+                const bool oend = __shfl_down((int)end, d, 16) != 0;
+                if (!end && p + d < 16) {
 #pragma unroll
-                for (int c = 0; c < 16; ++c) acc[c] += o[c];
-                end = oend;
+                    for (int c = 0; c < 16; ++c) acc[c] += o[c];
+                    end = oend;
+                }
             }
         }
         const uint32_t u0 = __shfl_up(key0, 1, 16), u1 = __shfl_up(key1, 1, 16),
                        u2 = __shfl_up(key2, 1, 16);
         const bool head = on && (p == 0 || !(u0 == key0 && u1 == key1 && u2 == key2));
-        emit_corners(et, head, lp, cell, acc);
+        emit_corners<COUNT>(et, head, lp, cell, acc);
+    }
+}
+
+// Rebuilds the per-sample state (grad, normal, tangent, curvature terms) of
+// the lane's sample from the 12 stored SDF values.
+__device__ __forceinline__ void geo_rebuild(const GeoArgs& a, const float* __restrict__ pts,
+                                            const float* __restrict__ randn,
+                                            const float* __restrict__ fdv, int64_t n, bool valid,
+                                            GeoSample& s) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) s.P[d] = valid ? pts[3 * n + d] : 0.0f;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) s.x0[d] = contract1(s.P[d], a);
+    if (a.with_grad) {
+        float g[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int j = 0; j < 6; ++j) diff_pair(g, j, valid ? fdv[n * 12 + j] : 0.0f);
+        finish_grad(a, g);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) s.grad[d] = g[d];
+        normalize3(s.grad, s.n);
+    }
+    if (a.with_laplace) {
+        geo_tangent(a, randn, n, valid, s);
+        float gs[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int j = 0; j < 6; ++j) diff_pair(gs, j, valid ? fdv[n * 12 + 6 + j] : 0.0f);
+        finish_grad(a, gs);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) s.grad_s[d] = gs[d];
+        normalize3(s.grad_s, s.ns);
+        s.dot = (s.n[0] * s.ns[0] + s.n[1] * s.ns[1]) + s.n[2] * s.ns[2];
+    }
+}
+
+// Count pass: the number of (row, value) pairs each tile of 16 samples
+// emits, from positions alone (the same decisions as the backward).
+template <int LJ>
+__global__ void __launch_bounds__(256) k_geo_count(GeoArgs a, const float* __restrict__ pts,
+                                                   const float* __restrict__ randn, int64_t N,
+                                                   const float* __restrict__ fdv,
+                                                   int64_t* __restrict__ counts) {
+    __shared__ LevelParams lv[AM_MAX_LEVELS];
+#pragma unroll
+    for (int q = 0; q < AM_MAX_LEVELS; ++q)
+        if ((int)threadIdx.x == q && q < a.g.L) lv[q] = a.g.lv[q];
+    __syncthreads();
+    const int l = lane_id(), p = l & 15;
+    const int nw = blockDim.x >> 6, wave = threadIdx.x >> 6;
+    const int64_t n_tiles = (N + 15) / 16;
+    for (int64_t tile = (int64_t)blockIdx.x * nw + wave; tile < n_tiles;
+         tile += (int64_t)gridDim.x * nw) {
+        const int64_t n = tile * 16 + p;
+        const bool valid = n < N;
+        GeoSample s;
+        geo_rebuild(a, pts, randn, fdv, n, valid, s);
+        EmitTile et{nullptr, nullptr, 0, 0};
+        if (a.with_laplace) flush_cluster<LJ, 1, true>(a, lv, s, valid, nullptr, 6, 1, et);
+        flush_cluster<LJ, 0, true>(a, lv, s, valid, nullptr, a.with_grad ? 7 : 1, 0, et);
+        if (l == 0) counts[tile] = et.cursor;
     }
 }
 
@@ -537,8 +610,9 @@ __global__ void __launch_bounds__(256) k_geo_bwd(
     const float* __restrict__ params, const float* __restrict__ fdv,
     const float* __restrict__ g_sdf, const float* __restrict__ g_grad,
     const float* __restrict__ g_normal, const float* __restrict__ g_feature,
-    const float* __restrict__ g_laplace, uint32_t* __restrict__ rkeys,
-    Vals<2>* __restrict__ rvals, int64_t region, int32_t* __restrict__ counts,
+    const float* __restrict__ g_laplace, uint32_t* __restrict__ keys,
+    Vals<2>* __restrict__ vals, const int64_t* __restrict__ offsets,
+    const int64_t* __restrict__ counts, int64_t* __restrict__ status,
     float* __restrict__ partial) {
     extern __shared__ float sm[];
     __shared__ LevelParams lv[AM_MAX_LEVELS];
@@ -570,33 +644,10 @@ __global__ void __launch_bounds__(256) k_geo_bwd(
          tile += (int64_t)gridDim.x * nw) {
         const int64_t n = tile * 16 + p;
         const bool valid = n < N;
-        EmitTile et{rkeys + tile * region, rvals + tile * region, 0};
+        const int64_t off = offsets[tile];
+        EmitTile et{keys + off, vals + off, 0, (int)counts[tile]};
         GeoSample s;
-#pragma unroll
-        for (int d = 0; d < 3; ++d) s.P[d] = valid ? pts[3 * n + d] : 0.0f;
-#pragma unroll
-        for (int d = 0; d < 3; ++d) s.x0[d] = contract1(s.P[d], a);
-        // rebuild grad / normal / tangent / curvature terms from the stored values
-        if (a.with_grad) {
-            float g[3] = {0.0f, 0.0f, 0.0f};
-#pragma unroll
-            for (int j = 0; j < 6; ++j) diff_pair(g, j, valid ? fdv[n * 12 + j] : 0.0f);
-            finish_grad(a, g);
-#pragma unroll
-            for (int d = 0; d < 3; ++d) s.grad[d] = g[d];
-            normalize3(s.grad, s.n);
-        }
-        if (a.with_laplace) {
-            geo_tangent(a, randn, n, valid, s);
-            float gs[3] = {0.0f, 0.0f, 0.0f};
-#pragma unroll
-            for (int j = 0; j < 6; ++j) diff_pair(gs, j, valid ? fdv[n * 12 + 6 + j] : 0.0f);
-            finish_grad(a, gs);
-#pragma unroll
-            for (int d = 0; d < 3; ++d) s.grad_s[d] = gs[d];
-            normalize3(s.grad_s, s.ns);
-            s.dot = (s.n[0] * s.ns[0] + s.n[1] * s.ns[1]) + s.n[2] * s.ns[2];
-        }
+        geo_rebuild(a, pts, randn, fdv, n, valid, s);
         // upstream gradients of this sample
         float gg[3] = {0, 0, 0}, gnrm[3] = {0, 0, 0};
         if (valid && g_grad)
@@ -653,7 +704,7 @@ __global__ void __launch_bounds__(256) k_geo_bwd(
                 for (int d = 0; d < 3; ++d)
                     if (m[d]) g_ps[d] += gpos[d] * a.inv2r;
             }
-            flush_cluster<LJ, 1>(a, lv, s, valid, genc, 6, 1, et);
+            flush_cluster<LJ, 1, false>(a, lv, s, valid, genc, 6, 1, et);
             float g_t[3], gcr[3];
 #pragma unroll
             for (int d = 0; d < 3; ++d) g_t[d] = g_ps[d] * a.eps;
@@ -678,8 +729,10 @@ __global__ void __launch_bounds__(256) k_geo_bwd(
                                             genc + (1 + j) * LJ * 128, nullptr);
             }
         }
-        flush_cluster<LJ, 0>(a, lv, s, valid, genc, a.with_grad ? 7 : 1, 0, et);
-        if (l == 0) counts[tile] = et.cursor;
+        flush_cluster<LJ, 0, false>(a, lv, s, valid, genc, a.with_grad ? 7 : 1, 0, et);
+        // the count pass and this pass emit the same sequence; a mismatch is
+        // reported (never written past: emit_corners stops at the limit)
+        if (l == 0 && et.cursor != et.limit) status[1] = 1;
     }
     // per-wave partial gradient row in parameter layout
     float* out = partial + ((int64_t)blockIdx.x * nw + wave) * a.n_params;
@@ -731,23 +784,14 @@ __global__ void __launch_bounds__(256) k_geo_bwd(
             if (4 * k + r < a.n_out) out[a.pbo + 4 * k + r] = so[r];
 }
 
-// Compaction: tile regions -> contiguous pairs at the tiles' prefix offsets.
-__global__ void k_compact_pairs(const uint32_t* __restrict__ rkeys, const Vals<2>* __restrict__ rvals,
-                                int64_t region, const int32_t* __restrict__ counts,
-                                const int64_t* __restrict__ offsets, uint32_t* __restrict__ keys,
-                                Vals<2>* __restrict__ vals) {
-    const int64_t tile = blockIdx.x;
-    const int c = counts[tile];
-    const int64_t o = offsets[tile];
-    for (int q = threadIdx.x; q < c; q += blockDim.x) {
-        keys[o + q] = rkeys[tile * region + q];
-        vals[o + q] = rvals[tile * region + q];
+// status[0] = number of pairs (offset + count of the last tile); status[1]
+// = 0, set by the backward on a count mismatch.
+__global__ void k_pair_total(const int64_t* __restrict__ offsets, const int64_t* __restrict__ counts,
+                             int64_t tiles, int64_t* __restrict__ status) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        status[0] = offsets[tiles - 1] + counts[tiles - 1];
+        status[1] = 0;
     }
-}
-
-__global__ void k_counts_to_i64(const int32_t* __restrict__ c, int64_t n, int64_t* __restrict__ o) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) o[i] = c[i];
 }
 
 // ------------------------------------------------------------------------
@@ -818,40 +862,53 @@ static int geo_fwd_launch(const GeoArgs& a, const float* pts,
     return AM_OK;
 }
 
-// Upper bound of the pairs one tile can emit (every evaluation, every level
-// slot, every lane unmerged).
-static int64_t geo_region(const GeoArgs& a, int LJ) { return (int64_t)a.n_evals * LJ * 8 * 64; }
+template <int LJ>
+static int geo_count_launch(const GeoArgs& a, const float* pts, const float* randn, int64_t n,
+                            const float* fdv, int64_t* counts, hipStream_t s) {
+    KernelTimer kt_("pair_count", s);
+    hipLaunchKernelGGL((k_geo_count<LJ>), dim3(field_blocks(n)), dim3(256), 0, s, a, pts, randn, n,
+                       fdv, counts);
+    AM_LAUNCH_CHECK();
+    return AM_OK;
+}
 
+// The backward's workspace before the pair arrays: the pair count is only
+// known after the count pass, so the sort buffers are carved afterwards
+// from the rest (sized for the worst case by the size query).
 struct GeoBwdWs {
     float* fdv;
     float* sdf_tmp;
     float* partial;
     float* ctmp;
-    int32_t* counts;
-    int64_t* offsets64;
-    int64_t* counts64;
+    int64_t* counts;
+    int64_t* offsets;
+    int64_t* status;
     void* scan_tmp;
     size_t scan_bytes;
-    TableReduceWs tw;
 };
 
-static int carve_geo_bwd(WsCarver& c, const GeoArgs& a, int LJ, const am_grid_desc* g, int64_t n,
-                         bool need_fdv, GeoBwdWs* w) {
+static void carve_geo_bwd_head(WsCarver& c, const GeoArgs& a, int64_t n, bool need_fdv,
+                               GeoBwdWs* w) {
     const int waves = field_blocks(n) * 4;
     const int64_t tiles = (n + 15) / 16;
     w->fdv = need_fdv ? c.take<float>((size_t)n * 12 + 1) : nullptr;
     w->sdf_tmp = need_fdv ? c.take<float>((size_t)n + 1) : nullptr;
     w->partial = c.take<float>((size_t)waves * a.n_params);
     w->ctmp = c.take<float>((size_t)colsum_tmp_floats(waves, a.n_params) + 1);
-    w->counts = c.take<int32_t>((size_t)tiles + 1);
-    w->counts64 = c.take<int64_t>((size_t)tiles + 1);
-    w->offsets64 = c.take<int64_t>((size_t)tiles + 1);
+    w->counts = c.take<int64_t>((size_t)tiles + 1);
+    w->offsets = c.take<int64_t>((size_t)tiles + 1);
+    w->status = c.take<int64_t>(2);
     size_t sb = 0;
     rocprim::exclusive_scan(nullptr, sb, (const int64_t*)nullptr, (int64_t*)nullptr, (int64_t)0,
                             (size_t)(tiles > 0 ? tiles : 1), rocprim::plus<int64_t>(), 0, false);
     w->scan_bytes = sb;
     w->scan_tmp = c.take<char>(sb + 256);
-    return table_reduce_carve(c, 2, tiles * geo_region(a, LJ), g->n_entries, &w->tw);
+}
+
+// Most pairs a tile can emit: every evaluation, every level slot, every
+// lane unmerged.
+static int64_t geo_max_pairs(const GeoArgs& a, int LJ, int64_t n) {
+    return (n + 15) / 16 * ((int64_t)a.n_evals * LJ * 8 * 64);
 }
 
 template <int LJ, int NH>
@@ -867,7 +924,7 @@ static int geo_bwd_launch(const GeoArgs& a, const am_grid_desc* g, const am_mlp_
     const bool need_fdv = a.with_grad && fdv_in == nullptr;
     WsCarver c(ws, ws_bytes);
     GeoBwdWs w;
-    if (int rc = carve_geo_bwd(c, a, LJ, g, n, need_fdv, &w)) return rc;
+    carve_geo_bwd_head(c, a, n, need_fdv, &w);
     if (!c.ok()) {
         set_error("workspace too small: need %zu, got %zu", c.used, ws_bytes);
         return AM_ERR_WORKSPACE;
@@ -879,11 +936,25 @@ static int geo_bwd_launch(const GeoArgs& a, const am_grid_desc* g, const am_mlp_
             return rc;
         fdv = w.fdv;
     }
-    const int64_t region = geo_region(a, LJ);
-    // tile regions live in the sort's alternate buffers; compacted pairs go to
-    // the primary ones
-    uint32_t* rkeys = w.tw.keys[1];
-    Vals<2>* rvals = reinterpret_cast<Vals<2>*>(w.tw.vals[1]);
+    // 1. pairs per tile, 2. exclusive prefix sum -> each tile's first pair
+    if (int rc = geo_count_launch<LJ>(a, pts, randn, n, fdv, w.counts, s)) return rc;
+    size_t sb = w.scan_bytes;
+    hipError_t e = rocprim::exclusive_scan(w.scan_tmp, sb, w.counts, w.offsets, (int64_t)0,
+                                           (size_t)tiles, rocprim::plus<int64_t>(), s, false);
+    if (e != hipSuccess) {
+        set_error("tile-count scan failed: %s", hipGetErrorString(e));
+        return AM_ERR_HIP;
+    }
+    hipLaunchKernelGGL(k_pair_total, dim3(1), dim3(64), 0, s, w.offsets, w.counts, tiles, w.status);
+    AM_LAUNCH_CHECK();
+    // 3. the pair arrays are carved for the worst case (the size query's
+    // bound); the backward writes each tile's pairs at its offset
+    TableReduceWs tw;
+    if (int rc = table_reduce_carve(c, 2, geo_max_pairs(a, LJ, n), g->n_entries, &tw)) return rc;
+    if (!c.ok()) {
+        set_error("workspace too small: need %zu, got %zu", c.used, ws_bytes);
+        return AM_ERR_WORKSPACE;
+    }
     const size_t lds = (size_t)(a.lds_floats + 4 * 128 * mlp::kScratchLd + 4 * 7 * LJ * 128) *
                        sizeof(float);
     AM_CHECK_ARG(lds <= 160 * 1024, "geometry backward needs %zu bytes of LDS", lds);
@@ -893,36 +964,21 @@ static int geo_bwd_launch(const GeoArgs& a, const am_grid_desc* g, const am_mlp_
         KernelTimer kt_("geo_bwd", s);
         hipLaunchKernelGGL((k_geo_bwd<LJ, NH>), dim3(blocks), dim3(256), lds, s, a, pts, randn, n,
                            table, params, fdv, g_sdf, g_grad, g_normal, g_feature, g_laplace,
-                           rkeys, rvals, region, w.counts, w.partial);
+                           tw.keys[0], reinterpret_cast<Vals<2>*>(tw.vals[0]), w.offsets, w.counts,
+                           w.status, w.partial);
     }
     AM_LAUNCH_CHECK();
     if (int rc = colsum_f32(w.partial, waves, a.n_params, dparams, w.ctmp, s)) return rc;
-    // deterministic compaction: integer prefix sum of the per-tile counts
-    hipLaunchKernelGGL(k_counts_to_i64, dim3((unsigned)ceil_div(tiles, 256)), dim3(256), 0, s,
-                       w.counts, tiles, w.counts64);
-    AM_LAUNCH_CHECK();
-    size_t sb = w.scan_bytes;
-    hipError_t e = rocprim::exclusive_scan(w.scan_tmp, sb, w.counts64, w.offsets64, (int64_t)0,
-                                           (size_t)tiles, rocprim::plus<int64_t>(), s, false);
-    if (e != hipSuccess) {
-        set_error("tile-count scan failed: %s", hipGetErrorString(e));
-        return AM_ERR_HIP;
-    }
-    int64_t last_off = 0;
-    int32_t last_cnt = 0;
-    AM_HIP(hipMemcpyAsync(&last_off, w.offsets64 + tiles - 1, sizeof(int64_t),
-                          hipMemcpyDeviceToHost, s));
-    AM_HIP(hipMemcpyAsync(&last_cnt, w.counts + tiles - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    {
-        KernelTimer kt_("pair_compact", s);
-        hipLaunchKernelGGL(k_compact_pairs, dim3((unsigned)tiles), dim3(256), 0, s, rkeys, rvals,
-                           region, w.counts, w.offsets64, w.tw.keys[0],
-                           reinterpret_cast<Vals<2>*>(w.tw.vals[0]));
-    }
-    AM_LAUNCH_CHECK();
+    int64_t st[2] = {0, 0};
+    AM_HIP(hipMemcpyAsync(st, w.status, sizeof(st), hipMemcpyDeviceToHost, s));
     AM_HIP(hipStreamSynchronize(s));  // one host read: the number of pairs
-    w.tw.m = last_off + last_cnt;
-    return table_reduce_run(w.tw, 2, g->n_entries, dtable, s);
+    if (st[1] != 0) {
+        set_error("table-gradient pair count mismatch between count and backward passes");
+        return AM_ERR_INTERNAL;
+    }
+    tw.m = st[0];
+    timer_units("table_pairs", tw.m);
+    return table_reduce_run(tw, 2, g->n_entries, dtable, s);
 }
 
 #define AM_GEO_DISPATCH(FN, ...)                                                   \
@@ -961,7 +1017,9 @@ extern "C" size_t am_sdf_bwd_workspace_size(const am_grid_desc* g, const am_mlp_
     if (make_geo_args(g, m, sd, &a, &LJ) || n < 0) return 0;
     WsCarver c(nullptr, 0);
     GeoBwdWs w;
-    if (carve_geo_bwd(c, a, LJ, g, n, true, &w)) return 0;
+    carve_geo_bwd_head(c, a, n, true, &w);
+    TableReduceWs tw;
+    if (table_reduce_carve(c, 2, geo_max_pairs(a, LJ, n), g->n_entries, &tw)) return 0;
     return c.used + 256;
 }
 

@@ -12,6 +12,8 @@ CONFIGS = {
     # name: (L, F, log2T, base, scale)
     "c0": (8, 2, 15, 16, 1.5),
     "c1_small": (16, 2, 16, 32, 1.3195079107728942),
+    # full Neuralangelo table: 24.7 M rows, 25-bit sort keys
+    "c1_full": (16, 2, 21, 32, 1.3195079107728942),
 }
 
 
