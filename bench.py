@@ -11,7 +11,8 @@ before timing; the timed region is bracketed by device synchronisation.
 Prints one JSON line (metric, value, ms_per_step, roofline of the dominant
 kernel from live HIP-event timers, and the CPU oracle baseline).
 
-    python bench.py [--steps K] [--warmup W] [--workload neuralangelo-fp32|sphere-fp32]
+    python bench.py [--steps K] [--warmup W]
+                    [--workload neuralangelo-fp32|neuralangelo-mixed|sphere-fp32]
 """
 import argparse
 import json
@@ -34,30 +35,34 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(kernel, spec, n_samples, n_rays, n_pairs):
+def algorithmic_bytes(kernel, spec, n_samples, n_rays, n_pairs, mixed=False):
     """Algorithmic HBM bytes of one launch (per-step averages).
 
-    geo_fwd / geo_bwd: per encode evaluation of L levels, F=2 features, fp32
-      table: 12 B position + 8*L*F*4 B corner reads (the fused kernels keep
+    geo_fwd / geo_bwd: per encode evaluation of L levels, F=2 features, b_t B
+      table entries (4 fp32, 2 in the mixed form's fp16 rows): 12 B position
+      + 8*L*F*b_t B corner reads (the fused kernels keep
       the encoding on chip, so the L*F output term is dropped); 13
       evaluations per sample (base + 6 FD + 6 curvature) + per-sample I/O:
       forward writes sdf, grad, normal, feature (19), laplace and the 12
       stored FD values (39 floats); backward reads the same upstream
       gradients + FD values and writes 12 B per (row, value) pair.
-    radix_sort: each pair (4 B row + 8 B value) read and written once.
+    radix_sort: each pair (4 B row + 8 B value; 4 B value in the mixed form)
+      read and written once.
     seg_reduce: each pair read once + one 8 B row written per pair (bound).
     composite: 40 B per sample + 32 B per ray.
     """
     L, F = spec["levels"], 2
-    per_eval = 12 + 8 * L * F * 4
+    bt = 2 if mixed else 4
+    pair = 8 if mixed else 12
+    per_eval = 12 + 8 * L * F * bt
     if kernel == "geo_fwd":
         return n_samples * (13 * per_eval + 39 * 4)
     if kernel == "geo_bwd":
-        return n_samples * (13 * per_eval + 39 * 4) + n_pairs * 12
+        return n_samples * (13 * per_eval + 39 * 4) + n_pairs * pair
     if kernel == "radix_sort":
-        return n_pairs * 12 * 2
+        return n_pairs * pair * 2
     if kernel == "seg_reduce":
-        return n_pairs * (12 + 8)
+        return n_pairs * (pair + 8)
     if kernel in ("composite_fwd", "composite_bwd"):
         return n_samples * 40 + n_rays * 32
     return None
@@ -124,6 +129,36 @@ def cpu_baseline(spec_name, workload, seconds=15.0):
                       f"torch CPU fp32, median of {len(times)} steps ({med:.2f} s/step)"}
 
 
+def serving_bench(wl, n_requests, rate_hz, seed=5):
+    """Continuous-batching serving of a Poisson mix of renders (256^2, 512^2,
+    1024^2 from seeded poses) and SDF lattices (128^3, 256^3, 512^3 in 64^3
+    chunks) on the workload's field, resident on the device, in the fp32 and
+    the mixed-precision form.  Returns p50/p99 latency and throughput."""
+    import torch
+    from angelo_amd import serving
+    model = wl.model
+    model.eval()
+    out = {}
+    for prec in ("fp32", "mixed"):
+        model.set_precision(prec)
+        srv = serving.FieldServer(model, rays_per_step=1 << 17, sdf_chunks_per_step=8)
+        warm = serving.poisson_requests(4, 1000.0, seed=seed + 100)
+        srv.run(warm, realtime=False)       # warm-up: workspaces, ray-direction tables
+        srv.rays_rendered = srv.points_queried = 0
+        reqs = serving.poisson_requests(n_requests, rate_hz, seed=seed)
+        wall = srv.run(reqs, realtime=True)
+        st = serving.latency_stats(reqs)
+        out[prec] = {"p50_ms": round(st["p50_s"] * 1e3, 2), "p99_ms": round(st["p99_s"] * 1e3, 2),
+                     "mean_ms": round(st["mean_s"] * 1e3, 2), "requests": st["n"],
+                     "rays_per_s": round(srv.rays_rendered / wall, 1),
+                     "sdf_points_per_s": round(srv.points_queried / wall, 1),
+                     "wall_s": round(wall, 3), "arrival_rate_hz": rate_hz,
+                     "graph_replay": False}
+        torch.cuda.synchronize()
+    model.set_precision("fp32")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -132,6 +167,9 @@ def main():
     ap.add_argument("--workload", default="neuralangelo-fp32")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--rays", type=int, default=None)
+    ap.add_argument("--serve-requests", type=int, default=16,
+                    help="requests in the serving measurement (0: skip it)")
+    ap.add_argument("--serve-rate", type=float, default=4.0, help="Poisson arrival rate (1/s)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
@@ -185,7 +223,7 @@ def main():
     # (launch averages: one launch per step for each of these kernels)
     roof, per_kernel = None, {}
     for k, v in kern.items():
-        nbytes = algorithmic_bytes(k, spec, samples_per_step, rays, pairs_per_step)
+        nbytes = algorithmic_bytes(k, spec, samples_per_step, rays, pairs_per_step, wl.mixed)
         if nbytes is None:
             continue
         ach = nbytes / (v["avg_ms"] / 1e3) / 1e9
@@ -198,6 +236,11 @@ def main():
             roof = {"kernel": dom, "bound": "hbm", "achieved": pk["achieved"], "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": pk["frac"], "traffic": None,
                     "algorithmic_bytes": pk["bytes"], "avg_ms": pk["avg_ms"]}
+    serve = None
+    if args.serve_requests > 0:
+        log(f"[bench] serving: {args.serve_requests} requests at {args.serve_rate}/s ...")
+        serve = serving_bench(wl, args.serve_requests, args.serve_rate)
+        log(f"[bench] serving: {json.dumps(serve)}")
     cpu = None
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline (oracle) ...")
@@ -209,17 +252,21 @@ def main():
         "metric": "train samples/s (fwd+bwd)", "value": round(value, 1), "unit": "samples/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(ms_per_step, 4), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "fp32",
+        "vs_baseline": None,
+        "dtype": "fp16 table / bf16 MFMA, fp32 accumulate" if wl.mixed else "fp32",
         "data": "synthetic (analytic scene, seeded cameras)",
         "config": {"workload": args.workload, "rays_per_step": rays,
                    "samples_per_step": round(samples_per_step, 1), "levels": spec["levels"],
                    "log2_hashmap_size": spec["log2T"], "evals_per_sample": 13,
                    "optimizer": "AdamW (in the timed step)"},
         "rays_per_s": round(rays * args.steps / (gpu_ms / 1e3), 1),
+        "loss_scaling": ({"final_scale": wl.scaler.scale, "skipped_steps": wl.skipped_steps}
+                         if wl.mixed else None),
         "kernels_ms": {k: round(v["avg_ms"], 4) for k, v in kern.items()},
         "kernels_roofline": per_kernel,
         "table_pairs_per_step": round(pairs_per_step, 1),
         "roofline": roof,
+        "serving": serve,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -104,6 +104,9 @@ int am_hashgrid_bwd(const am_grid_desc* g, const float* x, int64_t n, const floa
 #define AM_ACT_SOFTPLUS 2 /* Softplus(beta), threshold 20 (torch default) */
 #define AM_ACT_SIGMOID 3
 
+#define AM_PREC_FP32 0
+#define AM_PREC_MIXED 1
+
 typedef struct am_mlp_desc {
     int32_t n_in;            /* <= 64 */
     int32_t n_out;           /* <= 64 */
@@ -112,6 +115,10 @@ typedef struct am_mlp_desc {
     int32_t hidden_act;      /* AM_ACT_RELU or AM_ACT_SOFTPLUS */
     int32_t output_act;      /* AM_ACT_NONE / RELU / SIGMOID */
     float softplus_beta;
+    int32_t precision;       /* AM_PREC_FP32, or AM_PREC_MIXED: matmuls on bf16
+                                MFMA with fp32 accumulation (the fused radiance
+                                kernels; the SDF MLP and the stand-alone MLP
+                                take fp32 only) */
 } am_mlp_desc;
 
 int64_t am_mlp_n_params(const am_mlp_desc* m);
@@ -171,29 +178,45 @@ typedef struct am_sdf_desc {
     int32_t recontract;   /* 1: shifted point re-contracted as the reference does
                              (models/geometry.py:246,264-265); 0: shifted in world space */
     int32_t active_levels;
+    int32_t precision;    /* AM_PREC_FP32, or AM_PREC_MIXED: table rows stored
+                             in fp16 ([n_entries,2] __half) and table-gradient
+                             pair values in fp16 (expects loss-scaled upstream
+                             gradients); blends, sums and dtable stay fp32.
+                             The SDF MLP descriptor must be AM_PREC_FP32 in
+                             both forms: the central differences of the FD
+                             normal divide each evaluation's rounding by eps */
 } am_sdf_desc;
 
 /* Outputs (each nullable except sdf): sdf[n], grad[n,3], normal[n,3] =
  * F.normalize(grad), feature[n, n_out+3] = cat(mlp_out, x01*2-1),
  * laplace[n], fd_values[n,12] (the 6 central-difference and 6 curvature SDF
- * values, kept for the backward).  randn[n,3]: the curvature directions
- * (torch.randn_like). */
+ * values, kept for the backward), enc_store[am_sdf_store_floats(n)] (the
+ * encodings of every evaluation and the position derivatives of the
+ * curvature evaluations, kept so the backward does not gather the table
+ * again).  randn[n,3]: the curvature directions (torch.randn_like). */
 int am_sdf_fwd(const am_grid_desc* g, const am_mlp_desc* m, const am_sdf_desc* s,
-               const float* pts, const float* randn, int64_t n, const float* table,
+               const float* pts, const float* randn, int64_t n, const void* table,
                const float* params, float* sdf, float* grad, float* normal, float* feature,
-               float* laplace, float* fd_values, void* stream);
+               float* laplace, float* fd_values, float* enc_store, void* stream);
+/* Floats of enc_store for n samples (-1 on invalid descriptors). */
+int64_t am_sdf_store_floats(const am_grid_desc* g, const am_mlp_desc* m, const am_sdf_desc* s,
+                            int64_t n);
 size_t am_sdf_bwd_workspace_size(const am_grid_desc* g, const am_mlp_desc* m,
                                  const am_sdf_desc* s, int64_t n);
 /* Upstream gradients are nullable (= 0).  fd_values: from the forward
- * (nullable: recomputed).  dtable [n_entries,F] and dparams are fully
+ * (nullable: recomputed).  enc_store: from the forward (nullable: the
+ * backward re-gathers the table, the memory-lean "recompute" form; needs
+ * fd_values when given).  dtable [n_entries,F] and dparams are fully
  * overwritten; both reductions are deterministic.  The call reads the
- * number of table-gradient pairs back to the host once (a stream sync). */
+ * number of table-gradient pairs back to the host once (a stream sync).
+ * Replaces the autograd backward of VolumeSDF.forward
+ * (models/geometry.py:195-287) through tcnn's encoding backward. */
 int am_sdf_bwd(const am_grid_desc* g, const am_mlp_desc* m, const am_sdf_desc* s,
-               const float* pts, const float* randn, int64_t n, const float* table,
-               const float* params, const float* fd_values, const float* g_sdf,
-               const float* g_grad, const float* g_normal, const float* g_feature,
-               const float* g_laplace, float* dtable, float* dparams, void* workspace,
-               size_t workspace_bytes, void* stream);
+               const float* pts, const float* randn, int64_t n, const void* table,
+               const float* params, const float* fd_values, const float* enc_store,
+               const float* g_sdf, const float* g_grad, const float* g_normal,
+               const float* g_feature, const float* g_laplace, float* dtable, float* dparams,
+               void* workspace, size_t workspace_bytes, void* stream);
 
 /* ------------------------------------------------------------------------ */
 /* Fused VolumeRadiance.forward (models/texture.py:23-30): SH(deg) of        */
@@ -253,7 +276,7 @@ int am_composite_bwd(const am_composite_desc* c, const float* sdf, const float* 
 /* order) [m] int32; jitter [m,3] in [0,1).                                   */
 size_t am_occ_update_workspace_size(int32_t grid_res, int64_t m);
 int am_occ_update(const am_grid_desc* g, const am_mlp_desc* mlp, const am_sdf_desc* s,
-                  const float* table, const float* params, const float* inv_s,
+                  const void* table, const float* params, const float* inv_s,
                   float step, const int32_t* cells, const float* jitter, int64_t m,
                   int32_t grid_res, float ema_decay, float occ_thre, float* occs,
                   uint8_t* binary, void* workspace, size_t workspace_bytes, void* stream);

@@ -14,6 +14,7 @@ _lib = None
 
 AM_MAX_LEVELS = 32
 ACT = {"none": 0, "relu": 1, "softplus": 2, "sigmoid": 3}
+PREC = {"fp32": 0, "mixed": 1}
 
 
 class GridDesc(ctypes.Structure):
@@ -41,6 +42,7 @@ class MlpDesc(ctypes.Structure):
         ("hidden_act", ctypes.c_int32),
         ("output_act", ctypes.c_int32),
         ("softplus_beta", ctypes.c_float),
+        ("precision", ctypes.c_int32),
     ]
 
 
@@ -52,6 +54,7 @@ class SdfDesc(ctypes.Structure):
         ("with_laplace", ctypes.c_int32),
         ("recontract", ctypes.c_int32),
         ("active_levels", ctypes.c_int32),
+        ("precision", ctypes.c_int32),
     ]
 
 
@@ -86,9 +89,10 @@ _SIGS = {
     "am_march_count": (ctypes.c_int, [_P, _I64, _P, _P, _P, _P, _P, _I32, _F, _I32, _P, _P, _P, _SZ, _P]),
     "am_march_fill": (ctypes.c_int, [_P, _I64, _P, _P, _P, _P, _P, _I32, _F, _I32, _P, _P, _P, _P, _P]),
     "am_sample_points": (ctypes.c_int, [_P, _P, _P, _P, _I64, _P, _P, _P, _P]),
-    "am_sdf_fwd": (ctypes.c_int, [_P] * 5 + [_I64] + [_P] * 8 + [_P]),
+    "am_sdf_fwd": (ctypes.c_int, [_P] * 5 + [_I64] + [_P] * 9 + [_P]),
+    "am_sdf_store_floats": (_I64, [_P, _P, _P, _I64]),
     "am_sdf_bwd_workspace_size": (_SZ, [_P, _P, _P, _I64]),
-    "am_sdf_bwd": (ctypes.c_int, [_P] * 5 + [_I64] + [_P] * 10 + [_P, _SZ, _P]),
+    "am_sdf_bwd": (ctypes.c_int, [_P] * 5 + [_I64] + [_P] * 11 + [_P, _SZ, _P]),
     "am_radiance_fwd": (ctypes.c_int, [_P, _I32, _P, _I32, _P, _P, _P, _I64, _P, _P, _P]),
     "am_radiance_bwd_workspace_size": (_SZ, [_P, _I64]),
     "am_radiance_bwd": (ctypes.c_int, [_P, _I32, _P, _I32, _P, _P, _P, _I64, _P, _P, _P, _P, _P, _P, _SZ, _P]),

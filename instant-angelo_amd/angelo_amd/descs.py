@@ -29,8 +29,9 @@ def grid_arrays(g):
 
 
 def mlp_desc(n_in, n_out, n_hidden_layers, n_neurons=64, hidden_act="relu", output_act="none",
-             softplus_beta=100.0):
+             softplus_beta=100.0, precision="fp32"):
     m = nat.MlpDesc()
+    m.precision = nat.PREC[precision]
     m.n_in, m.n_out, m.n_hidden_layers, m.n_neurons = int(n_in), int(n_out), int(n_hidden_layers), int(n_neurons)
     m.hidden_act = nat.ACT[hidden_act]
     m.output_act = nat.ACT[output_act]

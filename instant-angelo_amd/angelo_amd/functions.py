@@ -101,14 +101,47 @@ def sh_encode(dirs01, degree):
     return out
 
 
+# Backward form of the fused SDF field.  "store": the forward keeps every
+# evaluation's encoding (and the curvature evaluations' position
+# derivatives), so the backward reads them back in coalesced rows instead of
+# gathering the table again; "recompute": the backward re-gathers (no extra
+# memory); "auto": store while the store fits store_limit_bytes.
+SDF_BACKWARD = {"mode": "auto", "store_limit_bytes": 24 << 30}
+
+
+def set_sdf_backward(mode="auto", store_limit_bytes=None):
+    if mode not in ("auto", "store", "recompute"):
+        raise ValueError(f"unknown SDF backward form {mode!r}")
+    SDF_BACKWARD["mode"] = mode
+    if store_limit_bytes is not None:
+        SDF_BACKWARD["store_limit_bytes"] = int(store_limit_bytes)
+
+
+def _sdf_store_floats(lib, grid, mdesc, sdesc, n):
+    mode = SDF_BACKWARD["mode"]
+    if mode == "recompute":
+        return 0
+    floats = int(lib.am_sdf_store_floats(nat.sptr(grid), nat.sptr(mdesc), nat.sptr(sdesc), n))
+    if floats < 0:
+        nat.check(-1, "am_sdf_store_floats")
+    if mode == "auto" and floats * 4 > SDF_BACKWARD["store_limit_bytes"]:
+        return 0
+    return floats
+
+
 class SDFFieldFn(torch.autograd.Function):
     """Fused VolumeSDF.forward (models/geometry.py:195-287): outputs sdf [n],
     grad [n,3], normal [n,3] = F.normalize(grad), feature [n, n_out+3],
     laplace [n] (grad/normal/laplace empty when not requested)."""
 
     @staticmethod
-    def forward(ctx, pts, randn, table, params, grid, mdesc, sdesc):
-        nat.require_cuda(pts, table, params)
+    def forward(ctx, pts, randn, table, params, grid, mdesc, sdesc, table_k=None):
+        """table: the fp32 master (its gradient is returned); table_k: the
+        rows the kernels read (the master, or its fp16 copy in the mixed
+        form)."""
+        if table_k is None:
+            table_k = table
+        nat.require_cuda(pts, table, params, table_k)
         lib = nat.load()
         pts = _f32c(pts)
         n = pts.shape[0]
@@ -122,21 +155,24 @@ class SDFFieldFn(torch.autograd.Function):
         rnd = _f32c(randn) if (sdesc.with_laplace and randn is not None) else None
         keep = torch.is_grad_enabled() or any(ctx.needs_input_grad)
         fdv = torch.empty(n, 12, **f32) if (sdesc.with_grad and keep) else None
+        st_floats = _sdf_store_floats(lib, grid, mdesc, sdesc, n) if (keep and n > 0) else 0
+        store = torch.empty(st_floats, **f32) if st_floats > 0 else None
         nat.check(lib.am_sdf_fwd(nat.sptr(grid), nat.sptr(mdesc), nat.sptr(sdesc), nat.ptr(pts),
-                                 nat.ptr(rnd), n, nat.ptr(table), nat.ptr(params), nat.ptr(sdf),
+                                 nat.ptr(rnd), n, nat.ptr(table_k), nat.ptr(params), nat.ptr(sdf),
                                  nat.ptr(grad) if sdesc.with_grad else None,
                                  nat.ptr(normal) if sdesc.with_grad else None, nat.ptr(feature),
                                  nat.ptr(lap) if sdesc.with_laplace else None, nat.ptr(fdv),
-                                 nat.stream()), "am_sdf_fwd")
+                                 nat.ptr(store), nat.stream()), "am_sdf_fwd")
         empty = pts.new_empty(0)
         ctx.save_for_backward(pts, rnd if rnd is not None else empty, table, params,
-                              fdv if fdv is not None else empty)
+                              fdv if fdv is not None else empty,
+                              store if store is not None else empty, table_k)
         ctx.descs = (grid, mdesc, sdesc)
         return sdf, grad, normal, feature, lap
 
     @staticmethod
     def backward(ctx, g_sdf, g_grad, g_normal, g_feature, g_lap):
-        pts, rnd, table, params, fdv = ctx.saved_tensors
+        pts, rnd, table, params, fdv, store, table_k = ctx.saved_tensors
         grid, mdesc, sdesc = ctx.descs
         lib = nat.load()
         n = pts.shape[0]
@@ -154,12 +190,13 @@ class SDFFieldFn(torch.autograd.Function):
         ws = nat.workspace(lib.am_sdf_bwd_workspace_size(nat.sptr(grid), nat.sptr(mdesc),
                                                          nat.sptr(sdesc), n), pts.device)
         nat.check(lib.am_sdf_bwd(nat.sptr(grid), nat.sptr(mdesc), nat.sptr(sdesc), nat.ptr(pts),
-                                 nat.ptr(rnd) if rnd.numel() else None, n, nat.ptr(table),
+                                 nat.ptr(rnd) if rnd.numel() else None, n, nat.ptr(table_k),
                                  nat.ptr(params), nat.ptr(fdv) if fdv.numel() else None,
+                                 nat.ptr(store) if store.numel() else None,
                                  nat.ptr(gs), nat.ptr(gg), nat.ptr(gn),
                                  nat.ptr(gf), nat.ptr(gl), nat.ptr(dtable), nat.ptr(dparams),
                                  nat.ptr(ws), ws.numel(), nat.stream()), "am_sdf_bwd")
-        return None, None, dtable, dparams, None, None, None
+        return None, None, dtable, dparams, None, None, None, None
 
 
 class RadianceFn(torch.autograd.Function):

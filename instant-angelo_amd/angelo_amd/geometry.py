@@ -59,6 +59,20 @@ class VolumeSDF(BaseImplicitGeometry):
         # 1: curvature points are re-contracted exactly as the reference does
         # (models/geometry.py:246,264-265); 0: shift in world space.
         self.curvature_recontract = int(self.config.get("curvature_recontract", 1))
+        self.precision = "fp32"
+        self.set_precision(self.config.get("precision", "fp32"))
+
+    def set_precision(self, precision):
+        """"fp32" or "mixed" (fp16 table rows, fp16 table-gradient pairs
+        summed in fp32).  The SDF MLP stays fp32 in both: the central
+        differences of the FD normal divide each evaluation's rounding by
+        eps, so bf16 activations there would corrupt the normals."""
+        if precision not in ("fp32", "mixed"):
+            raise ValueError(f"precision {precision!r}")
+        self.precision = precision
+        self.network.set_precision("fp32")
+        if isinstance(self.grid_module, HashGrid):
+            self.grid_module.precision = precision
 
     # -- helpers ------------------------------------------------------------
     @property
@@ -78,6 +92,7 @@ class VolumeSDF(BaseImplicitGeometry):
         d.with_laplace = int(with_laplace)
         d.recontract = self.curvature_recontract
         d.active_levels = int(self.active_levels())
+        d.precision = nat.PREC[self.precision]
         return d
 
     def _fused_ok(self):
@@ -105,7 +120,8 @@ class VolumeSDF(BaseImplicitGeometry):
             randn = torch.randn_like(pts)
         g = self.grid_module
         return fn.SDFFieldFn.apply(pts, randn, g.table, self.network.flat_params(), g.grid,
-                                   self.network.desc, self._sdf_desc(with_grad, with_laplace))
+                                   self.network.desc, self._sdf_desc(with_grad, with_laplace),
+                                   g.kernel_table())
 
     # -- reference API ------------------------------------------------------
     def forward(self, points, with_grad=True, with_feature=True, with_laplace=False,

@@ -13,6 +13,7 @@ import math
 import torch
 import torch.nn as nn
 
+from . import _native as nat
 from . import descs
 from . import functions as fn
 from .utils import config_to_primitive, get_activation, update_module_step
@@ -78,10 +79,28 @@ class HashGrid(nn.Module):
         init = (torch.rand(n, generator=gen) * 2.0 - 1.0) * 1e-4
         self.params = nn.Parameter(init)
         self.active_levels = self.n_levels
+        # "mixed": the fused kernels read an fp16 copy of the fp32 master
+        # table, refreshed whenever the master changes (the optimizer's
+        # in-place update bumps its version).
+        self.precision = "fp32"
+        self._half = None
+        self._half_key = None
 
     @property
     def table(self):
         return self.params.view(-1, self.n_features_per_level)
+
+    def kernel_table(self):
+        """The table the fused kernels read: the fp32 master, or its fp16
+        copy in the mixed-precision form."""
+        if self.precision != "mixed":
+            return self.table
+        key = (self.params.data_ptr(), self.params._version)
+        if self._half is None or self._half_key != key:
+            with torch.no_grad():
+                self._half = self.table.detach().to(torch.float16).contiguous()
+            self._half_key = key
+        return self._half
 
     def level_table(self):
         return descs.grid_arrays(self.grid)
@@ -209,6 +228,11 @@ class VanillaMLP(nn.Module):
         self._kernel_out_act = kernel_out
         self.desc = descs.mlp_desc(dim_in, dim_out, self.n_hidden_layers, self.n_neurons, hidden,
                                    kernel_out, 100.0)
+
+    def set_precision(self, precision):
+        """"fp32", or "mixed": the fused field kernels run this network's
+        matmuls on bf16 MFMA with fp32 accumulation."""
+        self.desc.precision = nat.PREC[precision]
 
     def make_linear(self, dim_in, dim_out, is_first, is_last):
         layer = nn.Linear(dim_in, dim_out, bias=True)

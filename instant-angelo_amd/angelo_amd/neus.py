@@ -120,6 +120,18 @@ class NeuSModel(nn.Module):
         self.max_samples_per_ray = int(cfg.get("max_samples_per_ray", 0))
         self.cos_anneal_ratio = 1.0
         self.kahan = int(cfg.get("kahan", 0))
+        self.precision = "fp32"
+        self.set_precision(cfg.get("precision", "fp32"))
+
+    def set_precision(self, precision):
+        """"fp32": the plain fp32 form.  "mixed": fp16 table rows, bf16 MFMA
+        matmuls with fp32 accumulation (SDF and radiance MLPs), fp16
+        table-gradient pairs summed in fp32 (the trainer applies dynamic loss
+        scaling), and Kahan-compensated transmittance and compositing sums."""
+        self.geometry.set_precision(precision)
+        self.texture.set_precision(precision)
+        self.precision = precision
+        self.kahan = 1 if precision == "mixed" else int(self.config.get("kahan", 0))
 
     # -- per-step protocol (models/neus.py:79-111) --------------------------
     def update_step(self, epoch, global_step):
@@ -147,7 +159,7 @@ class NeuSModel(nn.Module):
         ws = nat.workspace(lib.am_occ_update_workspace_size(og.resolution, m), og.occs.device)
         params = geo.network.flat_params().detach().contiguous()
         nat.check(lib.am_occ_update(nat.sptr(g.grid), nat.sptr(geo.network.desc), nat.sptr(sd),
-                                    nat.ptr(g.table), nat.ptr(params), nat.ptr(inv_s),
+                                    nat.ptr(g.kernel_table()), nat.ptr(params), nat.ptr(inv_s),
                                     float(self.render_step_size), nat.ptr(cells),
                                     nat.ptr(jitter.contiguous()), m, og.resolution,
                                     float(ema_decay), float(occ_thre), nat.ptr(og.occs),

@@ -38,18 +38,26 @@ class NeuSTrainer:
             self.train_num_rays = min(int(self.train_num_rays * 0.9 + train_num_rays * 0.1),
                                       self.mcfg.max_train_num_rays)
 
+    def _mean(self, t):
+        """Loss reductions: fp32, or fp64 in the mixed-precision form (the
+        compensated reductions of that form run in fp64 on the GPU)."""
+        if getattr(self.model, "precision", "fp32") == "mixed":
+            return t.double().mean().float()
+        return t.mean()
+
     def losses(self, out, batch):
         """systems/neus.py:130-194; returns (loss, dict of terms)."""
         L = self.cfg.loss
         terms = {}
         loss = 0.0
         valid = out["rays_valid_full"][..., 0]
-        terms["rgb_mse"] = F.mse_loss(out["comp_rgb_full"][valid], batch["rgb"][valid])
+        terms["rgb_mse"] = self._mean((out["comp_rgb_full"][valid] - batch["rgb"][valid]) ** 2)
         loss = loss + terms["rgb_mse"] * self.C(L.lambda_rgb_mse)
         if self.C(L.get("lambda_rgb_l1", 0.0)) > 0:
             terms["rgb_l1"] = F.l1_loss(out["comp_rgb_full"][valid], batch["rgb"][valid])
             loss = loss + terms["rgb_l1"] * self.C(L.lambda_rgb_l1)
-        terms["eikonal"] = ((torch.linalg.norm(out["sdf_grad_samples"], ord=2, dim=-1) - 1.0) ** 2).mean()
+        terms["eikonal"] = self._mean(
+            (torch.linalg.norm(out["sdf_grad_samples"], ord=2, dim=-1) - 1.0) ** 2)
         loss = loss + terms["eikonal"] * self.C(L.lambda_eikonal)
         opacity = torch.clamp(out["opacity"].squeeze(-1), 1e-3, 1.0 - 1e-3)
         if self.C(L.get("lambda_mask", 0.0)) > 0 and "fg_mask" in batch:
@@ -62,7 +70,7 @@ class NeuSTrainer:
             terms["sparsity"] = torch.exp(-L.sparsity_scale * out["sdf_samples"].abs()).mean()
             loss = loss + terms["sparsity"] * self.C(L.lambda_sparsity)
         if self.C(L.get("lambda_curvature", 0.0)) > 0:
-            terms["curvature"] = out["sdf_laplace_samples"].abs().mean()
+            terms["curvature"] = self._mean(out["sdf_laplace_samples"].abs())
             loss = loss + terms["curvature"] * self.C(L.lambda_curvature)
         for k in ("lambda_distortion", "lambda_sdf_l1"):
             if self.C(L.get(k, 0.0)) > 0:

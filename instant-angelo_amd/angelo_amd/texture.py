@@ -27,6 +27,14 @@ class VolumeRadiance(nn.Module):
         self.encoding = encoding
         self.network = network
         self.color_activation = self.config.get("color_activation", None)
+        self.set_precision(self.config.get("precision", "fp32"))
+
+    def set_precision(self, precision):
+        """"fp32" or "mixed" (radiance MLP on bf16 MFMA, fp32 accumulation)."""
+        if precision not in ("fp32", "mixed"):
+            raise ValueError(f"precision {precision!r}")
+        self.precision = precision
+        self.network.set_precision(precision)
 
     def _fused_ok(self, features, args):
         enc = self.encoding

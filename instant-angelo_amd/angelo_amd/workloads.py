@@ -16,7 +16,7 @@ synthetic data that feeds them.
 import torch
 
 from . import make, scenes
-from .system import NeuSTrainer
+from .system import DynamicLossScaler, NeuSTrainer
 from .utils import to_config
 
 NEURALANGELO_SCALE = 1.3195079107728942
@@ -24,14 +24,15 @@ NEURALANGELO_SCALE = 1.3195079107728942
 
 def model_config(levels, log2T, base, scale, rays, samples_per_ray, step=None, cap=0,
                  grid_prune=False, geo_hidden=1, feature_dim=16, radius=1.0, start_level=None,
-                 kahan=0, seed=0):
+                 kahan=0, seed=0, precision="fp32"):
     return to_config({
         "name": "neus", "radius": radius, "num_samples_per_ray": samples_per_ray,
         "train_num_rays": rays, "max_train_num_rays": rays, "dynamic_ray_sampling": False,
         "ray_chunk": 2048, "randomized": True, "grid_prune": grid_prune,
         "grid_prune_occ_thre": 0.001, "cos_anneal_end": 20000, "learned_background": False,
         "background_color": "white", "render_step_size": step, "max_samples_per_ray": cap,
-        "kahan": kahan, "variance": {"init_val": 0.3, "modulate": False},
+        "kahan": kahan, "precision": precision,
+        "variance": {"init_val": 0.3, "modulate": False},
         "geometry": {
             "name": "volume-sdf", "radius": radius, "feature_dim": feature_dim,
             "grad_type": "finite_difference", "finite_difference_eps": "progressive",
@@ -82,6 +83,11 @@ WORKLOADS = {
     "sphere-fp32": dict(levels=8, log2T=15, base=16, scale=1.5, rays=1024, samples_per_ray=64,
                         step=None, cap=0, grid_prune=False),
 }
+# The mixed-precision form of the Neuralangelo step (same shapes): fp16
+# table rows, bf16 MFMA matmuls with fp32 accumulation, fp16 table-gradient
+# pairs with dynamic loss scaling, Kahan-compensated compositing, fp64 loss
+# reductions.
+WORKLOADS["neuralangelo-mixed"] = dict(WORKLOADS["neuralangelo-fp32"], precision="mixed")
 
 
 class Workload:
@@ -95,13 +101,16 @@ class Workload:
         self.uniform = not spec["grid_prune"]
         mcfg = model_config(spec["levels"], spec["log2T"], spec["base"], spec["scale"], spec["rays"],
                             spec["samples_per_ray"], spec["step"], spec["cap"], spec["grid_prune"],
-                            seed=seed)
+                            seed=seed, precision=spec.get("precision", "fp32"))
         torch.manual_seed(seed)
         self.mcfg, self.scfg = mcfg, system_config()
         self.model = make("neus", mcfg).to(device)
         self.model.train()
         self.trainer = NeuSTrainer(self.model, self.scfg, mcfg)
         self.optimizer = make_optimizer(self.model, self.scfg)
+        self.mixed = spec.get("precision", "fp32") == "mixed"
+        self.scaler = DynamicLossScaler() if self.mixed else None
+        self.skipped_steps = 0
         self.global_step = global_step
         self.batches = []
         gen = torch.Generator().manual_seed(seed + 17)
@@ -131,6 +140,11 @@ class Workload:
         out = self.model(b["rays"], samples=b["samples"])
         loss, _ = self.trainer.losses(out, b)
         self.optimizer.zero_grad(set_to_none=True)
-        loss.backward()
-        self.optimizer.step()
+        if self.scaler is not None:
+            self.scaler.scale_loss(loss).backward()
+            if not self.scaler.step(self.optimizer, list(self.model.parameters())):
+                self.skipped_steps += 1
+        else:
+            loss.backward()
+            self.optimizer.step()
         return out["ray_indices"].shape[0]

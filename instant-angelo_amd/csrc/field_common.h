@@ -20,6 +20,8 @@ static inline int check_mlp_common(const am_mlp_desc* m) {
                  "hidden_act %d", m->hidden_act);
     AM_CHECK_ARG(m->output_act == AM_ACT_NONE, "fused field kernels need output_act none");
     AM_CHECK_ARG(m->n_out >= 1 && m->n_out <= 16, "n_out %d out of [1,16]", m->n_out);
+    AM_CHECK_ARG(m->precision == AM_PREC_FP32 || m->precision == AM_PREC_MIXED, "precision %d",
+                 m->precision);
     return AM_OK;
 }
 

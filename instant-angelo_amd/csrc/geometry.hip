@@ -13,12 +13,14 @@
 // backward rebuilds grad, normal, tangent and the curvature terms from them,
 // back-propagates each evaluation through MLP and encode, and reduces the
 // table gradient deterministically:
-//   1. per lane, consecutive evaluations of a cluster (base + FD around the
-//      sample, or the 6 curvature points) that fall in the same cell at a
-//      level are summed into one set of 8 corner values;
-//   2. the last run of each lane is merged with equal-cell runs of the
-//      neighbouring samples (lanes) of the same lane group, a segmented
-//      reduction in a fixed shuffle tree;
+//   1. per lane and level, the evaluations of a cluster (base + 6 FD, or the
+//      6 curvature points) are merged into the cell B of the cluster centre
+//      with a star stencil: an evaluation in B adds its 8 corners, one in a
+//      face neighbour of B adds the 4 corners of the shared face and emits
+//      its 4 far corners, any other emits its 8 corners;
+//   2. B's corner sums are merged with equal-B lanes of the same lane group
+//      (neighbouring samples of a ray) by a segmented reduction in a fixed
+//      shuffle tree;
 //   3. a count pass makes the same merge decisions from positions alone;
 //      an integer prefix sum of the per-tile counts gives each tile its
 //      first pair, the backward writes the surviving (row, value) pairs
@@ -30,6 +32,7 @@
 #include "mlp_dev.h"
 #include "table_grad.h"
 #include "field_common.h"
+#include <type_traits>
 #include <rocprim/device/device_scan.hpp>
 
 namespace am {
@@ -45,6 +48,7 @@ struct GeoArgs {
     int w1, b1, w2, b2, wo, bo, ld1, lds_floats;  // LDS offsets (floats)
     int64_t pw1, pb1, pw2, pb2, pwo, pbo, n_params;
     int n_evals;  // 1, 7 or 13
+    int prec;     // AM_PREC_FP32 / AM_PREC_MIXED
 };
 
 // Row-space column of the SDF MLP first layer for lane-group level slots:
@@ -108,12 +112,12 @@ __device__ __forceinline__ float pick3(const float v[3], int k) {
     return k == 0 ? v[0] : (k == 1 ? v[1] : v[2]);
 }
 
-// Encode + SDF MLP of this lane's sample at contracted point x.
-template <int LJ, int NH, bool DX>
-__device__ __forceinline__ void geo_eval(const GeoArgs& a, const float* sm,
-                                         const LevelParams* lv, const float2* __restrict__ table,
-                                         const float x[3], GeoEval<LJ, NH, DX>& e) {
-    constexpr int KS1 = GeoDims<LJ>::KS1;
+// Encode of this lane's level slots at contracted point x (e.xb[0..2LJ),
+// and with DX the position derivatives e.dx).
+template <int LJ, int NH, bool DX, class TT>
+__device__ __forceinline__ void geo_encode(const GeoArgs& a, const LevelParams* lv,
+                                           const TT* __restrict__ table, const float x[3],
+                                           GeoEval<LJ, NH, DX>& e) {
     const int k = lane_id() >> 4;
 #pragma unroll
     for (int j = 0; j < LJ; ++j) {
@@ -133,22 +137,106 @@ __device__ __forceinline__ void geo_eval(const GeoArgs& a, const float* sm,
             }
         }
     }
+}
+
+// xyz input (last K-step) + SDF MLP forward from the encoded slots.
+template <int LJ, int NH, bool DX, int PR>
+__device__ __forceinline__ void geo_mlp(const GeoArgs& a, const float* sm, const float x[3],
+                                        GeoEval<LJ, NH, DX>& e) {
+    constexpr int KS1 = GeoDims<LJ>::KS1;
+    const int k = lane_id() >> 4;
     e.xb[2 * LJ] = k < 3 ? pick3(x, k) * 2.0f - 1.0f : 0.0f;
 #pragma unroll
     for (int t = 0; t < 4; ++t) e.z1[t] = mlp::bias_tile(sm + a.b1, t, 64);
-    mlp::mm_fwd<KS1, 4>(sm + a.w1, a.ld1, mlp::FragF{e.xb}, e.z1);
+    mlp::mm_fwd<KS1, 4, mlp::FragF, PR>(sm + a.w1, a.ld1, mlp::FragF{e.xb}, e.z1);
     v4f h[4];
 #pragma unroll
     for (int t = 0; t < 4; ++t) h[t] = mlp::act4(a.act, e.z1[t], a.beta);
     if constexpr (NH == 2) {
 #pragma unroll
         for (int t = 0; t < 4; ++t) e.z2[t] = mlp::bias_tile(sm + a.b2, t, 64);
-        mlp::mm_fwd<16, 4>(sm + a.w2, 65, mlp::FragV{h}, e.z2);
+        mlp::mm_fwd<16, 4, mlp::FragV, PR>(sm + a.w2, 65, mlp::FragV{h}, e.z2);
 #pragma unroll
         for (int t = 0; t < 4; ++t) h[t] = mlp::act4(a.act, e.z2[t], a.beta);
     }
     e.out = mlp::bias_tile(sm + a.bo, 0, a.n_out);
-    mlp::mm_fwd<16, 1>(sm + a.wo, 65, mlp::FragV{h}, &e.out);
+    mlp::mm_fwd<16, 1, mlp::FragV, PR>(sm + a.wo, 65, mlp::FragV{h}, &e.out);
+}
+
+// Encode + SDF MLP of this lane's sample at contracted point x.
+template <int LJ, int NH, bool DX, int PR, class TT>
+__device__ __forceinline__ void geo_eval(const GeoArgs& a, const float* sm,
+                                         const LevelParams* lv, const TT* __restrict__ table,
+                                         const float x[3], GeoEval<LJ, NH, DX>& e) {
+    geo_encode<LJ, NH, DX>(a, lv, table, x, e);
+    geo_mlp<LJ, NH, DX, PR>(a, sm, x, e);
+}
+
+// Stored encodings (forward -> backward, the "store" form of the backward):
+// per tile of 16 samples, evaluation ev (0 base, 1-6 FD, 7-12 curvature) and
+// slot value s < 2LJ, one float per lane at
+//   enc[((tile * n_evals + ev) * 2LJ + s) * 64 + lane];
+// with laplace, the position derivatives of the 6 curvature evaluations
+// follow at enc + geo_enc_floats(...):
+//   [((tile * 6 + c) * 6LJ + (j * 2 + f) * 3 + d) * 64 + lane].
+// Every access is a 256 B coalesced row.
+__host__ __device__ inline int64_t geo_enc_floats(int64_t tiles, int n_evals, int LJ) {
+    return tiles * n_evals * 2 * LJ * 64;
+}
+__host__ __device__ inline int64_t geo_dx_floats(int64_t tiles, int with_laplace, int LJ) {
+    return with_laplace ? tiles * 6 * 6 * LJ * 64 : 0;
+}
+
+template <int LJ, int NH, bool DX>
+__device__ __forceinline__ void geo_store(float* __restrict__ st, const GeoArgs& a, int64_t tile,
+                                          int64_t n_tiles, int ev, const GeoEval<LJ, NH, DX>& e) {
+    const int l = lane_id();
+    float* p = st + (tile * a.n_evals + ev) * (2 * LJ) * 64 + l;
+#pragma unroll
+    for (int q = 0; q < 2 * LJ; ++q) p[q * 64] = e.xb[q];
+    if constexpr (DX) {
+        float* d0 = st + geo_enc_floats(n_tiles, a.n_evals, LJ) +
+                    (tile * 6 + (ev - 7)) * (6 * LJ) * 64 + l;
+#pragma unroll
+        for (int j = 0; j < LJ; ++j)
+#pragma unroll
+            for (int f = 0; f < 2; ++f)
+#pragma unroll
+                for (int d = 0; d < 3; ++d) d0[((j * 2 + f) * 3 + d) * 64] = e.dx[j][f][d];
+    }
+}
+
+template <int LJ, int NH, bool DX>
+__device__ __forceinline__ void geo_load(const float* __restrict__ st, const GeoArgs& a,
+                                         int64_t tile, int64_t n_tiles, int ev,
+                                         GeoEval<LJ, NH, DX>& e) {
+    const int l = lane_id();
+    const float* p = st + (tile * a.n_evals + ev) * (2 * LJ) * 64 + l;
+#pragma unroll
+    for (int q = 0; q < 2 * LJ; ++q) e.xb[q] = p[q * 64];
+    if constexpr (DX) {
+        const float* d0 = st + geo_enc_floats(n_tiles, a.n_evals, LJ) +
+                          (tile * 6 + (ev - 7)) * (6 * LJ) * 64 + l;
+#pragma unroll
+        for (int j = 0; j < LJ; ++j)
+#pragma unroll
+            for (int f = 0; f < 2; ++f)
+#pragma unroll
+                for (int d = 0; d < 3; ++d) e.dx[j][f][d] = d0[((j * 2 + f) * 3 + d) * 64];
+    }
+}
+
+// Backward-side evaluation: the stored encoding (ST) or a fresh encode, then
+// the MLP forward.
+template <int LJ, int NH, bool DX, bool ST, int PR, class TT>
+__device__ __forceinline__ void geo_eval_b(const GeoArgs& a, const float* sm, const LevelParams* lv,
+                                           const TT* __restrict__ table,
+                                           const float* __restrict__ st, int64_t tile,
+                                           int64_t n_tiles, int ev, const float x[3],
+                                           GeoEval<LJ, NH, DX>& e) {
+    if constexpr (ST) geo_load<LJ, NH, DX>(st, a, tile, n_tiles, ev, e);
+    else geo_encode<LJ, NH, DX>(a, lv, table, x, e);
+    geo_mlp<LJ, NH, DX, PR>(a, sm, x, e);
 }
 
 __device__ __forceinline__ float contract1(float v, const GeoArgs& a) { return (v + a.r) * a.inv2r; }
@@ -233,21 +321,36 @@ __device__ __forceinline__ void geo_tangent(const GeoArgs& a, const float* __res
     }
 }
 
-template <int LJ, int NH>
+// Precision forms of the fused field (template PR).  Mixed: the table rows
+// are read in fp16 and the table-gradient pair values are stored in fp16
+// (summed in fp32); the SDF MLP stays fp32 in both forms, because the
+// central differences of the FD normal divide each evaluation's rounding
+// by eps (bf16 activations there gave normals off by 0.3 on average at the
+// Neuralangelo shapes, while a fixed fp16 rounding of the table keeps the
+// field smooth).
+// Table storage of a precision form: fp32 rows, or fp16 rows (mixed form).
+template <int PR>
+using TableT = typename std::conditional<PR == mlp::kF32, float2, __half2>::type;
+// Table-gradient pair values: fp32, or fp16 (mixed form, loss-scaled).
+template <int PR>
+using PairV = typename std::conditional<PR == mlp::kF32, Vals<2>, ValsH<2>>::type;
+
+template <int LJ, int NH, bool ST, int PR>
 __global__ void __launch_bounds__(256) k_geo_fwd(GeoArgs a, const float* __restrict__ pts,
                                                  const float* __restrict__ randn, int64_t N,
-                                                 const float* __restrict__ table_f,
+                                                 const void* __restrict__ table_v,
                                                  const float* __restrict__ params,
                                                  float* __restrict__ sdf, float* __restrict__ grad,
                                                  float* __restrict__ normal,
                                                  float* __restrict__ feature,
                                                  float* __restrict__ laplace,
-                                                 float* __restrict__ fdv) {
+                                                 float* __restrict__ fdv,
+                                                 float* __restrict__ store) {
     extern __shared__ float sm[];
     __shared__ LevelParams lv[AM_MAX_LEVELS];
     geo_load_lds<LJ, NH>(a, params, sm, lv);
     __syncthreads();
-    const float2* table = reinterpret_cast<const float2*>(table_f);
+    const TableT<PR>* table = reinterpret_cast<const TableT<PR>*>(table_v);
     const int l = lane_id(), p = l & 15, k = l >> 4;
     const int nw = blockDim.x >> 6, wave = threadIdx.x >> 6;
     const int64_t n_tiles = (N + 15) / 16;
@@ -263,7 +366,8 @@ __global__ void __launch_bounds__(256) k_geo_fwd(GeoArgs a, const float* __restr
         for (int d = 0; d < 3; ++d) s.x0[d] = contract1(s.P[d], a);
         {
             GeoEval<LJ, NH, false> e;
-            geo_eval<LJ, NH, false>(a, sm, lv, table, s.x0, e);
+            geo_eval<LJ, NH, false, mlp::kF32>(a, sm, lv, table, s.x0, e);
+            if constexpr (ST) geo_store<LJ, NH, false>(store, a, tile, n_tiles, 0, e);
             if (valid) {
                 if (k == 0) sdf[n] = e.out[0];
                 if (feature) {
@@ -284,7 +388,8 @@ __global__ void __launch_bounds__(256) k_geo_fwd(GeoArgs a, const float* __restr
             bool m[3];
             offset_point(a, s.P, j, x, m);
             GeoEval<LJ, NH, false> e;
-            geo_eval<LJ, NH, false>(a, sm, lv, table, x, e);
+            geo_eval<LJ, NH, false, mlp::kF32>(a, sm, lv, table, x, e);
+            if constexpr (ST) geo_store<LJ, NH, false>(store, a, tile, n_tiles, 1 + j, e);
             const float v = shfl(e.out[0], p);
             diff_pair(g, j, v);
             if (fdv && valid && k == 0) fdv[n * 12 + j] = v;
@@ -309,8 +414,9 @@ __global__ void __launch_bounds__(256) k_geo_fwd(GeoArgs a, const float* __restr
             float x[3];
             bool m[3];
             offset_point(a, s.ps, j, x, m);
-            GeoEval<LJ, NH, false> e;
-            geo_eval<LJ, NH, false>(a, sm, lv, table, x, e);
+            GeoEval<LJ, NH, ST> e;  // the store keeps the position derivatives
+            geo_eval<LJ, NH, ST, mlp::kF32>(a, sm, lv, table, x, e);
+            if constexpr (ST) geo_store<LJ, NH, true>(store, a, tile, n_tiles, 7 + j, e);
             const float v = shfl(e.out[0], p);
             diff_pair(gs, j, v);
             if (fdv && valid && k == 0) fdv[n * 12 + 6 + j] = v;
@@ -347,7 +453,7 @@ __device__ __forceinline__ v4f go_row0(float g) {
 // outputs): accumulates weight gradients, stores the encode gradient of the
 // lane's level slots to genc (LDS, slot-major [LJ][2][64]) and, with DX,
 // returns the contracted-position gradient (summed over the 4 lane groups).
-template <int LJ, int NH, bool DX>
+template <int LJ, int NH, bool DX, int PR>
 __device__ __forceinline__ void geo_eval_bwd(const GeoArgs& a, const float* sm,
                                              const GeoEval<LJ, NH, DX>& e, v4f go,
                                              float* scratch, GeoGrad<LJ, NH>& G, float* genc,
@@ -358,12 +464,12 @@ __device__ __forceinline__ void geo_eval_bwd(const GeoArgs& a, const float* sm,
     const v4f* zl = NH == 2 ? e.z2 : e.z1;
 #pragma unroll
     for (int t = 0; t < 4; ++t) h[t] = mlp::act4(a.act, zl[t], a.beta);
-    mlp::mm_dw<1, 4, 16>(&go, mlp::FragV{h}, scratch, G.dWo);
+    mlp::mm_dw<1, 4, 16, mlp::FragV, PR>(&go, mlp::FragV{h}, scratch, G.dWo);
     G.dbo += go;
     v4f gh[4];
 #pragma unroll
     for (int t = 0; t < 4; ++t) gh[t] = v4f{0, 0, 0, 0};
-    mlp::mm_bwd_in<4, 4>(sm + a.wo, 65, mlp::FragV{&go}, gh);
+    mlp::mm_bwd_in<4, 4, mlp::FragV, PR>(sm + a.wo, 65, mlp::FragV{&go}, gh);
     if constexpr (NH == 2) {
         v4f gz2[4], h1[4];
 #pragma unroll
@@ -373,10 +479,10 @@ __device__ __forceinline__ void geo_eval_bwd(const GeoArgs& a, const float* sm,
             h1[t] = mlp::act4(a.act, e.z1[t], a.beta);
             G.db2[t] += gz2[t];
         }
-        mlp::mm_dw<4, 4, 16>(gz2, mlp::FragV{h1}, scratch, G.dW2);
+        mlp::mm_dw<4, 4, 16, mlp::FragV, PR>(gz2, mlp::FragV{h1}, scratch, G.dW2);
 #pragma unroll
         for (int t = 0; t < 4; ++t) gh[t] = v4f{0, 0, 0, 0};
-        mlp::mm_bwd_in<16, 4>(sm + a.w2, 65, mlp::FragV{gz2}, gh);
+        mlp::mm_bwd_in<16, 4, mlp::FragV, PR>(sm + a.w2, 65, mlp::FragV{gz2}, gh);
     }
     v4f gz1[4];
 #pragma unroll
@@ -385,11 +491,11 @@ __device__ __forceinline__ void geo_eval_bwd(const GeoArgs& a, const float* sm,
         for (int r = 0; r < 4; ++r) gz1[t][r] = gh[t][r] * mlp::act_bwd(a.act, e.z1[t][r], a.beta);
         G.db1[t] += gz1[t];
     }
-    mlp::mm_dw<4, IT1, KS1>(gz1, mlp::FragF{e.xb}, scratch, G.dW1);
+    mlp::mm_dw<4, IT1, KS1, mlp::FragF, PR>(gz1, mlp::FragF{e.xb}, scratch, G.dW1);
     v4f gin[IT1];
 #pragma unroll
     for (int t = 0; t < IT1; ++t) gin[t] = v4f{0, 0, 0, 0};
-    mlp::mm_bwd_in<16, IT1>(sm + a.w1, a.ld1, mlp::FragV{gz1}, gin);
+    mlp::mm_bwd_in<16, IT1, mlp::FragV, PR>(sm + a.w1, a.ld1, mlp::FragV{gz1}, gin);
     const mlp::FragV gb{gin};  // gb[s] = gradient of B fragment s
 #pragma unroll
     for (int j = 0; j < LJ; ++j) {
@@ -417,9 +523,10 @@ __device__ __forceinline__ void geo_eval_bwd(const GeoArgs& a, const float* sm,
 // A tile's pairs go to [keys, keys + limit): limit is the count the count
 // pass found for the tile; the backward emits the same sequence, and never
 // writes past limit.
+template <class VT>
 struct EmitTile {
     uint32_t* keys;   // tile's first pair (keys)
-    Vals<2>* vals;    // tile's first pair (values)
+    VT* vals;         // tile's first pair (values)
     int cursor;       // wave-uniform write position
     int limit;        // pairs reserved for the tile
 };
@@ -429,26 +536,47 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
     return l == 0 ? 0ull : (~0ull >> (64 - l));
 }
 
-// Lanes with `on` write 8 pairs (corner rows of `cell` at level `lp`,
-// values acc) at consecutive positions in lane order; COUNT only advances
-// the cursor.
-template <bool COUNT>
-__device__ __forceinline__ void emit_corners(EmitTile& et, bool on, const LevelParams& lp,
-                                             const uint32_t cell[3], const float acc[16]) {
+// Lanes with `on` write K pairs (rows, values) at consecutive positions in
+// lane order; COUNT only advances the cursor.
+__device__ __forceinline__ void set_pair(Vals<2>& v, float a, float b) {
+    v.v[0] = a;
+    v.v[1] = b;
+}
+__device__ __forceinline__ void set_pair(ValsH<2>& v, float a, float b) {
+    v.v[0] = __float2half_rn(a);
+    v.v[1] = __float2half_rn(b);
+}
+
+template <int K, bool COUNT, class VT>
+__device__ __forceinline__ void emit_pairs(EmitTile<VT>& et, bool on, const uint32_t rows[K],
+                                           const float vals[2 * K]) {
     const uint64_t mask = __ballot(on);
-    const int off = et.cursor + 8 * __popcll(mask & lanemask_lt());
-    if (!COUNT && on && off + 8 <= et.limit) {
+    const int off = et.cursor + K * __popcll(mask & lanemask_lt());
+    if (!COUNT && on && off + K <= et.limit) {
 #pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            et.keys[off + c] = lp.offset + grid_row(lp, cell[0] + (c & 1), cell[1] + ((c >> 1) & 1),
-                                                    cell[2] + ((c >> 2) & 1));
-            Vals<2> v;
-            v.v[0] = acc[2 * c];
-            v.v[1] = acc[2 * c + 1];
+        for (int c = 0; c < K; ++c) {
+            et.keys[off + c] = rows[c];
+            VT v;
+            set_pair(v, vals[2 * c], vals[2 * c + 1]);
             et.vals[off + c] = v;
         }
     }
-    et.cursor += 8 * __popcll(mask);
+    et.cursor += K * __popcll(mask);
+}
+
+// The 8 corner rows of `cell` at level `lp` (corner c = cell + bits of c).
+__device__ __forceinline__ void corner_rows(const LevelParams& lp, const uint32_t cell[3],
+                                            uint32_t rows[8]) {
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+        rows[c] = lp.offset + grid_row(lp, cell[0] + (c & 1), cell[1] + ((c >> 1) & 1),
+                                       cell[2] + ((c >> 2) & 1));
+}
+
+// Corner i (0..3) of the face of a cell with bit `ax` fixed to `b`.
+__device__ __forceinline__ int face_corner(int i, int ax, int b) {
+    const int low = i & ((1 << ax) - 1);
+    return low | (b << ax) | ((i >> ax) << (ax + 1));
 }
 
 // Position of cluster evaluation q (0..n-1) for this lane.
@@ -468,14 +596,33 @@ __device__ __forceinline__ void cluster_point(const GeoArgs& a, const GeoSample&
     }
 }
 
+// Centre of a cluster: the base point (FD cluster) or the tangent-shifted
+// point (curvature cluster; not itself evaluated), clamped and contracted as
+// its offsets are.
+template <int CL>
+__device__ __forceinline__ void cluster_center(const GeoArgs& a, const GeoSample& s, float x[3]) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) x[d] = CL == 0 ? s.x0[d] : contract1(clampr(s.ps[d], a.r), a);
+}
+
 // Reduces the stored encode gradients of one cluster of evaluations into
-// merged corner pairs.  genc_base: LDS [eval][LJ][2][64] of this wave; the
-// cluster's q-th evaluation is genc slot slot0 + q.  COUNT: positions only
-// (no gradients), the cursor ends at the tile's pair count.
-template <int LJ, int CL, bool COUNT>
+// (row, value) pairs with a star-stencil merge.  Per level, B = the cell of
+// the cluster centre; an evaluation whose cell C
+//   * is B adds its 8 corner values into B's accumulators,
+//   * is B +- e_axis (a face neighbour, the usual case of an offset of about
+//     one cell) adds its 4 corners on the shared face into B's accumulators
+//     and emits its 4 far corners,
+//   * is anything else emits its 8 corners.
+// B's accumulators are then merged with equal-B lanes of the same lane group
+// (neighbouring samples of a ray) by a segmented sum in a fixed shuffle
+// tree, and the segment head emits them.  Every sum has a fixed order.
+// genc_base: LDS [eval][LJ][2][64] of this wave; the cluster's q-th
+// evaluation is genc slot slot0 + q.  COUNT: positions only (no gradients);
+// the cursor ends at the tile's pair count.
+template <int LJ, int CL, bool COUNT, class VT>
 __device__ __forceinline__ void flush_cluster(const GeoArgs& a, const LevelParams* lv,
                                               const GeoSample& s, bool valid, const float* genc_base,
-                                              int n_q, int slot0, EmitTile& et) {
+                                              int n_q, int slot0, EmitTile<VT>& et) {
     const int l = lane_id(), p = l & 15, k = l >> 4;
 #pragma unroll 1
     for (int j = 0; j < LJ; ++j) {
@@ -483,8 +630,15 @@ __device__ __forceinline__ void flush_cluster(const GeoArgs& a, const LevelParam
         const bool on = valid && level < a.active;
         LevelParams lp{};
         if (level < a.g.L) lp = lv[level];
-        uint32_t cell[3] = {0u, 0u, 0u};
-        bool have = false;
+        uint32_t B[3];
+        {
+            float xc[3];
+            cluster_center<CL>(a, s, xc);
+            const LevelPos pb = level_pos(lp, xc);
+#pragma unroll
+            for (int d = 0; d < 3; ++d) B[d] = pb.c[d];
+        }
+        bool touched = false;
         float acc[16];
 #pragma unroll
         for (int q = 0; q < 16; ++q) acc[q] = 0.0f;
@@ -492,33 +646,87 @@ __device__ __forceinline__ void flush_cluster(const GeoArgs& a, const LevelParam
         for (int q = 0; q < n_q; ++q) {
             float x[3];
             cluster_point<CL>(a, s, q, x);
-            LevelPos pp = level_pos(lp, x);
-            const bool same = have && pp.c[0] == cell[0] && pp.c[1] == cell[1] && pp.c[2] == cell[2];
-            // a run ends: emit it (no cross-lane merge for interior runs)
-            emit_corners<COUNT>(et, on && have && !same, lp, cell, acc);
-            if (!same) {
+            const LevelPos pp = level_pos(lp, x);
+            // classify C - B
+            int ax = -1, sgn = 0, nz = 0;
+            bool unit = true;
 #pragma unroll
-                for (int d = 0; d < 3; ++d) cell[d] = pp.c[d];
-#pragma unroll
-                for (int c = 0; c < 16; ++c) acc[c] = 0.0f;
-                have = true;
+            for (int d = 0; d < 3; ++d) {
+                const int dd = (int)(pp.c[d] - B[d]);
+                if (dd != 0) {
+                    ++nz;
+                    ax = d;
+                    sgn = dd;
+                    unit = unit && (dd == 1 || dd == -1);
+                }
             }
+            const bool same = nz == 0;
+            const bool adj = nz == 1 && unit;
+            const int axm = ax < 0 ? 0 : ax;
+            const int far_bit = sgn > 0 ? 1 : 0;  // C's corners off B's face
+            touched = touched || same || adj;
+            float g0 = 0.0f, g1 = 0.0f;
             if constexpr (!COUNT) {
                 const int slot = slot0 + q;
-                const float g0 = genc_base[(slot * LJ + j) * 128 + l];
-                const float g1 = genc_base[(slot * LJ + j) * 128 + 64 + l];
+                g0 = genc_base[(slot * LJ + j) * 128 + l];
+                g1 = genc_base[(slot * LJ + j) * 128 + 64 + l];
+            }
+            // all 8 corners of C on their own
+            {
+                const bool em = on && !same && !adj;
+                uint32_t rows[8];
+                float v[16];
+                if (!COUNT && em) {
+                    corner_rows(lp, pp.c, rows);
 #pragma unroll
-                for (int c = 0; c < 8; ++c) {
+                    for (int c = 0; c < 8; ++c) {
+                        const float w = corner_weight(pp, c);
+                        v[2 * c] = w * g0;
+                        v[2 * c + 1] = w * g1;
+                    }
+                }
+                emit_pairs<8, COUNT, VT>(et, em, rows, v);
+            }
+            // the 4 far corners of a face neighbour
+            {
+                const bool em = on && adj;
+                uint32_t rows[4];
+                float v[8];
+                if (!COUNT && em) {
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const int c = face_corner(i, axm, far_bit);
+                        rows[i] = lp.offset + grid_row(lp, pp.c[0] + (c & 1),
+                                                       pp.c[1] + ((c >> 1) & 1),
+                                                       pp.c[2] + ((c >> 2) & 1));
+                        const float w = corner_weight(pp, c);
+                        v[2 * i] = w * g0;
+                        v[2 * i + 1] = w * g1;
+                    }
+                }
+                emit_pairs<4, COUNT, VT>(et, em, rows, v);
+            }
+            // into B: corner t of B is corner t of C (same) or corner
+            // t ^ (1 << ax) of C when t lies on the shared face (adj)
+            if constexpr (!COUNT) {
+#pragma unroll
+                for (int t = 0; t < 8; ++t) {
+                    const bool take = same || (adj && ((t >> axm) & 1) == far_bit);
+                    const int c = same ? t : (t ^ (1 << axm));
                     const float w = corner_weight(pp, c);
-                    acc[2 * c] = __fmaf_rn(w, g0, acc[2 * c]);
-                    acc[2 * c + 1] = __fmaf_rn(w, g1, acc[2 * c + 1]);
+                    if (take) {
+                        acc[2 * t] = __fmaf_rn(w, g0, acc[2 * t]);
+                        acc[2 * t + 1] = __fmaf_rn(w, g1, acc[2 * t + 1]);
+                    }
                 }
             }
         }
-        // last run: merge equal cells of neighbouring lanes (same lane group),
+        // B: merge equal cells of neighbouring lanes (same lane group),
         // suffix segmented sum in a fixed tree, segment head emits.
         const uint32_t INV = 0xFFFFFFFFu;
-        uint32_t key0 = on ? cell[0] : INV, key1 = on ? cell[1] : INV, key2 = on ? cell[2] : INV;
+        const bool live = on && touched;
+        const uint32_t key0 = live ? B[0] : INV, key1 = live ? B[1] : INV,
+                       key2 = live ? B[2] : INV;
         const uint32_t n0 = __shfl_down(key0, 1, 16), n1 = __shfl_down(key1, 1, 16),
                        n2 = __shfl_down(key2, 1, 16);
         bool end = (p == 15) || !(n0 == key0 && n1 == key1 && n2 == key2) || key0 == INV;
@@ -538,8 +746,10 @@ __device__ __forceinline__ void flush_cluster(const GeoArgs& a, const LevelParam
         }
         const uint32_t u0 = __shfl_up(key0, 1, 16), u1 = __shfl_up(key1, 1, 16),
                        u2 = __shfl_up(key2, 1, 16);
-        const bool head = on && (p == 0 || !(u0 == key0 && u1 == key1 && u2 == key2));
-        emit_corners<COUNT>(et, head, lp, cell, acc);
+        const bool head = live && (p == 0 || !(u0 == key0 && u1 == key1 && u2 == key2));
+        uint32_t rows[8];
+        if (!COUNT && head) corner_rows(lp, B, rows);
+        emit_pairs<8, COUNT, VT>(et, head, rows, acc);
     }
 }
 
@@ -596,29 +806,30 @@ __global__ void __launch_bounds__(256) k_geo_count(GeoArgs a, const float* __res
         const bool valid = n < N;
         GeoSample s;
         geo_rebuild(a, pts, randn, fdv, n, valid, s);
-        EmitTile et{nullptr, nullptr, 0, 0};
+        EmitTile<Vals<2>> et{nullptr, nullptr, 0, 0};
         if (a.with_laplace) flush_cluster<LJ, 1, true>(a, lv, s, valid, nullptr, 6, 1, et);
         flush_cluster<LJ, 0, true>(a, lv, s, valid, nullptr, a.with_grad ? 7 : 1, 0, et);
         if (l == 0) counts[tile] = et.cursor;
     }
 }
 
-template <int LJ, int NH>
+template <int LJ, int NH, bool ST, int PR>
 __global__ void __launch_bounds__(256) k_geo_bwd(
     GeoArgs a, const float* __restrict__ pts,
-    const float* __restrict__ randn, int64_t N, const float* __restrict__ table_f,
+    const float* __restrict__ randn, int64_t N, const void* __restrict__ table_v,
     const float* __restrict__ params, const float* __restrict__ fdv,
+    const float* __restrict__ store,
     const float* __restrict__ g_sdf, const float* __restrict__ g_grad,
     const float* __restrict__ g_normal, const float* __restrict__ g_feature,
     const float* __restrict__ g_laplace, uint32_t* __restrict__ keys,
-    Vals<2>* __restrict__ vals, const int64_t* __restrict__ offsets,
+    PairV<PR>* __restrict__ vals, const int64_t* __restrict__ offsets,
     const int64_t* __restrict__ counts, int64_t* __restrict__ status,
     float* __restrict__ partial) {
     extern __shared__ float sm[];
     __shared__ LevelParams lv[AM_MAX_LEVELS];
     geo_load_lds<LJ, NH>(a, params, sm, lv);
     __syncthreads();
-    const float2* table = reinterpret_cast<const float2*>(table_f);
+    const TableT<PR>* table = reinterpret_cast<const TableT<PR>*>(table_v);
     const int l = lane_id(), p = l & 15, k = l >> 4;
     const int nw = blockDim.x >> 6, wave = threadIdx.x >> 6;
     float* scratch = sm + a.lds_floats + wave * (128 * mlp::kScratchLd);
@@ -645,7 +856,7 @@ __global__ void __launch_bounds__(256) k_geo_bwd(
         const int64_t n = tile * 16 + p;
         const bool valid = n < N;
         const int64_t off = offsets[tile];
-        EmitTile et{keys + off, vals + off, 0, (int)counts[tile]};
+        EmitTile<PairV<PR>> et{keys + off, vals + off, 0, (int)counts[tile]};
         GeoSample s;
         geo_rebuild(a, pts, randn, fdv, n, valid, s);
         // upstream gradients of this sample
@@ -670,8 +881,9 @@ __global__ void __launch_bounds__(256) k_geo_bwd(
                 go[r] = v;
             }
             GeoEval<LJ, NH, false> e;
-            geo_eval<LJ, NH, false>(a, sm, lv, table, s.x0, e);
-            geo_eval_bwd<LJ, NH, false>(a, sm, e, go, scratch, G, genc, nullptr);
+            geo_eval_b<LJ, NH, false, ST, mlp::kF32>(a, sm, lv, table, store, tile, n_tiles, 0,
+                                                     s.x0, e);
+            geo_eval_bwd<LJ, NH, false, mlp::kF32>(a, sm, e, go, scratch, G, genc, nullptr);
         }
         if (a.with_laplace) {
             const float gl = (valid && g_laplace) ? g_laplace[n] : 0.0f;
@@ -696,9 +908,10 @@ __global__ void __launch_bounds__(256) k_geo_bwd(
                 bool m[3];
                 offset_point(a, s.ps, j, x, m);
                 GeoEval<LJ, NH, true> e;
-                geo_eval<LJ, NH, true>(a, sm, lv, table, x, e);
+                geo_eval_b<LJ, NH, true, ST, mlp::kF32>(a, sm, lv, table, store, tile, n_tiles,
+                                                        7 + j, x, e);
                 float gpos[3];
-                geo_eval_bwd<LJ, NH, true>(a, sm, e, go_row0(gf), scratch, G,
+                geo_eval_bwd<LJ, NH, true, mlp::kF32>(a, sm, e, go_row0(gf), scratch, G,
                                            genc + (1 + j) * LJ * 128, gpos);
 #pragma unroll
                 for (int d = 0; d < 3; ++d)
@@ -724,14 +937,15 @@ __global__ void __launch_bounds__(256) k_geo_bwd(
                 bool m[3];
                 offset_point(a, s.P, j, x, m);
                 GeoEval<LJ, NH, false> e;
-                geo_eval<LJ, NH, false>(a, sm, lv, table, x, e);
-                geo_eval_bwd<LJ, NH, false>(a, sm, e, go_row0(gf), scratch, G,
+                geo_eval_b<LJ, NH, false, ST, mlp::kF32>(a, sm, lv, table, store, tile, n_tiles,
+                                                         1 + j, x, e);
+                geo_eval_bwd<LJ, NH, false, mlp::kF32>(a, sm, e, go_row0(gf), scratch, G,
                                             genc + (1 + j) * LJ * 128, nullptr);
             }
         }
         flush_cluster<LJ, 0, false>(a, lv, s, valid, genc, a.with_grad ? 7 : 1, 0, et);
         // the count pass and this pass emit the same sequence; a mismatch is
-        // reported (never written past: emit_corners stops at the limit)
+        // reported (never written past: emit_pairs stops at the limit)
         if (l == 0 && et.cursor != et.limit) status[1] = 1;
     }
     // per-wave partial gradient row in parameter layout
@@ -809,6 +1023,16 @@ static int make_geo_args(const am_grid_desc* g, const am_mlp_desc* m, const am_s
     AM_CHECK_ARG(s->radius > 0.0f, "radius <= 0");
     AM_CHECK_ARG(!s->with_grad || s->fd_eps > 0.0f, "fd_eps <= 0");
     AM_CHECK_ARG(!s->with_laplace || s->with_grad, "laplace needs with_grad");
+    AM_CHECK_ARG(s->precision == AM_PREC_FP32 || s->precision == AM_PREC_MIXED,
+                 "precision %d", s->precision);
+    AM_CHECK_ARG(m->precision == AM_PREC_FP32,
+                 "the SDF MLP runs in fp32 in every form (got precision %d): central "
+                 "differences amplify per-evaluation rounding by 1/eps", m->precision);
+    if (s->precision == AM_PREC_MIXED && m->n_hidden_layers != 1) {
+        set_error("the mixed-precision SDF field supports 1 hidden layer (got %d)",
+                  m->n_hidden_layers);
+        return AM_ERR_UNSUPPORTED;
+    }
     GeoArgs a{};
     a.g = make_grid_args(g);
     a.r = s->radius;
@@ -840,26 +1064,52 @@ static int make_geo_args(const am_grid_desc* g, const am_mlp_desc* m, const am_s
     layout_params(m->n_in, m->n_out, m->n_hidden_layers, &a.pw1, &a.pb1, &a.pw2, &a.pb2, &a.pwo,
                   &a.pbo, &a.n_params);
     a.n_evals = 1 + (s->with_grad ? 6 : 0) + (s->with_laplace ? 6 : 0);
+    a.prec = s->precision;
     *out = a;
     *LJ = lj;
     return AM_OK;
 }
 
-template <int LJ, int NH>
-static int geo_fwd_launch(const GeoArgs& a, const float* pts,
-                          const float* randn, int64_t n, const float* table, const float* params,
-                          float* sdf, float* grad, float* normal, float* feature, float* laplace,
-                          float* fdv, hipStream_t s) {
+template <int LJ, int NH, bool ST, int PR>
+static int geo_fwd_launch_t(const GeoArgs& a, const float* pts, const float* randn, int64_t n,
+                            const void* table, const float* params, float* sdf, float* grad,
+                            float* normal, float* feature, float* laplace, float* fdv,
+                            float* store, hipStream_t s) {
     const size_t lds = (size_t)a.lds_floats * sizeof(float);
-    AM_HIP(hipFuncSetAttribute((const void*)k_geo_fwd<LJ, NH>,
+    AM_HIP(hipFuncSetAttribute((const void*)k_geo_fwd<LJ, NH, ST, PR>,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     {
         KernelTimer kt_("geo_fwd", s);
-        hipLaunchKernelGGL((k_geo_fwd<LJ, NH>), dim3(field_blocks(n)), dim3(256), lds, s, a, pts,
-                           randn, n, table, params, sdf, grad, normal, feature, laplace, fdv);
+        hipLaunchKernelGGL((k_geo_fwd<LJ, NH, ST, PR>), dim3(field_blocks(n)), dim3(256), lds, s,
+                           a, pts, randn, n, table, params, sdf, grad, normal, feature, laplace,
+                           fdv, store);
     }
     AM_LAUNCH_CHECK();
     return AM_OK;
+}
+
+template <int LJ, int NH>
+static int geo_fwd_launch(const GeoArgs& a, const float* pts, const float* randn, int64_t n,
+                          const void* table, const float* params, float* sdf, float* grad,
+                          float* normal, float* feature, float* laplace, float* fdv, float* store,
+                          hipStream_t s) {
+    if constexpr (NH == 1) {
+        if (a.prec == AM_PREC_MIXED) {
+            if (store)
+                return geo_fwd_launch_t<LJ, NH, true, mlp::kBF16>(
+                    a, pts, randn, n, table, params, sdf, grad, normal, feature, laplace, fdv,
+                    store, s);
+            return geo_fwd_launch_t<LJ, NH, false, mlp::kBF16>(
+                a, pts, randn, n, table, params, sdf, grad, normal, feature, laplace, fdv, nullptr,
+                s);
+        }
+    }
+    if (store)
+        return geo_fwd_launch_t<LJ, NH, true, mlp::kF32>(a, pts, randn, n, table, params, sdf,
+                                                         grad, normal, feature, laplace, fdv,
+                                                         store, s);
+    return geo_fwd_launch_t<LJ, NH, false, mlp::kF32>(a, pts, randn, n, table, params, sdf, grad,
+                                                      normal, feature, laplace, fdv, nullptr, s);
 }
 
 template <int LJ>
@@ -905,23 +1155,25 @@ static void carve_geo_bwd_head(WsCarver& c, const GeoArgs& a, int64_t n, bool ne
     w->scan_tmp = c.take<char>(sb + 256);
 }
 
-// Most pairs a tile can emit: every evaluation, every level slot, every
-// lane unmerged.
+// Most pairs a tile can emit, per lane and level slot: FD cluster 6 x 8
+// (the base point is always in B) + 8 for B; curvature cluster 6 x 8, or
+// 5 x 8 + 8 when B is touched; base only: 8.  All within n_evals x 8.
 static int64_t geo_max_pairs(const GeoArgs& a, int LJ, int64_t n) {
     return (n + 15) / 16 * ((int64_t)a.n_evals * LJ * 8 * 64);
 }
 
-template <int LJ, int NH>
-static int geo_bwd_launch(const GeoArgs& a, const am_grid_desc* g, const am_mlp_desc* m,
-                          const am_sdf_desc* sd, const float* pts, const float* randn, int64_t n,
-                          const float* table, const float* params, const float* fdv_in,
-                          const float* g_sdf, const float* g_grad, const float* g_normal,
-                          const float* g_feature, const float* g_laplace, float* dtable,
-                          float* dparams, void* ws, size_t ws_bytes, hipStream_t s) {
+template <int LJ, int NH, int PR>
+static int geo_bwd_launch_t(const GeoArgs& a, const am_grid_desc* g, const float* pts,
+                            const float* randn, int64_t n, const void* table, const float* params,
+                            const float* fdv_in,
+                          const float* store, const float* g_sdf, const float* g_grad,
+                          const float* g_normal, const float* g_feature, const float* g_laplace,
+                          float* dtable, float* dparams, void* ws, size_t ws_bytes, hipStream_t s) {
     const int blocks = field_blocks(n);
     const int waves = blocks * 4;
     const int64_t tiles = (n + 15) / 16;
     const bool need_fdv = a.with_grad && fdv_in == nullptr;
+    AM_CHECK_ARG(!store || !need_fdv, "a stored encoding needs the stored FD values too");
     WsCarver c(ws, ws_bytes);
     GeoBwdWs w;
     carve_geo_bwd_head(c, a, n, need_fdv, &w);
@@ -932,7 +1184,7 @@ static int geo_bwd_launch(const GeoArgs& a, const am_grid_desc* g, const am_mlp_
     const float* fdv = fdv_in;
     if (need_fdv) {  // recompute the stored SDF values with a forward pass
         if (int rc = geo_fwd_launch<LJ, NH>(a, pts, randn, n, table, params, w.sdf_tmp, nullptr,
-                                            nullptr, nullptr, nullptr, w.fdv, s))
+                                            nullptr, nullptr, nullptr, w.fdv, nullptr, s))
             return rc;
         fdv = w.fdv;
     }
@@ -950,7 +1202,9 @@ static int geo_bwd_launch(const GeoArgs& a, const am_grid_desc* g, const am_mlp_
     // 3. the pair arrays are carved for the worst case (the size query's
     // bound); the backward writes each tile's pairs at its offset
     TableReduceWs tw;
-    if (int rc = table_reduce_carve(c, 2, geo_max_pairs(a, LJ, n), g->n_entries, &tw)) return rc;
+    if (int rc = table_reduce_carve(c, 2, geo_max_pairs(a, LJ, n), g->n_entries, &tw,
+                                    PR == mlp::kBF16))
+        return rc;
     if (!c.ok()) {
         set_error("workspace too small: need %zu, got %zu", c.used, ws_bytes);
         return AM_ERR_WORKSPACE;
@@ -958,13 +1212,14 @@ static int geo_bwd_launch(const GeoArgs& a, const am_grid_desc* g, const am_mlp_
     const size_t lds = (size_t)(a.lds_floats + 4 * 128 * mlp::kScratchLd + 4 * 7 * LJ * 128) *
                        sizeof(float);
     AM_CHECK_ARG(lds <= 160 * 1024, "geometry backward needs %zu bytes of LDS", lds);
-    AM_HIP(hipFuncSetAttribute((const void*)k_geo_bwd<LJ, NH>,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    auto kern = store ? k_geo_bwd<LJ, NH, true, PR> : k_geo_bwd<LJ, NH, false, PR>;
+    AM_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)lds));
     {
         KernelTimer kt_("geo_bwd", s);
-        hipLaunchKernelGGL((k_geo_bwd<LJ, NH>), dim3(blocks), dim3(256), lds, s, a, pts, randn, n,
-                           table, params, fdv, g_sdf, g_grad, g_normal, g_feature, g_laplace,
-                           tw.keys[0], reinterpret_cast<Vals<2>*>(tw.vals[0]), w.offsets, w.counts,
+        hipLaunchKernelGGL(kern, dim3(blocks), dim3(256), lds, s, a, pts, randn, n, table, params,
+                           fdv, store, g_sdf, g_grad, g_normal, g_feature, g_laplace, tw.keys[0],
+                           reinterpret_cast<PairV<PR>*>(tw.vals[0]), w.offsets, w.counts,
                            w.status, w.partial);
     }
     AM_LAUNCH_CHECK();
@@ -981,6 +1236,25 @@ static int geo_bwd_launch(const GeoArgs& a, const am_grid_desc* g, const am_mlp_
     return table_reduce_run(tw, 2, g->n_entries, dtable, s);
 }
 
+template <int LJ, int NH>
+static int geo_bwd_launch(const GeoArgs& a, const am_grid_desc* g, const am_mlp_desc* m,
+                          const am_sdf_desc* sd, const float* pts, const float* randn, int64_t n,
+                          const void* table, const float* params, const float* fdv_in,
+                          const float* store, const float* g_sdf, const float* g_grad,
+                          const float* g_normal, const float* g_feature, const float* g_laplace,
+                          float* dtable, float* dparams, void* ws, size_t ws_bytes, hipStream_t s) {
+    if constexpr (NH == 1) {
+        if (a.prec == AM_PREC_MIXED)
+            return geo_bwd_launch_t<LJ, NH, mlp::kBF16>(a, g, pts, randn, n, table, params, fdv_in,
+                                                        store, g_sdf, g_grad, g_normal, g_feature,
+                                                        g_laplace, dtable, dparams, ws, ws_bytes,
+                                                        s);
+    }
+    return geo_bwd_launch_t<LJ, NH, mlp::kF32>(a, g, pts, randn, n, table, params, fdv_in, store,
+                                               g_sdf, g_grad, g_normal, g_feature, g_laplace,
+                                               dtable, dparams, ws, ws_bytes, s);
+}
+
 #define AM_GEO_DISPATCH(FN, ...)                                                   \
     do {                                                                           \
         const bool two = m->n_hidden_layers == 2;                                  \
@@ -994,10 +1268,20 @@ static int geo_bwd_launch(const GeoArgs& a, const am_grid_desc* g, const am_mlp_
 
 using namespace am;
 
+extern "C" int64_t am_sdf_store_floats(const am_grid_desc* g, const am_mlp_desc* m,
+                                       const am_sdf_desc* sd, int64_t n) {
+    GeoArgs a;
+    int LJ;
+    if (make_geo_args(g, m, sd, &a, &LJ) || n < 0) return -1;
+    const int64_t tiles = (n + 15) / 16;
+    return geo_enc_floats(tiles, a.n_evals, LJ) + geo_dx_floats(tiles, a.with_laplace, LJ);
+}
+
 extern "C" int am_sdf_fwd(const am_grid_desc* g, const am_mlp_desc* m, const am_sdf_desc* sd,
-                          const float* pts, const float* randn, int64_t n, const float* table,
+                          const float* pts, const float* randn, int64_t n, const void* table,
                           const float* params, float* sdf, float* grad, float* normal,
-                          float* feature, float* laplace, float* fd_values, void* stream) {
+                          float* feature, float* laplace, float* fd_values, float* enc_store,
+                          void* stream) {
     GeoArgs a;
     int LJ;
     if (int rc = make_geo_args(g, m, sd, &a, &LJ)) return rc;
@@ -1007,7 +1291,7 @@ extern "C" int am_sdf_fwd(const am_grid_desc* g, const am_mlp_desc* m, const am_
     AM_CHECK_ARG(!sd->with_laplace || randn, "laplace needs randn");
     hipStream_t s = to_stream(stream);
     AM_GEO_DISPATCH(geo_fwd_launch, a, pts, randn, n, table, params, sdf, grad, normal, feature,
-                    laplace, fd_values, s);
+                    laplace, fd_values, enc_store, s);
 }
 
 extern "C" size_t am_sdf_bwd_workspace_size(const am_grid_desc* g, const am_mlp_desc* m,
@@ -1024,11 +1308,11 @@ extern "C" size_t am_sdf_bwd_workspace_size(const am_grid_desc* g, const am_mlp_
 }
 
 extern "C" int am_sdf_bwd(const am_grid_desc* g, const am_mlp_desc* m, const am_sdf_desc* sd,
-                          const float* pts, const float* randn, int64_t n, const float* table,
-                          const float* params, const float* fd_values, const float* g_sdf,
-                          const float* g_grad, const float* g_normal, const float* g_feature,
-                          const float* g_laplace, float* dtable, float* dparams, void* workspace,
-                          size_t workspace_bytes, void* stream) {
+                          const float* pts, const float* randn, int64_t n, const void* table,
+                          const float* params, const float* fd_values, const float* enc_store,
+                          const float* g_sdf, const float* g_grad, const float* g_normal,
+                          const float* g_feature, const float* g_laplace, float* dtable,
+                          float* dparams, void* workspace, size_t workspace_bytes, void* stream) {
     GeoArgs a;
     int LJ;
     if (int rc = make_geo_args(g, m, sd, &a, &LJ)) return rc;
@@ -1041,7 +1325,7 @@ extern "C" int am_sdf_bwd(const am_grid_desc* g, const am_mlp_desc* m, const am_
     }
     AM_CHECK_ARG(pts, "NULL pts");
     AM_CHECK_ARG(!sd->with_laplace || randn, "laplace needs randn");
-    AM_GEO_DISPATCH(geo_bwd_launch, a, g, m, sd, pts, randn, n, table, params, fd_values, g_sdf,
-                    g_grad, g_normal, g_feature, g_laplace, dtable, dparams, workspace,
-                    workspace_bytes, s);
+    AM_GEO_DISPATCH(geo_bwd_launch, a, g, m, sd, pts, randn, n, table, params, fd_values,
+                    enc_store, g_sdf, g_grad, g_normal, g_feature, g_laplace, dtable, dparams,
+                    workspace, workspace_bytes, s);
 }

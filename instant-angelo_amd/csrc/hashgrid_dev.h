@@ -9,6 +9,7 @@
 // order.  Every step is a single rounded fp32 op (-ffp-contract=off), so the
 // indices and positions are bit-exact with the C restatement in the oracle.
 #pragma once
+#include <hip/hip_fp16.h>
 #include "am_common.h"
 
 namespace am {
@@ -87,16 +88,23 @@ __device__ __forceinline__ uint32_t corner_row(const LevelParams& lp, const Leve
                                 p.c[2] + ((c >> 2) & 1));
 }
 
+// A table row of two features as fp32 (fp32 storage, or fp16 storage of
+// the mixed-precision form: the blend stays fp32).
+__device__ __forceinline__ float2 load_row2(const float2* __restrict__ t, uint32_t r) { return t[r]; }
+__device__ __forceinline__ float2 load_row2(const __half2* __restrict__ t, uint32_t r) {
+    return __half22float2(t[r]);
+}
+
 // Two-feature level encode (the configuration on the NeuS path).  Optionally
 // returns d enc_f / d x_d (including the level scale) for input gradients.
-template <bool WITH_DX>
+template <bool WITH_DX, class TT>
 __device__ __forceinline__ void encode_level_f2(const LevelParams& lp, const float x[3],
-                                                const float2* __restrict__ table, float* out2,
+                                                const TT* __restrict__ table, float* out2,
                                                 float (*dx)[3]) {
     const LevelPos p = level_pos(lp, x);
     float2 v[8];
 #pragma unroll
-    for (int c = 0; c < 8; ++c) v[c] = table[corner_row(lp, p, c)];
+    for (int c = 0; c < 8; ++c) v[c] = load_row2(table, corner_row(lp, p, c));
     float a0 = 0.0f, a1 = 0.0f;
 #pragma unroll
     for (int c = 0; c < 8; ++c) {

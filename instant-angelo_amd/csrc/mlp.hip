@@ -18,6 +18,10 @@ static int make_mlp_args(const am_mlp_desc* m, MlpArgs* a) {
     AM_CHECK_ARG(m != nullptr, "mlp descriptor is NULL");
     AM_CHECK_ARG(m->n_neurons == 64, "n_neurons %d: only 64 is implemented", m->n_neurons);
     AM_CHECK_ARG(m->n_in >= 1 && m->n_in <= 64, "n_in %d out of [1,64]", m->n_in);
+    if (m->precision != AM_PREC_FP32) {
+        set_error("the stand-alone MLP runs in fp32 (precision %d)", m->precision);
+        return AM_ERR_UNSUPPORTED;
+    }
     AM_CHECK_ARG(m->n_out >= 1 && m->n_out <= 64, "n_out %d out of [1,64]", m->n_out);
     AM_CHECK_ARG(m->n_hidden_layers >= 1 && m->n_hidden_layers <= 4, "n_hidden_layers %d",
                  m->n_hidden_layers);

@@ -15,6 +15,13 @@
 // Fragments are read through FragF (plain float arrays) or FragV (D tiles,
 // fragment s = tile s/4, register s%4) so register arrays are never
 // reinterpreted through pointers (which would push them to scratch).
+//
+// BF16 form (PR = kBF16): v_mfma_f32_16x16x16_bf16 takes in lane group k
+// the 4 operand values k' = 4k..4k+3 of a 16-deep K step, and its D layout
+// is the fp32 one.  Four consecutive fp32 K-steps 4u..4u+3 of a lane group
+// therefore become one bf16 step with the values packed in step order: the
+// same row space, weights and fragments, the products summed in fp32.
+// Operands are rounded to bf16 (round to nearest even, v_cvt_pk_bf16_f32).
 #pragma once
 #include "am_common.h"
 
@@ -22,6 +29,24 @@ namespace am {
 namespace mlp {
 
 __device__ __forceinline__ int krow(int s, int k) { return ((s >> 2) << 4) + (k << 2) + (s & 3); }
+
+constexpr int kF32 = 0;
+constexpr int kBF16 = 1;
+
+typedef short s4v __attribute__((ext_vector_type(4)));
+typedef __bf16 bf4v __attribute__((ext_vector_type(4)));
+
+// One 16-deep bf16 step: a[q], b[q] are this lane's operands of fp32 K-step
+// 4u + q.
+__device__ __forceinline__ v4f mfma_bf16_step(v4f a, v4f b, v4f c) {
+    const bf4v ab = __builtin_convertvector(a, bf4v);
+    const bf4v bb = __builtin_convertvector(b, bf4v);
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(s4v, ab),
+                                                     __builtin_bit_cast(s4v, bb), c, 0, 0, 0);
+}
+
+// Value as the bf16 MFMA sees it (round to nearest even).
+__device__ __forceinline__ float round_bf16(float x) { return (float)(__bf16)x; }
 
 struct FragF {
     const float* p;
@@ -86,33 +111,67 @@ __device__ __forceinline__ v4f bias_tile(const float* __restrict__ b, int t, int
 }
 
 // acc[ot] += W[16ot + i][krow(s,k)] * b[s]
-template <int KS, int OT, class Frag>
+template <int KS, int OT, class Frag, int PR = kF32>
 __device__ __forceinline__ void mm_fwd(const float* __restrict__ W, int ld, Frag b, v4f* acc) {
     const int l = lane_id();
     const int i = l & 15, k = l >> 4;
+    if constexpr (PR == kF32) {
 #pragma unroll
-    for (int s = 0; s < KS; ++s) {
-        const float bs = b[s];
+        for (int s = 0; s < KS; ++s) {
+            const float bs = b[s];
 #pragma unroll
-        for (int ot = 0; ot < OT; ++ot) {
-            const float a = W[(16 * ot + i) * ld + krow(s, k)];
-            acc[ot] = mfma4(a, bs, acc[ot]);
+            for (int ot = 0; ot < OT; ++ot) {
+                const float a = W[(16 * ot + i) * ld + krow(s, k)];
+                acc[ot] = mfma4(a, bs, acc[ot]);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int u = 0; u < (KS + 3) / 4; ++u) {
+            v4f bv;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) bv[q] = 4 * u + q < KS ? b[4 * u + q] : 0.0f;
+#pragma unroll
+            for (int ot = 0; ot < OT; ++ot) {
+                v4f av;
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    av[q] = 4 * u + q < KS ? W[(16 * ot + i) * ld + krow(4 * u + q, k)] : 0.0f;
+                acc[ot] = mfma_bf16_step(av, bv, acc[ot]);
+            }
         }
     }
 }
 
 // gin[it] += W[krow(s,k)][16it + i] * g[s]   (W^T * G)
-template <int KS, int IT, class Frag>
+template <int KS, int IT, class Frag, int PR = kF32>
 __device__ __forceinline__ void mm_bwd_in(const float* __restrict__ W, int ld, Frag g, v4f* gin) {
     const int l = lane_id();
     const int i = l & 15, k = l >> 4;
+    if constexpr (PR == kF32) {
 #pragma unroll
-    for (int s = 0; s < KS; ++s) {
-        const float gs = g[s];
+        for (int s = 0; s < KS; ++s) {
+            const float gs = g[s];
 #pragma unroll
-        for (int it = 0; it < IT; ++it) {
-            const float a = W[krow(s, k) * ld + 16 * it + i];
-            gin[it] = mfma4(a, gs, gin[it]);
+            for (int it = 0; it < IT; ++it) {
+                const float a = W[krow(s, k) * ld + 16 * it + i];
+                gin[it] = mfma4(a, gs, gin[it]);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int u = 0; u < (KS + 3) / 4; ++u) {
+            v4f gv;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) gv[q] = 4 * u + q < KS ? g[4 * u + q] : 0.0f;
+#pragma unroll
+            for (int it = 0; it < IT; ++it) {
+                v4f av;
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    av[q] = 4 * u + q < KS ? W[krow(4 * u + q, k) * ld + 16 * it + i] : 0.0f;
+                gin[it] = mfma_bf16_step(av, gv, gin[it]);
+            }
         }
     }
 }
@@ -123,7 +182,7 @@ constexpr int kScratchLd = 17;
 // input row space) summed over the 16 points of the tile.  Transposes go
 // through the wave-private LDS scratch (16*(OT+IT) rows x 17).
 // Result layout: lane (i,k), register r -> dW[16ot + 4k + r][16it + i].
-template <int OT, int IT, int KS, class Frag>
+template <int OT, int IT, int KS, class Frag, int PR = kF32>
 __device__ __forceinline__ void mm_dw(const v4f* G, Frag X, float* scratch, v4f* dW) {
     const int l = lane_id();
     const int i = l & 15, k = l >> 4;
@@ -141,11 +200,21 @@ __device__ __forceinline__ void mm_dw(const v4f* G, Frag X, float* scratch, v4f*
 #pragma unroll
         for (int it = 0; it < IT; ++it) {
             v4f acc = dW[ot * IT + it];
+            if constexpr (PR == kF32) {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const float a = Gs[(16 * ot + i) * kScratchLd + 4 * q + k];
-                const float b = Xs[(16 * it + i) * kScratchLd + 4 * q + k];
-                acc = mfma4(a, b, acc);
+                for (int q = 0; q < 4; ++q) {
+                    const float a = Gs[(16 * ot + i) * kScratchLd + 4 * q + k];
+                    const float b = Xs[(16 * it + i) * kScratchLd + 4 * q + k];
+                    acc = mfma4(a, b, acc);
+                }
+            } else {
+                v4f av, bv;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    av[q] = Gs[(16 * ot + i) * kScratchLd + 4 * q + k];
+                    bv[q] = Xs[(16 * it + i) * kScratchLd + 4 * q + k];
+                }
+                acc = mfma_bf16_step(av, bv, acc);
             }
             dW[ot * IT + it] = acc;
         }

@@ -10,9 +10,10 @@
 #include "am_common.h"
 
 extern "C" int am_sdf_fwd(const am_grid_desc* g, const am_mlp_desc* m, const am_sdf_desc* sd,
-                          const float* pts, const float* randn, int64_t n, const float* table,
+                          const float* pts, const float* randn, int64_t n, const void* table,
                           const float* params, float* sdf, float* grad, float* normal,
-                          float* feature, float* laplace, float* fd_values, void* stream);
+                          float* feature, float* laplace, float* fd_values, float* enc_store,
+                          void* stream);
 
 namespace am {
 
@@ -106,7 +107,7 @@ extern "C" size_t am_occ_update_workspace_size(int32_t grid_res, int64_t m) {
 }
 
 extern "C" int am_occ_update(const am_grid_desc* g, const am_mlp_desc* mlp, const am_sdf_desc* s,
-                             const float* table, const float* params, const float* inv_s,
+                             const void* table, const float* params, const float* inv_s,
                              float step, const int32_t* cells, const float* jitter, int64_t m,
                              int32_t grid_res, float ema_decay, float occ_thre, float* occs,
                              uint8_t* binary, void* workspace, size_t workspace_bytes,
@@ -143,7 +144,7 @@ extern "C" int am_occ_update(const am_grid_desc* g, const am_mlp_desc* mlp, cons
         AM_LAUNCH_CHECK();
         TimerAlias alias("occ_sdf");
         if (int rc = am_sdf_fwd(g, mlp, &sd, pts, nullptr, m, table, params, sdf, nullptr, nullptr,
-                                nullptr, nullptr, nullptr, stream))
+                                nullptr, nullptr, nullptr, nullptr, stream))
             return rc;
         hipLaunchKernelGGL(k_occ_alpha, dim3((unsigned)ceil_div(m, 256)), dim3(256), 0, st, cells,
                            sdf, m, inv_s, step * 0.5f, best);

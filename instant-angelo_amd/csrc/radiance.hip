@@ -102,7 +102,7 @@ struct RadEval {
     v4f out;
 };
 
-template <int KS, int NH>
+template <int KS, int NH, int PR>
 __device__ __forceinline__ void rad_eval(const RadArgs& a, const float* sm,
                                          const float* __restrict__ feature,
                                          const float* __restrict__ rays,
@@ -139,22 +139,22 @@ __device__ __forceinline__ void rad_eval(const RadArgs& a, const float* sm,
     }
 #pragma unroll
     for (int t = 0; t < 4; ++t) e.z1[t] = mlp::bias_tile(sm + a.b1, t, 64);
-    mlp::mm_fwd<KS, 4>(sm + a.w1, a.ld1, mlp::FragF{e.xb}, e.z1);
+    mlp::mm_fwd<KS, 4, mlp::FragF, PR>(sm + a.w1, a.ld1, mlp::FragF{e.xb}, e.z1);
     v4f h[4];
 #pragma unroll
     for (int t = 0; t < 4; ++t) h[t] = mlp::act4(a.act, e.z1[t], a.beta);
     if constexpr (NH == 2) {
 #pragma unroll
         for (int t = 0; t < 4; ++t) e.z2[t] = mlp::bias_tile(sm + a.b2, t, 64);
-        mlp::mm_fwd<16, 4>(sm + a.w2, 65, mlp::FragV{h}, e.z2);
+        mlp::mm_fwd<16, 4, mlp::FragV, PR>(sm + a.w2, 65, mlp::FragV{h}, e.z2);
 #pragma unroll
         for (int t = 0; t < 4; ++t) h[t] = mlp::act4(a.act, e.z2[t], a.beta);
     }
     e.out = mlp::bias_tile(sm + a.bo, 0, a.n_out);
-    mlp::mm_fwd<16, 1>(sm + a.wo, 65, mlp::FragV{h}, &e.out);
+    mlp::mm_fwd<16, 1, mlp::FragV, PR>(sm + a.wo, 65, mlp::FragV{h}, &e.out);
 }
 
-template <int KS, int NH>
+template <int KS, int NH, int PR>
 __global__ void __launch_bounds__(256) k_rad_fwd(RadArgs a, const float* __restrict__ feature,
                                                  const float* __restrict__ rays,
                                                  const int32_t* __restrict__ ridx,
@@ -172,7 +172,7 @@ __global__ void __launch_bounds__(256) k_rad_fwd(RadArgs a, const float* __restr
         const int64_t n = tile * 16 + p;
         const bool valid = n < N;
         RadEval<KS, NH> e;
-        rad_eval<KS, NH>(a, sm, feature, rays, ridx, normal, n, valid, e);
+        rad_eval<KS, NH, PR>(a, sm, feature, rays, ridx, normal, n, valid, e);
         if (valid && k == 0)
 #pragma unroll
             for (int r = 0; r < 4; ++r)
@@ -180,7 +180,7 @@ __global__ void __launch_bounds__(256) k_rad_fwd(RadArgs a, const float* __restr
     }
 }
 
-template <int KS, int NH>
+template <int KS, int NH, int PR>
 __global__ void __launch_bounds__(256) k_rad_bwd(RadArgs a, const float* __restrict__ feature,
                                                  const float* __restrict__ rays,
                                                  const int32_t* __restrict__ ridx,
@@ -213,7 +213,7 @@ __global__ void __launch_bounds__(256) k_rad_bwd(RadArgs a, const float* __restr
         const int64_t n = tile * 16 + p;
         const bool valid = n < N;
         RadEval<KS, NH> e;
-        rad_eval<KS, NH>(a, sm, feature, rays, ridx, normal, n, valid, e);
+        rad_eval<KS, NH, PR>(a, sm, feature, rays, ridx, normal, n, valid, e);
         v4f go;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
@@ -229,12 +229,12 @@ __global__ void __launch_bounds__(256) k_rad_bwd(RadArgs a, const float* __restr
         const v4f* zl = NH == 2 ? e.z2 : e.z1;
 #pragma unroll
         for (int t = 0; t < 4; ++t) h[t] = mlp::act4(a.act, zl[t], a.beta);
-        mlp::mm_dw<1, 4, 16>(&go, mlp::FragV{h}, scratch, dWo);
+        mlp::mm_dw<1, 4, 16, mlp::FragV, PR>(&go, mlp::FragV{h}, scratch, dWo);
         dbo += go;
         v4f gh[4];
 #pragma unroll
         for (int t = 0; t < 4; ++t) gh[t] = v4f{0, 0, 0, 0};
-        mlp::mm_bwd_in<4, 4>(sm + a.wo, 65, mlp::FragV{&go}, gh);
+        mlp::mm_bwd_in<4, 4, mlp::FragV, PR>(sm + a.wo, 65, mlp::FragV{&go}, gh);
         if constexpr (NH == 2) {
             v4f gz2[4], h1[4];
 #pragma unroll
@@ -244,10 +244,10 @@ __global__ void __launch_bounds__(256) k_rad_bwd(RadArgs a, const float* __restr
                 h1[t] = mlp::act4(a.act, e.z1[t], a.beta);
                 db2[t] += gz2[t];
             }
-            mlp::mm_dw<4, 4, 16>(gz2, mlp::FragV{h1}, scratch, dW2);
+            mlp::mm_dw<4, 4, 16, mlp::FragV, PR>(gz2, mlp::FragV{h1}, scratch, dW2);
 #pragma unroll
             for (int t = 0; t < 4; ++t) gh[t] = v4f{0, 0, 0, 0};
-            mlp::mm_bwd_in<16, 4>(sm + a.w2, 65, mlp::FragV{gz2}, gh);
+            mlp::mm_bwd_in<16, 4, mlp::FragV, PR>(sm + a.w2, 65, mlp::FragV{gz2}, gh);
         }
         v4f gz1[4];
 #pragma unroll
@@ -256,11 +256,11 @@ __global__ void __launch_bounds__(256) k_rad_bwd(RadArgs a, const float* __restr
             for (int r = 0; r < 4; ++r) gz1[t][r] = gh[t][r] * mlp::act_bwd(a.act, e.z1[t][r], a.beta);
             db1[t] += gz1[t];
         }
-        mlp::mm_dw<4, IT1, KS>(gz1, mlp::FragF{e.xb}, scratch, dW1);
+        mlp::mm_dw<4, IT1, KS, mlp::FragF, PR>(gz1, mlp::FragF{e.xb}, scratch, dW1);
         v4f gin[IT1];
 #pragma unroll
         for (int t = 0; t < IT1; ++t) gin[t] = v4f{0, 0, 0, 0};
-        mlp::mm_bwd_in<16, IT1>(sm + a.w1, a.ld1, mlp::FragV{gz1}, gin);
+        mlp::mm_bwd_in<16, IT1, mlp::FragV, PR>(sm + a.w1, a.ld1, mlp::FragV{gz1}, gin);
         if (valid) {
 #pragma unroll
             for (int t = 0; t < IT1; ++t)
@@ -371,23 +371,23 @@ static int make_rad_args(const am_mlp_desc* m, int sh_degree, int fdim, RadArgs*
     return AM_OK;
 }
 
-template <int KS, int NH>
+template <int KS, int NH, int PR>
 static int rad_fwd_launch(const RadArgs& a, const float* feature, const float* rays,
                           const int32_t* ridx, const float* normal, int64_t n,
                           const float* params, float* rgb, hipStream_t s) {
     const size_t lds = (size_t)a.lds_floats * sizeof(float);
-    AM_HIP(hipFuncSetAttribute((const void*)k_rad_fwd<KS, NH>,
+    AM_HIP(hipFuncSetAttribute((const void*)k_rad_fwd<KS, NH, PR>,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     {
         KernelTimer kt_("rad_fwd", s);
-        hipLaunchKernelGGL((k_rad_fwd<KS, NH>), dim3(field_blocks(n)), dim3(256), lds, s, a, feature,
+        hipLaunchKernelGGL((k_rad_fwd<KS, NH, PR>), dim3(field_blocks(n)), dim3(256), lds, s, a, feature,
                            rays, ridx, normal, n, params, rgb);
     }
     AM_LAUNCH_CHECK();
     return AM_OK;
 }
 
-template <int KS, int NH>
+template <int KS, int NH, int PR>
 static int rad_bwd_launch(const RadArgs& a, const float* feature, const float* rays,
                           const int32_t* ridx, const float* normal, int64_t n,
                           const float* params, const float* g_rgb, float* g_feature,
@@ -403,25 +403,32 @@ static int rad_bwd_launch(const RadArgs& a, const float* feature, const float* r
         return AM_ERR_WORKSPACE;
     }
     const size_t lds = (size_t)(a.lds_floats + 4 * 128 * mlp::kScratchLd) * sizeof(float);
-    AM_HIP(hipFuncSetAttribute((const void*)k_rad_bwd<KS, NH>,
+    AM_HIP(hipFuncSetAttribute((const void*)k_rad_bwd<KS, NH, PR>,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     {
         KernelTimer kt_("rad_bwd", s);
-        hipLaunchKernelGGL((k_rad_bwd<KS, NH>), dim3(blocks), dim3(256), lds, s, a, feature, rays, ridx,
+        hipLaunchKernelGGL((k_rad_bwd<KS, NH, PR>), dim3(blocks), dim3(256), lds, s, a, feature, rays, ridx,
                            normal, n, params, g_rgb, g_feature, g_normal, partial);
     }
     AM_LAUNCH_CHECK();
     return colsum_f32(partial, waves, a.n_params, dparams, ctmp, s);
 }
 
-#define AM_RAD_DISPATCH(FN, ...)                                                   \
-    do {                                                                           \
-        const bool two = m->n_hidden_layers == 2;                                  \
-        switch (KS) {                                                              \
-            case 8: return two ? FN<8, 2>(__VA_ARGS__) : FN<8, 1>(__VA_ARGS__);    \
-            case 10: return two ? FN<10, 2>(__VA_ARGS__) : FN<10, 1>(__VA_ARGS__); \
-            default: return two ? FN<12, 2>(__VA_ARGS__) : FN<12, 1>(__VA_ARGS__); \
-        }                                                                          \
+#define AM_RAD_DISPATCH_PR(FN, PR, ...)                                                   \
+    do {                                                                                  \
+        const bool two = m->n_hidden_layers == 2;                                         \
+        switch (KS) {                                                                     \
+            case 8: return two ? FN<8, 2, PR>(__VA_ARGS__) : FN<8, 1, PR>(__VA_ARGS__);    \
+            case 10: return two ? FN<10, 2, PR>(__VA_ARGS__) : FN<10, 1, PR>(__VA_ARGS__); \
+            default: return two ? FN<12, 2, PR>(__VA_ARGS__) : FN<12, 1, PR>(__VA_ARGS__); \
+        }                                                                                 \
+    } while (0)
+
+#define AM_RAD_DISPATCH(FN, ...)                                      \
+    do {                                                              \
+        if (m->precision == AM_PREC_MIXED)                            \
+            AM_RAD_DISPATCH_PR(FN, mlp::kBF16, __VA_ARGS__);          \
+        AM_RAD_DISPATCH_PR(FN, mlp::kF32, __VA_ARGS__);               \
     } while (0)
 
 }  // namespace am

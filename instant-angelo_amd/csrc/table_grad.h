@@ -5,6 +5,7 @@
 // The emitters write pairs into carve.keys[0] / carve.vals[0].  Keys equal to
 // sentinel() mark empty slots; they sort last and are skipped.
 #pragma once
+#include <hip/hip_fp16.h>
 #include "am_common.h"
 
 namespace am {
@@ -12,6 +13,13 @@ namespace am {
 template <int F>
 struct Vals {
     float v[F];
+};
+
+// Pair values stored in fp16 (the mixed-precision form: loss-scaled
+// gradients, summed in fp32 by the segmented reduce).
+template <int F>
+struct ValsH {
+    __half v[F];
 };
 
 constexpr int kSegChunk = 17;  // items reduced sequentially per thread (odd: LDS banks)
@@ -27,6 +35,7 @@ struct TableReduceWs {
     uint8_t* tail_open;
     int64_t m;
     int end_bit;
+    bool half_vals;  // values are ValsH<F> instead of Vals<F>
 };
 
 // Number of key bits: smallest b with 2^b > n_entries (so the sentinel
@@ -37,7 +46,8 @@ static inline uint32_t table_sentinel(int end_bit) { return (uint32_t)((1ull << 
 // Carves the reduction workspace for m pairs of F features out of `c`.
 // Returns the sort's temp byte count through TableReduceWs (0 when c has no
 // base pointer: size query; the sort size query needs a device).
-int table_reduce_carve(WsCarver& c, int F, int64_t m, uint32_t n_entries, TableReduceWs* out);
+int table_reduce_carve(WsCarver& c, int F, int64_t m, uint32_t n_entries, TableReduceWs* out,
+                       bool half_vals = false);
 
 // Writes v to p[0..n) on `stream` (a fill kernel, safe under graph capture).
 int fill_f32(float* p, int64_t n, float v, hipStream_t stream);

@@ -19,12 +19,11 @@ using Sort9Config = rocprim::radix_sort_config<
                                         rocprim::kernel_config<1024, 8>, 9,
                                         rocprim::block_radix_rank_algorithm::match>>;
 
-template <int F>
+template <class VT>
 static hipError_t sort_pairs(void* tmp, size_t& bytes, TableReduceWs& w, uint32_t** kout,
-                             Vals<F>** vout, hipStream_t s) {
+                             VT** vout, hipStream_t s) {
     rocprim::double_buffer<uint32_t> kb(w.keys[0], w.keys[1]);
-    rocprim::double_buffer<Vals<F>> vb(reinterpret_cast<Vals<F>*>(w.vals[0]),
-                                       reinterpret_cast<Vals<F>*>(w.vals[1]));
+    rocprim::double_buffer<VT> vb(reinterpret_cast<VT*>(w.vals[0]), reinterpret_cast<VT*>(w.vals[1]));
     hipError_t e;
     if (w.end_bit > 24 && w.end_bit <= 27)
         e = rocprim::radix_sort_pairs<Sort9Config>(tmp, bytes, kb, vb, (size_t)w.m, 0u,
@@ -37,11 +36,13 @@ static hipError_t sort_pairs(void* tmp, size_t& bytes, TableReduceWs& w, uint32_
     return e;
 }
 
-int table_reduce_carve(WsCarver& c, int F, int64_t m, uint32_t n_entries, TableReduceWs* out) {
+int table_reduce_carve(WsCarver& c, int F, int64_t m, uint32_t n_entries, TableReduceWs* out,
+                       bool half_vals) {
     TableReduceWs w{};
     w.m = m;
     w.end_bit = table_key_bits(n_entries);
-    const size_t vbytes = (size_t)F * sizeof(float);
+    w.half_vals = half_vals;
+    const size_t vbytes = (size_t)F * (half_vals ? sizeof(__half) : sizeof(float));
     w.keys[0] = c.take<uint32_t>((size_t)m);
     w.keys[1] = c.take<uint32_t>((size_t)m);
     w.vals[0] = c.take<char>((size_t)m * vbytes);
@@ -53,11 +54,16 @@ int table_reduce_carve(WsCarver& c, int F, int64_t m, uint32_t n_entries, TableR
     w.tail_open = c.take<uint8_t>((size_t)nc);
     size_t sb = 0;
     hipError_t e = hipSuccess;
-    switch (F) {
-        case 1: e = sort_pairs<1>(nullptr, sb, w, nullptr, nullptr, 0); break;
-        case 2: e = sort_pairs<2>(nullptr, sb, w, nullptr, nullptr, 0); break;
-        case 4: e = sort_pairs<4>(nullptr, sb, w, nullptr, nullptr, 0); break;
-        case 8: e = sort_pairs<8>(nullptr, sb, w, nullptr, nullptr, 0); break;
+    if (half_vals && F != 2) {
+        set_error("fp16 gradient pairs need n_features = 2 (got %d)", F);
+        return AM_ERR_UNSUPPORTED;
+    }
+    switch (half_vals ? -F : F) {
+        case 1: e = sort_pairs<Vals<1>>(nullptr, sb, w, nullptr, nullptr, 0); break;
+        case 2: e = sort_pairs<Vals<2>>(nullptr, sb, w, nullptr, nullptr, 0); break;
+        case 4: e = sort_pairs<Vals<4>>(nullptr, sb, w, nullptr, nullptr, 0); break;
+        case 8: e = sort_pairs<Vals<8>>(nullptr, sb, w, nullptr, nullptr, 0); break;
+        case -2: e = sort_pairs<ValsH<2>>(nullptr, sb, w, nullptr, nullptr, 0); break;
         default: set_error("unsupported n_features %d", F); return AM_ERR_UNSUPPORTED;
     }
     if (e != hipSuccess) {
@@ -87,9 +93,9 @@ struct SegCfg {
     static constexpr int threads = F <= 2 ? 256 : 64;
 };
 
-template <int F>
+template <int F, class VT>
 __global__ void __launch_bounds__(SegCfg<F>::threads) k_seg_a(
-    const uint32_t* __restrict__ keys, const Vals<F>* __restrict__ vals, int64_t m,
+    const uint32_t* __restrict__ keys, const VT* __restrict__ vals, int64_t m,
     uint32_t n_entries, float* __restrict__ out, float* __restrict__ lead,
     float* __restrict__ tail, uint8_t* __restrict__ lead_full, uint8_t* __restrict__ tail_open) {
     constexpr int kSegThreads = SegCfg<F>::threads;
@@ -99,8 +105,13 @@ __global__ void __launch_bounds__(SegCfg<F>::threads) k_seg_a(
     const int64_t blk_n = m - blk0 < (int64_t)kSegThreads * kSegChunk ? m - blk0
                                                                      : (int64_t)kSegThreads * kSegChunk;
     for (int q = threadIdx.x; q < blk_n; q += kSegThreads) sk[q] = keys[blk0 + q];
-    const float* vf = reinterpret_cast<const float*>(vals) + blk0 * F;
-    for (int q = threadIdx.x; q < blk_n * F; q += kSegThreads) sv[q] = vf[q];
+    if constexpr (sizeof(VT) == F * sizeof(float)) {
+        const float* vf = reinterpret_cast<const float*>(vals) + blk0 * F;
+        for (int q = threadIdx.x; q < blk_n * F; q += kSegThreads) sv[q] = vf[q];
+    } else {
+        const __half* vh = reinterpret_cast<const __half*>(vals) + blk0 * F;
+        for (int q = threadIdx.x; q < blk_n * F; q += kSegThreads) sv[q] = __half2float(vh[q]);
+    }
     __syncthreads();
     const int64_t t = (int64_t)blockIdx.x * kSegThreads + threadIdx.x;
     const int64_t nc = (m + kSegChunk - 1) / kSegChunk;
@@ -183,17 +194,17 @@ int fill_f32(float* p, int64_t n, float v, hipStream_t s) {
     return AM_OK;
 }
 
-template <int F>
+template <int F, class VT>
 static int run_f(TableReduceWs& w, uint32_t n_entries, float* dtable, hipStream_t s) {
     if (int rc = fill_f32(dtable, (int64_t)n_entries * F, 0.0f, s)) return rc;
     if (w.m == 0) return AM_OK;
     uint32_t* ks = nullptr;
-    Vals<F>* vs = nullptr;
+    VT* vs = nullptr;
     size_t sb = w.sort_bytes;
     hipError_t e;
     {
         KernelTimer kt_("radix_sort", s);
-        e = sort_pairs<F>(w.sort_tmp, sb, w, &ks, &vs, s);
+        e = sort_pairs<VT>(w.sort_tmp, sb, w, &ks, &vs, s);
     }
     if (e != hipSuccess) {
         set_error("radix sort failed: %s", hipGetErrorString(e));
@@ -204,7 +215,7 @@ static int run_f(TableReduceWs& w, uint32_t n_entries, float* dtable, hipStream_
     {
         KernelTimer kt_("seg_reduce", s);
         constexpr int T = SegCfg<F>::threads;
-        hipLaunchKernelGGL(k_seg_a<F>, dim3((unsigned)ceil_div(nc, T)), dim3(T), 0, s, ks, vs, w.m,
+        hipLaunchKernelGGL((k_seg_a<F, VT>), dim3((unsigned)ceil_div(nc, T)), dim3(T), 0, s, ks, vs, w.m,
                            n_entries, dtable, w.lead, w.tail, w.lead_full, w.tail_open);
     }
     AM_LAUNCH_CHECK();
@@ -216,11 +227,16 @@ static int run_f(TableReduceWs& w, uint32_t n_entries, float* dtable, hipStream_
 
 int table_reduce_run(TableReduceWs& w, int F, uint32_t n_entries, float* dtable,
                      hipStream_t stream) {
+    if (w.half_vals) {
+        if (F == 2) return run_f<2, ValsH<2>>(w, n_entries, dtable, stream);
+        set_error("fp16 gradient pairs need n_features = 2 (got %d)", F);
+        return AM_ERR_UNSUPPORTED;
+    }
     switch (F) {
-        case 1: return run_f<1>(w, n_entries, dtable, stream);
-        case 2: return run_f<2>(w, n_entries, dtable, stream);
-        case 4: return run_f<4>(w, n_entries, dtable, stream);
-        case 8: return run_f<8>(w, n_entries, dtable, stream);
+        case 1: return run_f<1, Vals<1>>(w, n_entries, dtable, stream);
+        case 2: return run_f<2, Vals<2>>(w, n_entries, dtable, stream);
+        case 4: return run_f<4, Vals<4>>(w, n_entries, dtable, stream);
+        case 8: return run_f<8, Vals<8>>(w, n_entries, dtable, stream);
         default: set_error("unsupported n_features %d", F); return AM_ERR_UNSUPPORTED;
     }
 }

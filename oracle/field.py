@@ -36,7 +36,7 @@ OFFSETS = [[1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1]]
 
 def _scalar_div(x, s, mode):
     if mode == "f32":
-        return x * torch.tensor(np.float32(1.0) / np.float32(s), dtype=x.dtype)
+        return x * torch.tensor(np.float32(1.0) / np.float32(s), dtype=x.dtype, device=x.device)
     return x / s
 
 
@@ -57,7 +57,8 @@ def volume_sdf(points, field, r, eps, with_grad=True, with_laplace=False, randn=
     if not with_grad:
         return res
     e = np.float32(eps) if mode == "f32" else float(eps)
-    offsets = torch.tensor(OFFSETS, dtype=dt) * torch.tensor(e, dtype=dt)
+    offsets = (torch.tensor(OFFSETS, dtype=dt, device=points.device)
+               * torch.tensor(e, dtype=dt, device=points.device))
     pd = (points[..., None, :] + offsets).clamp(-r, r)
     xd = contract(pd, r, mode)
     fd = field(xd.view(-1, 3))[..., 0].view(-1, 6)
@@ -69,7 +70,7 @@ def volume_sdf(points, field, r, eps, with_grad=True, with_laplace=False, randn=
     rd = F.normalize(randn, dim=-1)
     t = torch.linalg.cross(normal, rd, dim=-1)
     base = x if recontract else points
-    ps = base.clone() + t * torch.tensor(e, dtype=dt)
+    ps = base.clone() + t * torch.tensor(e, dtype=dt, device=points.device)
     psd = (ps[..., None, :] + offsets).clamp(-r, r)
     xsd = contract(psd, r, mode)
     fsd = field(xsd.view(-1, 3))[..., 0].view(-1, 6)

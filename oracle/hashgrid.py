@@ -94,7 +94,7 @@ def _rows_torch(cell, lt):
     """Level-local rows of the 8 corners from integer cells [n,L,3] (int64),
     uint32 wrap emulated with masks."""
     m = 0xFFFFFFFF
-    out = torch.zeros(cell.shape[0], lt.n_levels, 8, dtype=torch.int64)
+    out = torch.zeros(cell.shape[0], lt.n_levels, 8, dtype=torch.int64, device=cell.device)
     for l in range(lt.n_levels):
         res, size = int(lt.res[l]), int(lt.size[l])
         for c in range(8):
@@ -118,22 +118,23 @@ def encode_torch(x, params, lt, active_levels=None, mode="f32"):
     Levels at or above active_levels are multiplied by zero (the mask)."""
     xs = x.reshape(-1, 3)
     n, L, F = xs.shape[0], lt.n_levels, lt.n_features
-    scale = torch.from_numpy(lt.scale.astype(np.float64)).to(xs.dtype)   # [L]
+    dev = xs.device
+    scale = torch.from_numpy(lt.scale.astype(np.float64)).to(dev, xs.dtype)   # [L]
     if mode == "f32":
         _, frac, idx, _ = corners(xs.detach().cpu().numpy(), lt)
-        frac_t = torch.from_numpy(frac).to(xs.dtype)                     # [n,L,3]
+        frac_t = torch.from_numpy(frac).to(dev, xs.dtype)                # [n,L,3]
         frac_t = frac_t + scale[None, :, None] * (xs - xs.detach())[:, None, :]
-        idx = torch.from_numpy(idx.astype(np.int64))
+        idx = torch.from_numpy(idx.astype(np.int64)).to(dev)
     else:
         pos = scale[None, :, None] * xs[:, None, :] + 0.5
         cell = torch.floor(pos.detach())
         frac_t = pos - cell
         idx = _rows_torch(cell.to(torch.int64), lt)
-    rows = idx + torch.from_numpy(lt.offset.astype(np.int64))[None, :, None]
-    one = torch.ones((), dtype=xs.dtype)
+    rows = idx + torch.from_numpy(lt.offset.astype(np.int64)).to(dev)[None, :, None]
+    one = torch.ones((), dtype=xs.dtype, device=dev)
     outs = []
     for l in range(L):
-        acc = torch.zeros(n, F, dtype=params.dtype)
+        acc = torch.zeros(n, F, dtype=params.dtype, device=dev)
         for c in range(8):
             w = one
             for d in range(3):
@@ -143,7 +144,7 @@ def encode_torch(x, params, lt, active_levels=None, mode="f32"):
         outs.append(acc)
     out = torch.cat(outs, dim=-1)
     if active_levels is not None and active_levels < L:
-        mask = torch.zeros(L * F, dtype=out.dtype)
+        mask = torch.zeros(L * F, dtype=out.dtype, device=dev)
         mask[: active_levels * F] = 1.0
         out = out * mask
     return out.reshape(*x.shape[:-1], L * F)

@@ -53,8 +53,8 @@ def transmittance(alpha, ray_indices, n_rays):
     if L == 0:
         return torch.ones_like(alpha)
     starts = packed[:, 0].long()
-    pos = torch.arange(alpha.shape[0]) - starts[ray_indices.long()]
-    om = torch.ones(n_rays, L + 1, dtype=alpha.dtype)
+    pos = torch.arange(alpha.shape[0], device=alpha.device) - starts[ray_indices.long()]
+    om = torch.ones(n_rays, L + 1, dtype=alpha.dtype, device=alpha.device)
     om = om.index_put((ray_indices.long(), pos + 1), 1.0 - alpha)
     T = torch.cumprod(om, dim=1)
     return T[ray_indices.long(), pos]
@@ -62,7 +62,7 @@ def transmittance(alpha, ray_indices, n_rays):
 
 def accumulate(weights, ray_indices, n_rays, values=None):
     src = weights[:, None] if values is None else weights[:, None] * values
-    out = torch.zeros(n_rays, src.shape[-1], dtype=src.dtype)
+    out = torch.zeros(n_rays, src.shape[-1], dtype=src.dtype, device=src.device)
     return out.index_add(0, ray_indices.long(), src)
 
 
@@ -97,7 +97,8 @@ def neus_forward(rays, ray_indices, t_starts, t_ends, n_rays, field, radiance, i
     normal = geo["normal"]
     rgb = radiance(geo["feature"], t_dirs, normal)
     c = composite(geo["sdf"], normal, rgb, t_dirs, t_starts, t_ends, ridx, n_rays, inv_s, ratio)
-    bg = background if background is not None else torch.ones(3, dtype=rays.dtype)
+    bg = background if background is not None else torch.ones(3, dtype=rays.dtype,
+                                                                 device=rays.device)
     full = c["comp_rgb"] + bg[None, :] * (1.0 - c["opacity"])
     return dict(c, sdf=geo["sdf"], grad=geo["grad"], normal=normal, feature=geo["feature"],
                 laplace=geo["laplace"], rgb=rgb, positions=positions, comp_rgb_full=full,

@@ -78,7 +78,10 @@ def test_neus_step_matches_oracle(dev, n_rays, geo_hidden, L):
     chk(out["sdf_samples"], o["sdf"], 1e-5, "sdf")
     chk(out["sdf_grad_samples"], o["grad"], 1e-3, "grad")
     chk(out["sdf_laplace_samples"], o["laplace"], 5e-3, "laplace")
-    chk(out["weights"], o["weights"], 1e-4, "weights")
+    # per-sample weights depend on the FD normals through the cosine term of
+    # alpha; the normals carry fp32 FD cancellation error (bounded at 1e-3
+    # above), so the weights get 5e-4
+    chk(out["weights"], o["weights"], 5e-4, "weights")
     assert abs(loss.item() - o_loss.item()) / abs(o_loss.item()) < 1e-4
     assert set(["comp_rgb", "comp_normal", "opacity", "depth", "rays_valid", "num_samples",
                 "sdf_samples", "sdf_grad_samples", "weights", "points", "intervals",

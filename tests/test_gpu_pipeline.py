@@ -219,6 +219,34 @@ def test_sdf_bwd_deterministic(dev):
         assert torch.equal(a, b)
 
 
+@pytest.mark.parametrize("laplace", [False, True])
+def test_sdf_store_and_recompute_backward_are_identical(dev, laplace):
+    """The stored-encoding backward reads back the very floats the recompute
+    form gathers again, so both give bit-identical gradients."""
+    from angelo_amd import functions
+    model = _field_model(dev)
+    geo = model.geometry
+    g = torch.Generator().manual_seed(6)
+    n = 4099
+    pts = ((torch.rand(n, 3, generator=g) * 2 - 1) * 0.95).to(dev)
+    randn = torch.randn(n, 3, generator=g).to(dev)
+    grads = {}
+    try:
+        for mode in ("store", "recompute"):
+            functions.set_sdf_backward(mode)
+            model.zero_grad()
+            sdf, grad, normal, feature, lap = geo.fused(pts, True, laplace, randn)
+            loss = sdf.sum() + grad.square().sum() + normal.sum() + feature.sum()
+            if laplace:
+                loss = loss + lap.sum()
+            loss.backward()
+            grads[mode] = [p.grad.clone() for p in geo.parameters()]
+    finally:
+        functions.set_sdf_backward("auto")
+    for a, b in zip(grads["store"], grads["recompute"]):
+        assert torch.equal(a, b)
+
+
 @pytest.mark.parametrize("rad_hidden", [1, 2])
 def test_radiance_fused_fwd_bwd(dev, rad_hidden):
     model = _field_model(dev, rad_hidden=rad_hidden)

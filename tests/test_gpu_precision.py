@@ -1,0 +1,174 @@
+"""The mixed-precision form against the FP64 oracle, beside the FP32 form.
+
+Mixed form: fp16 table rows (fp32 blend), radiance MLP on bf16 MFMA with
+fp32 accumulation (the SDF MLP stays fp32: its central differences divide
+activation rounding by eps), fp16 table-gradient pairs summed in fp32 (the
+upstream gradient carries a loss scale, as in training), Kahan-compensated
+transmittance and compositing, fp64 loss reductions.  Shapes are the
+Neuralangelo step's (16 levels x 2 features, 2^21 rows per level, base 32,
+scale 1.3195..., SDF MLP 1x64, radiance MLP 2x64); 65,536 random points for
+the field and 2,048 rays for the render, as the precision study specifies.
+
+Each test states its error bounds.  The errors are written to
+$ANGELO_REPORT_DIR/precision_report.json when that variable is set.
+"""
+import json
+import os
+
+import pytest
+import torch
+
+from oracle import field as ofield
+from oracle import render as orender
+
+import helpers
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = dict(L=16, log2T=21, base=32, scale=1.3195079107728942)
+LOSS_SCALE = 1024.0
+
+
+def _report(section, values):
+    d = os.environ.get("ANGELO_REPORT_DIR")
+    if not d:
+        return
+    os.makedirs(d, exist_ok=True)
+    path = os.path.join(d, "precision_report.json")
+    rep = json.load(open(path)) if os.path.exists(path) else {}
+    rep[section] = values
+    with open(path, "w") as f:
+        json.dump(rep, f, indent=1, sort_keys=True)
+
+
+def _model(dev):
+    import angelo_amd
+    cfg = helpers.model_config(**SHAPES)
+    model = helpers.randomize_(angelo_amd.make("neus", cfg), seed=11)
+    model.train()
+    model.update_step(0, 30000)
+    return model.to(dev), cfg
+
+
+def _err(a, b):
+    a = a.detach().double().cpu()
+    b = b.detach().double().cpu()
+    d = (a - b).abs()
+    return {"max_abs": float(d.max()), "mean_abs": float(d.mean()),
+            "max_rel_to_max": float(d.max() / b.abs().max().clamp_min(1e-30))}
+
+
+def test_mixed_field_errors_vs_fp64(dev):
+    """SDF, normal and feature of 65,536 points (base + 6 FD evaluations)."""
+    model, _ = _model(dev)
+    geo = model.geometry
+    g = torch.Generator().manual_seed(21)
+    n = 65536
+    pts = (torch.rand(n, 3, generator=g) * 2 - 1) * 0.95
+    field, _, _ = helpers.oracle_fns(model, torch.float64, "f64")
+    eps = float(geo._finite_difference_eps)
+    with torch.no_grad():
+        ref = ofield.volume_sdf(pts.double(), field, 1.0, eps, True, False, None, True, "f64")
+    res = {}
+    for prec in ("fp32", "mixed"):
+        model.set_precision(prec)
+        with torch.no_grad():
+            sdf, grad, normal, feature, _ = geo.fused(pts.to(dev), True, False)
+        res[prec] = {"sdf": _err(sdf, ref["sdf"]), "normal": _err(normal, ref["normal"]),
+                     "feature": _err(feature, ref["feature"]), "grad": _err(grad, ref["grad"])}
+    model.set_precision("fp32")
+    _report("field_65536_points", res)
+    print(json.dumps(res, indent=1))
+    # fp32 form: the plain parity bars
+    assert res["fp32"]["sdf"]["max_abs"] < 1e-5
+    assert res["fp32"]["normal"]["max_abs"] < 2e-3
+    # mixed form: fp16 rows (rel. 2^-11), a fixed rounding of the table
+    assert res["mixed"]["sdf"]["max_abs"] < 5e-4
+    assert res["mixed"]["feature"]["max_abs"] < 1e-3
+    assert res["mixed"]["normal"]["mean_abs"] < 2e-3
+    assert res["mixed"]["normal"]["max_abs"] < 5e-2
+
+
+def test_mixed_render_and_gradient_errors_vs_fp64(dev):
+    """RGB / opacity / depth of 2,048 rays x 32 samples, and the gradients of
+    the training loss (table, SDF MLP, radiance MLP), loss-scaled as in
+    training and unscaled before comparison."""
+    from angelo_amd import scenes
+    from angelo_amd.system import NeuSTrainer
+    from angelo_amd.utils import to_config
+    model, cfg = _model(dev)
+    n_rays = 2048
+    cfg.train_num_rays = n_rays
+    rays, gt = scenes.config0_batch(n_rays=n_rays, n_views=8, seed=23)
+    ridx, t0, t1, packed = scenes.uniform_samples(rays, 32)
+    n = ridx.shape[0]
+    randn = torch.randn(n, 3, generator=torch.Generator().manual_seed(24))
+    field, radiance, ol = helpers.oracle_fns(model, torch.float64, "f64")
+    var = model.variance.variance.detach().cpu().double()
+    inv_s = torch.exp(var * 10.0).clip(1e-6, 1e6)
+    eps = float(model.geometry._finite_difference_eps)
+    o = orender.neus_forward(rays.double(), ridx, t0.double(), t1.double(), n_rays, field,
+                             radiance, inv_s, 1.0, eps, model.cos_anneal_ratio, randn.double())
+    o_loss, _ = orender.losses(o, gt.double(), 10.0, 0.1, 0.5)
+    o_loss.backward()
+    scfg = to_config({"loss": {"lambda_rgb_mse": 10.0, "lambda_eikonal": 0.1,
+                               "lambda_curvature": [0, 0, 0.5, 5000]}})
+    samples = (ridx.to(dev), t0.to(dev), t1.to(dev), packed.to(dev))
+    res = {}
+    for prec in ("fp32", "mixed"):
+        model.set_precision(prec)
+        model.zero_grad(set_to_none=True)
+        leaves = {}
+        saved = {}
+        for name, net in (("geo", model.geometry.network), ("rad", model.texture.network)):
+            leaf = net.flat_params().detach().clone().requires_grad_(True)
+            saved[name] = net.flat_params
+            net.flat_params = (lambda l=leaf: l)
+            leaves[name] = leaf
+        trainer = NeuSTrainer(model, scfg, cfg)
+        trainer.global_step = 30000
+        out = model(rays.to(dev), samples=samples, randn=randn.to(dev))
+        loss, _ = trainer.losses(out, {"rgb": gt.to(dev)})
+        scale = LOSS_SCALE if prec == "mixed" else 1.0
+        (loss * scale).backward()
+        table_grad = model.geometry.grid_module.params.grad.view(-1, 2) / scale
+        res[prec] = {
+            "comp_rgb": _err(out["comp_rgb"], o["comp_rgb"]),
+            "opacity": _err(out["opacity"], o["opacity"]),
+            "depth": _err(out["depth"], o["depth"]),
+            "sdf_samples": _err(out["sdf_samples"], o["sdf"]),
+            "loss": abs(float(loss) - float(o_loss)),
+            "grad_table": _err(table_grad, ol["table"].grad),
+            "grad_sdf_mlp": _err(leaves["geo"].grad / scale, ol["geo"].grad),
+            "grad_radiance_mlp": _err(leaves["rad"].grad / scale, ol["rad"].grad),
+        }
+        for name, net in (("geo", model.geometry.network), ("rad", model.texture.network)):
+            net.flat_params = saved[name]
+    model.set_precision("fp32")
+    _report("render_2048_rays", res)
+    print(json.dumps(res, indent=1))
+    assert res["fp32"]["comp_rgb"]["max_abs"] < 1e-5
+    assert res["fp32"]["grad_table"]["max_rel_to_max"] < 1e-3
+    assert res["mixed"]["comp_rgb"]["max_abs"] < 5e-3
+    assert res["mixed"]["opacity"]["max_abs"] < 5e-3
+    assert res["mixed"]["grad_table"]["max_rel_to_max"] < 5e-2
+    assert res["mixed"]["grad_sdf_mlp"]["max_rel_to_max"] < 5e-2
+    assert res["mixed"]["grad_radiance_mlp"]["max_rel_to_max"] < 5e-2
+
+
+def test_mixed_backward_is_deterministic(dev):
+    """The mixed form keeps the run-to-run bit-identical table gradient."""
+    model, _ = _model(dev)
+    model.set_precision("mixed")
+    geo = model.geometry
+    g = torch.Generator().manual_seed(25)
+    pts = ((torch.rand(5000, 3, generator=g) * 2 - 1) * 0.9).to(dev)
+    randn = torch.randn(5000, 3, generator=g).to(dev)
+    grads = []
+    for _ in range(2):
+        model.zero_grad()
+        sdf, grad, normal, feature, lap = geo.fused(pts, True, True, randn)
+        ((sdf.sum() + grad.square().sum() + lap.sum() + feature.sum()) * LOSS_SCALE).backward()
+        grads.append([p.grad.clone() for p in geo.parameters()])
+    for a, b in zip(*grads):
+        assert torch.equal(a, b)

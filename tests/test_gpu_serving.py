@@ -1,0 +1,146 @@
+"""Serving loop: continuous batching of renders and SDF-lattice queries.
+
+* Every request's result is bit-identical alone and in a mix (per-ray and
+  per-point computations do not depend on the rest of the batch).
+* Graph replay gives the same results as eager execution (skipped: graph
+  replay is switched off, see FieldServer).
+* Admission by footprint never holds more than the HBM budget (beyond a
+  single request that exceeds it on its own) and serves every request.
+* A served render equals the eval-mode NeuSModel.forward of the same rays
+  within the fp32 tolerance, and the served SDF lattice equals
+  VolumeSDF.sdf_grid.
+"""
+import pytest
+import torch
+
+import helpers
+
+pytestmark = pytest.mark.gpu
+
+
+def _model(dev, grid_prune=False):
+    import angelo_amd
+    cfg = helpers.model_config(L=8, log2T=15, base=16, scale=1.5, grid_prune=grid_prune,
+                               step=2.0 / 256 if grid_prune else None)
+    model = helpers.randomize_(angelo_amd.make("neus", cfg), seed=81)
+    model.update_step(0, 30000)
+    model.eval()
+    return model.to(dev)
+
+
+def _requests(seed=82):
+    from angelo_amd import serving
+    from angelo_amd.scenes import look_at_c2w
+    g = torch.Generator().manual_seed(seed)
+    reqs = []
+    for i, size in enumerate((32, 48, 40, 24)):
+        v = torch.randn(3, generator=g)
+        reqs.append(serving.Request("render", 0.001 * i, c2w=look_at_c2w(v / v.norm() * 2.5),
+                                    W=size, H=size, fov=40.0))
+    for j, res in enumerate((24, 40)):
+        reqs.append(serving.Request("sdf_grid", 0.0015 + 0.001 * j, resolution=res))
+    return reqs
+
+
+def _clone_reqs(reqs):
+    from angelo_amd import serving
+    return [serving.Request(r.kind, r.arrival, **r.params) for r in reqs]
+
+
+def _serve(model, reqs, **kw):
+    from angelo_amd import serving
+    srv = serving.FieldServer(model, rays_per_step=1500, sdf_chunks_per_step=1,
+                              sample_bucket=1 << 14, **kw)
+    srv.run(reqs, realtime=False)
+    assert all(r.finished is not None for r in reqs)
+    return srv
+
+
+GRAPHS_OFF = ("graph replay of the serving steps is switched off: a captured run left a device "
+              "memory fault whose cause is not established")
+
+
+@pytest.mark.parametrize("use_graphs", [False, pytest.param(True, marks=pytest.mark.skip(
+    reason=GRAPHS_OFF))])
+def test_results_identical_alone_and_in_mix(dev, use_graphs):
+    model = _model(dev)
+    mix = _clone_reqs(_requests())
+    _serve(model, mix, use_graphs=use_graphs)
+    for i, r in enumerate(_clone_reqs(_requests())):
+        _serve(model, [r], use_graphs=use_graphs)
+        m = mix[i]   # the same request (same kind and parameters) in the mix
+        assert m.kind == r.kind and m.n_units() == r.n_units()
+        for k in r.result:
+            assert torch.equal(r.result[k], m.result[k]), (r.kind, k)
+
+
+@pytest.mark.skip(reason=GRAPHS_OFF)
+def test_graph_replay_matches_eager(dev):
+    model = _model(dev, grid_prune=True)
+    og = model.occupancy_grid
+    from angelo_amd import scenes
+    og.binary.copy_(scenes.prewarm_occupancy(scenes.sdf_union, og.resolution, 1.0, 1.5, dev))
+    a = _clone_reqs(_requests(83))
+    b = _clone_reqs(_requests(83))
+    _serve(model, a, use_graphs=False)
+    srv = _serve(model, b, use_graphs=True)
+    assert srv._render_graphs or srv._sdf_graph is not None
+    for x, y in zip(a, b):
+        for k in x.result:
+            assert torch.equal(x.result[k], y.result[k]), (x.kind, k)
+
+
+def test_admission_by_footprint(dev):
+    from angelo_amd import serving
+    model = _model(dev)
+    reqs = _clone_reqs(_requests(84))
+    budget = max(r.footprint_bytes() for r in reqs) + 1
+    srv = serving.FieldServer(model, rays_per_step=1000, sdf_chunks_per_step=1,
+                              hbm_budget_bytes=budget, sample_bucket=1 << 14, use_graphs=False)
+    peak = 0
+    orig = srv._admit
+
+    def admit(now):
+        nonlocal peak
+        orig(now)
+        peak = max(peak, srv.in_flight_bytes)
+
+    srv._admit = admit
+    srv.run(reqs, realtime=False)
+    assert all(r.finished is not None for r in reqs)
+    assert peak <= budget
+    assert srv.in_flight_bytes == 0
+    # FIFO admission: requests were admitted in arrival order
+    adm = [r.admitted for r in sorted(reqs, key=lambda r: r.arrival)]
+    assert adm == sorted(adm)
+
+
+def test_served_render_and_lattice_match_model(dev):
+    from angelo_amd import serving
+    model = _model(dev)
+    reqs = _clone_reqs(_requests(85))
+    _serve(model, reqs, use_graphs=False)
+    r = next(q for q in reqs if q.kind == "render")
+    srv = serving.FieldServer(model, rays_per_step=1500, sdf_chunks_per_step=1,
+                              sample_bucket=1 << 14, use_graphs=False)
+    srv._start(serving.Request("render", 0.0, **r.params), 0.0)
+    rays = srv.active[0].rays
+    with torch.no_grad():
+        out = model(rays)
+    torch.testing.assert_close(r.result["rgb"], out["comp_rgb_full"], rtol=1e-5, atol=1e-5)
+    torch.testing.assert_close(r.result["opacity"], out["opacity"][:, 0], rtol=1e-5, atol=1e-5)
+    g = next(q for q in reqs if q.kind == "sdf_grid")
+    ref = model.geometry.sdf_grid(g.params["resolution"])
+    assert torch.equal(g.result["sdf"].view_as(ref), ref)
+
+
+def test_latency_stats_and_poisson_mix():
+    from angelo_amd import serving
+    reqs = serving.poisson_requests(40, 10.0, seed=1)
+    assert [r.arrival for r in reqs] == sorted(r.arrival for r in reqs)
+    kinds = {r.kind for r in reqs}
+    assert kinds == {"render", "sdf_grid"}
+    for i, r in enumerate(reqs):
+        r.finished = r.arrival + 0.01 * (i + 1)
+    st = serving.latency_stats(reqs)
+    assert st["n"] == 40 and st["p50_s"] <= st["p99_s"]
