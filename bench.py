@@ -26,9 +26,9 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 FP32_MFMA_PEAK_TFLOPS = 157.3
-TIMED_KERNELS = ["geo_fwd", "geo_bwd", "pair_count", "radix_sort", "seg_reduce", "rad_fwd",
-                 "rad_bwd", "composite_fwd", "composite_bwd", "march_count", "march_fill",
-                 "occ_sdf"]
+TIMED_KERNELS = ["geo_fwd", "geo_bwd", "pair_count", "pair_emit", "radix_sort", "seg_reduce",
+                 "rad_fwd", "rad_bwd", "composite_fwd", "composite_bwd", "march_count",
+                 "march_fill", "occ_sdf"]
 
 
 def log(*a):
@@ -38,14 +38,17 @@ def log(*a):
 def algorithmic_bytes(kernel, spec, n_samples, n_rays, n_pairs, mixed=False):
     """Algorithmic HBM bytes of one launch (per-step averages).
 
-    geo_fwd / geo_bwd: per encode evaluation of L levels, F=2 features, b_t B
-      table entries (4 fp32, 2 in the mixed form's fp16 rows): 12 B position
-      + 8*L*F*b_t B corner reads (the fused kernels keep
-      the encoding on chip, so the L*F output term is dropped); 13
-      evaluations per sample (base + 6 FD + 6 curvature) + per-sample I/O:
-      forward writes sdf, grad, normal, feature (19), laplace and the 12
-      stored FD values (39 floats); backward reads the same upstream
-      gradients + FD values and writes 12 B per (row, value) pair.
+    geo_fwd: per encode evaluation of L levels, F=2 features, b_t B table
+      entries (4 fp32, 2 in the mixed form's fp16 rows): 12 B position +
+      8*L*F*b_t B corner reads; 13 evaluations per sample (base + 6 FD + 6
+      curvature) + outputs sdf, grad, normal, feature (19), laplace and the
+      12 FD values (39 floats) + the stored encodings (13 x L*F floats) and
+      the curvature evaluations' position derivatives (6 x 3*L*F floats).
+    geo_bwd (store form): reads the stored encodings and derivatives, the 12
+      FD values, position and curvature direction (24 B) and the upstream
+      gradients (27 floats); writes the encode gradients (13 x L*F floats).
+    pair_count: position, direction, FD values.  pair_emit: the same + the
+      encode gradients + 12 B (8 B mixed) per (row, value) pair written.
     radix_sort: each pair (4 B row + 8 B value; 4 B value in the mixed form)
       read and written once.
     seg_reduce: each pair read once + one 8 B row written per pair (bound).
@@ -55,10 +58,16 @@ def algorithmic_bytes(kernel, spec, n_samples, n_rays, n_pairs, mixed=False):
     bt = 2 if mixed else 4
     pair = 8 if mixed else 12
     per_eval = 12 + 8 * L * F * bt
+    store = (13 * L * F + 6 * 3 * L * F) * 4
+    genc = 13 * L * F * 4
     if kernel == "geo_fwd":
-        return n_samples * (13 * per_eval + 39 * 4)
+        return n_samples * (13 * per_eval + 39 * 4 + store)
     if kernel == "geo_bwd":
-        return n_samples * (13 * per_eval + 39 * 4) + n_pairs * pair
+        return n_samples * (store + 12 * 4 + 24 + 27 * 4 + genc)
+    if kernel == "pair_count":
+        return n_samples * (24 + 12 * 4)
+    if kernel == "pair_emit":
+        return n_samples * (24 + 12 * 4 + genc) + n_pairs * pair
     if kernel == "radix_sort":
         return n_pairs * pair * 2
     if kernel == "seg_reduce":

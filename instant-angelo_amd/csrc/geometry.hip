@@ -9,25 +9,27 @@
 // fragments of the first layer) + the SDF MLP on MFMA.  The evaluations run
 // in runtime loops so one copy of the encode + MLP body is compiled.
 //
-// The forward stores the 12 FD / curvature SDF values per sample; the
-// backward rebuilds grad, normal, tangent and the curvature terms from them,
-// back-propagates each evaluation through MLP and encode, and reduces the
-// table gradient deterministically:
-//   1. per lane and level, the evaluations of a cluster (base + 6 FD, or the
-//      6 curvature points) are merged into the cell B of the cluster centre
-//      with a star stencil: an evaluation in B adds its 8 corners, one in a
-//      face neighbour of B adds the 4 corners of the shared face and emits
-//      its 4 far corners, any other emits its 8 corners;
-//   2. B's corner sums are merged with equal-B lanes of the same lane group
-//      (neighbouring samples of a ray) by a segmented reduction in a fixed
-//      shuffle tree;
-//   3. a count pass makes the same merge decisions from positions alone;
-//      an integer prefix sum of the per-tile counts gives each tile its
-//      first pair, the backward writes the surviving (row, value) pairs
-//      there in a fixed order, and a stable radix sort + fixed-order
-//      segmented reduce gives dtable.
-// MLP weight gradients are per-wave register accumulators summed in a fixed
-// order.
+// The forward stores the 12 FD / curvature SDF values per sample and, in
+// the store form, every evaluation's encoding.  The backward:
+//   k_geo_bwd    rebuilds grad, normal, tangent and the curvature terms from
+//                the stored values, re-runs each evaluation's MLP forward
+//                (from the stored encoding, or a fresh gather in the
+//                recompute form), back-propagates it on MFMA and writes the
+//                encode gradient to HBM; MLP weight gradients are per-wave
+//                register accumulators summed in a fixed order;
+//   k_geo_pairs  turns the encode gradients into table-gradient pairs:
+//     1. per lane and level, the evaluations of a cluster (base + 6 FD, or
+//        the 6 curvature points) are merged into the cell B of the cluster
+//        centre with a star stencil: an evaluation in B adds its 8 corners,
+//        one in a face neighbour of B adds the 4 corners of the shared face
+//        and emits its 4 far corners, any other emits its 8 corners;
+//     2. B's corner sums are merged with equal-B lanes of the same lane
+//        group (neighbouring samples of a ray) by a segmented reduction in a
+//        fixed shuffle tree;
+//     3. the same kernel in count mode makes these decisions from positions
+//        alone first; an integer prefix sum of the per-tile counts gives
+//        each tile its first pair, so the pairs land compacted, and a stable
+//        radix sort + fixed-order segmented reduce gives dtable.
 #include "hashgrid_dev.h"
 #include "mlp_dev.h"
 #include "table_grad.h"
@@ -785,13 +787,23 @@ __device__ __forceinline__ void geo_rebuild(const GeoArgs& a, const float* __res
     }
 }
 
-// Count pass: the number of (row, value) pairs each tile of 16 samples
-// emits, from positions alone (the same decisions as the backward).
-template <int LJ>
-__global__ void __launch_bounds__(256) k_geo_count(GeoArgs a, const float* __restrict__ pts,
-                                                   const float* __restrict__ randn, int64_t N,
-                                                   const float* __restrict__ fdv,
-                                                   int64_t* __restrict__ counts) {
+// Encode-gradient slots of a tile in HBM (written by the backward, read by
+// the pair pass): slot 0 base, 1-6 FD, 7-12 curvature; a slot is [LJ][2][64]
+// floats (level slot, feature, lane).
+__host__ __device__ inline int64_t geo_genc_floats(int64_t tiles, int n_evals, int LJ) {
+    return tiles * n_evals * LJ * 128;
+}
+
+// Pair pass.  COUNT: the number of (row, value) pairs each tile of 16
+// samples emits, from positions alone.  Otherwise: the same decisions with
+// the encode gradients, each tile's pairs written from offsets[tile] (never
+// past counts[tile]; a different count sets status[1]).
+template <int LJ, bool COUNT, class VT>
+__global__ void __launch_bounds__(256) k_geo_pairs(
+    GeoArgs a, const float* __restrict__ pts, const float* __restrict__ randn, int64_t N,
+    const float* __restrict__ fdv, const float* __restrict__ genc, uint32_t* __restrict__ keys,
+    VT* __restrict__ vals, const int64_t* __restrict__ offsets, int64_t* __restrict__ counts,
+    int64_t* __restrict__ status) {
     __shared__ LevelParams lv[AM_MAX_LEVELS];
 #pragma unroll
     for (int q = 0; q < AM_MAX_LEVELS; ++q)
@@ -806,10 +818,20 @@ __global__ void __launch_bounds__(256) k_geo_count(GeoArgs a, const float* __res
         const bool valid = n < N;
         GeoSample s;
         geo_rebuild(a, pts, randn, fdv, n, valid, s);
-        EmitTile<Vals<2>> et{nullptr, nullptr, 0, 0};
-        if (a.with_laplace) flush_cluster<LJ, 1, true>(a, lv, s, valid, nullptr, 6, 1, et);
-        flush_cluster<LJ, 0, true>(a, lv, s, valid, nullptr, a.with_grad ? 7 : 1, 0, et);
-        if (l == 0) counts[tile] = et.cursor;
+        EmitTile<VT> et{nullptr, nullptr, 0, 0};
+        const float* gt = nullptr;
+        if constexpr (!COUNT) {
+            const int64_t off = offsets[tile];
+            et = EmitTile<VT>{keys + off, vals + off, 0, (int)counts[tile]};
+            gt = genc + tile * a.n_evals * LJ * 128;
+        }
+        if (a.with_laplace) flush_cluster<LJ, 1, COUNT>(a, lv, s, valid, gt, 6, 7, et);
+        flush_cluster<LJ, 0, COUNT>(a, lv, s, valid, gt, a.with_grad ? 7 : 1, 0, et);
+        if constexpr (COUNT) {
+            if (l == 0) counts[tile] = et.cursor;
+        } else {
+            if (l == 0 && et.cursor != et.limit) status[1] = 1;
+        }
     }
 }
 
@@ -821,9 +843,7 @@ __global__ void __launch_bounds__(256) k_geo_bwd(
     const float* __restrict__ store,
     const float* __restrict__ g_sdf, const float* __restrict__ g_grad,
     const float* __restrict__ g_normal, const float* __restrict__ g_feature,
-    const float* __restrict__ g_laplace, uint32_t* __restrict__ keys,
-    PairV<PR>* __restrict__ vals, const int64_t* __restrict__ offsets,
-    const int64_t* __restrict__ counts, int64_t* __restrict__ status,
+    const float* __restrict__ g_laplace, float* __restrict__ genc_g,
     float* __restrict__ partial) {
     extern __shared__ float sm[];
     __shared__ LevelParams lv[AM_MAX_LEVELS];
@@ -833,8 +853,6 @@ __global__ void __launch_bounds__(256) k_geo_bwd(
     const int l = lane_id(), p = l & 15, k = l >> 4;
     const int nw = blockDim.x >> 6, wave = threadIdx.x >> 6;
     float* scratch = sm + a.lds_floats + wave * (128 * mlp::kScratchLd);
-    // genc storage: 7 evaluation slots x LJ x 2 x 64 lanes per wave
-    float* genc = sm + a.lds_floats + nw * (128 * mlp::kScratchLd) + wave * (7 * LJ * 128);
     const int64_t n_tiles = (N + 15) / 16;
     const int fdim = a.n_out + 3;
     GeoGrad<LJ, NH> G;
@@ -855,8 +873,8 @@ __global__ void __launch_bounds__(256) k_geo_bwd(
          tile += (int64_t)gridDim.x * nw) {
         const int64_t n = tile * 16 + p;
         const bool valid = n < N;
-        const int64_t off = offsets[tile];
-        EmitTile<PairV<PR>> et{keys + off, vals + off, 0, (int)counts[tile]};
+        // this tile's encode-gradient slots (0 base, 1-6 FD, 7-12 curvature)
+        float* genc = genc_g + tile * a.n_evals * LJ * 128;
         GeoSample s;
         geo_rebuild(a, pts, randn, fdv, n, valid, s);
         // upstream gradients of this sample
@@ -912,12 +930,11 @@ __global__ void __launch_bounds__(256) k_geo_bwd(
                                                         7 + j, x, e);
                 float gpos[3];
                 geo_eval_bwd<LJ, NH, true, mlp::kF32>(a, sm, e, go_row0(gf), scratch, G,
-                                           genc + (1 + j) * LJ * 128, gpos);
+                                                      genc + (7 + j) * LJ * 128, gpos);
 #pragma unroll
                 for (int d = 0; d < 3; ++d)
                     if (m[d]) g_ps[d] += gpos[d] * a.inv2r;
             }
-            flush_cluster<LJ, 1, false>(a, lv, s, valid, genc, 6, 1, et);
             float g_t[3], gcr[3];
 #pragma unroll
             for (int d = 0; d < 3; ++d) g_t[d] = g_ps[d] * a.eps;
@@ -940,13 +957,9 @@ __global__ void __launch_bounds__(256) k_geo_bwd(
                 geo_eval_b<LJ, NH, false, ST, mlp::kF32>(a, sm, lv, table, store, tile, n_tiles,
                                                          1 + j, x, e);
                 geo_eval_bwd<LJ, NH, false, mlp::kF32>(a, sm, e, go_row0(gf), scratch, G,
-                                            genc + (1 + j) * LJ * 128, nullptr);
+                                                       genc + (1 + j) * LJ * 128, nullptr);
             }
         }
-        flush_cluster<LJ, 0, false>(a, lv, s, valid, genc, a.with_grad ? 7 : 1, 0, et);
-        // the count pass and this pass emit the same sequence; a mismatch is
-        // reported (never written past: emit_pairs stops at the limit)
-        if (l == 0 && et.cursor != et.limit) status[1] = 1;
     }
     // per-wave partial gradient row in parameter layout
     float* out = partial + ((int64_t)blockIdx.x * nw + wave) * a.n_params;
@@ -1116,8 +1129,8 @@ template <int LJ>
 static int geo_count_launch(const GeoArgs& a, const float* pts, const float* randn, int64_t n,
                             const float* fdv, int64_t* counts, hipStream_t s) {
     KernelTimer kt_("pair_count", s);
-    hipLaunchKernelGGL((k_geo_count<LJ>), dim3(field_blocks(n)), dim3(256), 0, s, a, pts, randn, n,
-                       fdv, counts);
+    hipLaunchKernelGGL((k_geo_pairs<LJ, true, Vals<2>>), dim3(field_blocks(n)), dim3(256), 0, s, a,
+                       pts, randn, n, fdv, nullptr, nullptr, nullptr, nullptr, counts, nullptr);
     AM_LAUNCH_CHECK();
     return AM_OK;
 }
@@ -1128,6 +1141,7 @@ static int geo_count_launch(const GeoArgs& a, const float* pts, const float* ran
 struct GeoBwdWs {
     float* fdv;
     float* sdf_tmp;
+    float* genc;
     float* partial;
     float* ctmp;
     int64_t* counts;
@@ -1137,12 +1151,13 @@ struct GeoBwdWs {
     size_t scan_bytes;
 };
 
-static void carve_geo_bwd_head(WsCarver& c, const GeoArgs& a, int64_t n, bool need_fdv,
+static void carve_geo_bwd_head(WsCarver& c, const GeoArgs& a, int LJ, int64_t n, bool need_fdv,
                                GeoBwdWs* w) {
     const int waves = field_blocks(n) * 4;
     const int64_t tiles = (n + 15) / 16;
     w->fdv = need_fdv ? c.take<float>((size_t)n * 12 + 1) : nullptr;
     w->sdf_tmp = need_fdv ? c.take<float>((size_t)n + 1) : nullptr;
+    w->genc = c.take<float>((size_t)geo_genc_floats(tiles, a.n_evals, LJ) + 1);
     w->partial = c.take<float>((size_t)waves * a.n_params);
     w->ctmp = c.take<float>((size_t)colsum_tmp_floats(waves, a.n_params) + 1);
     w->counts = c.take<int64_t>((size_t)tiles + 1);
@@ -1176,7 +1191,7 @@ static int geo_bwd_launch_t(const GeoArgs& a, const am_grid_desc* g, const float
     AM_CHECK_ARG(!store || !need_fdv, "a stored encoding needs the stored FD values too");
     WsCarver c(ws, ws_bytes);
     GeoBwdWs w;
-    carve_geo_bwd_head(c, a, n, need_fdv, &w);
+    carve_geo_bwd_head(c, a, LJ, n, need_fdv, &w);
     if (!c.ok()) {
         set_error("workspace too small: need %zu, got %zu", c.used, ws_bytes);
         return AM_ERR_WORKSPACE;
@@ -1209,8 +1224,7 @@ static int geo_bwd_launch_t(const GeoArgs& a, const am_grid_desc* g, const float
         set_error("workspace too small: need %zu, got %zu", c.used, ws_bytes);
         return AM_ERR_WORKSPACE;
     }
-    const size_t lds = (size_t)(a.lds_floats + 4 * 128 * mlp::kScratchLd + 4 * 7 * LJ * 128) *
-                       sizeof(float);
+    const size_t lds = (size_t)(a.lds_floats + 4 * 128 * mlp::kScratchLd) * sizeof(float);
     AM_CHECK_ARG(lds <= 160 * 1024, "geometry backward needs %zu bytes of LDS", lds);
     auto kern = store ? k_geo_bwd<LJ, NH, true, PR> : k_geo_bwd<LJ, NH, false, PR>;
     AM_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -1218,9 +1232,15 @@ static int geo_bwd_launch_t(const GeoArgs& a, const am_grid_desc* g, const float
     {
         KernelTimer kt_("geo_bwd", s);
         hipLaunchKernelGGL(kern, dim3(blocks), dim3(256), lds, s, a, pts, randn, n, table, params,
-                           fdv, store, g_sdf, g_grad, g_normal, g_feature, g_laplace, tw.keys[0],
-                           reinterpret_cast<PairV<PR>*>(tw.vals[0]), w.offsets, w.counts,
-                           w.status, w.partial);
+                           fdv, store, g_sdf, g_grad, g_normal, g_feature, g_laplace, w.genc,
+                           w.partial);
+    }
+    AM_LAUNCH_CHECK();
+    {
+        KernelTimer kt_("pair_emit", s);
+        hipLaunchKernelGGL((k_geo_pairs<LJ, false, PairV<PR>>), dim3(blocks), dim3(256), 0, s, a, pts,
+                           randn, n, fdv, w.genc, tw.keys[0],
+                           reinterpret_cast<PairV<PR>*>(tw.vals[0]), w.offsets, w.counts, w.status);
     }
     AM_LAUNCH_CHECK();
     if (int rc = colsum_f32(w.partial, waves, a.n_params, dparams, w.ctmp, s)) return rc;
@@ -1301,7 +1321,7 @@ extern "C" size_t am_sdf_bwd_workspace_size(const am_grid_desc* g, const am_mlp_
     if (make_geo_args(g, m, sd, &a, &LJ) || n < 0) return 0;
     WsCarver c(nullptr, 0);
     GeoBwdWs w;
-    carve_geo_bwd_head(c, a, n, true, &w);
+    carve_geo_bwd_head(c, a, LJ, n, true, &w);
     TableReduceWs tw;
     if (table_reduce_carve(c, 2, geo_max_pairs(a, LJ, n), g->n_entries, &tw)) return 0;
     return c.used + 256;

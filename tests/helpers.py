@@ -42,17 +42,35 @@ def model_config(L=8, log2T=15, base=16, scale=1.5, geo_hidden=1, feature_dim=16
     })
 
 
-def randomize_(model, table_scale=0.05, seed=0):
-    """Replaces the near-zero tcnn-style table init by a larger one so the
-    parity tests exercise every term."""
+def randomize_(model, table_scale=0.05, seed=0, geo_feature_scale=0.5):
+    """Replaces the near-zero tcnn-style table init by a larger one and gives
+    the SDF MLP non-zero weights on the encoding (sphere init sets them to
+    exactly 0, which would make every table gradient 0), so the parity tests
+    exercise every term.  The SDF keeps its sphere-like xyz part."""
     g = torch.Generator().manual_seed(seed)
     grid = model.geometry.grid_module
     with torch.no_grad():
         grid.params.copy_((torch.rand(grid.params.shape, generator=g) * 2 - 1) * table_scale)
+        lins = model.geometry.network.linears()
+        first = lins[0]
+        if hasattr(first, "weight_v"):
+            v = first.weight_v
+            v[:, 3:] = torch.randn(v[:, 3:].shape, generator=g) * geo_feature_scale
+            first.weight_g.copy_(v.norm(dim=1, keepdim=True))
+        else:
+            first.weight[:, 3:] = torch.randn(first.weight[:, 3:].shape, generator=g) * geo_feature_scale
         for lin in model.texture.network.linears():
             lin.weight.copy_(torch.randn(lin.weight.shape, generator=g) / lin.weight.shape[1] ** 0.5)
             lin.bias.copy_(torch.randn(lin.bias.shape, generator=g) * 0.1)
     return model
+
+
+def assert_nontrivial(t, name, min_frac=0.01):
+    """A gradient check means something only if the gradient is not ~0."""
+    t = t.detach().double().cpu()
+    assert float(t.abs().max()) > 0, f"{name} is identically zero"
+    assert float((t != 0).double().mean()) >= min_frac or t.numel() < 100, \
+        f"{name}: fewer than {min_frac:.0%} non-zero entries"
 
 
 def level_table(model):

@@ -255,5 +255,6 @@ def test_store_and_recompute_backward_identical_at_2pow20_samples(dev):
             del sdf, grad, normal, feature, lap
     finally:
         functions.set_sdf_backward("auto")
+    helpers.assert_nontrivial(geo.grid_module.params.grad, "table gradient")
     for a, b in zip(grads["store"], grads["recompute"]):
         assert torch.equal(a, b)

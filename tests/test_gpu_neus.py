@@ -57,6 +57,15 @@ def test_neus_step_matches_oracle(dev, n_rays, geo_hidden, L):
                              radiance, inv_s, 1.0, eps, model.cos_anneal_ratio, randn.double())
     o_loss, o_terms = orender.losses(o, gt.double(), 10.0, 0.1, 0.5)
     o_loss.backward()
+    # the same arithmetic restated in fp32 (fp32 positions and blends): its
+    # distance to FP64 is the scale each HIP error is measured against
+    f32, r32, ol32 = helpers.oracle_fns(model, torch.float32, "f32")
+    var32 = model.variance.variance.detach().cpu().float().clone().requires_grad_(True)
+    o32 = orender.neus_forward(rays, ridx, t0, t1, n_rays, f32, r32,
+                               torch.exp(var32 * 10.0).clip(1e-6, 1e6), 1.0, eps,
+                               model.cos_anneal_ratio, randn)
+    l32, _ = orender.losses(o32, gt, 10.0, 0.1, 0.5)
+    l32.backward()
 
     # HIP path
     leaves = _leaf_params(model)
@@ -67,22 +76,26 @@ def test_neus_step_matches_oracle(dev, n_rays, geo_hidden, L):
     loss, terms = trainer.losses(out, {"rgb": gt.to(dev)})
     loss.backward()
 
-    def chk(a, b, tol, name):
+    def chk(a, b, floor, name, b32=None):
+        """HIP error vs FP64 within 8x the fp32 restatement's error, plus a
+        floor (the fp32 restatement and the kernels round differently)."""
         e = helpers.rel_err(a, b)
-        assert e < tol, (name, e)
+        e32 = helpers.rel_err(b32, b) if b32 is not None else 0.0
+        assert e <= 8 * e32 + floor, (name, e, e32)
 
-    chk(out["comp_rgb"], o["comp_rgb"], 1e-4, "comp_rgb")
-    chk(out["opacity"], o["opacity"], 1e-4, "opacity")
-    chk(out["depth"], o["depth"], 1e-4, "depth")
-    chk(out["comp_normal"], o["comp_normal"], 1e-3, "comp_normal")
-    chk(out["sdf_samples"], o["sdf"], 1e-5, "sdf")
-    chk(out["sdf_grad_samples"], o["grad"], 1e-3, "grad")
-    chk(out["sdf_laplace_samples"], o["laplace"], 5e-3, "laplace")
+    chk(out["comp_rgb"], o["comp_rgb"], 1e-4, "comp_rgb", o32["comp_rgb"])
+    chk(out["opacity"], o["opacity"], 1e-4, "opacity", o32["opacity"])
+    chk(out["depth"], o["depth"], 1e-4, "depth", o32["depth"])
+    chk(out["comp_normal"], o["comp_normal"], 1e-3, "comp_normal", o32["comp_normal"])
+    chk(out["sdf_samples"], o["sdf"], 1e-5, "sdf", o32["sdf"])
+    chk(out["sdf_grad_samples"], o["grad"], 1e-3, "grad", o32["grad"])
+    chk(out["sdf_laplace_samples"], o["laplace"], 5e-3, "laplace", o32["laplace"])
     # per-sample weights depend on the FD normals through the cosine term of
     # alpha; the normals carry fp32 FD cancellation error (bounded at 1e-3
-    # above), so the weights get 5e-4
-    chk(out["weights"], o["weights"], 5e-4, "weights")
-    assert abs(loss.item() - o_loss.item()) / abs(o_loss.item()) < 1e-4
+    # above), so the weights get a 5e-4 floor
+    chk(out["weights"], o["weights"], 5e-4, "weights", o32["weights"])
+    e_l32 = abs(l32.item() - o_loss.item()) / abs(o_loss.item())
+    assert abs(loss.item() - o_loss.item()) / abs(o_loss.item()) <= 8 * e_l32 + 1e-4
     assert set(["comp_rgb", "comp_normal", "opacity", "depth", "rays_valid", "num_samples",
                 "sdf_samples", "sdf_grad_samples", "weights", "points", "intervals",
                 "ray_indices", "sdf_laplace_samples", "comp_rgb_bg", "num_samples_bg",
@@ -90,9 +103,11 @@ def test_neus_step_matches_oracle(dev, n_rays, geo_hidden, L):
                 "inv_s"]) <= set(out.keys())
     assert int(out["num_samples"].item()) == n
     # gradients of every parameter group
-    chk(model.geometry.grid_module.params.grad.view(-1, 2), ol["table"].grad, 2e-3, "d table")
-    chk(leaves["geo"].grad, ol["geo"].grad, 2e-3, "d sdf mlp")
-    chk(leaves["rad"].grad, ol["rad"].grad, 1e-3, "d radiance mlp")
+    helpers.assert_nontrivial(ol["table"].grad, "oracle table gradient")
+    chk(model.geometry.grid_module.params.grad.view(-1, 2), ol["table"].grad, 2e-3, "d table",
+        ol32["table"].grad)
+    chk(leaves["geo"].grad, ol["geo"].grad, 2e-3, "d sdf mlp", ol32["geo"].grad)
+    chk(leaves["rad"].grad, ol["rad"].grad, 1e-3, "d radiance mlp", ol32["rad"].grad)
     chk(model.variance.variance.grad.reshape(1), var.grad.reshape(1), 1e-3, "d variance")
 
 

@@ -198,6 +198,8 @@ def test_sdf_fused_fwd_bwd(dev, geo_hidden, L, log2T, base, scale):
         assert e_hip <= 8 * e_t32 + 1e-5, (k, e_hip, e_t32)
     # gradients: table, SDF MLP (weight-norm folded: compare d/dW flat)
     gt = table.grad.view(-1, 2).cpu()
+    helpers.assert_nontrivial(l64["table"].grad, "oracle table gradient")
+    helpers.assert_nontrivial(gt, "table gradient")
     e_hip = helpers.rel_err(gt, l64["table"].grad)
     e_t32 = helpers.rel_err(l32["table"].grad, l64["table"].grad)
     assert e_hip <= 8 * e_t32 + 1e-5, ("table", e_hip, e_t32)
@@ -215,6 +217,7 @@ def test_sdf_bwd_deterministic(dev):
         sdf, grad, normal, feature, lap = geo.fused(pts, True, True, randn)
         (sdf.sum() + grad.square().sum() + lap.sum() + feature.sum()).backward()
         grads.append([p.grad.clone() for p in geo.parameters()])
+    helpers.assert_nontrivial(geo.grid_module.params.grad, "table gradient")
     for a, b in zip(*grads):
         assert torch.equal(a, b)
 
@@ -243,6 +246,7 @@ def test_sdf_store_and_recompute_backward_are_identical(dev, laplace):
             grads[mode] = [p.grad.clone() for p in geo.parameters()]
     finally:
         functions.set_sdf_backward("auto")
+    helpers.assert_nontrivial(geo.grid_module.params.grad, "table gradient")
     for a, b in zip(grads["store"], grads["recompute"]):
         assert torch.equal(a, b)
 

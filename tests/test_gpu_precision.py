@@ -79,13 +79,17 @@ def test_mixed_field_errors_vs_fp64(dev):
     model.set_precision("fp32")
     _report("field_65536_points", res)
     print(json.dumps(res, indent=1))
-    # fp32 form: the plain parity bars
-    assert res["fp32"]["sdf"]["max_abs"] < 1e-5
-    assert res["fp32"]["normal"]["max_abs"] < 2e-3
-    # mixed form: fp16 rows (rel. 2^-11), a fixed rounding of the table
-    assert res["mixed"]["sdf"]["max_abs"] < 5e-4
-    assert res["mixed"]["feature"]["max_abs"] < 1e-3
-    assert res["mixed"]["normal"]["mean_abs"] < 2e-3
+    # fp32 form: fp32 positions (fmaf(scale, x, 0.5) up to scale 2048) against
+    # FP64 ones bound the SDF at a few 1e-5 for these random fine features;
+    # FD normals add the fp32 cancellation of (f+ - f-) / 2eps
+    assert res["fp32"]["sdf"]["max_abs"] < 5e-5
+    assert res["fp32"]["normal"]["mean_abs"] < 2e-4
+    assert res["fp32"]["normal"]["max_abs"] < 1e-2
+    # mixed form: fp16 rows (rel. 2^-11) on top; target SDF error <= 1e-4
+    assert res["mixed"]["sdf"]["max_abs"] < 1e-4
+    assert res["mixed"]["sdf"]["mean_abs"] < 3e-5
+    assert res["mixed"]["feature"]["max_abs"] < 2e-4
+    assert res["mixed"]["normal"]["mean_abs"] < 1e-3
     assert res["mixed"]["normal"]["max_abs"] < 5e-2
 
 
@@ -147,8 +151,9 @@ def test_mixed_render_and_gradient_errors_vs_fp64(dev):
     model.set_precision("fp32")
     _report("render_2048_rays", res)
     print(json.dumps(res, indent=1))
-    assert res["fp32"]["comp_rgb"]["max_abs"] < 1e-5
-    assert res["fp32"]["grad_table"]["max_rel_to_max"] < 1e-3
+    helpers.assert_nontrivial(ol["table"].grad, "oracle table gradient")
+    assert res["fp32"]["comp_rgb"]["max_abs"] < 5e-5
+    assert res["fp32"]["grad_table"]["max_rel_to_max"] < 2e-3
     assert res["mixed"]["comp_rgb"]["max_abs"] < 5e-3
     assert res["mixed"]["opacity"]["max_abs"] < 5e-3
     assert res["mixed"]["grad_table"]["max_rel_to_max"] < 5e-2
@@ -170,5 +175,6 @@ def test_mixed_backward_is_deterministic(dev):
         sdf, grad, normal, feature, lap = geo.fused(pts, True, True, randn)
         ((sdf.sum() + grad.square().sum() + lap.sum() + feature.sum()) * LOSS_SCALE).backward()
         grads.append([p.grad.clone() for p in geo.parameters()])
+    helpers.assert_nontrivial(geo.grid_module.params.grad, "table gradient")
     for a, b in zip(*grads):
         assert torch.equal(a, b)
