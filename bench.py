@@ -179,6 +179,9 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=16,
                     help="requests in the serving measurement (0: skip it)")
     ap.add_argument("--serve-rate", type=float, default=4.0, help="Poisson arrival rate (1/s)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first run a plain 1 GiB device-to-device copy three times (the "
+                         "reference kernel for scaling HBM counters of a profile)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
@@ -193,6 +196,14 @@ def main():
         sys.exit(1)
     dev = torch.device("cuda:0")
     nat.load()
+    if args.calibrate_copy:
+        src = torch.empty(1 << 28, device=dev, dtype=torch.float32).fill_(1.0)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log("[bench] calibration copy: 3 x 1 GiB device-to-device")
+        del src, dst
     log(f"[bench] building workload {args.workload}")
     wl = Workload(args.workload, dev, rays=args.rays)
     spec = wl.spec

@@ -344,10 +344,11 @@ def ray_marching(rays, t_near, t_far, roi, grid_binary, step, jitter=None, cap=0
     ridx = torch.empty(N, device=dev, dtype=torch.int32)
     t0 = torch.empty(N, device=dev, dtype=torch.float32)
     t1 = torch.empty(N, device=dev, dtype=torch.float32)
-    nat.check(lib.am_march_fill(nat.ptr(rays), R, nat.ptr(t_near), nat.ptr(t_far), nat.ptr(jit),
-                                roi_p, gptr, res, float(step), int(cap), nat.ptr(packed),
-                                nat.ptr(ridx), nat.ptr(t0), nat.ptr(t1), nat.stream()),
-              "am_march_fill")
+    if N > 0:   # no ray has a sample: packed already holds zero counts
+        nat.check(lib.am_march_fill(nat.ptr(rays), R, nat.ptr(t_near), nat.ptr(t_far),
+                                    nat.ptr(jit), roi_p, gptr, res, float(step), int(cap),
+                                    nat.ptr(packed), nat.ptr(ridx), nat.ptr(t0), nat.ptr(t1),
+                                    nat.stream()), "am_march_fill")
     return ridx, t0, t1, packed
 
 

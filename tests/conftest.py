@@ -19,3 +19,11 @@ def dev():
     if not torch.cuda.is_available():
         pytest.skip("no GPU")
     return torch.device("cuda:0")
+
+
+@pytest.fixture(autouse=True)
+def _seed_torch():
+    """Every test starts from the same global RNG state (module construction
+    draws weight inits from it), so each run sees the same models."""
+    import torch
+    torch.manual_seed(0)

