@@ -218,8 +218,11 @@ def main():
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     start.record()
+    step_wall = []
     for i in range(args.steps):
+        ts = time.perf_counter()
         total_samples += wl.step(args.warmup + i)
+        step_wall.append(time.perf_counter() - ts)
     stop.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
@@ -236,6 +239,9 @@ def main():
     rays = spec["rays"]
     value = total_samples / (gpu_ms / 1e3)
     log(f"[bench] {ms_per_step:.3f} ms/step, {samples_per_step:.0f} samples/step, wall {wall:.2f}s")
+    sw = sorted(step_wall)
+    log(f"[bench] host time per step: min {1e3 * sw[0]:.2f} ms, median "
+        f"{1e3 * sw[len(sw) // 2]:.2f} ms, max {1e3 * sw[-1]:.2f} ms")
     for k, v in sorted(kern.items(), key=lambda kv: -kv[1]["total_ms"]):
         log(f"[bench]   {k:14s} {v['avg_ms']:.3f} ms x {v['launches']}  "
             f"({100 * v['total_ms'] / gpu_ms:.1f}% of step time)")

@@ -38,6 +38,11 @@ extern "C" {
 #define AM_MAX_LEVELS 32
 
 const char* am_last_error(void);
+/* After AM_ERR_WORKSPACE: the workspace bytes the failed call needs
+ * (thread-local).  am_sdf_bwd's need depends on the data (its number of
+ * table-gradient pairs), so a caller may pass less than the worst-case size
+ * query, keep a grow-only workspace and call again with this many bytes. */
+size_t am_last_workspace_need(void);
 int am_version(void);
 
 /* Kernel timers for measurement: when enabled, the named hot kernels are
@@ -201,14 +206,19 @@ int am_sdf_fwd(const am_grid_desc* g, const am_mlp_desc* m, const am_sdf_desc* s
 /* Floats of enc_store for n samples (-1 on invalid descriptors). */
 int64_t am_sdf_store_floats(const am_grid_desc* g, const am_mlp_desc* m, const am_sdf_desc* s,
                             int64_t n);
+/* Worst-case bound: every sample's every corner a pair of its own. */
 size_t am_sdf_bwd_workspace_size(const am_grid_desc* g, const am_mlp_desc* m,
                                  const am_sdf_desc* s, int64_t n);
 /* Upstream gradients are nullable (= 0).  fd_values: from the forward
  * (nullable: recomputed).  enc_store: from the forward (nullable: the
  * backward re-gathers the table, the memory-lean "recompute" form; needs
  * fd_values when given).  dtable [n_entries,F] and dparams are fully
- * overwritten; both reductions are deterministic.  The call reads the
- * number of table-gradient pairs back to the host once (a stream sync).
+ * overwritten; both reductions are deterministic.  The call counts the
+ * table-gradient pairs first and reads the count back to the host (a stream
+ * sync), carving the pair arrays for exactly that many; a workspace below
+ * that need returns AM_ERR_WORKSPACE before any output is written
+ * (am_last_workspace_need).  A second stream sync reads the emit pass's
+ * consistency flag before the sort.
  * Replaces the autograd backward of VolumeSDF.forward
  * (models/geometry.py:195-287) through tcnn's encoding backward. */
 int am_sdf_bwd(const am_grid_desc* g, const am_mlp_desc* m, const am_sdf_desc* s,

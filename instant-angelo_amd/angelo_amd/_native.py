@@ -71,6 +71,7 @@ _SZ = ctypes.c_size_t
 # name -> (restype, argtypes)
 _SIGS = {
     "am_last_error": (ctypes.c_char_p, []),
+    "am_last_workspace_need": (_SZ, []),
     "am_version": (ctypes.c_int, []),
     "am_timer_enable": (ctypes.c_int, [_I32]),
     "am_timer_read": (ctypes.c_int, [ctypes.c_char_p, _P, _P]),
@@ -166,6 +167,45 @@ def require_cuda(*tensors):
 
 def workspace(nbytes, device):
     return torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device=device)
+
+
+AM_ERR_WORKSPACE = -3
+
+
+class GrowingWorkspace:
+    """Grow-only device workspace for a call whose need depends on the data
+    (am_sdf_bwd: its table-gradient pairs).  The call runs with what is held;
+    when that is short it fails before writing any output, reports its need
+    (am_last_workspace_need) and runs again with that plus `headroom`.  One
+    buffer per (device, stream): the call's kernels use it asynchronously."""
+
+    def __init__(self, initial_bytes=1 << 20, headroom=1.25):
+        self.initial = int(initial_bytes)
+        self.headroom = float(headroom)
+        self.grows = 0
+        self._bufs = {}
+
+    def held_bytes(self):
+        return sum(b.numel() for b in self._bufs.values())
+
+    def run(self, device, call, what):
+        """call(ws_ptr, ws_bytes) -> status."""
+        key = (device.index, torch.cuda.current_stream(device).cuda_stream)
+        buf = self._bufs.get(key)
+        if buf is None:
+            buf = self._bufs[key] = workspace(self.initial, device)
+        for _ in range(4):   # head short, then pairs short, then fits
+            rc = call(ptr(buf), buf.numel())
+            if rc != AM_ERR_WORKSPACE:
+                check(rc, what)
+                return
+            need = int(load().am_last_workspace_need())
+            if need <= buf.numel():
+                check(rc, what)
+            self._bufs[key] = None
+            buf = self._bufs[key] = workspace(int(need * self.headroom), device)
+            self.grows += 1
+        check(rc, what)
 
 
 def timer_enable(on=True):

@@ -108,6 +108,10 @@ def sh_encode(dirs01, degree):
 # memory); "auto": store while the store fits store_limit_bytes.
 SDF_BACKWARD = {"mode": "auto", "store_limit_bytes": 24 << 30}
 
+# The SDF backward's workspace is sized by its actual table-gradient pairs
+# (known only after its count pass), held and grown across steps.
+SDF_BWD_WORKSPACE = nat.GrowingWorkspace()
+
 
 def set_sdf_backward(mode="auto", store_limit_bytes=None):
     if mode not in ("auto", "store", "recompute"):
@@ -187,15 +191,12 @@ class SDFFieldFn(torch.autograd.Function):
         gl = opt(g_lap, n) if sdesc.with_laplace else None
         dtable = torch.empty_like(table)
         dparams = torch.empty_like(params)
-        ws = nat.workspace(lib.am_sdf_bwd_workspace_size(nat.sptr(grid), nat.sptr(mdesc),
-                                                         nat.sptr(sdesc), n), pts.device)
-        nat.check(lib.am_sdf_bwd(nat.sptr(grid), nat.sptr(mdesc), nat.sptr(sdesc), nat.ptr(pts),
-                                 nat.ptr(rnd) if rnd.numel() else None, n, nat.ptr(table_k),
-                                 nat.ptr(params), nat.ptr(fdv) if fdv.numel() else None,
-                                 nat.ptr(store) if store.numel() else None,
-                                 nat.ptr(gs), nat.ptr(gg), nat.ptr(gn),
-                                 nat.ptr(gf), nat.ptr(gl), nat.ptr(dtable), nat.ptr(dparams),
-                                 nat.ptr(ws), ws.numel(), nat.stream()), "am_sdf_bwd")
+        SDF_BWD_WORKSPACE.run(pts.device, lambda wp, wb: lib.am_sdf_bwd(
+            nat.sptr(grid), nat.sptr(mdesc), nat.sptr(sdesc), nat.ptr(pts),
+            nat.ptr(rnd) if rnd.numel() else None, n, nat.ptr(table_k), nat.ptr(params),
+            nat.ptr(fdv) if fdv.numel() else None, nat.ptr(store) if store.numel() else None,
+            nat.ptr(gs), nat.ptr(gg), nat.ptr(gn), nat.ptr(gf), nat.ptr(gl), nat.ptr(dtable),
+            nat.ptr(dparams), wp, wb, nat.stream()), "am_sdf_bwd")
         return None, None, dtable, dparams, None, None, None, None
 
 

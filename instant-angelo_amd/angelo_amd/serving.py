@@ -107,6 +107,7 @@ class FieldServer:
         self.aabb = model.scene_aabb.tolist()
         self.pending = deque()
         self.active = []
+        self._clock = time.perf_counter   # run() rebases it on its start time
         self.done = []
         self.in_flight_bytes = 0
         self._dirs = {}
@@ -253,7 +254,7 @@ class FieldServer:
         """Admits what fits, runs one render batch and one SDF batch over the
         admitted requests' next work units, scatters the results.  Returns
         False when there was nothing to do."""
-        now = time.perf_counter() if now is None else now
+        now = self._clock() if now is None else now
         self._admit(now)
         if not self.active:
             return False
@@ -302,7 +303,7 @@ class FieldServer:
                 r._done += take
             self.points_queried += fill
         torch.cuda.current_stream().synchronize()
-        t = time.perf_counter()
+        t = self._clock()   # the same time base as `now` and the arrivals
         for r in list(self.active):
             if r._done >= r._total:
                 self.active.remove(r)
@@ -316,19 +317,20 @@ class FieldServer:
         passed on the wall clock; otherwise all are submitted at once."""
         reqs = sorted(requests, key=lambda r: r.arrival)
         t0 = time.perf_counter()
-        for r in reqs:
-            r.arrival_abs = r.arrival
+        self._clock = lambda: time.perf_counter() - t0
         i = 0
-        while i < len(reqs) or self.pending or self.active:
-            now = time.perf_counter() - t0
-            while i < len(reqs) and (not realtime or reqs[i].arrival <= now):
-                self.submit(reqs[i])
-                i += 1
-            if not self.step(now=time.perf_counter() - t0):
-                if i < len(reqs):
-                    time.sleep(max(0.0, min(0.002, reqs[i].arrival - now)))
-        wall = time.perf_counter() - t0
-        return wall
+        try:
+            while i < len(reqs) or self.pending or self.active:
+                now = self._clock()
+                while i < len(reqs) and (not realtime or reqs[i].arrival <= now):
+                    self.submit(reqs[i])
+                    i += 1
+                if not self.step(now=self._clock()):
+                    if i < len(reqs):
+                        time.sleep(max(0.0, min(0.002, reqs[i].arrival - now)))
+        finally:
+            self._clock = time.perf_counter
+        return time.perf_counter() - t0
 
 
 def latency_stats(requests):

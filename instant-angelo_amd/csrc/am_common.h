@@ -10,6 +10,10 @@ namespace am {
 // ---- error reporting (thread-local message, int status through the C-ABI) ----
 void set_error(const char* fmt, ...);
 const char* last_error();
+// Records the workspace bytes a call needs (am_last_workspace_need) and the
+// message; returns AM_ERR_WORKSPACE.  `used`: the carver's bytes.
+int workspace_short(size_t used, size_t got);
+size_t last_workspace_need();
 
 #define AM_CHECK_ARG(cond, ...)                 \
     do {                                        \

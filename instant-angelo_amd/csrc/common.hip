@@ -72,6 +72,16 @@ void set_error(const char* fmt, ...) {
 
 const char* last_error() { return g_err; }
 
+static thread_local size_t g_ws_need = 0;
+
+int workspace_short(size_t used, size_t got) {
+    g_ws_need = used + 256;   // the size queries' alignment slack
+    set_error("workspace too small: need %zu bytes, got %zu", g_ws_need, got);
+    return AM_ERR_WORKSPACE;
+}
+
+size_t last_workspace_need() { return g_ws_need; }
+
 static constexpr int64_t kColsumSlice = 64;  // rows per slice
 
 int64_t colsum_tmp_floats(int64_t rows, int64_t cols) {
@@ -113,6 +123,8 @@ int colsum_f32(const float* in, int64_t rows, int64_t cols, float* out, float* t
 }  // namespace am
 
 extern "C" const char* am_last_error(void) { return am::last_error(); }
+
+extern "C" size_t am_last_workspace_need(void) { return am::last_workspace_need(); }
 
 extern "C" int am_version(void) { return 10000; /* 1.0.0 */ }
 

@@ -836,8 +836,8 @@ __global__ void __launch_bounds__(256) k_geo_pairs(
 }
 
 template <int LJ, int NH, bool ST, int PR>
-__global__ void __launch_bounds__(256) k_geo_bwd(
-    GeoArgs a, const float* __restrict__ pts,
+__device__ __forceinline__ void geo_bwd_body(
+    const GeoArgs& a, const float* __restrict__ pts,
     const float* __restrict__ randn, int64_t N, const void* __restrict__ table_v,
     const float* __restrict__ params, const float* __restrict__ fdv,
     const float* __restrict__ store,
@@ -1009,6 +1009,32 @@ __global__ void __launch_bounds__(256) k_geo_bwd(
 #pragma unroll
         for (int r = 0; r < 4; ++r)
             if (4 * k + r < a.n_out) out[a.pbo + 4 * k + r] = so[r];
+}
+
+#define AM_GEO_BWD_PARAMS                                                                  \
+    GeoArgs a, const float* __restrict__ pts, const float* __restrict__ randn, int64_t N,      \
+        const void* __restrict__ table_v, const float* __restrict__ params,                    \
+        const float* __restrict__ fdv, const float* __restrict__ store,                        \
+        const float* __restrict__ g_sdf, const float* __restrict__ g_grad,                     \
+        const float* __restrict__ g_normal, const float* __restrict__ g_feature,               \
+        const float* __restrict__ g_laplace, float* __restrict__ genc_g,                       \
+        float* __restrict__ partial
+#define AM_GEO_BWD_ARGS                                                                     \
+    a, pts, randn, N, table_v, params, fdv, store, g_sdf, g_grad, g_normal, g_feature,          \
+        g_laplace, genc_g, partial
+
+template <int LJ, int NH, bool ST, int PR>
+__global__ void __launch_bounds__(256) k_geo_bwd(AM_GEO_BWD_PARAMS) {
+    geo_bwd_body<LJ, NH, ST, PR>(AM_GEO_BWD_ARGS);
+}
+
+// The 1-hidden-layer form at two waves per SIMD: its state fits the 256
+// registers a lane has at that occupancy with a small spill, and the second
+// wave hides the LDS and memory latency of the first.
+template <int LJ, bool ST, int PR>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
+k_geo_bwd_o2(AM_GEO_BWD_PARAMS) {
+    geo_bwd_body<LJ, 1, ST, PR>(AM_GEO_BWD_ARGS);
 }
 
 // status[0] = number of pairs (offset + count of the last tile); status[1]
@@ -1192,10 +1218,7 @@ static int geo_bwd_launch_t(const GeoArgs& a, const am_grid_desc* g, const float
     WsCarver c(ws, ws_bytes);
     GeoBwdWs w;
     carve_geo_bwd_head(c, a, LJ, n, need_fdv, &w);
-    if (!c.ok()) {
-        set_error("workspace too small: need %zu, got %zu", c.used, ws_bytes);
-        return AM_ERR_WORKSPACE;
-    }
+    if (!c.ok()) return workspace_short(c.used, ws_bytes);
     const float* fdv = fdv_in;
     if (need_fdv) {  // recompute the stored SDF values with a forward pass
         if (int rc = geo_fwd_launch<LJ, NH>(a, pts, randn, n, table, params, w.sdf_tmp, nullptr,
@@ -1214,19 +1237,20 @@ static int geo_bwd_launch_t(const GeoArgs& a, const am_grid_desc* g, const float
     }
     hipLaunchKernelGGL(k_pair_total, dim3(1), dim3(64), 0, s, w.offsets, w.counts, tiles, w.status);
     AM_LAUNCH_CHECK();
-    // 3. the pair arrays are carved for the worst case (the size query's
-    // bound); the backward writes each tile's pairs at its offset
+    // 3. one host read of the pair count; the pair arrays are carved for
+    // exactly that many pairs (a workspace of the size query's worst-case
+    // bound always suffices; a smaller one is rejected here, before any
+    // output is written, with am_last_workspace_need() = what this call needs)
+    int64_t st[2] = {0, 0};
+    AM_HIP(hipMemcpyAsync(st, w.status, sizeof(st), hipMemcpyDeviceToHost, s));
+    AM_HIP(hipStreamSynchronize(s));
     TableReduceWs tw;
-    if (int rc = table_reduce_carve(c, 2, geo_max_pairs(a, LJ, n), g->n_entries, &tw,
-                                    PR == mlp::kBF16))
-        return rc;
-    if (!c.ok()) {
-        set_error("workspace too small: need %zu, got %zu", c.used, ws_bytes);
-        return AM_ERR_WORKSPACE;
-    }
+    if (int rc = table_reduce_carve(c, 2, st[0], g->n_entries, &tw, PR == mlp::kBF16)) return rc;
+    if (!c.ok()) return workspace_short(c.used, ws_bytes);
     const size_t lds = (size_t)(a.lds_floats + 4 * 128 * mlp::kScratchLd) * sizeof(float);
     AM_CHECK_ARG(lds <= 160 * 1024, "geometry backward needs %zu bytes of LDS", lds);
     auto kern = store ? k_geo_bwd<LJ, NH, true, PR> : k_geo_bwd<LJ, NH, false, PR>;
+    if constexpr (NH == 1) kern = store ? k_geo_bwd_o2<LJ, true, PR> : k_geo_bwd_o2<LJ, false, PR>;
     AM_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)lds));
     {
@@ -1244,9 +1268,10 @@ static int geo_bwd_launch_t(const GeoArgs& a, const am_grid_desc* g, const float
     }
     AM_LAUNCH_CHECK();
     if (int rc = colsum_f32(w.partial, waves, a.n_params, dparams, w.ctmp, s)) return rc;
-    int64_t st[2] = {0, 0};
+    // the emit pass flags a tile whose pairs differ from its count: checked
+    // before the sort reads the pair arrays
     AM_HIP(hipMemcpyAsync(st, w.status, sizeof(st), hipMemcpyDeviceToHost, s));
-    AM_HIP(hipStreamSynchronize(s));  // one host read: the number of pairs
+    AM_HIP(hipStreamSynchronize(s));
     if (st[1] != 0) {
         set_error("table-gradient pair count mismatch between count and backward passes");
         return AM_ERR_INTERNAL;

@@ -183,10 +183,7 @@ static int hg_bwd(const GridArgs& ga, uint32_t n_entries, const float* x, int64_
         TableReduceWs w;
         int rc = table_reduce_carve(c, F, m, n_entries, &w);
         if (rc) return rc;
-        if (!c.ok()) {
-            set_error("workspace too small: need %zu bytes, got %zu", c.used, ws_bytes);
-            return AM_ERR_WORKSPACE;
-        }
+        if (!c.ok()) return workspace_short(c.used, ws_bytes);
         {
             KernelTimer kt_("hg_emit", s);
             hipLaunchKernelGGL(k_hg_emit<F>, dim3((unsigned)ceil_div(n * active, 256)), dim3(256), 0,

@@ -251,10 +251,7 @@ extern "C" int am_march_count(const float* rays, int64_t n_rays, const float* t_
     int32_t* starts = c.take<int32_t>((size_t)n_rays + 1);
     size_t sb = scan_tmp_bytes(n_rays);
     char* stmp = c.take<char>(sb + 256);
-    if (!c.ok()) {
-        set_error("workspace too small: need %zu, got %zu", c.used, workspace_bytes);
-        return AM_ERR_WORKSPACE;
-    }
+    if (!c.ok()) return workspace_short(c.used, workspace_bytes);
     const MarchArgs a = march_args(roi6_host, grid, grid_res, step, cap);
     const unsigned nb = (unsigned)ceil_div(n_rays, 128);
     {

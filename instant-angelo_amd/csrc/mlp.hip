@@ -331,10 +331,7 @@ static int launch_bwd(const MlpArgs& a, const float* x, int64_t n, const float* 
     WsCarver c(ws, ws_bytes);
     float* partial = c.take<float>((size_t)blocks * a.n_params);
     float* tmp = c.take<float>((size_t)colsum_tmp_floats(blocks, a.n_params) + 1);
-    if (!c.ok()) {
-        set_error("workspace too small: need %zu, got %zu", c.used, ws_bytes);
-        return AM_ERR_WORKSPACE;
-    }
+    if (!c.ok()) return workspace_short(c.used, ws_bytes);
     AM_HIP(hipFuncSetAttribute((const void*)k_mlp_bwd<KS, OT>,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL((k_mlp_bwd<KS, OT>), dim3(blocks), dim3(64), lds, s, a, x, n, params, dy, dx,

@@ -128,10 +128,7 @@ extern "C" int am_occ_update(const am_grid_desc* g, const am_mlp_desc* mlp, cons
     float* bsum = c.take<float>((size_t)blocks);
     float* ctmp = c.take<float>((size_t)colsum_tmp_floats(blocks, 1) + 1);
     float* total = c.take<float>(1);
-    if (!c.ok()) {
-        set_error("workspace too small: need %zu, got %zu", c.used, workspace_bytes);
-        return AM_ERR_WORKSPACE;
-    }
+    if (!c.ok()) return workspace_short(c.used, workspace_bytes);
     am_sdf_desc sd = *s;
     sd.with_grad = 0;
     sd.with_laplace = 0;

@@ -398,10 +398,7 @@ static int rad_bwd_launch(const RadArgs& a, const float* feature, const float* r
     WsCarver c(ws, ws_bytes);
     float* partial = c.take<float>((size_t)waves * a.n_params);
     float* ctmp = c.take<float>((size_t)colsum_tmp_floats(waves, a.n_params) + 1);
-    if (!c.ok()) {
-        set_error("workspace too small: need %zu, got %zu", c.used, ws_bytes);
-        return AM_ERR_WORKSPACE;
-    }
+    if (!c.ok()) return workspace_short(c.used, ws_bytes);
     const size_t lds = (size_t)(a.lds_floats + 4 * 128 * mlp::kScratchLd) * sizeof(float);
     AM_HIP(hipFuncSetAttribute((const void*)k_rad_bwd<KS, NH, PR>,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));

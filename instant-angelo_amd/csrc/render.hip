@@ -392,10 +392,7 @@ extern "C" int am_composite_bwd(const am_composite_desc* c, const float* sdf, co
     WsCarver wc(workspace, workspace_bytes);
     float* per_ray = wc.take<float>((size_t)n_rays);
     float* tmp = wc.take<float>((size_t)colsum_tmp_floats(n_rays, 1) + 1);
-    if (!wc.ok()) {
-        set_error("workspace too small: need %zu, got %zu", wc.used, workspace_bytes);
-        return AM_ERR_WORKSPACE;
-    }
+    if (!wc.ok()) return workspace_short(wc.used, workspace_bytes);
     {
         KernelTimer kt_("composite_bwd", s);
         hipLaunchKernelGGL(k_composite_bwd, dim3((unsigned)ceil_div(n_rays, 4)), dim3(256), 0, s,

@@ -263,6 +263,42 @@ def test_sdf_store_and_recompute_backward_are_identical(dev, laplace):
         assert torch.equal(a, b)
 
 
+def test_sdf_bwd_workspace_sized_by_pairs(dev):
+    """The backward carves its pair arrays for the counted pairs: a short
+    workspace is rejected before any output is written and grown to the
+    reported need; results equal those with the worst-case workspace."""
+    from angelo_amd import _native as nat
+    from angelo_amd import functions
+    model = _field_model(dev)
+    geo = model.geometry
+    g = torch.Generator().manual_seed(7)
+    held = functions.SDF_BWD_WORKSPACE
+
+    def grads(pts, randn):
+        model.zero_grad()
+        sdf, grad, normal, feature, lap = geo.fused(pts, True, True, randn)
+        (sdf.sum() + grad.square().sum() + lap.sum() + feature.sum()).backward()
+        return [p.grad.clone() for p in geo.parameters()]
+
+    try:
+        tight = nat.GrowingWorkspace(initial_bytes=256)
+        for n in (3000, 1100):   # the second, smaller call reuses the grown buffer
+            pts = ((torch.rand(n, 3, generator=g) * 2 - 1) * 0.9).to(dev)
+            randn = torch.randn(n, 3, generator=g).to(dev)
+            grows_before = tight.grows
+            functions.SDF_BWD_WORKSPACE = tight
+            small = grads(pts, randn)
+            assert (tight.grows >= 1) if n == 3000 else (tight.grows == grows_before)
+            functions.SDF_BWD_WORKSPACE = nat.GrowingWorkspace(initial_bytes=4 << 30)
+            big = grads(pts, randn)
+            assert functions.SDF_BWD_WORKSPACE.grows == 0
+            helpers.assert_nontrivial(geo.grid_module.params.grad, "table gradient")
+            for a, b in zip(small, big):
+                assert torch.equal(a, b)
+    finally:
+        functions.SDF_BWD_WORKSPACE = held
+
+
 @pytest.mark.parametrize("rad_hidden", [1, 2])
 def test_radiance_fused_fwd_bwd(dev, rad_hidden):
     model = _field_model(dev, rad_hidden=rad_hidden)
