@@ -2,6 +2,12 @@
 // backward kernel -> stable radix sort by key -> segmented reduce in a fixed
 // order -> one write per touched row; untouched rows are written 0.
 //
+// Two-feature tables (the hash grids here) sort only the key bits above
+// `begin_bit` (two 9-bit passes for 25-27-bit keys instead of three): the
+// pairs of each bucket of 2^begin_bit rows are then contiguous and in
+// emission order, and one wave per bucket finishes the job with an in-wave
+// stable split sort and a segmented scan into per-row accumulators in LDS.
+//
 // The emitters write pairs into carve.keys[0] / carve.vals[0].  Keys equal to
 // sentinel() mark empty slots; they sort last and are skipped.
 #pragma once
@@ -33,10 +39,17 @@ struct TableReduceWs {
     float* tail;
     uint8_t* lead_full;
     uint8_t* tail_open;
+    int64_t* bucket_start;  // [n_buckets + 1] (bucketed form)
+    int64_t n_buckets;      // 0: full sort + chunked segmented reduce
     int64_t m;
+    int begin_bit;          // key bits sorted: [begin_bit, end_bit)
     int end_bit;
     bool half_vals;  // values are ValsH<F> instead of Vals<F>
 };
+
+// Bucket bits of the bucketed form: the low key bits left unsorted (at most
+// 9, so a wave's accumulators span at most 512 rows).
+int table_bucket_bits(int end_bit);
 
 // Number of key bits: smallest b with 2^b > n_entries (so the sentinel
 // (2^b - 1) is >= n_entries and never a valid row).

@@ -11,8 +11,15 @@ int table_key_bits(uint32_t n_entries) {
     return b;
 }
 
-// Row keys of 25-27 bits (tables of 16.7 M - 134 M rows) sort in three
-// 9-bit onesweep passes instead of four 8-bit ones.
+int table_bucket_bits(int end_bit) {
+    const int passes = end_bit > 9 ? (end_bit - 9 + 8) / 9 : 1;
+    const int b = end_bit - 9 * passes;
+    return b > 0 ? b : 0;
+}
+
+// 9-bit onesweep digits: row keys of 25-27 bits (tables of 16.7 M - 134 M
+// rows) sort in three passes instead of four 8-bit ones, and the bucketed
+// form's high bits [7..9, 25..27) in two.
 using Sort9Config = rocprim::radix_sort_config<
     rocprim::default_config, rocprim::default_config,
     rocprim::radix_sort_onesweep_config<rocprim::kernel_config<1024, 8>,
@@ -25,9 +32,10 @@ static hipError_t sort_pairs(void* tmp, size_t& bytes, TableReduceWs& w, uint32_
     rocprim::double_buffer<uint32_t> kb(w.keys[0], w.keys[1]);
     rocprim::double_buffer<VT> vb(reinterpret_cast<VT*>(w.vals[0]), reinterpret_cast<VT*>(w.vals[1]));
     hipError_t e;
-    if (w.end_bit > 24 && w.end_bit <= 27)
-        e = rocprim::radix_sort_pairs<Sort9Config>(tmp, bytes, kb, vb, (size_t)w.m, 0u,
-                                                   (unsigned)w.end_bit, s, false);
+    if (w.n_buckets > 0 || (w.end_bit > 24 && w.end_bit <= 27))
+        e = rocprim::radix_sort_pairs<Sort9Config>(tmp, bytes, kb, vb, (size_t)w.m,
+                                                   (unsigned)w.begin_bit, (unsigned)w.end_bit, s,
+                                                   false);
     else
         e = rocprim::radix_sort_pairs(tmp, bytes, kb, vb, (size_t)w.m, 0u, (unsigned)w.end_bit, s,
                                       false);
@@ -47,11 +55,17 @@ int table_reduce_carve(WsCarver& c, int F, int64_t m, uint32_t n_entries, TableR
     w.keys[1] = c.take<uint32_t>((size_t)m);
     w.vals[0] = c.take<char>((size_t)m * vbytes);
     w.vals[1] = c.take<char>((size_t)m * vbytes);
-    const int64_t nc = ceil_div(m, kSegChunk);
-    w.lead = c.take<float>((size_t)nc * F);
-    w.tail = c.take<float>((size_t)nc * F);
-    w.lead_full = c.take<uint8_t>((size_t)nc);
-    w.tail_open = c.take<uint8_t>((size_t)nc);
+    if (F == 2) {   // bucketed form
+        w.begin_bit = table_bucket_bits(w.end_bit);
+        w.n_buckets = ceil_div((int64_t)n_entries, (int64_t)1 << w.begin_bit);
+        w.bucket_start = c.take<int64_t>((size_t)w.n_buckets + 1);
+    } else {        // chunked segmented reduce
+        const int64_t nc = ceil_div(m, kSegChunk);
+        w.lead = c.take<float>((size_t)nc * F);
+        w.tail = c.take<float>((size_t)nc * F);
+        w.lead_full = c.take<uint8_t>((size_t)nc);
+        w.tail_open = c.take<uint8_t>((size_t)nc);
+    }
     size_t sb = 0;
     hipError_t e = hipSuccess;
     if (half_vals && F != 2) {
@@ -186,6 +200,184 @@ __global__ void k_seg_b(const uint32_t* __restrict__ keys, int64_t m, uint32_t n
     for (int f = 0; f < F; ++f) out[(int64_t)key * F + f] = acc[f];
 }
 
+// Bucketed form.  start[k] = first sorted item of bucket k (keys sorted on
+// bits >= B, so key >> B is non-decreasing), k in [0, n_buckets].
+__global__ void k_bucket_bounds(const uint32_t* __restrict__ keys, int64_t m, int B,
+                                int64_t n_buckets, int64_t* __restrict__ start) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t > n_buckets) return;
+    // start[n_buckets] is searched too: sentinel keys of empty slots may
+    // lie past the last bucket and must stay out of its range
+    int64_t lo = 0, hi = m;
+    while (lo < hi) {
+        const int64_t mid = lo + ((hi - lo) >> 1);
+        if ((int64_t)(keys[mid] >> B) < t) lo = mid + 1;
+        else hi = mid;
+    }
+    start[t] = lo;
+}
+
+__device__ __forceinline__ int popc_below(uint64_t mask) {
+    return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                          __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+__device__ __forceinline__ float permute_f(int dst_byte, float v) {
+    return __int_as_float(__builtin_amdgcn_ds_permute(dst_byte, __float_as_int(v)));
+}
+
+__device__ __forceinline__ float bpermute_f(int src_byte, float v) {
+    return __int_as_float(__builtin_amdgcn_ds_bpermute(src_byte, __float_as_int(v)));
+}
+
+__device__ __forceinline__ void load_pair(const Vals<2>* v, int64_t i, float& a, float& b) {
+    const float2 x = *reinterpret_cast<const float2*>(v + i);
+    a = x.x;
+    b = x.y;
+}
+
+__device__ __forceinline__ void load_pair(const ValsH<2>* v, int64_t i, float& a, float& b) {
+    const __half2 x = *reinterpret_cast<const __half2*>(v + i);
+    a = __low2float(x);
+    b = __high2float(x);
+}
+
+// One step of a segmented inclusive scan over lanes sorted by r: the DPP
+// source lane's partial is added when it holds the same row (lanes whose
+// source is outside the row, or whose row the mask excludes, read the
+// never-matching sentinel).
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ void seg_step(uint32_t r, float& a, float& b) {
+    const uint32_t rs = (uint32_t)__builtin_amdgcn_update_dpp((int)0xffffffffu, (int)r, CTRL,
+                                                              ROW_MASK, 0xf, false);
+    const float as = __int_as_float(
+        __builtin_amdgcn_update_dpp(0, __float_as_int(a), CTRL, ROW_MASK, 0xf, false));
+    const float bs = __int_as_float(
+        __builtin_amdgcn_update_dpp(0, __float_as_int(b), CTRL, ROW_MASK, 0xf, false));
+    if (rs == r) {
+        a += as;
+        b += bs;
+    }
+}
+
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+constexpr int kBucketWaves = 4;
+constexpr int kMaxBucketBits = 9;
+
+// One wave per bucket of 2^B rows (grid-stride over buckets).  Each chunk of
+// 64 items (in emission order) is stable-sorted in registers by its row
+// within the bucket (B split rounds of ballot + permute; a partial chunk sorts
+// one more bit that moves its empty lanes last), reduced by a segmented
+// inclusive scan (DPP row shifts and broadcasts: a fixed tree per segment),
+// and each segment's total is
+// added to the row's LDS accumulator by the segment's last lane (rows are
+// unique per chunk, chunks go in order): the summation order is a fixed
+// function of the sorted input.  Every row of the bucket is then written,
+// zeros included, so dtable needs no separate fill.
+template <class VT>
+__global__ void __launch_bounds__(64 * kBucketWaves) k_bucket_reduce(
+    const uint32_t* __restrict__ keys, const VT* __restrict__ vals,
+    const int64_t* __restrict__ start, int64_t n_buckets, int B, uint32_t n_entries,
+    float* __restrict__ out) {
+    __shared__ float acc_s[kBucketWaves][2 << kMaxBucketBits];
+    const int wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    float* acc = acc_s[wave];
+    const int rows = 1 << B;
+    for (int64_t bk = (int64_t)blockIdx.x * kBucketWaves + wave; bk < n_buckets;
+         bk += (int64_t)gridDim.x * kBucketWaves) {
+        for (int q = lane; q < 2 * rows; q += 64) acc[q] = 0.0f;
+        wave_lds_sync();
+        const int64_t s = start[bk], e = start[bk + 1];
+        uint32_t nk = 0;
+        float na = 0.0f, nb = 0.0f;
+        if (s + lane < e) {
+            nk = keys[s + lane];
+            load_pair(vals, s + lane, na, nb);
+        }
+        for (int64_t c = s; c < e; c += 64) {
+            const bool valid = c + lane < e;
+            const uint32_t r0 = valid ? (nk & (uint32_t)(rows - 1)) : (uint32_t)rows;
+            const float a0 = valid ? na : 0.0f, b0 = valid ? nb : 0.0f;
+            if (c + 64 + lane < e) {   // prefetch the next chunk
+                nk = keys[c + 64 + lane];
+                load_pair(vals, c + 64 + lane, na, nb);
+            }
+            // stable split sort of (row << 6 | source lane): one permute per
+            // round; the values follow once at the end
+            uint32_t rk = (r0 << 6) | (uint32_t)lane;
+            const int nbits = (e - c < 64) ? B + 1 : B;
+            for (int bit = 0; bit < nbits; ++bit) {
+                const bool one = (rk >> (6 + bit)) & 1u;
+                const uint64_t ones = __ballot(one);
+                const uint64_t zeros = ~ones;
+                const int dst = one ? __popcll(zeros) + popc_below(ones) : popc_below(zeros);
+                rk = (uint32_t)__builtin_amdgcn_ds_permute(dst * 4, (int)rk);
+            }
+            const uint32_t r = rk >> 6;
+            const int from = (int)(rk & 63u) * 4;
+            float a = bpermute_f(from, a0);
+            float b = bpermute_f(from, b0);
+            // segmented inclusive scan over the sorted lanes: row shifts of
+            // 1, 2, 4, 8 within 16-lane rows, then the last lane of row 0
+            // (row 2) into row 1 (row 3), then lane 31 into rows 2 and 3
+            seg_step<0x111, 0xf>(r, a, b);
+            seg_step<0x112, 0xf>(r, a, b);
+            seg_step<0x114, 0xf>(r, a, b);
+            seg_step<0x118, 0xf>(r, a, b);
+            seg_step<0x142, 0xa>(r, a, b);
+            seg_step<0x143, 0xc>(r, a, b);
+            const uint32_t rn =
+                (uint32_t)__builtin_amdgcn_ds_bpermute(((lane + 1) & 63) * 4, (int)r);
+            if ((lane == 63 || rn != r) && r < (uint32_t)rows) {
+                acc[2 * r] += a;
+                acc[2 * r + 1] += b;
+            }
+            wave_lds_sync();
+        }
+        const int64_t row0 = bk << B;
+        for (int q = lane; q < rows; q += 64) {
+            const int64_t row = row0 + q;
+            if (row < (int64_t)n_entries)
+                *reinterpret_cast<float2*>(out + 2 * row) = make_float2(acc[2 * q], acc[2 * q + 1]);
+        }
+        wave_lds_sync();
+    }
+}
+
+template <class VT>
+static int run_bucketed(TableReduceWs& w, uint32_t n_entries, float* dtable, hipStream_t s) {
+    uint32_t* ks = w.keys[0];
+    VT* vs = reinterpret_cast<VT*>(w.vals[0]);
+    if (w.m > 0) {
+        size_t sb = w.sort_bytes;
+        hipError_t e;
+        {
+            KernelTimer kt_("radix_sort", s);
+            e = sort_pairs<VT>(w.sort_tmp, sb, w, &ks, &vs, s);
+        }
+        if (e != hipSuccess) {
+            set_error("radix sort failed: %s", hipGetErrorString(e));
+            return AM_ERR_HIP;
+        }
+    }
+    KernelTimer kt_("seg_reduce", s);
+    hipLaunchKernelGGL(k_bucket_bounds, dim3((unsigned)ceil_div(w.n_buckets + 1, 256)), dim3(256),
+                       0, s, ks, w.m, w.begin_bit, w.n_buckets, w.bucket_start);
+    AM_LAUNCH_CHECK();
+    const int64_t want = ceil_div(w.n_buckets, kBucketWaves);
+    const unsigned grid = (unsigned)(want < 8192 ? want : 8192);
+    hipLaunchKernelGGL((k_bucket_reduce<VT>), dim3(grid), dim3(64 * kBucketWaves), 0, s, ks, vs,
+                       w.bucket_start, w.n_buckets, w.begin_bit, n_entries, dtable);
+    AM_LAUNCH_CHECK();
+    return AM_OK;
+}
+
 int fill_f32(float* p, int64_t n, float v, hipStream_t s) {
     if (n <= 0) return AM_OK;
     int blocks = (int)(ceil_div(n, 256) < 8192 ? ceil_div(n, 256) : 8192);
@@ -227,6 +419,14 @@ static int run_f(TableReduceWs& w, uint32_t n_entries, float* dtable, hipStream_
 
 int table_reduce_run(TableReduceWs& w, int F, uint32_t n_entries, float* dtable,
                      hipStream_t stream) {
+    if (F == 2 && w.n_buckets > 0) {
+        if (w.begin_bit > kMaxBucketBits) {
+            set_error("bucket bits %d above %d", w.begin_bit, kMaxBucketBits);
+            return AM_ERR_INTERNAL;
+        }
+        return w.half_vals ? run_bucketed<ValsH<2>>(w, n_entries, dtable, stream)
+                           : run_bucketed<Vals<2>>(w, n_entries, dtable, stream);
+    }
     if (w.half_vals) {
         if (F == 2) return run_f<2, ValsH<2>>(w, n_entries, dtable, stream);
         set_error("fp16 gradient pairs need n_features = 2 (got %d)", F);
