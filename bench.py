@@ -77,6 +77,21 @@ def algorithmic_bytes(kernel, spec, n_samples, n_rays, n_pairs, mixed=False):
     return None
 
 
+def profiled_traffic(workload):
+    """HBM bytes per step of each timed kernel, from the committed summary of
+    two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) of this same bench
+    command, each scaled by the counter's reading for a 1 GiB copy.  None when
+    no summary exists for the workload."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{workload}.md")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("<!-- traffic-json: "):
+                return json.loads(line[len("<!-- traffic-json: "):].rsplit("-->", 1)[0])
+    return None
+
+
 def cpu_baseline(spec_name, workload, seconds=15.0):
     """The oracle (torch CPU, fp32, CUDA-semantics restatement) on a bounded
     sample of the same workload: fwd + losses + bwd, samples/s."""
@@ -255,13 +270,25 @@ def main():
         ach = nbytes / (v["avg_ms"] / 1e3) / 1e9
         per_kernel[k] = {"achieved": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4),
                          "bytes": int(nbytes), "avg_ms": round(v["avg_ms"], 4)}
+    # measured HBM traffic (profiled run of this command; per launch, scaled
+    # by samples/step when this run's sample count differs from the profile's)
+    prof = profiled_traffic(args.workload)
+    if prof:
+        ratio = samples_per_step / max(prof["samples_per_step"], 1.0)
+        for k, pk in per_kernel.items():
+            t = prof["kernels"].get(k)
+            if t:
+                pk["traffic"] = int((t["fetch_bytes"] + t["write_bytes"]) * ratio)
     if kern:
         dom = max(kern, key=lambda k: kern[k]["total_ms"])
         if dom in per_kernel:
             pk = per_kernel[dom]
             roof = {"kernel": dom, "bound": "hbm", "achieved": pk["achieved"], "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": pk["frac"], "traffic": None,
+                    "unit": "GB/s", "frac": pk["frac"], "traffic": pk.get("traffic"),
                     "algorithmic_bytes": pk["bytes"], "avg_ms": pk["avg_ms"]}
+            if pk.get("traffic"):
+                roof["traffic_source"] = f"profiles/hbm_traffic_{args.workload}.md"
+                roof["traffic_gbs"] = round(pk["traffic"] / (pk["avg_ms"] / 1e3) / 1e9, 1)
     serve = None
     if args.serve_requests > 0:
         log(f"[bench] serving: {args.serve_requests} requests at {args.serve_rate}/s ...")

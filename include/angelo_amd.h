@@ -217,8 +217,8 @@ size_t am_sdf_bwd_workspace_size(const am_grid_desc* g, const am_mlp_desc* m,
  * table-gradient pairs first and reads the count back to the host (a stream
  * sync), carving the pair arrays for exactly that many; a workspace below
  * that need returns AM_ERR_WORKSPACE before any output is written
- * (am_last_workspace_need).  A second stream sync reads the emit pass's
- * consistency flag before the sort.
+ * (am_last_workspace_need).  A second stream sync, after the reduction is
+ * queued, reads the emit pass's consistency flag (AM_ERR_INTERNAL if set).
  * Replaces the autograd backward of VolumeSDF.forward
  * (models/geometry.py:195-287) through tcnn's encoding backward. */
 int am_sdf_bwd(const am_grid_desc* g, const am_mlp_desc* m, const am_sdf_desc* s,

@@ -1268,17 +1268,21 @@ static int geo_bwd_launch_t(const GeoArgs& a, const am_grid_desc* g, const float
     }
     AM_LAUNCH_CHECK();
     if (int rc = colsum_f32(w.partial, waves, a.n_params, dparams, w.ctmp, s)) return rc;
-    // the emit pass flags a tile whose pairs differ from its count: checked
-    // before the sort reads the pair arrays
+    tw.m = st[0];
+    timer_units("table_pairs", tw.m);
+    if (int rc = table_reduce_run(tw, 2, g->n_entries, dtable, s)) return rc;
+    // the emit pass flags a tile whose pairs differ from its count.  Read
+    // after the reduction is queued: every emitted pair stays inside its
+    // tile's reserved range and the sort and reduce only index rows below
+    // n_entries and items below the count whatever the keys hold, so a
+    // mismatch gives a wrong dtable (reported here), never a stray access.
     AM_HIP(hipMemcpyAsync(st, w.status, sizeof(st), hipMemcpyDeviceToHost, s));
     AM_HIP(hipStreamSynchronize(s));
     if (st[1] != 0) {
         set_error("table-gradient pair count mismatch between count and backward passes");
         return AM_ERR_INTERNAL;
     }
-    tw.m = st[0];
-    timer_units("table_pairs", tw.m);
-    return table_reduce_run(tw, 2, g->n_entries, dtable, s);
+    return AM_OK;
 }
 
 template <int LJ, int NH>
