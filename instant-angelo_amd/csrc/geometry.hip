@@ -141,26 +141,36 @@ __device__ __forceinline__ void geo_encode(const GeoArgs& a, const LevelParams* 
     }
 }
 
-// xyz input (last K-step) + SDF MLP forward from the encoded slots.
+// xyz input (last K-step) + SDF MLP forward from the encoded slots up to the
+// last hidden pre-activation (all the backward needs).
 template <int LJ, int NH, bool DX, int PR>
-__device__ __forceinline__ void geo_mlp(const GeoArgs& a, const float* sm, const float x[3],
-                                        GeoEval<LJ, NH, DX>& e) {
+__device__ __forceinline__ void geo_mlp_hidden(const GeoArgs& a, const float* sm,
+                                               const float x[3], GeoEval<LJ, NH, DX>& e) {
     constexpr int KS1 = GeoDims<LJ>::KS1;
     const int k = lane_id() >> 4;
     e.xb[2 * LJ] = k < 3 ? pick3(x, k) * 2.0f - 1.0f : 0.0f;
 #pragma unroll
     for (int t = 0; t < 4; ++t) e.z1[t] = mlp::bias_tile(sm + a.b1, t, 64);
     mlp::mm_fwd<KS1, 4, mlp::FragF, PR>(sm + a.w1, a.ld1, mlp::FragF{e.xb}, e.z1);
-    v4f h[4];
-#pragma unroll
-    for (int t = 0; t < 4; ++t) h[t] = mlp::act4(a.act, e.z1[t], a.beta);
     if constexpr (NH == 2) {
+        v4f h[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) h[t] = mlp::act4(a.act, e.z1[t], a.beta);
 #pragma unroll
         for (int t = 0; t < 4; ++t) e.z2[t] = mlp::bias_tile(sm + a.b2, t, 64);
         mlp::mm_fwd<16, 4, mlp::FragV, PR>(sm + a.w2, 65, mlp::FragV{h}, e.z2);
-#pragma unroll
-        for (int t = 0; t < 4; ++t) h[t] = mlp::act4(a.act, e.z2[t], a.beta);
     }
+}
+
+// The full SDF MLP forward (hidden layers, then the output layer into e.out).
+template <int LJ, int NH, bool DX, int PR>
+__device__ __forceinline__ void geo_mlp(const GeoArgs& a, const float* sm, const float x[3],
+                                        GeoEval<LJ, NH, DX>& e) {
+    geo_mlp_hidden<LJ, NH, DX, PR>(a, sm, x, e);
+    v4f h[4];
+    const v4f* zl = NH == 2 ? e.z2 : e.z1;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) h[t] = mlp::act4(a.act, zl[t], a.beta);
     e.out = mlp::bias_tile(sm + a.bo, 0, a.n_out);
     mlp::mm_fwd<16, 1, mlp::FragV, PR>(sm + a.wo, 65, mlp::FragV{h}, &e.out);
 }
@@ -229,7 +239,7 @@ __device__ __forceinline__ void geo_load(const float* __restrict__ st, const Geo
 }
 
 // Backward-side evaluation: the stored encoding (ST) or a fresh encode, then
-// the MLP forward.
+// the MLP forward up to the last hidden pre-activation.
 template <int LJ, int NH, bool DX, bool ST, int PR, class TT>
 __device__ __forceinline__ void geo_eval_b(const GeoArgs& a, const float* sm, const LevelParams* lv,
                                            const TT* __restrict__ table,
@@ -238,7 +248,7 @@ __device__ __forceinline__ void geo_eval_b(const GeoArgs& a, const float* sm, co
                                            GeoEval<LJ, NH, DX>& e) {
     if constexpr (ST) geo_load<LJ, NH, DX>(st, a, tile, n_tiles, ev, e);
     else geo_encode<LJ, NH, DX>(a, lv, table, x, e);
-    geo_mlp<LJ, NH, DX, PR>(a, sm, x, e);
+    geo_mlp_hidden<LJ, NH, DX, PR>(a, sm, x, e);
 }
 
 __device__ __forceinline__ float contract1(float v, const GeoArgs& a) { return (v + a.r) * a.inv2r; }
@@ -462,10 +472,10 @@ __device__ __forceinline__ void geo_eval_bwd(const GeoArgs& a, const float* sm,
                                              float gpos[3]) {
     constexpr int KS1 = GeoDims<LJ>::KS1, IT1 = GeoDims<LJ>::IT1;
     const int l = lane_id(), k = l >> 4;
-    v4f h[4];
+    v4f h[4], dh[4];   // last hidden activation and its derivative
     const v4f* zl = NH == 2 ? e.z2 : e.z1;
 #pragma unroll
-    for (int t = 0; t < 4; ++t) h[t] = mlp::act4(a.act, zl[t], a.beta);
+    for (int t = 0; t < 4; ++t) mlp::act_fb4(a.act, zl[t], a.beta, h[t], dh[t]);
     mlp::mm_dw<1, 4, 16, mlp::FragV, PR>(&go, mlp::FragV{h}, scratch, G.dWo);
     G.dbo += go;
     v4f gh[4];
@@ -476,9 +486,8 @@ __device__ __forceinline__ void geo_eval_bwd(const GeoArgs& a, const float* sm,
         v4f gz2[4], h1[4];
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-#pragma unroll
-            for (int r = 0; r < 4; ++r) gz2[t][r] = gh[t][r] * mlp::act_bwd(a.act, e.z2[t][r], a.beta);
-            h1[t] = mlp::act4(a.act, e.z1[t], a.beta);
+            gz2[t] = gh[t] * dh[t];
+            mlp::act_fb4(a.act, e.z1[t], a.beta, h1[t], dh[t]);
             G.db2[t] += gz2[t];
         }
         mlp::mm_dw<4, 4, 16, mlp::FragV, PR>(gz2, mlp::FragV{h1}, scratch, G.dW2);
@@ -486,11 +495,10 @@ __device__ __forceinline__ void geo_eval_bwd(const GeoArgs& a, const float* sm,
         for (int t = 0; t < 4; ++t) gh[t] = v4f{0, 0, 0, 0};
         mlp::mm_bwd_in<16, 4, mlp::FragV, PR>(sm + a.w2, 65, mlp::FragV{gz2}, gh);
     }
-    v4f gz1[4];
+    v4f gz1[4];   // dh now holds the first hidden layer's derivative
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
-#pragma unroll
-        for (int r = 0; r < 4; ++r) gz1[t][r] = gh[t][r] * mlp::act_bwd(a.act, e.z1[t][r], a.beta);
+        gz1[t] = gh[t] * dh[t];
         G.db1[t] += gz1[t];
     }
     mlp::mm_dw<4, IT1, KS1, mlp::FragF, PR>(gz1, mlp::FragF{e.xb}, scratch, G.dW1);
@@ -569,10 +577,7 @@ __device__ __forceinline__ void emit_pairs(EmitTile<VT>& et, bool on, const uint
 // The 8 corner rows of `cell` at level `lp` (corner c = cell + bits of c).
 __device__ __forceinline__ void corner_rows(const LevelParams& lp, const uint32_t cell[3],
                                             uint32_t rows[8]) {
-#pragma unroll
-    for (int c = 0; c < 8; ++c)
-        rows[c] = lp.offset + grid_row(lp, cell[0] + (c & 1), cell[1] + ((c >> 1) & 1),
-                                       cell[2] + ((c >> 2) & 1));
+    cell_rows(lp, cell, rows);
 }
 
 // Corner i (0..3) of the face of a cell with bit `ax` fixed to `b`.

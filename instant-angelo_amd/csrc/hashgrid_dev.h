@@ -35,17 +35,47 @@ struct GridArgs {
 
 GridArgs make_grid_args(const am_grid_desc* g);
 
+// idx mod rows.  Power-of-two levels (every hashed level) mask; a dense
+// level's index is below 2 rows for every cell of [0, 1]^3 (coordinates up to
+// res, rows >= res^3), so one conditional subtraction does; the division is
+// left for inputs outside the unit cube.
+__device__ __forceinline__ uint32_t wrap_row(uint32_t idx, uint32_t size_flags) {
+    const uint32_t sz = size_flags & kSizeMask;
+    if (size_flags & kPow2Bit) return idx & (sz - 1u);
+    if (idx < sz) return idx;
+    const uint32_t d = idx - sz;
+    if (__builtin_expect(d < sz, 1)) return d;
+    return idx % sz;
+}
+
 __device__ __forceinline__ uint32_t grid_row(const LevelParams& p, uint32_t cx, uint32_t cy,
                                              uint32_t cz) {
-    const uint32_t sz = p.size_flags & kSizeMask;
     uint32_t idx;
     if (p.size_flags & kHashedBit) {
         idx = cx ^ (cy * 2654435761u) ^ (cz * 805459861u);
     } else {
         idx = cx + cy * p.res + cz * (p.res * p.res);
     }
-    if (p.size_flags & kPow2Bit) return idx & (sz - 1u);
-    return idx < sz ? idx : idx % sz;
+    return wrap_row(idx, p.size_flags);
+}
+
+// The 8 corner rows of `cell` (corner c = cell + bits of c, x = bit 0) with
+// the per-axis products formed once: (c+1) * m = c * m + m (mod 2^32).
+__device__ __forceinline__ void cell_rows(const LevelParams& lp, const uint32_t cell[3],
+                                          uint32_t rows[8]) {
+    const bool hashed = (lp.size_flags & kHashedBit) != 0u;
+    const uint32_t my = hashed ? 2654435761u : lp.res;
+    const uint32_t mz = hashed ? 805459861u : lp.res * lp.res;
+    const uint32_t y0 = cell[1] * my, y1 = y0 + my;
+    const uint32_t z0 = cell[2] * mz, z1 = z0 + mz;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        const uint32_t x = cell[0] + (uint32_t)(c & 1);
+        const uint32_t y = (c & 2) ? y1 : y0;
+        const uint32_t z = (c & 4) ? z1 : z0;
+        const uint32_t idx = hashed ? (x ^ y ^ z) : (x + y + z);
+        rows[c] = lp.offset + wrap_row(idx, lp.size_flags);
+    }
 }
 
 // Cell and fractional position of x at one level.
@@ -102,9 +132,11 @@ __device__ __forceinline__ void encode_level_f2(const LevelParams& lp, const flo
                                                 const TT* __restrict__ table, float* out2,
                                                 float (*dx)[3]) {
     const LevelPos p = level_pos(lp, x);
+    uint32_t rows[8];
+    cell_rows(lp, p.c, rows);
     float2 v[8];
 #pragma unroll
-    for (int c = 0; c < 8; ++c) v[c] = load_row2(table, corner_row(lp, p, c));
+    for (int c = 0; c < 8; ++c) v[c] = load_row2(table, rows[c]);
     float a0 = 0.0f, a1 = 0.0f;
 #pragma unroll
     for (int c = 0; c < 8; ++c) {

@@ -65,12 +65,41 @@ __device__ __forceinline__ void wave_sync() {
 }
 
 // ---- activations (torch semantics) ----
+// log1p(t) for t in [0, 1]: t * ln(u) / (u - 1) with u = fl(1 + t), which
+// cancels the rounding of u (t itself when u == 1); hardware log2/rcp.
+__device__ __forceinline__ float log1p_unit(float t) {
+    const float u = 1.0f + t;
+    const float d = u - 1.0f;
+    const float l = __builtin_amdgcn_logf(u) * 0.693147180559945309f;
+    return d == 0.0f ? t : t * l * __builtin_amdgcn_rcpf(d);
+}
+
+// torch softplus(z, beta, threshold 20) = bz > 20 ? z : log1p(exp(bz)) / beta
+// with bz = beta z, in the overflow-free form (max(bz, 0) + log1p(t)) / beta,
+// t = exp(-|bz|) in (0, 1]; its derivative sigmoid(bz) is 1/(1+t) (bz >= 0)
+// or t/(1+t) from the same t.
+__device__ __forceinline__ float softplus_t(float bz) {
+    return __builtin_amdgcn_exp2f(-fabsf(bz) * 1.44269504088896341f);
+}
+
+__device__ __forceinline__ float softplus_fwd(float z, float beta) {
+    const float bz = z * beta;
+    const float t = softplus_t(bz);
+    return bz > 20.0f ? z : (fmaxf(bz, 0.0f) + log1p_unit(t)) * (1.0f / beta);
+}
+
+__device__ __forceinline__ void softplus_fb(float z, float beta, float& h, float& dh) {
+    const float bz = z * beta;
+    const float t = softplus_t(bz);
+    const float r = __builtin_amdgcn_rcpf(1.0f + t);
+    const bool big = bz > 20.0f;
+    h = big ? z : (fmaxf(bz, 0.0f) + log1p_unit(t)) * (1.0f / beta);
+    dh = big ? 1.0f : (bz >= 0.0f ? r : t * r);
+}
+
 __device__ __forceinline__ float act_fwd(int act, float z, float beta) {
     if (act == AM_ACT_RELU) return z > 0.0f ? z : 0.0f;
-    if (act == AM_ACT_SOFTPLUS) {
-        const float bz = z * beta;
-        return bz > 20.0f ? z : log1pf(expf(bz)) / beta;
-    }
+    if (act == AM_ACT_SOFTPLUS) return softplus_fwd(z, beta);
     if (act == AM_ACT_SIGMOID) return 1.0f / (1.0f + expf(-z));
     return z;
 }
@@ -79,16 +108,35 @@ __device__ __forceinline__ float act_fwd(int act, float z, float beta) {
 __device__ __forceinline__ float act_bwd(int act, float z, float beta) {
     if (act == AM_ACT_RELU) return z > 0.0f ? 1.0f : 0.0f;
     if (act == AM_ACT_SOFTPLUS) {
-        const float bz = z * beta;
-        if (bz > 20.0f) return 1.0f;
-        const float e = expf(bz);
-        return e / (e + 1.0f);
+        float h, dh;
+        softplus_fb(z, beta, h, dh);
+        return dh;
     }
     if (act == AM_ACT_SIGMOID) {
         const float y = 1.0f / (1.0f + expf(-z));
         return (1.0f - y) * y;
     }
     return 1.0f;
+}
+
+// Activation and its derivative at z together (one exp for softplus).
+__device__ __forceinline__ void act_fb(int act, float z, float beta, float& h, float& dh) {
+    if (act == AM_ACT_SOFTPLUS) {
+        softplus_fb(z, beta, h, dh);
+    } else {
+        h = act_fwd(act, z, beta);
+        dh = act_bwd(act, z, beta);
+    }
+}
+
+__device__ __forceinline__ void act_fb4(int act, v4f z, float beta, v4f& h, v4f& dh) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        float a, d;
+        act_fb(act, z[r], beta, a, d);
+        h[r] = a;
+        dh[r] = d;
+    }
 }
 
 __device__ __forceinline__ v4f act4(int act, v4f z, float beta) {
