@@ -241,6 +241,7 @@ def main():
     stop.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    nat.sdf_bwd_check()   # the pair passes of every timed backward agreed
     gpu_ms = start.elapsed_time(stop)
     kern = {}
     for k in TIMED_KERNELS:

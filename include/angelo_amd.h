@@ -217,10 +217,16 @@ size_t am_sdf_bwd_workspace_size(const am_grid_desc* g, const am_mlp_desc* m,
  * table-gradient pairs first and reads the count back to the host (a stream
  * sync), carving the pair arrays for exactly that many; a workspace below
  * that need returns AM_ERR_WORKSPACE before any output is written
- * (am_last_workspace_need).  A second stream sync, after the reduction is
- * queued, reads the emit pass's consistency flag (AM_ERR_INTERNAL if set).
+ * (am_last_workspace_need).  The emit pass's consistency flag (its pairs
+ * equal the count pass's) is copied to pinned host memory without a wait and
+ * read by later calls, after their own count sync, and by am_sdf_bwd_check;
+ * a set flag returns AM_ERR_INTERNAL there.
  * Replaces the autograd backward of VolumeSDF.forward
  * (models/geometry.py:195-287) through tcnn's encoding backward. */
+/* Waits for the consistency flags of every am_sdf_bwd call so far:
+ * AM_OK, or AM_ERR_INTERNAL if one was set.  No counterpart in the
+ * reference (its backward has no pair passes to cross-check). */
+int am_sdf_bwd_check(void);
 int am_sdf_bwd(const am_grid_desc* g, const am_mlp_desc* m, const am_sdf_desc* s,
                const float* pts, const float* randn, int64_t n, const void* table,
                const float* params, const float* fd_values, const float* enc_store,

@@ -72,6 +72,7 @@ _SZ = ctypes.c_size_t
 _SIGS = {
     "am_last_error": (ctypes.c_char_p, []),
     "am_last_workspace_need": (_SZ, []),
+    "am_sdf_bwd_check": (ctypes.c_int, []),
     "am_version": (ctypes.c_int, []),
     "am_timer_enable": (ctypes.c_int, [_I32]),
     "am_timer_read": (ctypes.c_int, [ctypes.c_char_p, _P, _P]),
@@ -206,6 +207,12 @@ class GrowingWorkspace:
             buf = self._bufs[key] = workspace(int(need * self.headroom), device)
             self.grows += 1
         check(rc, what)
+
+
+def sdf_bwd_check():
+    """Raises if any SDF backward so far had inconsistent pair passes (the
+    library reads those flags without waiting; this waits for all)."""
+    check(load().am_sdf_bwd_check(), "am_sdf_bwd_check")
 
 
 def timer_enable(on=True):

@@ -108,6 +108,18 @@ int64_t colsum_tmp_floats(int64_t rows, int64_t cols);
 int colsum_f32(const float* in, int64_t rows, int64_t cols, float* out, float* tmp,
                hipStream_t stream);
 
+// Workgroups are dispatched to the chip's 8 XCDs round-robin (block b runs
+// on XCD b % 8, each XCD with its own L2).  Renumbering b to
+// start(b % 8) + b / 8 gives each XCD a consecutive range of virtual block
+// indices, so work items that neighbour each other in index order (samples
+// along one ray) share an L2.  A bijection of [0, g); identity below 8 blocks.
+__device__ __forceinline__ int xcd_block(int b, int g) {
+    if (g < 8) return b;
+    const int x = b & 7, j = b >> 3;
+    const int q = g >> 3, r = g & 7;
+    return x * q + (x < r ? x : r) + j;
+}
+
 // Persistent-grid block count: enough blocks for `tiles` work items of
 // `per_block` each, capped at `max_blocks`.
 static inline int persistent_blocks(int64_t tiles, int per_block, int max_blocks) {

@@ -34,6 +34,7 @@
 #include "mlp_dev.h"
 #include "table_grad.h"
 #include "field_common.h"
+#include <mutex>
 #include <type_traits>
 #include <rocprim/device/device_scan.hpp>
 
@@ -367,7 +368,8 @@ __global__ void __launch_bounds__(256) k_geo_fwd(GeoArgs a, const float* __restr
     const int nw = blockDim.x >> 6, wave = threadIdx.x >> 6;
     const int64_t n_tiles = (N + 15) / 16;
     const int fdim = a.n_out + 3;
-    for (int64_t tile = (int64_t)blockIdx.x * nw + wave; tile < n_tiles;
+    const int vb = xcd_block((int)blockIdx.x, (int)gridDim.x);   // the table gathers share L2
+    for (int64_t tile = (int64_t)vb * nw + wave; tile < n_tiles;
          tile += (int64_t)gridDim.x * nw) {
         const int64_t n = tile * 16 + p;
         const bool valid = n < N;
@@ -1208,6 +1210,70 @@ static int64_t geo_max_pairs(const GeoArgs& a, int LJ, int64_t n) {
     return (n + 15) / 16 * ((int64_t)a.n_evals * LJ * 8 * 64);
 }
 
+// Consistency flags of recent backward calls.  The emit pass flags a tile
+// whose pairs differ from the count pass's; instead of a stream sync per
+// call, each call copies its flag into a pinned host slot behind an event,
+// and later calls (after their own pair-count sync, cheaply) and
+// am_sdf_bwd_check read the slots whose copies have landed.  Every emitted
+// pair stays inside its tile's reserved range and the sort and reduce only
+// index rows below n_entries and items below the count whatever the keys
+// hold, so a mismatch gives a wrong dtable, never a stray access.
+namespace {
+struct FlagSlot {
+    hipEvent_t ev = nullptr;
+    int64_t* flag = nullptr;   // pinned host memory
+    bool armed = false;
+};
+constexpr int kFlagSlots = 16;
+std::mutex g_flag_mu;
+FlagSlot g_flag_slots[kFlagSlots];
+int g_flag_next = 0;
+}  // namespace
+
+static int flag_result(FlagSlot& f) {
+    f.armed = false;
+    if (*f.flag != 0) {
+        set_error("table-gradient pair count mismatch between the count and emit passes of an "
+                  "earlier am_sdf_bwd call");
+        return AM_ERR_INTERNAL;
+    }
+    return AM_OK;
+}
+
+// Reads the armed slots: those whose copy has landed, or all (wait).
+static int collect_flags(bool wait) {
+    std::lock_guard<std::mutex> lk(g_flag_mu);
+    int rc = AM_OK;
+    for (FlagSlot& f : g_flag_slots) {
+        if (!f.armed) continue;
+        const hipError_t e = wait ? hipEventSynchronize(f.ev) : hipEventQuery(f.ev);
+        if (e == hipErrorNotReady) continue;
+        if (e != hipSuccess) {
+            set_error("consistency flag event failed: %s", hipGetErrorString(e));
+            return AM_ERR_HIP;
+        }
+        if (int r = flag_result(f)) rc = r;
+    }
+    return rc;
+}
+
+static int arm_flag(const int64_t* dev_flag, hipStream_t s) {
+    std::lock_guard<std::mutex> lk(g_flag_mu);
+    FlagSlot& f = g_flag_slots[g_flag_next];
+    g_flag_next = (g_flag_next + 1) % kFlagSlots;
+    if (f.ev == nullptr) {
+        AM_HIP(hipEventCreateWithFlags(&f.ev, hipEventDisableTiming));
+        AM_HIP(hipHostMalloc((void**)&f.flag, sizeof(int64_t), hipHostMallocDefault));
+    } else if (f.armed) {   // the ring wrapped onto a pending slot
+        AM_HIP(hipEventSynchronize(f.ev));
+        if (int rc = flag_result(f)) return rc;
+    }
+    AM_HIP(hipMemcpyAsync(f.flag, dev_flag, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    AM_HIP(hipEventRecord(f.ev, s));
+    f.armed = true;
+    return AM_OK;
+}
+
 template <int LJ, int NH, int PR>
 static int geo_bwd_launch_t(const GeoArgs& a, const am_grid_desc* g, const float* pts,
                             const float* randn, int64_t n, const void* table, const float* params,
@@ -1249,6 +1315,7 @@ static int geo_bwd_launch_t(const GeoArgs& a, const am_grid_desc* g, const float
     int64_t st[2] = {0, 0};
     AM_HIP(hipMemcpyAsync(st, w.status, sizeof(st), hipMemcpyDeviceToHost, s));
     AM_HIP(hipStreamSynchronize(s));
+    if (int rc = collect_flags(false)) return rc;
     TableReduceWs tw;
     if (int rc = table_reduce_carve(c, 2, st[0], g->n_entries, &tw, PR == mlp::kBF16)) return rc;
     if (!c.ok()) return workspace_short(c.used, ws_bytes);
@@ -1276,18 +1343,7 @@ static int geo_bwd_launch_t(const GeoArgs& a, const am_grid_desc* g, const float
     tw.m = st[0];
     timer_units("table_pairs", tw.m);
     if (int rc = table_reduce_run(tw, 2, g->n_entries, dtable, s)) return rc;
-    // the emit pass flags a tile whose pairs differ from its count.  Read
-    // after the reduction is queued: every emitted pair stays inside its
-    // tile's reserved range and the sort and reduce only index rows below
-    // n_entries and items below the count whatever the keys hold, so a
-    // mismatch gives a wrong dtable (reported here), never a stray access.
-    AM_HIP(hipMemcpyAsync(st, w.status, sizeof(st), hipMemcpyDeviceToHost, s));
-    AM_HIP(hipStreamSynchronize(s));
-    if (st[1] != 0) {
-        set_error("table-gradient pair count mismatch between count and backward passes");
-        return AM_ERR_INTERNAL;
-    }
-    return AM_OK;
+    return arm_flag(w.status + 1, s);
 }
 
 template <int LJ, int NH>
@@ -1360,6 +1416,8 @@ extern "C" size_t am_sdf_bwd_workspace_size(const am_grid_desc* g, const am_mlp_
     if (table_reduce_carve(c, 2, geo_max_pairs(a, LJ, n), g->n_entries, &tw)) return 0;
     return c.used + 256;
 }
+
+extern "C" int am_sdf_bwd_check(void) { return collect_flags(true); }
 
 extern "C" int am_sdf_bwd(const am_grid_desc* g, const am_mlp_desc* m, const am_sdf_desc* sd,
                           const float* pts, const float* randn, int64_t n, const void* table,

@@ -232,6 +232,8 @@ def test_sdf_bwd_deterministic(dev):
     helpers.assert_nontrivial(geo.grid_module.params.grad, "table gradient")
     for a, b in zip(*grads):
         assert torch.equal(a, b)
+    from angelo_amd import _native as nat
+    nat.sdf_bwd_check()   # count and emit passes agreed on every call
 
 
 @pytest.mark.parametrize("laplace", [False, True])
