@@ -3,7 +3,7 @@
 // order -> one write per touched row; untouched rows are written 0.
 //
 // Two-feature tables (the hash grids here) sort only the key bits above
-// `begin_bit` (two 9-bit passes for 25-27-bit keys instead of three): the
+// `begin_bit` (two passes for 25-27-bit keys instead of three): the
 // pairs of each bucket of 2^begin_bit rows are then contiguous and in
 // emission order, and one wave per bucket finishes the job with an in-wave
 // stable split sort and a segmented scan into per-row accumulators in LDS.
@@ -40,6 +40,11 @@ struct TableReduceWs {
     uint8_t* lead_full;
     uint8_t* tail_open;
     int64_t* bucket_start;  // [n_buckets + 1] (bucketed form)
+    uint64_t* units;        // [n_buckets + 1] packed work-unit counts
+    uint64_t* unit_off;     // [n_buckets + 1] their exclusive scan
+    float* partial;         // partial sums of multi-unit buckets
+    void* scan_tmp;
+    size_t scan_bytes;
     int64_t n_buckets;      // 0: full sort + chunked segmented reduce
     int64_t m;
     int begin_bit;          // key bits sorted: [begin_bit, end_bit)

@@ -1,9 +1,16 @@
 // Deterministic table-gradient reduction: stable radix sort of (row, value)
 // pairs, then a fixed-order segmented reduce (see table_grad.h).
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 #include "table_grad.h"
 
 namespace am {
+
+constexpr int64_t kUnitItems = 16384;   // sorted items per work unit (256 chunks)
+
+// Partial-sum slots the units of multi-unit buckets can need: each such
+// bucket holds more than kUnitItems items and fewer than 2n / kUnitItems units.
+static int64_t bucket_slots(int64_t m) { return 2 * (m / kUnitItems) + 1; }
 
 int table_key_bits(uint32_t n_entries) {
     int b = 1;
@@ -11,19 +18,32 @@ int table_key_bits(uint32_t n_entries) {
     return b;
 }
 
+// Bucketed form: up to 9 low key bits stay unsorted.  Keys of up to 25 bits
+// sort their high bits [9, end) in one or two 8-bit onesweep passes (the
+// fastest form measured for 16 bits: 8-bit digits rank with less LDS than
+// 9-bit ones); 26-27-bit keys in two 9-bit passes, longer ones in three.
 int table_bucket_bits(int end_bit) {
-    const int passes = end_bit > 9 ? (end_bit - 9 + 8) / 9 : 1;
-    const int b = end_bit - 9 * passes;
-    return b > 0 ? b : 0;
+    if (end_bit <= 9) return 0;
+    if (end_bit <= 25) return 9;
+    const int passes = (end_bit - 9 + 8) / 9;
+    return end_bit - 9 * passes;
 }
 
-// 9-bit onesweep digits: row keys of 25-27 bits (tables of 16.7 M - 134 M
-// rows) sort in three passes instead of four 8-bit ones, and the bucketed
-// form's high bits [7..9, 25..27) in two.
+static int digit_bits(const TableReduceWs& w) {
+    return (w.n_buckets > 0 && w.begin_bit == 9) ? 8 : 9;
+}
+
+// Onesweep configurations: 1024-thread blocks of 8 items per thread for the
+// sort passes, 32 items per thread for the histogram pass.
 using Sort9Config = rocprim::radix_sort_config<
     rocprim::default_config, rocprim::default_config,
-    rocprim::radix_sort_onesweep_config<rocprim::kernel_config<1024, 8>,
+    rocprim::radix_sort_onesweep_config<rocprim::kernel_config<1024, 32>,
                                         rocprim::kernel_config<1024, 8>, 9,
+                                        rocprim::block_radix_rank_algorithm::match>>;
+using Sort8Config = rocprim::radix_sort_config<
+    rocprim::default_config, rocprim::default_config,
+    rocprim::radix_sort_onesweep_config<rocprim::kernel_config<1024, 32>,
+                                        rocprim::kernel_config<1024, 8>, 8,
                                         rocprim::block_radix_rank_algorithm::match>>;
 
 template <class VT>
@@ -32,7 +52,11 @@ static hipError_t sort_pairs(void* tmp, size_t& bytes, TableReduceWs& w, uint32_
     rocprim::double_buffer<uint32_t> kb(w.keys[0], w.keys[1]);
     rocprim::double_buffer<VT> vb(reinterpret_cast<VT*>(w.vals[0]), reinterpret_cast<VT*>(w.vals[1]));
     hipError_t e;
-    if (w.n_buckets > 0 || (w.end_bit > 24 && w.end_bit <= 27))
+    if (w.n_buckets > 0 && digit_bits(w) == 8)
+        e = rocprim::radix_sort_pairs<Sort8Config>(tmp, bytes, kb, vb, (size_t)w.m,
+                                                   (unsigned)w.begin_bit, (unsigned)w.end_bit, s,
+                                                   false);
+    else if (w.n_buckets > 0 || (w.end_bit > 24 && w.end_bit <= 27))
         e = rocprim::radix_sort_pairs<Sort9Config>(tmp, bytes, kb, vb, (size_t)w.m,
                                                    (unsigned)w.begin_bit, (unsigned)w.end_bit, s,
                                                    false);
@@ -59,6 +83,18 @@ int table_reduce_carve(WsCarver& c, int F, int64_t m, uint32_t n_entries, TableR
         w.begin_bit = table_bucket_bits(w.end_bit);
         w.n_buckets = ceil_div((int64_t)n_entries, (int64_t)1 << w.begin_bit);
         w.bucket_start = c.take<int64_t>((size_t)w.n_buckets + 1);
+        w.units = c.take<uint64_t>((size_t)w.n_buckets + 1);
+        w.unit_off = c.take<uint64_t>((size_t)w.n_buckets + 1);
+        w.partial = c.take<float>((size_t)bucket_slots(m) * (2u << w.begin_bit));
+        size_t scb = 0;
+        if (rocprim::exclusive_scan(nullptr, scb, (uint64_t*)nullptr, (uint64_t*)nullptr,
+                                    (uint64_t)0, (size_t)w.n_buckets + 1,
+                                    rocprim::plus<uint64_t>(), 0, false) != hipSuccess) {
+            set_error("bucket unit scan size query failed");
+            return AM_ERR_HIP;
+        }
+        w.scan_bytes = scb;
+        w.scan_tmp = c.take<char>(scb + 256);
     } else {        // chunked segmented reduce
         const int64_t nc = ceil_div(m, kSegChunk);
         w.lead = c.take<float>((size_t)nc * F);
@@ -269,84 +305,160 @@ __device__ __forceinline__ void wave_lds_sync() {
 constexpr int kBucketWaves = 4;
 constexpr int kMaxBucketBits = 9;
 
-// One wave per bucket of 2^B rows (grid-stride over buckets).  Each chunk of
-// 64 items (in emission order) is stable-sorted in registers by its row
-// within the bucket (B split rounds of ballot + permute; a partial chunk sorts
-// one more bit that moves its empty lanes last), reduced by a segmented
-// inclusive scan (DPP row shifts and broadcasts: a fixed tree per segment),
-// and each segment's total is
-// added to the row's LDS accumulator by the segment's last lane (rows are
-// unique per chunk, chunks go in order): the summation order is a fixed
-// function of the sorted input.  Every row of the bucket is then written,
-// zeros included, so dtable needs no separate fill.
+// Work units: a bucket of n items is cut into max(1, ceil(n / kUnitItems))
+// consecutive ranges, so a bucket of a coarse (dense, heavily shared) level
+// is spread over many waves.  units[b] packs the bucket's unit count (low 32
+// bits) and, for a bucket of several units, the same count again (high 32
+// bits: its partial-sum slots); an exclusive scan of the packed words gives
+// each bucket's first unit and first slot at once.  units[n_buckets] = 0, so
+// the scan's last entry holds the totals.
+__global__ void k_bucket_units(const int64_t* __restrict__ start, int64_t n_buckets,
+                               uint64_t* __restrict__ units) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t > n_buckets) return;
+    uint64_t v = 0;
+    if (t < n_buckets) {
+        const int64_t n = start[t + 1] - start[t];
+        const uint64_t ns = n <= kUnitItems ? 1u : (uint64_t)((n + kUnitItems - 1) / kUnitItems);
+        v = ns | (ns > 1 ? ns << 32 : 0);
+    }
+    units[t] = v;
+}
+
+// Reduces sorted items [s, e) of one bucket into the wave's LDS accumulators
+// acc[2 << B].  Each chunk of 64 items (in emission order) is stable-sorted
+// in registers by its row within the bucket (B split rounds of ballot +
+// permute; a partial chunk sorts one more bit that moves its empty lanes
+// last), reduced by a segmented inclusive scan (DPP row shifts and
+// broadcasts: a fixed tree per segment), and each segment's total is added
+// to the row's accumulator by the segment's last lane (rows are unique per
+// chunk, chunks go in order): the summation order is a fixed function of the
+// sorted input.
+template <class VT>
+__device__ __forceinline__ void reduce_range(const uint32_t* __restrict__ keys,
+                                             const VT* __restrict__ vals, int64_t s, int64_t e,
+                                             int B, float* acc) {
+    const int lane = threadIdx.x & 63;
+    const int rows = 1 << B;
+    uint32_t nk = 0;
+    float na = 0.0f, nb = 0.0f;
+    if (s + lane < e) {
+        nk = keys[s + lane];
+        load_pair(vals, s + lane, na, nb);
+    }
+    for (int64_t c = s; c < e; c += 64) {
+        const bool valid = c + lane < e;
+        const uint32_t r0 = valid ? (nk & (uint32_t)(rows - 1)) : (uint32_t)rows;
+        const float a0 = valid ? na : 0.0f, b0 = valid ? nb : 0.0f;
+        if (c + 64 + lane < e) {   // prefetch the next chunk
+            nk = keys[c + 64 + lane];
+            load_pair(vals, c + 64 + lane, na, nb);
+        }
+        // stable split sort of (row << 6 | source lane): one permute per
+        // round; the values follow once at the end
+        uint32_t rk = (r0 << 6) | (uint32_t)lane;
+        const int nbits = (e - c < 64) ? B + 1 : B;
+        for (int bit = 0; bit < nbits; ++bit) {
+            const bool one = (rk >> (6 + bit)) & 1u;
+            const uint64_t ones = __ballot(one);
+            const uint64_t zeros = ~ones;
+            const int dst = one ? __popcll(zeros) + popc_below(ones) : popc_below(zeros);
+            rk = (uint32_t)__builtin_amdgcn_ds_permute(dst * 4, (int)rk);
+        }
+        const uint32_t r = rk >> 6;
+        const int from = (int)(rk & 63u) * 4;
+        float a = bpermute_f(from, a0);
+        float b = bpermute_f(from, b0);
+        // segmented inclusive scan over the sorted lanes: row shifts of
+        // 1, 2, 4, 8 within 16-lane rows, then the last lane of row 0
+        // (row 2) into row 1 (row 3), then lane 31 into rows 2 and 3
+        seg_step<0x111, 0xf>(r, a, b);
+        seg_step<0x112, 0xf>(r, a, b);
+        seg_step<0x114, 0xf>(r, a, b);
+        seg_step<0x118, 0xf>(r, a, b);
+        seg_step<0x142, 0xa>(r, a, b);
+        seg_step<0x143, 0xc>(r, a, b);
+        const uint32_t rn = (uint32_t)__builtin_amdgcn_ds_bpermute(((lane + 1) & 63) * 4, (int)r);
+        if ((lane == 63 || rn != r) && r < (uint32_t)rows) {
+            acc[2 * r] += a;
+            acc[2 * r + 1] += b;
+        }
+        wave_lds_sync();
+    }
+}
+
+// One wave per work unit (grid-stride).  A single-unit bucket writes its
+// rows to dtable, zeros included (dtable needs no fill); a unit of a
+// multi-unit bucket writes its accumulators to its partial slot.
 template <class VT>
 __global__ void __launch_bounds__(64 * kBucketWaves) k_bucket_reduce(
     const uint32_t* __restrict__ keys, const VT* __restrict__ vals,
-    const int64_t* __restrict__ start, int64_t n_buckets, int B, uint32_t n_entries,
-    float* __restrict__ out) {
+    const int64_t* __restrict__ start, const uint64_t* __restrict__ uoff, int64_t n_buckets,
+    int B, uint32_t n_entries, float* __restrict__ out, float* __restrict__ partial) {
     __shared__ float acc_s[kBucketWaves][2 << kMaxBucketBits];
     const int wave = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
     float* acc = acc_s[wave];
     const int rows = 1 << B;
-    for (int64_t bk = (int64_t)blockIdx.x * kBucketWaves + wave; bk < n_buckets;
-         bk += (int64_t)gridDim.x * kBucketWaves) {
+    const int64_t total = (int64_t)(uint32_t)uoff[n_buckets];
+    for (int64_t u = (int64_t)blockIdx.x * kBucketWaves + wave; u < total;
+         u += (int64_t)gridDim.x * kBucketWaves) {
+        // the bucket: the last b with first unit <= u
+        int64_t lo = 0, hi = n_buckets - 1;
+        while (lo < hi) {
+            const int64_t mid = (lo + hi + 1) >> 1;
+            if ((int64_t)(uint32_t)uoff[mid] <= u) lo = mid;
+            else hi = mid - 1;
+        }
+        const int64_t bk = lo;
+        const uint64_t ub = uoff[bk];
+        const int64_t sub = u - (int64_t)(uint32_t)ub;
+        const int64_t ns = (int64_t)(uint32_t)uoff[bk + 1] - (int64_t)(uint32_t)ub;
+        const int64_t s = start[bk] + sub * kUnitItems;
+        const int64_t e = s + kUnitItems < start[bk + 1] ? s + kUnitItems : start[bk + 1];
         for (int q = lane; q < 2 * rows; q += 64) acc[q] = 0.0f;
         wave_lds_sync();
-        const int64_t s = start[bk], e = start[bk + 1];
-        uint32_t nk = 0;
-        float na = 0.0f, nb = 0.0f;
-        if (s + lane < e) {
-            nk = keys[s + lane];
-            load_pair(vals, s + lane, na, nb);
-        }
-        for (int64_t c = s; c < e; c += 64) {
-            const bool valid = c + lane < e;
-            const uint32_t r0 = valid ? (nk & (uint32_t)(rows - 1)) : (uint32_t)rows;
-            const float a0 = valid ? na : 0.0f, b0 = valid ? nb : 0.0f;
-            if (c + 64 + lane < e) {   // prefetch the next chunk
-                nk = keys[c + 64 + lane];
-                load_pair(vals, c + 64 + lane, na, nb);
+        reduce_range(keys, vals, s, e, B, acc);
+        if (ns == 1) {
+            const int64_t row0 = bk << B;
+            for (int q = lane; q < rows; q += 64) {
+                const int64_t row = row0 + q;
+                if (row < (int64_t)n_entries)
+                    *reinterpret_cast<float2*>(out + 2 * row) =
+                        make_float2(acc[2 * q], acc[2 * q + 1]);
             }
-            // stable split sort of (row << 6 | source lane): one permute per
-            // round; the values follow once at the end
-            uint32_t rk = (r0 << 6) | (uint32_t)lane;
-            const int nbits = (e - c < 64) ? B + 1 : B;
-            for (int bit = 0; bit < nbits; ++bit) {
-                const bool one = (rk >> (6 + bit)) & 1u;
-                const uint64_t ones = __ballot(one);
-                const uint64_t zeros = ~ones;
-                const int dst = one ? __popcll(zeros) + popc_below(ones) : popc_below(zeros);
-                rk = (uint32_t)__builtin_amdgcn_ds_permute(dst * 4, (int)rk);
-            }
-            const uint32_t r = rk >> 6;
-            const int from = (int)(rk & 63u) * 4;
-            float a = bpermute_f(from, a0);
-            float b = bpermute_f(from, b0);
-            // segmented inclusive scan over the sorted lanes: row shifts of
-            // 1, 2, 4, 8 within 16-lane rows, then the last lane of row 0
-            // (row 2) into row 1 (row 3), then lane 31 into rows 2 and 3
-            seg_step<0x111, 0xf>(r, a, b);
-            seg_step<0x112, 0xf>(r, a, b);
-            seg_step<0x114, 0xf>(r, a, b);
-            seg_step<0x118, 0xf>(r, a, b);
-            seg_step<0x142, 0xa>(r, a, b);
-            seg_step<0x143, 0xc>(r, a, b);
-            const uint32_t rn =
-                (uint32_t)__builtin_amdgcn_ds_bpermute(((lane + 1) & 63) * 4, (int)r);
-            if ((lane == 63 || rn != r) && r < (uint32_t)rows) {
-                acc[2 * r] += a;
-                acc[2 * r + 1] += b;
-            }
-            wave_lds_sync();
-        }
-        const int64_t row0 = bk << B;
-        for (int q = lane; q < rows; q += 64) {
-            const int64_t row = row0 + q;
-            if (row < (int64_t)n_entries)
-                *reinterpret_cast<float2*>(out + 2 * row) = make_float2(acc[2 * q], acc[2 * q + 1]);
+        } else {
+            float* dst = partial + ((int64_t)(ub >> 32) + sub) * (2 * rows);
+            for (int q = lane; q < 2 * rows; q += 64) dst[q] = acc[q];
         }
         wave_lds_sync();
+    }
+}
+
+// Multi-unit buckets: each row is the sum of its units' partials in unit
+// order (one wave per bucket, grid-stride).
+__global__ void __launch_bounds__(256) k_bucket_combine(const uint64_t* __restrict__ uoff,
+                                                        int64_t n_buckets, int B,
+                                                        uint32_t n_entries, float* __restrict__ out,
+                                                        const float* __restrict__ partial) {
+    const int lane = threadIdx.x & 63;
+    const int rows = 1 << B;
+    for (int64_t bk = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+         bk < n_buckets; bk += (int64_t)gridDim.x * (blockDim.x >> 6)) {
+        const uint64_t ub = uoff[bk];
+        const int64_t ns = (int64_t)(uint32_t)uoff[bk + 1] - (int64_t)(uint32_t)ub;
+        if (ns < 2) continue;
+        const float* src = partial + (int64_t)(ub >> 32) * (2 * rows);
+        for (int q = lane; q < rows; q += 64) {
+            const int64_t row = (bk << B) + q;
+            if (row >= (int64_t)n_entries) continue;
+            float a = 0.0f, b = 0.0f;
+            for (int64_t j = 0; j < ns; ++j) {
+                a += src[j * 2 * rows + 2 * q];
+                b += src[j * 2 * rows + 2 * q + 1];
+            }
+            *reinterpret_cast<float2*>(out + 2 * row) = make_float2(a, b);
+        }
     }
 }
 
@@ -367,13 +479,32 @@ static int run_bucketed(TableReduceWs& w, uint32_t n_entries, float* dtable, hip
         }
     }
     KernelTimer kt_("seg_reduce", s);
-    hipLaunchKernelGGL(k_bucket_bounds, dim3((unsigned)ceil_div(w.n_buckets + 1, 256)), dim3(256),
-                       0, s, ks, w.m, w.begin_bit, w.n_buckets, w.bucket_start);
+    const unsigned gb = (unsigned)ceil_div(w.n_buckets + 1, 256);
+    hipLaunchKernelGGL(k_bucket_bounds, dim3(gb), dim3(256), 0, s, ks, w.m, w.begin_bit,
+                       w.n_buckets, w.bucket_start);
     AM_LAUNCH_CHECK();
-    const int64_t want = ceil_div(w.n_buckets, kBucketWaves);
+    hipLaunchKernelGGL(k_bucket_units, dim3(gb), dim3(256), 0, s, w.bucket_start, w.n_buckets,
+                       w.units);
+    AM_LAUNCH_CHECK();
+    size_t scb = w.scan_bytes;
+    hipError_t e = rocprim::exclusive_scan(w.scan_tmp, scb, w.units, w.unit_off, (uint64_t)0,
+                                           (size_t)w.n_buckets + 1, rocprim::plus<uint64_t>(), s,
+                                           false);
+    if (e != hipSuccess) {
+        set_error("bucket unit scan failed: %s", hipGetErrorString(e));
+        return AM_ERR_HIP;
+    }
+    // units <= n_buckets + m / kUnitItems + 1; the kernel reads the total
+    const int64_t max_units = w.n_buckets + w.m / kUnitItems + 1;
+    const int64_t want = ceil_div(max_units, kBucketWaves);
     const unsigned grid = (unsigned)(want < 8192 ? want : 8192);
     hipLaunchKernelGGL((k_bucket_reduce<VT>), dim3(grid), dim3(64 * kBucketWaves), 0, s, ks, vs,
-                       w.bucket_start, w.n_buckets, w.begin_bit, n_entries, dtable);
+                       w.bucket_start, w.unit_off, w.n_buckets, w.begin_bit, n_entries, dtable,
+                       w.partial);
+    AM_LAUNCH_CHECK();
+    const int64_t cw = ceil_div(w.n_buckets, 4);
+    hipLaunchKernelGGL(k_bucket_combine, dim3((unsigned)(cw < 4096 ? cw : 4096)), dim3(256), 0, s,
+                       w.unit_off, w.n_buckets, w.begin_bit, n_entries, dtable, w.partial);
     AM_LAUNCH_CHECK();
     return AM_OK;
 }

@@ -42,14 +42,20 @@ def test_hashgrid_fwd_bitexact(dev, name):
         assert np.array_equal(out, ref), np.abs(out - ref).max()
 
 
-@pytest.mark.parametrize("name", list(CONFIGS))
-def test_hashgrid_bwd_matches_autograd_and_is_deterministic(dev, name):
+@pytest.mark.parametrize("name,clustered", [(n, False) for n in CONFIGS] + [("c1_full", True)])
+def test_hashgrid_bwd_matches_autograd_and_is_deterministic(dev, name, clustered):
+    """clustered: 6000 points in a cube of side 0.004, so the coarse levels'
+    few rows collect tens of thousands of gradient pairs (a reduction bucket
+    split over several work units, combined in unit order)."""
     from angelo_amd import descs
     from angelo_amd.functions import hashgrid_encode
     L, F, log2T, base, s = CONFIGS[name]
     g = descs.grid_desc(L, F, log2T, base, s)
     lt = ohg.level_table(L, base, s, log2T, F)
-    x = _points(2000, 3)
+    if clustered:
+        x = 0.3 + 0.004 * torch.rand(6000, 3, generator=torch.Generator().manual_seed(9))
+    else:
+        x = _points(2000, 3)
     gen = torch.Generator().manual_seed(4)
     table = torch.rand(lt.n_entries, F, generator=gen) * 2 - 1
     dy = torch.randn(x.shape[0], L * F, generator=gen)
@@ -70,8 +76,20 @@ def test_hashgrid_bwd_matches_autograd_and_is_deterministic(dev, name):
     dtab, dx = res[0]
     assert torch.equal(res[0][0], res[1][0]), "table gradient not run-to-run identical"
     assert torch.equal(res[0][1], res[1][1])
-    torch.testing.assert_close(dtab.double(), t64.grad, rtol=1e-5, atol=1e-6)
-    torch.testing.assert_close(dx.double(), x64.grad, rtol=1e-4, atol=1e-4)
+    if clustered:
+        # rows summing thousands of terms, and position gradients of fine
+        # levels cancelling in fp32: the bar is torch's own fp32 autograd
+        # error against the float64 oracle
+        x32 = x.clone().requires_grad_(True)
+        t32 = table.clone().requires_grad_(True)
+        (ohg.encode_torch(x32, t32, lt, active) * dy).sum().backward()
+        for got, ref32, ref64 in ((dtab, t32.grad, t64.grad), (dx, x32.grad, x64.grad)):
+            e32 = float((ref32.double() - ref64).abs().max())
+            e_hip = float((got.double() - ref64).abs().max())
+            assert e_hip <= 8 * e32 + 1e-7, (e_hip, e32)
+    else:
+        torch.testing.assert_close(dtab.double(), t64.grad, rtol=1e-5, atol=1e-6)
+        torch.testing.assert_close(dx.double(), x64.grad, rtol=1e-4, atol=1e-4)
 
 
 @pytest.mark.parametrize("case", [
