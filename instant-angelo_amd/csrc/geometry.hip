@@ -533,7 +533,7 @@ __device__ __forceinline__ void geo_eval_bwd(const GeoArgs& a, const float* sm,
 
 // ---- pair emission with merging ----
 // A tile's pairs go to [keys, keys + limit): limit is the count the count
-// pass found for the tile; the backward emits the same sequence, and never
+// pass found for the tile; the emit pass makes the same decisions, and never
 // writes past limit.
 template <class VT>
 struct EmitTile {
@@ -548,8 +548,8 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
     return l == 0 ? 0ull : (~0ull >> (64 - l));
 }
 
-// Lanes with `on` write K pairs (rows, values) at consecutive positions in
-// lane order; COUNT only advances the cursor.
+// Lanes with `on` write K pairs (rows, values); COUNT only advances the
+// cursor.
 __device__ __forceinline__ void set_pair(Vals<2>& v, float a, float b) {
     v.v[0] = a;
     v.v[1] = b;
@@ -559,21 +559,26 @@ __device__ __forceinline__ void set_pair(ValsH<2>& v, float a, float b) {
     v.v[1] = __float2half_rn(b);
 }
 
+// The group's K * n pairs (n lanes on) fill [cursor, cursor + K n) corner-
+// major: pair c of the lane of rank r goes to cursor + c n + r, so each store
+// instruction writes consecutive addresses across the lanes.  The order is a
+// fixed function of the data (the reduction's summation order follows it).
 template <int K, bool COUNT, class VT>
 __device__ __forceinline__ void emit_pairs(EmitTile<VT>& et, bool on, const uint32_t rows[K],
                                            const float vals[2 * K]) {
     const uint64_t mask = __ballot(on);
-    const int off = et.cursor + K * __popcll(mask & lanemask_lt());
-    if (!COUNT && on && off + K <= et.limit) {
+    const int n_on = __popcll(mask);
+    if (!COUNT && on && et.cursor + K * n_on <= et.limit) {
+        const int off = et.cursor + __popcll(mask & lanemask_lt());
 #pragma unroll
         for (int c = 0; c < K; ++c) {
-            et.keys[off + c] = rows[c];
+            et.keys[off + c * n_on] = rows[c];
             VT v;
             set_pair(v, vals[2 * c], vals[2 * c + 1]);
-            et.vals[off + c] = v;
+            et.vals[off + c * n_on] = v;
         }
     }
-    et.cursor += K * __popcll(mask);
+    et.cursor += K * n_on;
 }
 
 // The 8 corner rows of `cell` at level `lp` (corner c = cell + bits of c).
@@ -587,6 +592,7 @@ __device__ __forceinline__ int face_corner(int i, int ax, int b) {
     const int low = i & ((1 << ax) - 1);
     return low | (b << ax) | ((i >> ax) << (ax + 1));
 }
+
 
 // Position of cluster evaluation q (0..n-1) for this lane.
 template <int CL>
@@ -612,6 +618,34 @@ template <int CL>
 __device__ __forceinline__ void cluster_center(const GeoArgs& a, const GeoSample& s, float x[3]) {
 #pragma unroll
     for (int d = 0; d < 3; ++d) x[d] = CL == 0 ? s.x0[d] : contract1(clampr(s.ps[d], a.r), a);
+}
+
+// Lane p + D (p - 1) of the lane's 16-lane row (a DPP row shift: no LDS
+// traffic); lanes whose source lies outside the row get 0 and must not use it.
+template <int D>
+__device__ __forceinline__ uint32_t row_down(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x100 + D, 0xf, 0xf, false);
+}
+__device__ __forceinline__ uint32_t row_up1(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false);
+}
+
+// One step of the suffix segmented sum over a 16-lane row: a lane that is
+// not the end of its segment adds the accumulators of lane p + D (and takes
+// over its end flag) when that lane is in the row.
+template <int D>
+__device__ __forceinline__ void merge_step(int p, float acc[16], bool& end) {
+    float o[16];
+#pragma unroll
+    for (int c = 0; c < 16; ++c)
+        o[c] = __int_as_float(
+            __builtin_amdgcn_update_dpp(0, __float_as_int(acc[c]), 0x100 + D, 0xf, 0xf, false));
+    const bool oend = row_down<D>((uint32_t)end) != 0u;
+    if (!end && p + D < 16) {
+#pragma unroll
+        for (int c = 0; c < 16; ++c) acc[c] += o[c];
+        end = oend;
+    }
 }
 
 // Reduces the stored encode gradients of one cluster of evaluations into
@@ -736,25 +770,15 @@ __device__ __forceinline__ void flush_cluster(const GeoArgs& a, const LevelParam
         const bool live = on && touched;
         const uint32_t key0 = live ? B[0] : INV, key1 = live ? B[1] : INV,
                        key2 = live ? B[2] : INV;
-        const uint32_t n0 = __shfl_down(key0, 1, 16), n1 = __shfl_down(key1, 1, 16),
-                       n2 = __shfl_down(key2, 1, 16);
+        const uint32_t n0 = row_down<1>(key0), n1 = row_down<1>(key1), n2 = row_down<1>(key2);
         bool end = (p == 15) || !(n0 == key0 && n1 == key1 && n2 == key2) || key0 == INV;
         if constexpr (!COUNT) {
-#pragma unroll
-            for (int d = 1; d < 16; d <<= 1) {
-                float o[16];
-#pragma unroll
-                for (int c = 0; c < 16; ++c) o[c] = __shfl_down(acc[c], d, 16);
-                const bool oend = __shfl_down((int)end, d, 16) != 0;
-                if (!end && p + d < 16) {
-#pragma unroll
-                    for (int c = 0; c < 16; ++c) acc[c] += o[c];
-                    end = oend;
-                }
-            }
+            merge_step<1>(p, acc, end);
+            merge_step<2>(p, acc, end);
+            merge_step<4>(p, acc, end);
+            merge_step<8>(p, acc, end);
         }
-        const uint32_t u0 = __shfl_up(key0, 1, 16), u1 = __shfl_up(key1, 1, 16),
-                       u2 = __shfl_up(key2, 1, 16);
+        const uint32_t u0 = row_up1(key0), u1 = row_up1(key1), u2 = row_up1(key2);
         const bool head = live && (p == 0 || !(u0 == key0 && u1 == key1 && u2 == key2));
         uint32_t rows[8];
         if (!COUNT && head) corner_rows(lp, B, rows);

@@ -8,6 +8,7 @@
 // sample intervals and ray indices are bit-exact with it.  Build-side
 // additions shared with the oracle: an optional per-ray sample cap and the
 // end of a ray when a step no longer advances t.
+#include <cmath>
 #include <rocprim/device/device_scan.hpp>
 #include "am_common.h"
 
@@ -19,7 +20,23 @@ struct MarchArgs {
     float step;
     int cap;
     int has_grid;
+    // Division by a power of two is an exact scaling, so x / range and
+    // x * (1 / range) are the same IEEE result: the divisions of the marching
+    // rule become multiplications where the roi extent (and the grid
+    // resolution) is a power of two, as for the usual [-r, r]^3 with r = 2^k.
+    float inv_range[3];
+    int range_pow2;  // bit d: roi extent of axis d is a power of two
+    float inv_res;
+    int res_pow2;
 };
+
+__device__ __forceinline__ float div_range(const MarchArgs& a, float x, int d) {
+    return ((a.range_pow2 >> d) & 1) ? x * a.inv_range[d] : x / (a.roi[3 + d] - a.roi[d]);
+}
+
+__device__ __forceinline__ float div_res(const MarchArgs& a, float x) {
+    return a.res_pow2 ? x * a.inv_res : x / (float)a.res;
+}
 
 __device__ __forceinline__ void aabb_one(const float* o, const float* d, const float* aabb,
                                          float* t_near, float* t_far) {
@@ -51,7 +68,7 @@ __device__ __forceinline__ bool occupied(const MarchArgs& a, const float xyz[3],
     int ix[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-        const float u = (xyz[d] - a.roi[d]) / (a.roi[3 + d] - a.roi[d]);
+        const float u = div_range(a, xyz[d] - a.roi[d], d);
         int v = (int)(u * (float)a.res);
         v = v < 0 ? 0 : (v > a.res - 1 ? a.res - 1 : v);
         ix[d] = v;
@@ -65,9 +82,9 @@ __device__ __forceinline__ float dist_next_voxel(const MarchArgs& a, const float
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
         const float range = a.roi[3 + d] - a.roi[d];
-        const float u = ((xyz[d] - a.roi[d]) / range) * (float)a.res;
+        const float u = div_range(a, xyz[d] - a.roi[d], d) * (float)a.res;
         const float sg = copysignf(1.0f, dir[d]);
-        const float tt = ((floorf(u + 0.5f + 0.5f * sg) - u) * inv[d]) / (float)a.res * range;
+        const float tt = div_res(a, (floorf(u + 0.5f + 0.5f * sg) - u) * inv[d]) * range;
         t = fminf(t, tt);
     }
     return fmaxf(t, 0.0f);
@@ -183,6 +200,18 @@ static MarchArgs march_args(const float* roi6, const uint8_t* grid, int res, flo
     a.step = step;
     a.cap = cap;
     a.has_grid = grid != nullptr;
+    a.range_pow2 = 0;
+    for (int d = 0; d < 3; ++d) {
+        const float range = roi6[3 + d] - roi6[d];
+        int ex = 0;
+        const float m = std::frexp(range, &ex);   // range = m * 2^ex, m in [0.5, 1)
+        const bool pow2 = std::isfinite(range) && range > 0.0f && m == 0.5f &&
+                          std::isnormal(1.0f / range);
+        a.inv_range[d] = pow2 ? 1.0f / range : 0.0f;
+        if (pow2) a.range_pow2 |= 1 << d;
+    }
+    a.res_pow2 = res > 0 && (res & (res - 1)) == 0;
+    a.inv_res = a.res_pow2 ? 1.0f / (float)res : 0.0f;
     return a;
 }
 
