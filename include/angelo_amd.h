@@ -112,6 +112,12 @@ int am_hashgrid_bwd(const am_grid_desc* g, const float* x, int64_t n, const floa
 #define AM_PREC_FP32 0
 #define AM_PREC_MIXED 1
 
+/* VolumeSDF grad_type (models/geometry.py:214-234) */
+#define AM_GRAD_FD 0       /* 'finite_difference': 6 central differences */
+#define AM_GRAD_ANALYTIC 1 /* 'analytic': d sdf / d x through the encoding and
+                              the MLP (one hidden layer), backward with the
+                              second-order terms of the gradient output */
+
 typedef struct am_mlp_desc {
     int32_t n_in;            /* <= 64 */
     int32_t n_out;           /* <= 64 */
@@ -190,12 +196,14 @@ typedef struct am_sdf_desc {
                              The SDF MLP descriptor must be AM_PREC_FP32 in
                              both forms: the central differences of the FD
                              normal divide each evaluation's rounding by eps */
+    int32_t grad_type;    /* AM_GRAD_FD or AM_GRAD_ANALYTIC (with_grad) */
 } am_sdf_desc;
 
 /* Outputs (each nullable except sdf): sdf[n], grad[n,3], normal[n,3] =
  * F.normalize(grad), feature[n, n_out+3] = cat(mlp_out, x01*2-1),
- * laplace[n], fd_values[n,12] (the 6 central-difference and 6 curvature SDF
- * values, kept for the backward), enc_store[am_sdf_store_floats(n)] (the
+ * laplace[n], fd_values[n,12] (the 6 central-difference SDF values, or the
+ * analytic grad in [0,3), then the 6 curvature SDF values, kept for the
+ * backward), enc_store[am_sdf_store_floats(n)] (the
  * encodings of every evaluation and the position derivatives of the
  * curvature evaluations, kept so the backward does not gather the table
  * again).  randn[n,3]: the curvature directions (torch.randn_like). */

@@ -46,6 +46,9 @@ class MlpDesc(ctypes.Structure):
     ]
 
 
+GRAD_TYPE = {"finite_difference": 0, "analytic": 1}
+
+
 class SdfDesc(ctypes.Structure):
     _fields_ = [
         ("radius", ctypes.c_float),
@@ -55,6 +58,7 @@ class SdfDesc(ctypes.Structure):
         ("recontract", ctypes.c_int32),
         ("active_levels", ctypes.c_int32),
         ("precision", ctypes.c_int32),
+        ("grad_type", ctypes.c_int32),
     ]
 
 

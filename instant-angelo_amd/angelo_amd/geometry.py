@@ -93,11 +93,14 @@ class VolumeSDF(BaseImplicitGeometry):
         d.recontract = self.curvature_recontract
         d.active_levels = int(self.active_levels())
         d.precision = nat.PREC[self.precision]
+        d.grad_type = nat.GRAD_TYPE[self.grad_type]
         return d
 
     def _fused_ok(self):
         enc = self.encoding
-        return (self.grad_type == "finite_difference" and enc.include_xyz
+        fd_or_analytic = (self.grad_type == "finite_difference"
+                          or (self.grad_type == "analytic" and self.network.n_hidden_layers == 1))
+        return (fd_or_analytic and enc.include_xyz
                 and isinstance(self.grid_module, HashGrid)
                 and self.grid_module.n_features_per_level == 2
                 and self.grid_module.n_levels <= 16
@@ -112,9 +115,9 @@ class VolumeSDF(BaseImplicitGeometry):
         feature [n, feature_dim+3], laplace [n]."""
         if not self._fused_ok():
             raise NotImplementedError(
-                "fused VolumeSDF needs grad_type=finite_difference, include_xyz, a 2-feature "
-                "hash grid with <= 16 levels and a 1-2 hidden-layer VanillaMLP without "
-                "output/sdf/feature activations")
+                "fused VolumeSDF needs grad_type=finite_difference (or analytic with one hidden "
+                "layer), include_xyz, a 2-feature hash grid with <= 16 levels and a 1-2 "
+                "hidden-layer VanillaMLP without output/sdf/feature activations")
         pts = points.reshape(-1, 3)
         if with_laplace and randn is None:
             randn = torch.randn_like(pts)
@@ -128,9 +131,6 @@ class VolumeSDF(BaseImplicitGeometry):
                 with_auxiliary_feature=False, randn=None):
         if with_auxiliary_feature:
             raise NotImplementedError("with_auxiliary_feature (sh-neus) is not part of this build")
-        if with_grad and self.grad_type == "analytic":
-            raise NotImplementedError("grad_type='analytic' needs the hash-grid double backward, "
-                                      "which this build does not provide; use finite_difference")
         shape = points.shape[:-1]
         with torch.set_grad_enabled(self.training and torch.is_grad_enabled()):
             sdf, grad, _, feature, lap = self.fused(points, with_grad, with_laplace, randn)

@@ -52,6 +52,13 @@ struct GeoArgs {
     int64_t pw1, pb1, pw2, pb2, pwo, pbo, n_params;
     int n_evals;  // 1, 7 or 13
     int prec;     // AM_PREC_FP32 / AM_PREC_MIXED
+    // analytic-gradient mode: the base evaluation carries d enc / d x and
+    // grad = d sdf / d x through the MLP; no FD evaluations.  Slots: curv0 =
+    // first curvature evaluation (7 FD mode, 1 analytic), n_dx = stored
+    // position-derivative slots (the analytic base, then the 6 curvature
+    // evaluations), genc_slots = encode-gradient slots per tile (n_evals, and
+    // 3 more in analytic mode: u * scale * g_x along each axis).
+    int analytic, curv0, n_dx, dx_curv0, genc_slots;
 };
 
 // Row-space column of the SDF MLP first layer for lane-group level slots:
@@ -186,30 +193,33 @@ __device__ __forceinline__ void geo_eval(const GeoArgs& a, const float* sm,
 }
 
 // Stored encodings (forward -> backward, the "store" form of the backward):
-// per tile of 16 samples, evaluation ev (0 base, 1-6 FD, 7-12 curvature) and
-// slot value s < 2LJ, one float per lane at
+// per tile of 16 samples, evaluation ev (0 base, then 1-6 FD and 7-12
+// curvature, or 1-6 curvature in analytic mode) and slot value s < 2LJ, one
+// float per lane at
 //   enc[((tile * n_evals + ev) * 2LJ + s) * 64 + lane];
-// with laplace, the position derivatives of the 6 curvature evaluations
-// follow at enc + geo_enc_floats(...):
-//   [((tile * 6 + c) * 6LJ + (j * 2 + f) * 3 + d) * 64 + lane].
+// the position derivatives of the evaluations that carry them (dx slot c:
+// the analytic base, then the curvature evaluations) follow at
+// enc + geo_enc_floats(...):
+//   [((tile * n_dx + c) * 6LJ + (j * 2 + f) * 3 + d) * 64 + lane].
 // Every access is a 256 B coalesced row.
 __host__ __device__ inline int64_t geo_enc_floats(int64_t tiles, int n_evals, int LJ) {
     return tiles * n_evals * 2 * LJ * 64;
 }
-__host__ __device__ inline int64_t geo_dx_floats(int64_t tiles, int with_laplace, int LJ) {
-    return with_laplace ? tiles * 6 * 6 * LJ * 64 : 0;
+__host__ __device__ inline int64_t geo_dx_floats(int64_t tiles, int n_dx, int LJ) {
+    return tiles * n_dx * 6 * LJ * 64;
 }
 
 template <int LJ, int NH, bool DX>
 __device__ __forceinline__ void geo_store(float* __restrict__ st, const GeoArgs& a, int64_t tile,
-                                          int64_t n_tiles, int ev, const GeoEval<LJ, NH, DX>& e) {
+                                          int64_t n_tiles, int ev, int dxs,
+                                          const GeoEval<LJ, NH, DX>& e) {
     const int l = lane_id();
     float* p = st + (tile * a.n_evals + ev) * (2 * LJ) * 64 + l;
 #pragma unroll
     for (int q = 0; q < 2 * LJ; ++q) p[q * 64] = e.xb[q];
     if constexpr (DX) {
         float* d0 = st + geo_enc_floats(n_tiles, a.n_evals, LJ) +
-                    (tile * 6 + (ev - 7)) * (6 * LJ) * 64 + l;
+                    (tile * a.n_dx + dxs) * (6 * LJ) * 64 + l;
 #pragma unroll
         for (int j = 0; j < LJ; ++j)
 #pragma unroll
@@ -221,7 +231,7 @@ __device__ __forceinline__ void geo_store(float* __restrict__ st, const GeoArgs&
 
 template <int LJ, int NH, bool DX>
 __device__ __forceinline__ void geo_load(const float* __restrict__ st, const GeoArgs& a,
-                                         int64_t tile, int64_t n_tiles, int ev,
+                                         int64_t tile, int64_t n_tiles, int ev, int dxs,
                                          GeoEval<LJ, NH, DX>& e) {
     const int l = lane_id();
     const float* p = st + (tile * a.n_evals + ev) * (2 * LJ) * 64 + l;
@@ -229,7 +239,7 @@ __device__ __forceinline__ void geo_load(const float* __restrict__ st, const Geo
     for (int q = 0; q < 2 * LJ; ++q) e.xb[q] = p[q * 64];
     if constexpr (DX) {
         const float* d0 = st + geo_enc_floats(n_tiles, a.n_evals, LJ) +
-                          (tile * 6 + (ev - 7)) * (6 * LJ) * 64 + l;
+                          (tile * a.n_dx + dxs) * (6 * LJ) * 64 + l;
 #pragma unroll
         for (int j = 0; j < LJ; ++j)
 #pragma unroll
@@ -245,9 +255,9 @@ template <int LJ, int NH, bool DX, bool ST, int PR, class TT>
 __device__ __forceinline__ void geo_eval_b(const GeoArgs& a, const float* sm, const LevelParams* lv,
                                            const TT* __restrict__ table,
                                            const float* __restrict__ st, int64_t tile,
-                                           int64_t n_tiles, int ev, const float x[3],
+                                           int64_t n_tiles, int ev, int dxs, const float x[3],
                                            GeoEval<LJ, NH, DX>& e) {
-    if constexpr (ST) geo_load<LJ, NH, DX>(st, a, tile, n_tiles, ev, e);
+    if constexpr (ST) geo_load<LJ, NH, DX>(st, a, tile, n_tiles, ev, dxs, e);
     else geo_encode<LJ, NH, DX>(a, lv, table, x, e);
     geo_mlp_hidden<LJ, NH, DX, PR>(a, sm, x, e);
 }
@@ -348,7 +358,72 @@ using TableT = typename std::conditional<PR == mlp::kF32, float2, __half2>::type
 template <int PR>
 using PairV = typename std::conditional<PR == mlp::kF32, Vals<2>, ValsH<2>>::type;
 
-template <int LJ, int NH, bool ST, int PR>
+// sdf and feature outputs of the base evaluation.
+template <int LJ, int NH, bool DX>
+__device__ __forceinline__ void write_base(const GeoArgs& a, const GeoEval<LJ, NH, DX>& e,
+                                           const GeoSample& s, int64_t n, bool valid, int k,
+                                           int fdim, float* __restrict__ sdf,
+                                           float* __restrict__ feature) {
+    if (!valid) return;
+    if (k == 0) sdf[n] = e.out[0];
+    if (feature) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = 4 * k + r;
+            if (row < a.n_out) feature[n * fdim + row] = e.out[r];
+        }
+        if (k < 3) feature[n * fdim + a.n_out + k] = pick3(s.x0, k) * 2.0f - 1.0f;
+    }
+}
+
+// d sdf / d x01 of the base evaluation (analytic mode, one hidden layer):
+// u = W1^T (sigma'(z1) * Wo[0,:]) is the gradient of the SDF row with respect
+// to the MLP input (encoding slots, then xyz*2-1), contracted with the
+// encoding's position derivatives e.dx and with 2 for the xyz input; summed
+// over the four lane groups (every lane of the sample gets the sum).  u is
+// returned in B-fragment layout.
+template <int LJ>
+__device__ __forceinline__ void geo_input_grad(const GeoArgs& a, const float* sm,
+                                               const GeoEval<LJ, 1, true>& e, float gpos[3],
+                                               float u[GeoDims<LJ>::KS1]) {
+    constexpr int IT1 = GeoDims<LJ>::IT1;
+    const int k = lane_id() >> 4;
+    v4f c1[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        v4f h, dh;
+        mlp::act_fb4(a.act, e.z1[t], a.beta, h, dh);
+        v4f w;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) w[r] = sm[a.wo + 16 * t + 4 * k + r];   // Wo[0, row]
+        c1[t] = dh * w;
+    }
+    v4f gin[IT1];
+#pragma unroll
+    for (int t = 0; t < IT1; ++t) gin[t] = v4f{0, 0, 0, 0};
+    mlp::mm_bwd_in<16, IT1, mlp::FragV, mlp::kF32>(sm + a.w1, a.ld1, mlp::FragV{c1}, gin);
+    const mlp::FragV gb{gin};
+#pragma unroll
+    for (int q = 0; q < GeoDims<LJ>::KS1; ++q) u[q] = gb[q];
+    float gp[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int j = 0; j < LJ; ++j)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) gp[d] += gb[2 * j] * e.dx[j][0][d] + gb[2 * j + 1] * e.dx[j][1][d];
+    if (k < 3) {
+        const float gxyz = gb[2 * LJ] * 2.0f;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) gp[d] += d == k ? gxyz : 0.0f;
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) gpos[d] = sum_groups(gp[d]);
+}
+
+// Forward of the fused field.  AN: analytic-gradient mode (grad from the
+// base evaluation's derivatives; one hidden layer), else central
+// differences.  fdv keeps, per sample, the 6 FD values (or the analytic
+// grad in [0, 3)) and the 6 curvature values, for the backward.
+template <int LJ, int NH, bool ST, int PR, bool AN>
 __global__ void __launch_bounds__(256) k_geo_fwd(GeoArgs a, const float* __restrict__ pts,
                                                  const float* __restrict__ randn, int64_t N,
                                                  const void* __restrict__ table_v,
@@ -359,6 +434,7 @@ __global__ void __launch_bounds__(256) k_geo_fwd(GeoArgs a, const float* __restr
                                                  float* __restrict__ laplace,
                                                  float* __restrict__ fdv,
                                                  float* __restrict__ store) {
+    static_assert(!AN || NH == 1, "analytic gradients take one hidden layer");
     extern __shared__ float sm[];
     __shared__ LevelParams lv[AM_MAX_LEVELS];
     geo_load_lds<LJ, NH>(a, params, sm, lv);
@@ -378,39 +454,44 @@ __global__ void __launch_bounds__(256) k_geo_fwd(GeoArgs a, const float* __restr
         for (int d = 0; d < 3; ++d) s.P[d] = valid ? pts[3 * n + d] : 0.0f;
 #pragma unroll
         for (int d = 0; d < 3; ++d) s.x0[d] = contract1(s.P[d], a);
-        {
-            GeoEval<LJ, NH, false> e;
-            geo_eval<LJ, NH, false, mlp::kF32>(a, sm, lv, table, s.x0, e);
-            if constexpr (ST) geo_store<LJ, NH, false>(store, a, tile, n_tiles, 0, e);
-            if (valid) {
-                if (k == 0) sdf[n] = e.out[0];
-                if (feature) {
+        if constexpr (AN) {
+            GeoEval<LJ, NH, true> e;   // with the position derivatives
+            geo_eval<LJ, NH, true, mlp::kF32>(a, sm, lv, table, s.x0, e);
+            if constexpr (ST) geo_store<LJ, NH, true>(store, a, tile, n_tiles, 0, 0, e);
+            write_base(a, e, s, n, valid, k, fdim, sdf, feature);
+            if (!a.with_grad) continue;
+            float gp[3], u[GeoDims<LJ>::KS1];
+            geo_input_grad<LJ>(a, sm, e, gp, u);
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        const int row = 4 * k + r;
-                        if (row < a.n_out) feature[n * fdim + row] = e.out[r];
-                    }
-                    if (k < 3) feature[n * fdim + a.n_out + k] = pick3(s.x0, k) * 2.0f - 1.0f;
-                }
+            for (int d = 0; d < 3; ++d) s.grad[d] = gp[d] * a.inv2r;   // d x01 / d x = 1 / 2r
+            if (fdv && valid && k == 0)
+#pragma unroll
+                for (int d = 0; d < 3; ++d) fdv[n * 12 + d] = s.grad[d];
+        } else {
+            {
+                GeoEval<LJ, NH, false> e;
+                geo_eval<LJ, NH, false, mlp::kF32>(a, sm, lv, table, s.x0, e);
+                if constexpr (ST) geo_store<LJ, NH, false>(store, a, tile, n_tiles, 0, 0, e);
+                write_base(a, e, s, n, valid, k, fdim, sdf, feature);
             }
-        }
-        if (!a.with_grad) continue;
-        float g[3] = {0.0f, 0.0f, 0.0f};
+            if (!a.with_grad) continue;
+            float g[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll 1
-        for (int j = 0; j < 6; ++j) {
-            float x[3];
-            bool m[3];
-            offset_point(a, s.P, j, x, m);
-            GeoEval<LJ, NH, false> e;
-            geo_eval<LJ, NH, false, mlp::kF32>(a, sm, lv, table, x, e);
-            if constexpr (ST) geo_store<LJ, NH, false>(store, a, tile, n_tiles, 1 + j, e);
-            const float v = shfl(e.out[0], p);
-            diff_pair(g, j, v);
-            if (fdv && valid && k == 0) fdv[n * 12 + j] = v;
-        }
-        finish_grad(a, g);
+            for (int j = 0; j < 6; ++j) {
+                float x[3];
+                bool m[3];
+                offset_point(a, s.P, j, x, m);
+                GeoEval<LJ, NH, false> e;
+                geo_eval<LJ, NH, false, mlp::kF32>(a, sm, lv, table, x, e);
+                if constexpr (ST) geo_store<LJ, NH, false>(store, a, tile, n_tiles, 1 + j, 0, e);
+                const float v = shfl(e.out[0], p);
+                diff_pair(g, j, v);
+                if (fdv && valid && k == 0) fdv[n * 12 + j] = v;
+            }
+            finish_grad(a, g);
 #pragma unroll
-        for (int d = 0; d < 3; ++d) s.grad[d] = g[d];
+            for (int d = 0; d < 3; ++d) s.grad[d] = g[d];
+        }
         normalize3(s.grad, s.n);
         if (valid && k == 0) {
             if (grad)
@@ -430,7 +511,8 @@ __global__ void __launch_bounds__(256) k_geo_fwd(GeoArgs a, const float* __restr
             offset_point(a, s.ps, j, x, m);
             GeoEval<LJ, NH, ST> e;  // the store keeps the position derivatives
             geo_eval<LJ, NH, ST, mlp::kF32>(a, sm, lv, table, x, e);
-            if constexpr (ST) geo_store<LJ, NH, true>(store, a, tile, n_tiles, 7 + j, e);
+            if constexpr (ST)
+                geo_store<LJ, NH, true>(store, a, tile, n_tiles, a.curv0 + j, a.dx_curv0 + j, e);
             const float v = shfl(e.out[0], p);
             diff_pair(gs, j, v);
             if (fdv && valid && k == 0) fdv[n * 12 + 6 + j] = v;
@@ -528,6 +610,90 @@ __device__ __forceinline__ void geo_eval_bwd(const GeoArgs& a, const float* sm,
         }
 #pragma unroll
         for (int d = 0; d < 3; ++d) gpos[d] = sum_groups(gp[d]);
+    }
+}
+
+// Backward of the analytic base evaluation (one hidden layer).  With
+// v = d input / d x01 . g_x (the encoding's position derivatives and 2 for
+// xyz, contracted with the upstream gradient of d sdf / d x01) and
+// s = W1 v, the gradient term of the loss is s . (sigma'(z1) * Wo[0,:]):
+//   dWo[0,:] += s * sigma',  dW1 += (sigma' * Wo[0,:]) (x) v,
+//   d/dz1 += s * Wo[0,:] * sigma'',  d/dtable via u = W1^T (sigma' * Wo[0,:]):
+//   each corner c of level l gets u[l,f] * scale_l * grad(w_c) . g_x,
+// stored as u[l,f] * scale_l * g_x[d] in three extra slots (gext) for the
+// pair pass; the ordinary output gradient go flows as in geo_eval_bwd.
+template <int LJ>
+__device__ __forceinline__ void geo_eval_bwd_an(const GeoArgs& a, const float* sm,
+                                                const LevelParams* lv,
+                                                const GeoEval<LJ, 1, true>& e, v4f go,
+                                                const float gx[3], float* scratch,
+                                                GeoGrad<LJ, 1>& G, float* genc, float* gext) {
+    constexpr int KS1 = GeoDims<LJ>::KS1, IT1 = GeoDims<LJ>::IT1;
+    const int l = lane_id(), k = l >> 4;
+    v4f h[4], dh[4], d2[4], w0[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        mlp::act_fb4(a.act, e.z1[t], a.beta, h[t], dh[t]);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            d2[t][r] = mlp::act_d2(a.act, e.z1[t][r], a.beta, dh[t][r]);
+            w0[t][r] = sm[a.wo + 16 * t + 4 * k + r];   // Wo[0, row]
+        }
+    }
+    float v[KS1];
+#pragma unroll
+    for (int j = 0; j < LJ; ++j)
+#pragma unroll
+        for (int f = 0; f < 2; ++f)
+            v[2 * j + f] = (e.dx[j][f][0] * gx[0] + e.dx[j][f][1] * gx[1]) + e.dx[j][f][2] * gx[2];
+    v[2 * LJ] = k < 3 ? 2.0f * pick3(gx, k) : 0.0f;
+    v4f sv[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) sv[t] = v4f{0, 0, 0, 0};
+    mlp::mm_fwd<KS1, 4, mlp::FragF, mlp::kF32>(sm + a.w1, a.ld1, mlp::FragF{v}, sv);
+    // output layer: the ordinary term and the gradient term's row 0
+    mlp::mm_dw<1, 4, 16, mlp::FragV, mlp::kF32>(&go, mlp::FragV{h}, scratch, G.dWo);
+    G.dbo += go;
+    {
+        v4f sdh[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) sdh[t] = sv[t] * dh[t];
+        const v4f e0 = go_row0(1.0f);
+        mlp::mm_dw<1, 4, 16, mlp::FragV, mlp::kF32>(&e0, mlp::FragV{sdh}, scratch, G.dWo);
+    }
+    v4f gh[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) gh[t] = v4f{0, 0, 0, 0};
+    mlp::mm_bwd_in<4, 4, mlp::FragV, mlp::kF32>(sm + a.wo, 65, mlp::FragV{&go}, gh);
+    v4f gz1[4], c1[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        gz1[t] = gh[t] * dh[t] + sv[t] * w0[t] * d2[t];
+        c1[t] = dh[t] * w0[t];
+        G.db1[t] += gz1[t];
+    }
+    mlp::mm_dw<4, IT1, KS1, mlp::FragF, mlp::kF32>(gz1, mlp::FragF{e.xb}, scratch, G.dW1);
+    mlp::mm_dw<4, IT1, KS1, mlp::FragF, mlp::kF32>(c1, mlp::FragF{v}, scratch, G.dW1);
+    v4f gin[IT1], uin[IT1];
+#pragma unroll
+    for (int t = 0; t < IT1; ++t) {
+        gin[t] = v4f{0, 0, 0, 0};
+        uin[t] = v4f{0, 0, 0, 0};
+    }
+    mlp::mm_bwd_in<16, IT1, mlp::FragV, mlp::kF32>(sm + a.w1, a.ld1, mlp::FragV{gz1}, gin);
+    mlp::mm_bwd_in<16, IT1, mlp::FragV, mlp::kF32>(sm + a.w1, a.ld1, mlp::FragV{c1}, uin);
+    const mlp::FragV gb{gin}, ub{uin};
+#pragma unroll
+    for (int j = 0; j < LJ; ++j) {
+        genc[(2 * j) * 64 + l] = gb[2 * j];
+        genc[(2 * j + 1) * 64 + l] = gb[2 * j + 1];
+        const int level = k + 4 * j;
+        const float sc = level < a.g.L ? lv[level].scale : 0.0f;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            gext[d * LJ * 128 + (2 * j) * 64 + l] = ub[2 * j] * sc * gx[d];
+            gext[d * LJ * 128 + (2 * j + 1) * 64 + l] = ub[2 * j + 1] * sc * gx[d];
+        }
     }
 }
 
@@ -762,6 +928,26 @@ __device__ __forceinline__ void flush_cluster(const GeoArgs& a, const LevelParam
                         acc[2 * t + 1] = __fmaf_rn(w, g1, acc[2 * t + 1]);
                     }
                 }
+                // analytic mode, base evaluation (always in B): the gradient
+                // term u * scale * grad(w_c) . g_x of each corner
+                if (CL == 0 && a.analytic && q == 0) {
+                    float ex[3][2];
+#pragma unroll
+                    for (int d = 0; d < 3; ++d)
+#pragma unroll
+                        for (int f = 0; f < 2; ++f)
+                            ex[d][f] = genc_base[((a.n_evals + d) * LJ + j) * 128 + f * 64 + l];
+#pragma unroll
+                    for (int t = 0; t < 8; ++t) {
+#pragma unroll
+                        for (int f = 0; f < 2; ++f) {
+                            const float dv = (corner_dweight(pp, t, 0) * ex[0][f] +
+                                              corner_dweight(pp, t, 1) * ex[1][f]) +
+                                             corner_dweight(pp, t, 2) * ex[2][f];
+                            acc[2 * t + f] += dv;
+                        }
+                    }
+                }
             }
         }
         // B: merge equal cells of neighbouring lanes (same lane group),
@@ -787,7 +973,8 @@ __device__ __forceinline__ void flush_cluster(const GeoArgs& a, const LevelParam
 }
 
 // Rebuilds the per-sample state (grad, normal, tangent, curvature terms) of
-// the lane's sample from the 12 stored SDF values.
+// the lane's sample from the 12 stored values (6 FD values, or the analytic
+// grad in the first 3, then the 6 curvature values).
 __device__ __forceinline__ void geo_rebuild(const GeoArgs& a, const float* __restrict__ pts,
                                             const float* __restrict__ randn,
                                             const float* __restrict__ fdv, int64_t n, bool valid,
@@ -798,9 +985,14 @@ __device__ __forceinline__ void geo_rebuild(const GeoArgs& a, const float* __res
     for (int d = 0; d < 3; ++d) s.x0[d] = contract1(s.P[d], a);
     if (a.with_grad) {
         float g[3] = {0.0f, 0.0f, 0.0f};
+        if (a.analytic) {
 #pragma unroll
-        for (int j = 0; j < 6; ++j) diff_pair(g, j, valid ? fdv[n * 12 + j] : 0.0f);
-        finish_grad(a, g);
+            for (int d = 0; d < 3; ++d) g[d] = valid ? fdv[n * 12 + d] : 0.0f;
+        } else {
+#pragma unroll
+            for (int j = 0; j < 6; ++j) diff_pair(g, j, valid ? fdv[n * 12 + j] : 0.0f);
+            finish_grad(a, g);
+        }
 #pragma unroll
         for (int d = 0; d < 3; ++d) s.grad[d] = g[d];
         normalize3(s.grad, s.n);
@@ -854,10 +1046,11 @@ __global__ void __launch_bounds__(256) k_geo_pairs(
         if constexpr (!COUNT) {
             const int64_t off = offsets[tile];
             et = EmitTile<VT>{keys + off, vals + off, 0, (int)counts[tile]};
-            gt = genc + tile * a.n_evals * LJ * 128;
+            gt = genc + tile * a.genc_slots * LJ * 128;
         }
-        if (a.with_laplace) flush_cluster<LJ, 1, COUNT>(a, lv, s, valid, gt, 6, 7, et);
-        flush_cluster<LJ, 0, COUNT>(a, lv, s, valid, gt, a.with_grad ? 7 : 1, 0, et);
+        if (a.with_laplace) flush_cluster<LJ, 1, COUNT>(a, lv, s, valid, gt, 6, a.curv0, et);
+        flush_cluster<LJ, 0, COUNT>(a, lv, s, valid, gt, a.with_grad && !a.analytic ? 7 : 1, 0,
+                                    et);
         if constexpr (COUNT) {
             if (l == 0) counts[tile] = et.cursor;
         } else {
@@ -866,7 +1059,7 @@ __global__ void __launch_bounds__(256) k_geo_pairs(
     }
 }
 
-template <int LJ, int NH, bool ST, int PR>
+template <int LJ, int NH, bool ST, int PR, bool AN>
 __device__ __forceinline__ void geo_bwd_body(
     const GeoArgs& a, const float* __restrict__ pts,
     const float* __restrict__ randn, int64_t N, const void* __restrict__ table_v,
@@ -904,8 +1097,9 @@ __device__ __forceinline__ void geo_bwd_body(
          tile += (int64_t)gridDim.x * nw) {
         const int64_t n = tile * 16 + p;
         const bool valid = n < N;
-        // this tile's encode-gradient slots (0 base, 1-6 FD, 7-12 curvature)
-        float* genc = genc_g + tile * a.n_evals * LJ * 128;
+        // this tile's encode-gradient slots (0 base, then FD and curvature, or
+        // curvature and the three analytic extra slots)
+        float* genc = genc_g + tile * a.genc_slots * LJ * 128;
         GeoSample s;
         geo_rebuild(a, pts, randn, fdv, n, valid, s);
         // upstream gradients of this sample
@@ -916,21 +1110,21 @@ __device__ __forceinline__ void geo_bwd_body(
         if (valid && g_normal)
 #pragma unroll
             for (int d = 0; d < 3; ++d) gnrm[d] = g_normal[3 * n + d];
-        // base evaluation (genc slot 0)
-        {
-            v4f go;
+        // output gradient of the base evaluation
+        v4f go;
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int row = 4 * k + r;
-                float v = 0.0f;
-                if (valid && row < a.n_out) {
-                    if (g_feature) v = g_feature[n * fdim + row];
-                    if (row == 0 && g_sdf) v += g_sdf[n];
-                }
-                go[r] = v;
+        for (int r = 0; r < 4; ++r) {
+            const int row = 4 * k + r;
+            float v = 0.0f;
+            if (valid && row < a.n_out) {
+                if (g_feature) v = g_feature[n * fdim + row];
+                if (row == 0 && g_sdf) v += g_sdf[n];
             }
+            go[r] = v;
+        }
+        if constexpr (!AN) {   // base evaluation (genc slot 0)
             GeoEval<LJ, NH, false> e;
-            geo_eval_b<LJ, NH, false, ST, mlp::kF32>(a, sm, lv, table, store, tile, n_tiles, 0,
+            geo_eval_b<LJ, NH, false, ST, mlp::kF32>(a, sm, lv, table, store, tile, n_tiles, 0, 0,
                                                      s.x0, e);
             geo_eval_bwd<LJ, NH, false, mlp::kF32>(a, sm, e, go, scratch, G, genc, nullptr);
         }
@@ -958,10 +1152,10 @@ __device__ __forceinline__ void geo_bwd_body(
                 offset_point(a, s.ps, j, x, m);
                 GeoEval<LJ, NH, true> e;
                 geo_eval_b<LJ, NH, true, ST, mlp::kF32>(a, sm, lv, table, store, tile, n_tiles,
-                                                        7 + j, x, e);
+                                                        a.curv0 + j, a.dx_curv0 + j, x, e);
                 float gpos[3];
                 geo_eval_bwd<LJ, NH, true, mlp::kF32>(a, sm, e, go_row0(gf), scratch, G,
-                                                      genc + (7 + j) * LJ * 128, gpos);
+                                                      genc + (a.curv0 + j) * LJ * 128, gpos);
 #pragma unroll
                 for (int d = 0; d < 3; ++d)
                     if (m[d]) g_ps[d] += gpos[d] * a.inv2r;
@@ -973,7 +1167,20 @@ __device__ __forceinline__ void geo_bwd_body(
 #pragma unroll
             for (int d = 0; d < 3; ++d) gnrm[d] += gcr[d];
         }
-        if (a.with_grad) {
+        if constexpr (AN) {
+            // analytic mode: the base evaluation with the second-order terms
+            // of the gradient output (with_grad holds in this mode)
+            float gn2g[3];
+            normalize3_bwd(s.grad, gnrm, gn2g);
+            float gx[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) gx[d] = (gg[d] + gn2g[d]) * a.inv2r;
+            GeoEval<LJ, NH, true> e;
+            geo_eval_b<LJ, NH, true, ST, mlp::kF32>(a, sm, lv, table, store, tile, n_tiles, 0, 0,
+                                                    s.x0, e);
+            geo_eval_bwd_an<LJ>(a, sm, lv, e, go, gx, scratch, G, genc,
+                                genc + a.n_evals * LJ * 128);
+        } else if (a.with_grad) {
             float gn2g[3];
             normalize3_bwd(s.grad, gnrm, gn2g);
 #pragma unroll
@@ -986,7 +1193,7 @@ __device__ __forceinline__ void geo_bwd_body(
                 offset_point(a, s.P, j, x, m);
                 GeoEval<LJ, NH, false> e;
                 geo_eval_b<LJ, NH, false, ST, mlp::kF32>(a, sm, lv, table, store, tile, n_tiles,
-                                                         1 + j, x, e);
+                                                         1 + j, 0, x, e);
                 geo_eval_bwd<LJ, NH, false, mlp::kF32>(a, sm, e, go_row0(gf), scratch, G,
                                                        genc + (1 + j) * LJ * 128, nullptr);
             }
@@ -1054,18 +1261,18 @@ __device__ __forceinline__ void geo_bwd_body(
     a, pts, randn, N, table_v, params, fdv, store, g_sdf, g_grad, g_normal, g_feature,          \
         g_laplace, genc_g, partial
 
-template <int LJ, int NH, bool ST, int PR>
+template <int LJ, int NH, bool ST, int PR, bool AN>
 __global__ void __launch_bounds__(256) k_geo_bwd(AM_GEO_BWD_PARAMS) {
-    geo_bwd_body<LJ, NH, ST, PR>(AM_GEO_BWD_ARGS);
+    geo_bwd_body<LJ, NH, ST, PR, AN>(AM_GEO_BWD_ARGS);
 }
 
 // The 1-hidden-layer form at two waves per SIMD: its state fits the 256
 // registers a lane has at that occupancy with a small spill, and the second
 // wave hides the LDS and memory latency of the first.
-template <int LJ, bool ST, int PR>
+template <int LJ, bool ST, int PR, bool AN>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
 k_geo_bwd_o2(AM_GEO_BWD_PARAMS) {
-    geo_bwd_body<LJ, 1, ST, PR>(AM_GEO_BWD_ARGS);
+    geo_bwd_body<LJ, 1, ST, PR, AN>(AM_GEO_BWD_ARGS);
 }
 
 // status[0] = number of pairs (offset + count of the last tile); status[1]
@@ -1091,8 +1298,17 @@ static int make_geo_args(const am_grid_desc* g, const am_mlp_desc* m, const am_s
     AM_CHECK_ARG(s->active_levels >= 0 && s->active_levels <= g->n_levels, "active_levels %d",
                  s->active_levels);
     AM_CHECK_ARG(s->radius > 0.0f, "radius <= 0");
-    AM_CHECK_ARG(!s->with_grad || s->fd_eps > 0.0f, "fd_eps <= 0");
+    AM_CHECK_ARG(s->grad_type == AM_GRAD_FD || s->grad_type == AM_GRAD_ANALYTIC, "grad_type %d",
+                 s->grad_type);
+    const bool analytic = s->with_grad && s->grad_type == AM_GRAD_ANALYTIC;
+    AM_CHECK_ARG(!(s->with_laplace || (s->with_grad && !analytic)) || s->fd_eps > 0.0f,
+                 "fd_eps <= 0");
     AM_CHECK_ARG(!s->with_laplace || s->with_grad, "laplace needs with_grad");
+    if (analytic && m->n_hidden_layers != 1) {
+        set_error("analytic gradients take a one-hidden-layer SDF MLP (got %d)",
+                  m->n_hidden_layers);
+        return AM_ERR_UNSUPPORTED;
+    }
     AM_CHECK_ARG(s->precision == AM_PREC_FP32 || s->precision == AM_PREC_MIXED,
                  "precision %d", s->precision);
     AM_CHECK_ARG(m->precision == AM_PREC_FP32,
@@ -1133,7 +1349,12 @@ static int make_geo_args(const am_grid_desc* g, const am_mlp_desc* m, const am_s
     a.lds_floats = o;
     layout_params(m->n_in, m->n_out, m->n_hidden_layers, &a.pw1, &a.pb1, &a.pw2, &a.pb2, &a.pwo,
                   &a.pbo, &a.n_params);
-    a.n_evals = 1 + (s->with_grad ? 6 : 0) + (s->with_laplace ? 6 : 0);
+    a.analytic = analytic ? 1 : 0;
+    a.n_evals = 1 + (s->with_grad && !analytic ? 6 : 0) + (s->with_laplace ? 6 : 0);
+    a.curv0 = analytic ? 1 : 7;
+    a.dx_curv0 = analytic ? 1 : 0;
+    a.n_dx = (analytic ? 1 : 0) + (s->with_laplace ? 6 : 0);
+    a.genc_slots = a.n_evals + (analytic ? 3 : 0);
     a.prec = s->precision;
     *out = a;
     *LJ = lj;
@@ -1146,13 +1367,21 @@ static int geo_fwd_launch_t(const GeoArgs& a, const float* pts, const float* ran
                             float* normal, float* feature, float* laplace, float* fdv,
                             float* store, hipStream_t s) {
     const size_t lds = (size_t)a.lds_floats * sizeof(float);
-    AM_HIP(hipFuncSetAttribute((const void*)k_geo_fwd<LJ, NH, ST, PR>,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    auto kern = k_geo_fwd<LJ, NH, ST, PR, false>;
+    if (a.analytic) {
+        if constexpr (NH == 1) {
+            kern = k_geo_fwd<LJ, NH, ST, PR, true>;
+        } else {
+            set_error("analytic gradients take a one-hidden-layer SDF MLP");
+            return AM_ERR_UNSUPPORTED;
+        }
+    }
+    AM_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)lds));
     {
         KernelTimer kt_("geo_fwd", s);
-        hipLaunchKernelGGL((k_geo_fwd<LJ, NH, ST, PR>), dim3(field_blocks(n)), dim3(256), lds, s,
-                           a, pts, randn, n, table, params, sdf, grad, normal, feature, laplace,
-                           fdv, store);
+        hipLaunchKernelGGL(kern, dim3(field_blocks(n)), dim3(256), lds, s, a, pts, randn, n, table,
+                           params, sdf, grad, normal, feature, laplace, fdv, store);
     }
     AM_LAUNCH_CHECK();
     return AM_OK;
@@ -1214,7 +1443,7 @@ static void carve_geo_bwd_head(WsCarver& c, const GeoArgs& a, int LJ, int64_t n,
     const int64_t tiles = (n + 15) / 16;
     w->fdv = need_fdv ? c.take<float>((size_t)n * 12 + 1) : nullptr;
     w->sdf_tmp = need_fdv ? c.take<float>((size_t)n + 1) : nullptr;
-    w->genc = c.take<float>((size_t)geo_genc_floats(tiles, a.n_evals, LJ) + 1);
+    w->genc = c.take<float>((size_t)geo_genc_floats(tiles, a.genc_slots, LJ) + 1);
     w->partial = c.take<float>((size_t)waves * a.n_params);
     w->ctmp = c.take<float>((size_t)colsum_tmp_floats(waves, a.n_params) + 1);
     w->counts = c.take<int64_t>((size_t)tiles + 1);
@@ -1345,8 +1574,14 @@ static int geo_bwd_launch_t(const GeoArgs& a, const am_grid_desc* g, const float
     if (!c.ok()) return workspace_short(c.used, ws_bytes);
     const size_t lds = (size_t)(a.lds_floats + 4 * 128 * mlp::kScratchLd) * sizeof(float);
     AM_CHECK_ARG(lds <= 160 * 1024, "geometry backward needs %zu bytes of LDS", lds);
-    auto kern = store ? k_geo_bwd<LJ, NH, true, PR> : k_geo_bwd<LJ, NH, false, PR>;
-    if constexpr (NH == 1) kern = store ? k_geo_bwd_o2<LJ, true, PR> : k_geo_bwd_o2<LJ, false, PR>;
+    auto kern = store ? k_geo_bwd<LJ, NH, true, PR, false> : k_geo_bwd<LJ, NH, false, PR, false>;
+    if constexpr (NH == 1) {
+        kern = store ? k_geo_bwd_o2<LJ, true, PR, false> : k_geo_bwd_o2<LJ, false, PR, false>;
+        if (a.analytic)
+            kern = store ? k_geo_bwd_o2<LJ, true, PR, true> : k_geo_bwd_o2<LJ, false, PR, true>;
+    } else {
+        AM_CHECK_ARG(!a.analytic, "analytic gradients take a one-hidden-layer SDF MLP");
+    }
     AM_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)lds));
     {
@@ -1408,7 +1643,7 @@ extern "C" int64_t am_sdf_store_floats(const am_grid_desc* g, const am_mlp_desc*
     int LJ;
     if (make_geo_args(g, m, sd, &a, &LJ) || n < 0) return -1;
     const int64_t tiles = (n + 15) / 16;
-    return geo_enc_floats(tiles, a.n_evals, LJ) + geo_dx_floats(tiles, a.with_laplace, LJ);
+    return geo_enc_floats(tiles, a.n_evals, LJ) + geo_dx_floats(tiles, a.n_dx, LJ);
 }
 
 extern "C" int am_sdf_fwd(const am_grid_desc* g, const am_mlp_desc* m, const am_sdf_desc* sd,

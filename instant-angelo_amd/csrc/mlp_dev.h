@@ -119,6 +119,19 @@ __device__ __forceinline__ float act_bwd(int act, float z, float beta) {
     return 1.0f;
 }
 
+// Second derivative of the activation at z given its first derivative dh
+// there (the analytic-gradient backward): softplus beta * s * (1 - s) with
+// s = sigmoid(beta z) = dh (0 above torch's threshold, where dh = 1), 0 for
+// relu and identity, sigmoid y (1 - y)(1 - 2y).
+__device__ __forceinline__ float act_d2(int act, float z, float beta, float dh) {
+    if (act == AM_ACT_SOFTPLUS) return z * beta > 20.0f ? 0.0f : beta * dh * (1.0f - dh);
+    if (act == AM_ACT_SIGMOID) {
+        const float y = 1.0f / (1.0f + expf(-z));
+        return dh * (1.0f - 2.0f * y);
+    }
+    return 0.0f;
+}
+
 // Activation and its derivative at z together (one exp for softplus).
 __device__ __forceinline__ void act_fb(int act, float z, float beta, float& h, float& dh) {
     if (act == AM_ACT_SOFTPLUS) {

@@ -8,6 +8,9 @@ Restates models/geometry.py:195-287 (VolumeSDF.forward, grad_type
                  feature = cat(out, x01*2-1)                         (:204-208)
   FD normal      clamp(P + offsets, -r, r) -> contract -> sdf;
                  grad = 0.5*(f+ - f-)/eps                            (:219-234)
+  analytic grad  grad_type 'analytic' (:214-218): grad = autograd.grad(sdf,
+                 points, create_graph=True), so losses on it differentiate to
+                 second order
   curvature      n = normalize(grad); t = cross(n, normalize(randn));
                  ps = x01 + t*eps (the contracted point, :246); the six
                  offsets are added to ps, clamped and contracted again
@@ -45,10 +48,13 @@ def contract(x, r, mode="f32"):
 
 
 def volume_sdf(points, field, r, eps, with_grad=True, with_laplace=False, randn=None,
-               recontract=True, mode="f32"):
+               recontract=True, mode="f32", grad_type="finite_difference"):
     """field(x01 [M,3]) -> out [M, n_out].  Returns dict of sdf [N], grad [N,3],
     normal [N,3], feature [N, n_out+3], laplace [N,1]."""
     dt = points.dtype
+    analytic = with_grad and grad_type == "analytic"
+    if analytic and not points.requires_grad:
+        points = points.detach().requires_grad_(True)
     x = contract(points, r, mode)
     out = field(x.view(-1, 3))
     sdf = out[..., 0]
@@ -59,10 +65,14 @@ def volume_sdf(points, field, r, eps, with_grad=True, with_laplace=False, randn=
     e = np.float32(eps) if mode == "f32" else float(eps)
     offsets = (torch.tensor(OFFSETS, dtype=dt, device=points.device)
                * torch.tensor(e, dtype=dt, device=points.device))
-    pd = (points[..., None, :] + offsets).clamp(-r, r)
-    xd = contract(pd, r, mode)
-    fd = field(xd.view(-1, 3))[..., 0].view(-1, 6)
-    grad = _scalar_div(0.5 * (fd[..., 0::2] - fd[..., 1::2]), float(e), mode)
+    if analytic:
+        grad = torch.autograd.grad(sdf, points, grad_outputs=torch.ones_like(sdf),
+                                   create_graph=True, retain_graph=True, only_inputs=True)[0]
+    else:
+        pd = (points[..., None, :] + offsets).clamp(-r, r)
+        xd = contract(pd, r, mode)
+        fd = field(xd.view(-1, 3))[..., 0].view(-1, 6)
+        grad = _scalar_div(0.5 * (fd[..., 0::2] - fd[..., 1::2]), float(e), mode)
     normal = F.normalize(grad, p=2, dim=-1)
     res.update(grad=grad, normal=normal)
     if not with_laplace:

@@ -9,7 +9,8 @@ from oracle import nets as onets
 
 
 def model_config(L=8, log2T=15, base=16, scale=1.5, geo_hidden=1, feature_dim=16, radius=1.0,
-                 start_level=None, rad_hidden=2, grid_prune=False, step=None, cap=0, seed=0):
+                 start_level=None, rad_hidden=2, grid_prune=False, step=None, cap=0, seed=0,
+                 grad_type="finite_difference"):
     from angelo_amd.utils import to_config
     return to_config({
         "name": "neus", "radius": radius, "num_samples_per_ray": 64, "ray_chunk": 4096,
@@ -19,7 +20,7 @@ def model_config(L=8, log2T=15, base=16, scale=1.5, geo_hidden=1, feature_dim=16
         "variance": {"init_val": 0.3, "modulate": False},
         "geometry": {
             "name": "volume-sdf", "radius": radius, "feature_dim": feature_dim,
-            "grad_type": "finite_difference", "finite_difference_eps": "progressive", "seed": seed,
+            "grad_type": grad_type, "finite_difference_eps": "progressive", "seed": seed,
             "isosurface": None,
             "xyz_encoding_config": {
                 "otype": "ProgressiveBandHashGrid", "n_levels": L, "n_features_per_level": 2,

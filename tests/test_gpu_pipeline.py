@@ -329,3 +329,64 @@ def test_radiance_fused_fwd_bwd(dev, rad_hidden):
     gflat = torch.cat([torch.cat([lin.weight.grad.reshape(-1), lin.bias.grad.reshape(-1)])
                        for lin in tex.network.linears()])
     assert helpers.rel_err(gflat, leaves["rad"].grad) < 1e-4
+
+
+@pytest.mark.parametrize("laplace", [False, True])
+@pytest.mark.parametrize("L,log2T,base,scale", [(8, 15, 16, 1.5), (16, 16, 32, 1.3195079107728942)])
+def test_sdf_analytic_fwd_bwd(dev, laplace, L, log2T, base, scale):
+    """grad_type 'analytic': sdf, the analytic grad (d sdf / d x through the
+    encoding and the MLP), normal, feature and laplace, and the gradients of
+    a loss linear in all of them -- the grad term needs second derivatives --
+    against the oracle's autograd (create_graph), FP64 and FP32."""
+    from angelo_amd import functions as fn
+    model = _field_model(dev, L=L, log2T=log2T, base=base, scale=scale, grad_type="analytic")
+    geo = model.geometry
+    g = torch.Generator().manual_seed(13)
+    n = 1000 + 3
+    pts = (torch.rand(n, 3, generator=g) * 2 - 1) * 0.9
+    pts[0] = torch.tensor([0.9, -0.9, 0.899])
+    randn = torch.randn(n, 3, generator=g)
+    eps = float(geo._finite_difference_eps)
+    gw = [torch.randn(n, generator=g), torch.randn(n, 3, generator=g),
+          torch.randn(n, 3, generator=g), torch.randn(n, geo.n_output_dims + 3, generator=g),
+          torch.randn(n, generator=g)]
+
+    def loss_of(o, dt):
+        w = [x.to(o["sdf"].device, dt) for x in gw]
+        loss = ((o["sdf"].reshape(-1) * w[0]).sum() + (o["grad"] * w[1]).sum()
+                + (o["normal"] * w[2]).sum() + (o["feature"] * w[3]).sum())
+        if laplace:
+            loss = loss + (o["laplace"].reshape(-1) * w[4]).sum()
+        return loss
+
+    def oracle(dtype, mode):
+        field, _, leaves = helpers.oracle_fns(model, dtype, mode)
+        o = ofield.volume_sdf(pts.to(dtype), field, 1.0, eps, True, laplace, randn.to(dtype), True,
+                              mode, grad_type="analytic")
+        loss_of(o, dtype).backward()
+        return o, leaves
+
+    o64, l64 = oracle(torch.float64, "f64")
+    o32, l32 = oracle(torch.float32, "f32")
+    grid = geo.grid_module
+    table = grid.params
+    flat = geo.network.flat_params().detach().clone().requires_grad_(True)
+    sdf, grad, normal, feature, lap = fn.SDFFieldFn.apply(
+        pts.to(dev), randn.to(dev), table, flat, grid.grid, geo.network.desc,
+        geo._sdf_desc(True, laplace), grid.kernel_table())
+    outs = dict(sdf=sdf, grad=grad, normal=normal, feature=feature)
+    if laplace:
+        outs["laplace"] = lap
+    loss_of({k: v for k, v in outs.items()}, torch.float32).backward()
+    for k, v in outs.items():
+        ref = o64[k].reshape(v.shape)
+        e_hip = helpers.rel_err(v, ref)
+        e_t32 = helpers.rel_err(o32[k].reshape(v.shape), ref)
+        assert e_hip <= 8 * e_t32 + 1e-5, (k, e_hip, e_t32)
+    gt = table.grad.view(-1, 2).cpu()
+    helpers.assert_nontrivial(gt, "table gradient")
+    for name, got, ref64, ref32 in (("table", gt, l64["table"].grad, l32["table"].grad),
+                                    ("sdf mlp", flat.grad.cpu(), l64["geo"].grad, l32["geo"].grad)):
+        e_hip = helpers.rel_err(got, ref64)
+        e_t32 = helpers.rel_err(ref32, ref64)
+        assert e_hip <= 8 * e_t32 + 1e-5, (name, e_hip, e_t32)

@@ -93,6 +93,53 @@ def test_volume_sdf_gradcheck():
                                     eps=1e-6, atol=1e-4)
 
 
+def test_analytic_gradient_closed_forms():
+    """grad_type 'analytic': a field linear in the world position gives its
+    coefficient exactly; for a quadratic field central differences are exact,
+    so both modes agree to rounding."""
+    a = torch.tensor([0.3, -1.2, 0.5], dtype=D)
+    r = 1.0
+
+    def linear(x01):
+        P = x01 * 2 * r - r
+        return torch.cat([(P @ a)[:, None], torch.zeros(x01.shape[0], 15, dtype=D)], -1)
+
+    P = torch.rand(40, 3, dtype=D) * 1.6 - 0.8
+    o = ofield.volume_sdf(P, linear, r, 1e-3, True, False, mode="f64", grad_type="analytic")
+    torch.testing.assert_close(o["grad"], a.expand(40, 3), rtol=1e-12, atol=1e-12)
+    c = torch.tensor([0.7, -0.4, 1.1], dtype=D)
+
+    def quadratic(x01):
+        P = x01 * 2 * r - r
+        return torch.cat([((P * P) @ c + P @ a)[:, None],
+                          torch.zeros(x01.shape[0], 15, dtype=D)], -1)
+
+    oa = ofield.volume_sdf(P, quadratic, r, 1e-2, True, False, mode="f64", grad_type="analytic")
+    of = ofield.volume_sdf(P, quadratic, r, 1e-2, True, False, mode="f64")
+    torch.testing.assert_close(oa["grad"], 2 * P * c + a, rtol=1e-12, atol=1e-12)
+    torch.testing.assert_close(oa["grad"], of["grad"], rtol=1e-9, atol=1e-9)
+
+
+def test_analytic_volume_sdf_second_order_gradcheck():
+    """Losses on the analytic grad (eikonal, normals, curvature tangent)
+    differentiate to second order: gradcheck of the parameter gradients."""
+    g = torch.Generator().manual_seed(1)
+    W1 = torch.randn(8, 3, generator=g, dtype=D)
+    W2 = torch.randn(16, 8, generator=g, dtype=D) * 0.3
+    P = (torch.rand(3, 3, generator=g, dtype=D) * 1.4 - 0.7)
+    rnd = torch.randn(3, 3, generator=g, dtype=D)
+
+    def fn(w1, w2):
+        field = lambda x: torch.tanh(x @ w1.T) @ w2.T
+        o = ofield.volume_sdf(P, field, 1.0, 1e-2, True, True, rnd, mode="f64",
+                              grad_type="analytic")
+        eik = ((o["grad"].norm(dim=-1) - 1) ** 2).sum()
+        return o["sdf"].sum() + eik + o["normal"].sum() + o["laplace"].sum()
+
+    assert torch.autograd.gradcheck(fn, (W1.requires_grad_(True), W2.requires_grad_(True)),
+                                    eps=1e-6, atol=1e-4)
+
+
 def test_alpha_hand_case():
     """sdf 0 at a head-on crossing (cos = -1, ratio 1): prev = +d/2, next = -d/2."""
     s, d = 20.0, 0.1
