@@ -53,15 +53,20 @@ def algorithmic_bytes(kernel, spec, n_samples, n_rays, n_pairs, mixed=False):
       read and written once.
     seg_reduce: each pair read once + one 8 B row written per pair (bound).
     composite: 40 B per sample + 32 B per ray.
+    Analytic-gradient mode: 7 evaluations (base with position derivatives +
+    6 curvature), 7 derivative slots stored, 3 more encode-gradient slots.
     """
     L, F = spec["levels"], 2
     bt = 2 if mixed else 4
     pair = 8 if mixed else 12
+    n_ev, n_dx, n_genc = evals_per_sample(spec), 6, evals_per_sample(spec)
+    if spec.get("grad_type") == "analytic":
+        n_dx, n_genc = 7, n_ev + 3
     per_eval = 12 + 8 * L * F * bt
-    store = (13 * L * F + 6 * 3 * L * F) * 4
-    genc = 13 * L * F * 4
+    store = (n_ev * L * F + n_dx * 3 * L * F) * 4
+    genc = n_genc * L * F * 4
     if kernel == "geo_fwd":
-        return n_samples * (13 * per_eval + 39 * 4 + store)
+        return n_samples * (n_ev * per_eval + 39 * 4 + store)
     if kernel == "geo_bwd":
         return n_samples * (store + 12 * 4 + 24 + 27 * 4 + genc)
     if kernel == "pair_count":
@@ -75,6 +80,12 @@ def algorithmic_bytes(kernel, spec, n_samples, n_rays, n_pairs, mixed=False):
     if kernel in ("composite_fwd", "composite_bwd"):
         return n_samples * 40 + n_rays * 32
     return None
+
+
+def evals_per_sample(spec):
+    """Encode + SDF-MLP evaluations per sample: base + 6 central
+    differences + 6 curvature, or base + 6 curvature in analytic mode."""
+    return 7 if spec.get("grad_type") == "analytic" else 13
 
 
 def profiled_traffic(workload):
@@ -311,7 +322,8 @@ def main():
         "data": "synthetic (analytic scene, seeded cameras)",
         "config": {"workload": args.workload, "rays_per_step": rays,
                    "samples_per_step": round(samples_per_step, 1), "levels": spec["levels"],
-                   "log2_hashmap_size": spec["log2T"], "evals_per_sample": 13,
+                   "log2_hashmap_size": spec["log2T"], "evals_per_sample": evals_per_sample(spec),
+                   "grad_type": spec.get("grad_type", "finite_difference"),
                    "optimizer": "AdamW (in the timed step)"},
         "rays_per_s": round(rays * args.steps / (gpu_ms / 1e3), 1),
         "loss_scaling": ({"final_scale": wl.scaler.scale, "skipped_steps": wl.skipped_steps}

@@ -24,7 +24,7 @@ NEURALANGELO_SCALE = 1.3195079107728942
 
 def model_config(levels, log2T, base, scale, rays, samples_per_ray, step=None, cap=0,
                  grid_prune=False, geo_hidden=1, feature_dim=16, radius=1.0, start_level=None,
-                 kahan=0, seed=0, precision="fp32"):
+                 kahan=0, seed=0, precision="fp32", grad_type="finite_difference"):
     return to_config({
         "name": "neus", "radius": radius, "num_samples_per_ray": samples_per_ray,
         "train_num_rays": rays, "max_train_num_rays": rays, "dynamic_ray_sampling": False,
@@ -35,7 +35,7 @@ def model_config(levels, log2T, base, scale, rays, samples_per_ray, step=None, c
         "variance": {"init_val": 0.3, "modulate": False},
         "geometry": {
             "name": "volume-sdf", "radius": radius, "feature_dim": feature_dim,
-            "grad_type": "finite_difference", "finite_difference_eps": "progressive",
+            "grad_type": grad_type, "finite_difference_eps": "progressive",
             "seed": seed, "isosurface": None,
             "xyz_encoding_config": {
                 "otype": "ProgressiveBandHashGrid", "n_levels": levels, "n_features_per_level": 2,
@@ -88,6 +88,10 @@ WORKLOADS = {
 # pairs with dynamic loss scaling, Kahan-compensated compositing, fp64 loss
 # reductions.
 WORKLOADS["neuralangelo-mixed"] = dict(WORKLOADS["neuralangelo-fp32"], precision="mixed")
+# grad_type 'analytic' (the default of configs/neuralangelo-colmap_sparse.yaml:38):
+# one base evaluation with d sdf / d x instead of six central differences,
+# curvature still from six evaluations around the shifted point.
+WORKLOADS["neuralangelo-analytic"] = dict(WORKLOADS["neuralangelo-fp32"], grad_type="analytic")
 
 
 class Workload:
@@ -101,7 +105,8 @@ class Workload:
         self.uniform = not spec["grid_prune"]
         mcfg = model_config(spec["levels"], spec["log2T"], spec["base"], spec["scale"], spec["rays"],
                             spec["samples_per_ray"], spec["step"], spec["cap"], spec["grid_prune"],
-                            seed=seed, precision=spec.get("precision", "fp32"))
+                            seed=seed, precision=spec.get("precision", "fp32"),
+                            grad_type=spec.get("grad_type", "finite_difference"))
         torch.manual_seed(seed)
         self.mcfg, self.scfg = mcfg, system_config()
         self.model = make("neus", mcfg).to(device)

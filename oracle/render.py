@@ -84,7 +84,8 @@ def composite(sdf, normal, rgb, dirs, t_starts, t_ends, ray_indices, n_rays, inv
 
 
 def neus_forward(rays, ray_indices, t_starts, t_ends, n_rays, field, radiance, inv_s, radius, eps,
-                 ratio, randn, background=None, recontract=True, mode="f32"):
+                 ratio, randn, background=None, recontract=True, mode="f32",
+                 grad_type="finite_difference"):
     """NeuSModel.forward_ from given samples.  field(x01) -> [M, n_out];
     radiance(feature, dirs, normal) -> rgb."""
     ridx = ray_indices.long()
@@ -93,7 +94,8 @@ def neus_forward(rays, ray_indices, t_starts, t_ends, n_rays, field, radiance, i
     t_dirs = rays_d[ridx]
     midpoints = (t_starts + t_ends) / 2.0
     positions = t_origins + t_dirs * midpoints[:, None]
-    geo = ofield.volume_sdf(positions, field, radius, eps, True, True, randn, recontract, mode)
+    geo = ofield.volume_sdf(positions, field, radius, eps, True, True, randn, recontract, mode,
+                            grad_type=grad_type)
     normal = geo["normal"]
     rgb = radiance(geo["feature"], t_dirs, normal)
     c = composite(geo["sdf"], normal, rgb, t_dirs, t_starts, t_ends, ridx, n_rays, inv_s, ratio)

@@ -28,12 +28,14 @@ def _leaf_params(model):
     return leaves
 
 
-@pytest.mark.parametrize("n_rays,geo_hidden,L", [(64, 1, 8), (96, 2, 16)])
-def test_neus_step_matches_oracle(dev, n_rays, geo_hidden, L):
+@pytest.mark.parametrize("n_rays,geo_hidden,L,grad_type", [(64, 1, 8, "finite_difference"),
+                                                           (96, 2, 16, "finite_difference"),
+                                                           (64, 1, 16, "analytic")])
+def test_neus_step_matches_oracle(dev, n_rays, geo_hidden, L, grad_type):
     import angelo_amd
     from angelo_amd import scenes
     from angelo_amd.system import NeuSTrainer
-    kw = dict(L=L, geo_hidden=geo_hidden)
+    kw = dict(L=L, geo_hidden=geo_hidden, grad_type=grad_type)
     if L == 16:
         kw.update(log2T=16, base=32, scale=1.3195079107728942)
     cfg = helpers.model_config(**kw)
@@ -54,7 +56,8 @@ def test_neus_step_matches_oracle(dev, n_rays, geo_hidden, L):
     inv_s = torch.exp(var * 10.0).clip(1e-6, 1e6)
     eps = float(model.geometry._finite_difference_eps)
     o = orender.neus_forward(rays.double(), ridx, t0.double(), t1.double(), n_rays, field,
-                             radiance, inv_s, 1.0, eps, model.cos_anneal_ratio, randn.double())
+                             radiance, inv_s, 1.0, eps, model.cos_anneal_ratio, randn.double(),
+                             grad_type=grad_type)
     o_loss, o_terms = orender.losses(o, gt.double(), 10.0, 0.1, 0.5)
     o_loss.backward()
     # the same arithmetic restated in fp32 (fp32 positions and blends): its
@@ -63,7 +66,7 @@ def test_neus_step_matches_oracle(dev, n_rays, geo_hidden, L):
     var32 = model.variance.variance.detach().cpu().float().clone().requires_grad_(True)
     o32 = orender.neus_forward(rays, ridx, t0, t1, n_rays, f32, r32,
                                torch.exp(var32 * 10.0).clip(1e-6, 1e6), 1.0, eps,
-                               model.cos_anneal_ratio, randn)
+                               model.cos_anneal_ratio, randn, grad_type=grad_type)
     l32, _ = orender.losses(o32, gt, 10.0, 0.1, 0.5)
     l32.backward()
 

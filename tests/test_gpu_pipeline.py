@@ -236,12 +236,13 @@ def test_sdf_bwd_deterministic(dev):
     nat.sdf_bwd_check()   # count and emit passes agreed on every call
 
 
+@pytest.mark.parametrize("grad_type", ["finite_difference", "analytic"])
 @pytest.mark.parametrize("laplace", [False, True])
-def test_sdf_store_and_recompute_backward_are_identical(dev, laplace):
+def test_sdf_store_and_recompute_backward_are_identical(dev, laplace, grad_type):
     """The stored-encoding backward reads back the very floats the recompute
     form gathers again, so both give bit-identical gradients."""
     from angelo_amd import functions
-    model = _field_model(dev)
+    model = _field_model(dev, grad_type=grad_type)
     geo = model.geometry
     g = torch.Generator().manual_seed(6)
     n = 4099
