@@ -245,15 +245,18 @@ int am_sdf_bwd(const am_grid_desc* g, const am_mlp_desc* m, const am_sdf_desc* s
 /* ------------------------------------------------------------------------ */
 /* Fused VolumeRadiance.forward (models/texture.py:23-30): SH(deg) of        */
 /* (d+1)/2, cat(feature, sh, normal) -> MLP -> sigmoid.  dirs are gathered    */
-/* from rays[ray_indices] (models/neus.py:224).                               */
+/* from rays[ray_indices] (models/neus.py:224).  base_color = 1:              */
+/* VolumeDualColor.forward (models/texture.py:38-55) adds sigmoid(feature[:,  */
+/* 1:4]) (feature_dim >= 4).                                                  */
 /* ------------------------------------------------------------------------ */
-int am_radiance_fwd(const am_mlp_desc* m, int32_t sh_degree, const float* feature,
-                    int32_t feature_dim, const float* rays, const int32_t* ray_indices,
-                    const float* normal, int64_t n, const float* params, float* rgb,
-                    void* stream);
+int am_radiance_fwd(const am_mlp_desc* m, int32_t sh_degree, int32_t base_color,
+                    const float* feature, int32_t feature_dim, const float* rays,
+                    const int32_t* ray_indices, const float* normal, int64_t n,
+                    const float* params, float* rgb, void* stream);
 size_t am_radiance_bwd_workspace_size(const am_mlp_desc* m, int64_t n);
-int am_radiance_bwd(const am_mlp_desc* m, int32_t sh_degree, const float* feature,
-                    int32_t feature_dim, const float* rays, const int32_t* ray_indices,
+int am_radiance_bwd(const am_mlp_desc* m, int32_t sh_degree, int32_t base_color,
+                    const float* feature, int32_t feature_dim, const float* rays,
+                    const int32_t* ray_indices,
                     const float* normal, int64_t n, const float* params, const float* g_rgb,
                     float* g_feature, float* g_normal, float* dparams, void* workspace,
                     size_t workspace_bytes, void* stream);

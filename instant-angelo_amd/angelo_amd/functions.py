@@ -202,22 +202,24 @@ class SDFFieldFn(torch.autograd.Function):
 
 class RadianceFn(torch.autograd.Function):
     """Fused VolumeRadiance.forward (models/texture.py:23-30) with sigmoid colour:
-    rgb = sigmoid(MLP(cat(feature, SH((d+1)/2), normal))), d = rays[ridx, 3:6]."""
+    rgb = sigmoid(MLP(cat(feature, SH((d+1)/2), normal))), d = rays[ridx, 3:6];
+    base_color (VolumeDualColor, :38-55) adds sigmoid(feature[:, 1:4])."""
 
     @staticmethod
-    def forward(ctx, feature, normal, params, rays, ray_indices, mdesc, sh_degree):
+    def forward(ctx, feature, normal, params, rays, ray_indices, mdesc, sh_degree, base_color=0):
         nat.require_cuda(feature, normal, params, rays, ray_indices)
         lib = nat.load()
         feature = _f32c(feature)
         normal = _f32c(normal)
         n = feature.shape[0]
         rgb = torch.empty(n, mdesc.n_out, device=feature.device, dtype=torch.float32)
-        nat.check(lib.am_radiance_fwd(nat.sptr(mdesc), int(sh_degree), nat.ptr(feature),
+        nat.check(lib.am_radiance_fwd(nat.sptr(mdesc), int(sh_degree), int(base_color),
+                                      nat.ptr(feature),
                                       feature.shape[1], nat.ptr(rays), nat.ptr(ray_indices),
                                       nat.ptr(normal), n, nat.ptr(params), nat.ptr(rgb),
                                       nat.stream()), "am_radiance_fwd")
         ctx.save_for_backward(feature, normal, params, rays, ray_indices)
-        ctx.mdesc, ctx.deg = mdesc, int(sh_degree)
+        ctx.mdesc, ctx.deg, ctx.base = mdesc, int(sh_degree), int(base_color)
         return rgb
 
     @staticmethod
@@ -231,12 +233,12 @@ class RadianceFn(torch.autograd.Function):
         dparams = torch.empty_like(params)
         ws = nat.workspace(lib.am_radiance_bwd_workspace_size(nat.sptr(ctx.mdesc), n),
                            feature.device)
-        nat.check(lib.am_radiance_bwd(nat.sptr(ctx.mdesc), ctx.deg, nat.ptr(feature),
+        nat.check(lib.am_radiance_bwd(nat.sptr(ctx.mdesc), ctx.deg, ctx.base, nat.ptr(feature),
                                       feature.shape[1], nat.ptr(rays), nat.ptr(ridx),
                                       nat.ptr(normal), n, nat.ptr(params), nat.ptr(g_rgb),
                                       nat.ptr(gf), nat.ptr(gn), nat.ptr(dparams), nat.ptr(ws),
                                       ws.numel(), nat.stream()), "am_radiance_bwd")
-        return gf, gn, dparams, None, None, None, None
+        return gf, gn, dparams, None, None, None, None, None
 
 
 class CompositeFn(torch.autograd.Function):

@@ -1,4 +1,6 @@
-"""VolumeRadiance (models/texture.py:10-36), registered as 'volume-radiance'.
+"""VolumeRadiance (models/texture.py:10-36), registered as 'volume-radiance',
+and VolumeDualColor (:38-64), 'volume-dual-color': the same network plus a
+base colour sigmoid(feature[..., 1:4]) when the colour activation is set.
 
 The SH direction encoding, the radiance MLP and the colour activation run as
 one fused HIP launch (am_radiance_fwd / am_radiance_bwd) when the extra
@@ -46,11 +48,17 @@ class VolumeRadiance(nn.Module):
                 and self.n_input_dims <= 48
                 and features.shape[-1] + enc.n_output_dims + 3 == self.network.dim_in)
 
+    base_color = 0
+
     def fused(self, features, normal, rays, ray_indices):
         """rgb of samples whose view directions are rays[ray_indices, 3:6]."""
+        if (self.color_activation or "").lower() != "sigmoid":
+            raise NotImplementedError("the fused radiance launch applies the sigmoid colour "
+                                      f"activation (got {self.color_activation!r})")
         return fn.RadianceFn.apply(features.reshape(-1, features.shape[-1]),
                                    normal.reshape(-1, 3), self.network.flat_params(), rays,
-                                   ray_indices, self.network.desc, self.encoding.encoding.degree)
+                                   ray_indices, self.network.desc, self.encoding.encoding.degree,
+                                   self.base_color)
 
     def forward(self, features, dirs, *args):
         """models/texture.py:23-30."""
@@ -67,6 +75,8 @@ class VolumeRadiance(nn.Module):
         color = self.network(net_in).view(*features.shape[:-1], self.n_output_dims).float()
         if self.color_activation is not None:
             color = get_activation(self.color_activation)(color)
+            if self.base_color:
+                color = color + get_activation(self.color_activation)(features[..., 1:4])
         return color
 
     def update_step(self, epoch, global_step):
@@ -74,3 +84,10 @@ class VolumeRadiance(nn.Module):
 
     def regularizations(self, out):
         return {}
+
+
+@register("volume-dual-color")
+class VolumeDualColor(VolumeRadiance):
+    """models/texture.py:38-64: color = act(MLP(...)) + act(features[..., 1:4])
+    with the colour activation (sigmoid); fused in the radiance launch."""
+    base_color = 1

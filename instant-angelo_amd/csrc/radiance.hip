@@ -10,6 +10,7 @@ namespace am {
 
 struct RadArgs {
     int n_in, n_out, act, fdim, sh_deg, n_sh;
+    int base_color;  // VolumeDualColor: + sigmoid(feature[:, 1:4])
     float beta;
     int w1, b1, w2, b2, wo, bo, ld1, lds_floats;
     int64_t pw1, pb1, pw2, pb2, pwo, pbo, n_params;
@@ -176,7 +177,11 @@ __global__ void __launch_bounds__(256) k_rad_fwd(RadArgs a, const float* __restr
         if (valid && k == 0)
 #pragma unroll
             for (int r = 0; r < 4; ++r)
-                if (r < a.n_out) rgb[n * a.n_out + r] = 1.0f / (1.0f + expf(-e.out[r]));
+                if (r < a.n_out) {
+                    float c = 1.0f / (1.0f + expf(-e.out[r]));
+                    if (a.base_color) c += 1.0f / (1.0f + expf(-feature[n * a.fdim + 1 + r]));
+                    rgb[n * a.n_out + r] = c;
+                }
     }
 }
 
@@ -270,7 +275,12 @@ __global__ void __launch_bounds__(256) k_rad_bwd(RadArgs a, const float* __restr
                     const int c = 4 * s + k;
                     if (s >= KS || c >= a.n_in) continue;
                     if (c < a.fdim) {
-                        if (g_feature) g_feature[n * a.fdim + c] = gin[t][r];
+                        float gv = gin[t][r];
+                        if (a.base_color && c >= 1 && c <= 3) {   // d sigmoid(feature[c]) term
+                            const float y = 1.0f / (1.0f + expf(-feature[n * a.fdim + c]));
+                            gv += g_rgb[n * a.n_out + c - 1] * ((1.0f - y) * y);
+                        }
+                        if (g_feature) g_feature[n * a.fdim + c] = gv;
                     } else if (c >= a.fdim + a.n_sh) {
                         if (g_normal) g_normal[3 * n + (c - a.fdim - a.n_sh)] = gin[t][r];
                     }
@@ -338,8 +348,12 @@ __global__ void k_sh(const float* __restrict__ d01, int64_t n, int deg, float* _
 }
 
 // host side
-static int make_rad_args(const am_mlp_desc* m, int sh_degree, int fdim, RadArgs* out, int* KS) {
+static int make_rad_args(const am_mlp_desc* m, int sh_degree, int base_color, int fdim,
+                         RadArgs* out, int* KS) {
     if (int rc = check_mlp_common(m)) return rc;
+    AM_CHECK_ARG(base_color == 0 || base_color == 1, "base_color %d", base_color);
+    AM_CHECK_ARG(!base_color || (fdim >= 4 && m->n_out == 3),
+                 "base colour needs feature_dim >= 4 and 3 outputs");
     AM_CHECK_ARG(sh_degree >= 1 && sh_degree <= 4, "sh_degree %d out of [1,4]", sh_degree);
     AM_CHECK_ARG(fdim >= 0 && m->n_in == fdim + sh_degree * sh_degree + 3,
                  "radiance n_in %d != feature_dim + degree^2 + 3", m->n_in);
@@ -350,6 +364,7 @@ static int make_rad_args(const am_mlp_desc* m, int sh_degree, int fdim, RadArgs*
     a.act = m->hidden_act;
     a.beta = m->softplus_beta;
     a.fdim = fdim;
+    a.base_color = base_color;
     a.sh_deg = sh_degree;
     a.n_sh = sh_degree * sh_degree;
     const int ks = (m->n_in + 3) / 4;
@@ -432,13 +447,13 @@ static int rad_bwd_launch(const RadArgs& a, const float* feature, const float* r
 
 using namespace am;
 
-extern "C" int am_radiance_fwd(const am_mlp_desc* m, int32_t sh_degree, const float* feature,
-                               int32_t feature_dim, const float* rays,
+extern "C" int am_radiance_fwd(const am_mlp_desc* m, int32_t sh_degree, int32_t base_color,
+                               const float* feature, int32_t feature_dim, const float* rays,
                                const int32_t* ray_indices, const float* normal, int64_t n,
                                const float* params, float* rgb, void* stream) {
     RadArgs a;
     int KS;
-    if (int rc = make_rad_args(m, sh_degree, feature_dim, &a, &KS)) return rc;
+    if (int rc = make_rad_args(m, sh_degree, base_color, feature_dim, &a, &KS)) return rc;
     AM_CHECK_ARG(n >= 0, "n < 0");
     if (n == 0) return AM_OK;
     AM_CHECK_ARG(feature && rays && ray_indices && normal && params && rgb, "NULL buffer");
@@ -457,15 +472,15 @@ extern "C" size_t am_radiance_bwd_workspace_size(const am_mlp_desc* m, int64_t n
     return c.used + 256;
 }
 
-extern "C" int am_radiance_bwd(const am_mlp_desc* m, int32_t sh_degree, const float* feature,
-                               int32_t feature_dim, const float* rays,
+extern "C" int am_radiance_bwd(const am_mlp_desc* m, int32_t sh_degree, int32_t base_color,
+                               const float* feature, int32_t feature_dim, const float* rays,
                                const int32_t* ray_indices, const float* normal, int64_t n,
                                const float* params, const float* g_rgb, float* g_feature,
                                float* g_normal, float* dparams, void* workspace,
                                size_t workspace_bytes, void* stream) {
     RadArgs a;
     int KS;
-    if (int rc = make_rad_args(m, sh_degree, feature_dim, &a, &KS)) return rc;
+    if (int rc = make_rad_args(m, sh_degree, base_color, feature_dim, &a, &KS)) return rc;
     AM_CHECK_ARG(n >= 0, "n < 0");
     AM_CHECK_ARG(params && dparams && workspace, "NULL buffer");
     AM_CHECK_ARG(n == 0 || (feature && rays && ray_indices && normal && g_rgb), "NULL input");

@@ -302,10 +302,16 @@ def test_sdf_bwd_workspace_sized_by_pairs(dev):
         functions.SDF_BWD_WORKSPACE = held
 
 
-@pytest.mark.parametrize("rad_hidden", [1, 2])
-def test_radiance_fused_fwd_bwd(dev, rad_hidden):
+@pytest.mark.parametrize("rad_hidden,dual", [(1, False), (2, False), (2, True)])
+def test_radiance_fused_fwd_bwd(dev, rad_hidden, dual):
+    """dual: VolumeDualColor (models/texture.py:38-64), + sigmoid(feature[:, 1:4])."""
     model = _field_model(dev, rad_hidden=rad_hidden)
     tex = model.texture
+    if dual:
+        from angelo_amd.texture import VolumeDualColor
+        dual_tex = VolumeDualColor(tex.config).to(dev)
+        dual_tex.load_state_dict(tex.state_dict())
+        tex = dual_tex
     g = torch.Generator().manual_seed(5)
     n = 777
     feature = torch.randn(n, 19, generator=g)
@@ -317,6 +323,8 @@ def test_radiance_fused_fwd_bwd(dev, rad_hidden):
     fl = feature.double().requires_grad_(True)
     nl = normal.double().requires_grad_(True)
     ref = radiance(fl, rays[ridx.long(), 3:6].double(), nl)
+    if dual:
+        ref = ref + torch.sigmoid(fl[:, 1:4])
     (ref * g_rgb.double()).sum().backward()
     fd = feature.to(dev).requires_grad_(True)
     nd = normal.to(dev).requires_grad_(True)
