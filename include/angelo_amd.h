@@ -168,6 +168,51 @@ int am_march_fill(const float* rays, int64_t n_rays, const float* t_near, const 
                   int32_t grid_res, float step, int32_t cap, const int32_t* packed,
                   int32_t* ray_indices, float* t_starts, float* t_ends, void* stream);
 
+/* Background marching: nerfacc ray_marching with ContractionType.
+ * UN_BOUNDED_SPHERE and cone angle as called by forward_bg_ (models/neus.py:
+ * 141-171, scene_aabb None): per ray [t_near, t_far] are given (near plane,
+ * far plane and stratified jitter applied by the caller); steps of
+ * dt = clamp(t0 * cone_angle, step, 1e10); a sample is kept when its
+ * midpoint's cell of the contracted grid (res^3 over roi6, or NULL: all) is
+ * occupied.  Same two passes and outputs as am_march_count / am_march_fill. */
+int am_march_bg_count(const float* rays, int64_t n_rays, const float* t_near, const float* t_far,
+                      const float* roi6_host, const uint8_t* grid, int32_t grid_res, float step,
+                      float cone_angle, int32_t cap, int32_t* packed, int32_t* total,
+                      void* workspace, size_t workspace_bytes, void* stream);
+int am_march_bg_fill(const float* rays, int64_t n_rays, const float* t_near, const float* t_far,
+                     const float* roi6_host, const uint8_t* grid, int32_t grid_res, float step,
+                     float cone_angle, int32_t cap, const int32_t* packed, int32_t* ray_indices,
+                     float* t_starts, float* t_ends, void* stream);
+/* Early-stop filter of the background samples (nerfacc 0.3.3 ray_marching
+ * with sigma_fn and alpha_thre 0, models/neus.py:141-167): with the samples'
+ * density sigma[n] (packed per ray), keeps each ray's samples while the
+ * transmittance prod_{i<j} (1 - alpha_i), alpha = 1 - exp(-sigma dt), is at
+ * least early_stop_eps (1e-4 in the reference).  The kept samples are a
+ * prefix of each ray's: packed_out[R,2] = (start, kept count), total[1];
+ * am_march_bg_fill with packed_out then writes exactly them.  Workspace:
+ * am_march_workspace_size(n_rays). */
+int am_march_bg_visible(const float* sigma, const float* t_starts, const float* t_ends,
+                        const int32_t* packed, int64_t n_rays, float early_stop_eps,
+                        int32_t* packed_out, int32_t* total, void* workspace,
+                        size_t workspace_bytes, void* stream);
+
+/* Background compositing (nerfacc render_weight_from_density +
+ * accumulate_along_rays, models/neus.py:174-178): weights[n] =
+ * exp(-sum_{j<i} s_j) (1 - exp(-s_i)), s = density (t_end - t_start);
+ * trans[n] (kept for the backward); opacity[R], depth[R] (midpoints),
+ * comp_rgb[R,3].  Backward to density[n] and rgb[n,3] (upstream gradients
+ * nullable = 0). */
+int am_composite_density_fwd(const float* density, const float* t_starts, const float* t_ends,
+                             const float* rgb, const int32_t* packed, int64_t n_rays, int64_t n,
+                             float* weights, float* trans, float* opacity, float* depth,
+                             float* comp_rgb, void* stream);
+int am_composite_density_bwd(const float* density, const float* t_starts, const float* t_ends,
+                             const float* rgb, const int32_t* packed, int64_t n_rays, int64_t n,
+                             const float* weights, const float* trans, const float* g_weights,
+                             const float* g_opacity, const float* g_depth,
+                             const float* g_comp_rgb, float* g_density, float* g_rgb,
+                             void* stream);
+
 /* Sample positions for models/neus.py:222-227: pts = o[idx] + d[idx]*mid,
  * mid = (t0+t1)/2, dists = t1-t0 (each a separate fp32 op, as torch). */
 int am_sample_points(const float* rays, const int32_t* ray_indices, const float* t_starts,

@@ -77,6 +77,11 @@ _SIGS = {
     "am_last_error": (ctypes.c_char_p, []),
     "am_last_workspace_need": (_SZ, []),
     "am_sdf_bwd_check": (ctypes.c_int, []),
+    "am_march_bg_count": (ctypes.c_int, [_P, _I64, _P, _P, _P, _P, _I32, _F, _F, _I32, _P, _P, _P, _SZ, _P]),
+    "am_march_bg_fill": (ctypes.c_int, [_P, _I64, _P, _P, _P, _P, _I32, _F, _F, _I32, _P, _P, _P, _P, _P]),
+    "am_march_bg_visible": (ctypes.c_int, [_P, _P, _P, _P, _I64, _F, _P, _P, _P, _SZ, _P]),
+    "am_composite_density_fwd": (ctypes.c_int, [_P, _P, _P, _P, _P, _I64, _I64, _P, _P, _P, _P, _P, _P]),
+    "am_composite_density_bwd": (ctypes.c_int, [_P, _P, _P, _P, _P, _I64, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "am_version": (ctypes.c_int, []),
     "am_timer_enable": (ctypes.c_int, [_I32]),
     "am_timer_read": (ctypes.c_int, [ctypes.c_char_p, _P, _P]),

@@ -355,6 +355,110 @@ def ray_marching(rays, t_near, t_far, roi, grid_binary, step, jitter=None, cap=0
     return ridx, t0, t1, packed
 
 
+def ray_marching_bg(rays, t_near, t_far, roi, grid_binary, step, cone_angle, cap=0,
+                    sigma_fn=None, early_stop_eps=1e-4):
+    """nerfacc ray_marching for the background (models/neus.py:141-171):
+    unbounded-sphere contraction over roi, cone-angle steps, [t_near, t_far]
+    per ray given.  With sigma_fn(t_starts, t_ends, ray_indices) -> density,
+    the samples behind transmittance < early_stop_eps are dropped (each ray
+    keeps a prefix: am_march_bg_visible, then the fill pass again with the
+    kept counts).  Returns (ray_indices int32, t_starts, t_ends, packed)."""
+    import ctypes
+    lib = nat.load()
+    nat.require_cuda(rays, t_near, t_far)
+    R = rays.shape[0]
+    dev = rays.device
+    roi_h = (ctypes.c_float * 6)(*[float(v) for v in roi])
+    roi_p = ctypes.cast(roi_h, ctypes.c_void_p)
+    res, gptr = 0, None
+    if grid_binary is not None:
+        res = int(grid_binary.shape[-1])
+        gptr = nat.ptr(grid_binary)
+    t_near, t_far = _f32c(t_near), _f32c(t_far)
+    packed = torch.empty(R, 2, device=dev, dtype=torch.int32)
+    total = torch.empty(1, device=dev, dtype=torch.int32)
+    ws = nat.workspace(lib.am_march_workspace_size(R), dev)
+    nat.check(lib.am_march_bg_count(nat.ptr(rays), R, nat.ptr(t_near), nat.ptr(t_far), roi_p, gptr,
+                                    res, float(step), float(cone_angle), int(cap), nat.ptr(packed),
+                                    nat.ptr(total), nat.ptr(ws), ws.numel(), nat.stream()),
+              "am_march_bg_count")
+
+    def fill(pk, n):
+        ridx = torch.empty(n, device=dev, dtype=torch.int32)
+        t0 = torch.empty(n, device=dev, dtype=torch.float32)
+        t1 = torch.empty(n, device=dev, dtype=torch.float32)
+        if n > 0:
+            nat.check(lib.am_march_bg_fill(nat.ptr(rays), R, nat.ptr(t_near), nat.ptr(t_far), roi_p,
+                                           gptr, res, float(step), float(cone_angle), int(cap),
+                                           nat.ptr(pk), nat.ptr(ridx), nat.ptr(t0), nat.ptr(t1),
+                                           nat.stream()), "am_march_bg_fill")
+        return ridx, t0, t1
+
+    N = int(total.item())
+    ridx, t0, t1 = fill(packed, N)
+    if sigma_fn is None or early_stop_eps <= 0.0 or N == 0:
+        return ridx, t0, t1, packed
+    sigma = _f32c(sigma_fn(t0, t1, ridx).reshape(-1))
+    if sigma.shape[0] != N:
+        raise ValueError(f"sigma_fn returned {sigma.shape[0]} values for {N} samples")
+    kept = torch.empty_like(packed)
+    nat.check(lib.am_march_bg_visible(nat.ptr(sigma), nat.ptr(t0), nat.ptr(t1), nat.ptr(packed), R,
+                                      float(early_stop_eps), nat.ptr(kept), nat.ptr(total),
+                                      nat.ptr(ws), ws.numel(), nat.stream()), "am_march_bg_visible")
+    K = int(total.item())
+    if K == N:
+        return ridx, t0, t1, packed
+    ridx, t0, t1 = fill(kept, K)
+    return ridx, t0, t1, kept
+
+
+class DensityCompositeFn(torch.autograd.Function):
+    """nerfacc render_weight_from_density + accumulate_along_rays
+    (models/neus.py:174-178) in one launch each way: weights, opacity,
+    depth (midpoints), comp_rgb; gradients to density and rgb."""
+
+    @staticmethod
+    def forward(ctx, density, rgb, t_starts, t_ends, packed):
+        nat.require_cuda(density, rgb, t_starts, t_ends, packed)
+        lib = nat.load()
+        density, rgb = _f32c(density.reshape(-1)), _f32c(rgb.reshape(-1, 3))
+        n, R = density.shape[0], packed.shape[0]
+        dev = density.device
+        w = torch.empty(n, device=dev, dtype=torch.float32)
+        trans = torch.empty(n, device=dev, dtype=torch.float32)
+        op = torch.empty(R, device=dev, dtype=torch.float32)
+        dep = torch.empty(R, device=dev, dtype=torch.float32)
+        crgb = torch.empty(R, 3, device=dev, dtype=torch.float32)
+        nat.check(lib.am_composite_density_fwd(nat.ptr(density), nat.ptr(t_starts), nat.ptr(t_ends),
+                                                nat.ptr(rgb), nat.ptr(packed), R, n, nat.ptr(w),
+                                                nat.ptr(trans), nat.ptr(op), nat.ptr(dep),
+                                                nat.ptr(crgb), nat.stream()),
+                  "am_composite_density_fwd")
+        ctx.save_for_backward(density, rgb, t_starts, t_ends, packed, w, trans)
+        return w, op, dep, crgb
+
+    @staticmethod
+    def backward(ctx, g_w, g_op, g_dep, g_rgb):
+        density, rgb, t_starts, t_ends, packed, w, trans = ctx.saved_tensors
+        lib = nat.load()
+        n, R = density.shape[0], packed.shape[0]
+        gd = torch.empty_like(density)
+        gr = torch.empty_like(rgb)
+        c = lambda t: _f32c(t) if t is not None else None
+        g_w, g_op, g_dep, g_rgb = c(g_w), c(g_op), c(g_dep), c(g_rgb)
+        nat.check(lib.am_composite_density_bwd(nat.ptr(density), nat.ptr(t_starts), nat.ptr(t_ends),
+                                                nat.ptr(rgb), nat.ptr(packed), R, n, nat.ptr(w),
+                                                nat.ptr(trans), nat.ptr(g_w), nat.ptr(g_op),
+                                                nat.ptr(g_dep), nat.ptr(g_rgb), nat.ptr(gd),
+                                                nat.ptr(gr), nat.stream()),
+                  "am_composite_density_bwd")
+        return gd, gr, None, None, None
+
+
+def composite_density(density, rgb, t_starts, t_ends, packed):
+    return DensityCompositeFn.apply(density, rgb, t_starts, t_ends, packed)
+
+
 def sample_points(rays, ray_indices, t_starts, t_ends):
     """models/neus.py:222-227: positions, midpoints, dists."""
     n = ray_indices.shape[0]

@@ -19,11 +19,20 @@ from .utils import get_activation, scale_anything, update_module_step
 
 
 def contract_to_unisphere(x, radius, contraction_type="aabb"):
-    """models/geometry.py:19-31 (AABB branch; the unbounded-sphere branch is
-    the background model's, not built here)."""
-    if contraction_type not in ("aabb", None):
-        raise NotImplementedError("only AABB contraction is provided")
-    return scale_anything(x, (-radius, radius), (0, 1))
+    """models/geometry.py:19-31: "aabb" maps [-r, r]^3 to [0, 1]^3;
+    "un_bounded_sphere" maps the cube to [-1, 1]^3, points of norm m > 1 to
+    (2 - 1/m) x/m, then [-2, 2]^3 to [0, 1]^3."""
+    if contraction_type in ("aabb", None):
+        return scale_anything(x, (-radius, radius), (0, 1))
+    if contraction_type != "un_bounded_sphere":
+        raise NotImplementedError(f"contraction {contraction_type!r}")
+    x = scale_anything(x, (-radius, radius), (0, 1))
+    x = x * 2 - 1
+    mag = x.norm(dim=-1, keepdim=True)
+    outside = mag > 1
+    m = torch.where(outside, mag, torch.ones_like(mag))   # no 1/0 in the unused branch
+    x = torch.where(outside, (2 - 1 / m) * (x / m), x)
+    return x / 4 + 0.5
 
 
 class BaseImplicitGeometry(nn.Module):
@@ -188,3 +197,60 @@ class VolumeSDF(BaseImplicitGeometry):
     def isosurface(self):
         raise NotImplementedError("triangulation (mcubes) is host-side and out of scope; "
                                   "use sdf_grid() for the SDF lattice")
+
+
+class EncodingWithNetwork(nn.Module):
+    """models/network_utils.py:188-198: network(encoding(x))."""
+
+    def __init__(self, encoding, network):
+        super().__init__()
+        self.encoding, self.network = encoding, network
+
+    def forward(self, x):
+        return self.network(self.encoding(x))
+
+    def update_step(self, epoch, global_step):
+        update_module_step(self.encoding, epoch, global_step)
+        update_module_step(self.network, epoch, global_step)
+
+
+@register("volume-density")
+class VolumeDensity(BaseImplicitGeometry):
+    """models/geometry.py:153-177, the learned background's field: the
+    unbounded-sphere contraction (elementwise torch), the hash-grid encode
+    and the MLP on their HIP kernels, then density = act(out[..., 0] + bias)
+    (trunc_exp) and feature = out."""
+
+    def setup(self):
+        self.n_input_dims = self.config.get("n_input_dims", 3)
+        self.n_output_dims = self.config.feature_dim
+        encoding = get_encoding(self.n_input_dims, self.config.xyz_encoding_config,
+                                seed=self.config.get("seed", None))
+        network = get_mlp(encoding.n_output_dims, self.n_output_dims, self.config.mlp_network_config)
+        self.encoding_with_network = EncodingWithNetwork(encoding, network)
+        self.contraction_type = "un_bounded_sphere"
+
+    def _density(self, out):
+        density = out[..., 0]
+        if "density_activation" in self.config:
+            density = get_activation(self.config.density_activation)(
+                density + float(self.config.density_bias))
+        return density
+
+    def forward(self, points):
+        points = contract_to_unisphere(points, self.radius, self.contraction_type)
+        out = self.encoding_with_network(points.view(-1, self.n_input_dims)).view(
+            *points.shape[:-1], self.n_output_dims).float()
+        density, feature = self._density(out), out
+        if "feature_activation" in self.config:
+            feature = get_activation(self.config.feature_activation)(feature)
+        return density, feature
+
+    def forward_level(self, points):
+        points = contract_to_unisphere(points, self.radius, self.contraction_type)
+        out = self.encoding_with_network(points.reshape(-1, self.n_input_dims)).reshape(
+            *points.shape[:-1], self.n_output_dims)
+        return -self._density(out)
+
+    def update_step(self, epoch, global_step):
+        update_module_step(self.encoding_with_network, epoch, global_step)

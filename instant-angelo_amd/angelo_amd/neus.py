@@ -54,6 +54,21 @@ class VarianceNetwork(nn.Module):
                                    + self.prev_inv_s, self.max_inv_s)
 
 
+def contract_inv_unbounded(x, roi):
+    """Inverse of the unbounded-sphere contraction (nerfacc 0.3.3
+    contract_inv, ContractionType.UN_BOUNDED_SPHERE): y = 4x - 2; for |y| > 1,
+    y <- y / (|y| (2 - |y|)); then y/2 + 1/2 scaled onto roi.  Defined for
+    |x - 1/2| < 1/2 (the contracted ball)."""
+    roi = torch.as_tensor(roi, dtype=x.dtype, device=x.device)
+    y = x * 4.0 - 2.0
+    n = y.norm(dim=-1, keepdim=True)
+    outside = n > 1.0
+    m = torch.where(outside, n, torch.ones_like(n))
+    y = torch.where(outside, y / (m * (2.0 - m)), y)
+    u = y * 0.5 + 0.5
+    return u * (roi[3:] - roi[:3]) + roi[:3]
+
+
 class OccupancyGrid(nn.Module):
     """Occupancy grid of nerfacc 0.3.3 (AABB contraction): occs [R^3] float32
     and a uint8 binary [R,R,R] (x-major), updated every n steps by
@@ -106,13 +121,22 @@ class NeuSModel(nn.Module):
         cfg = self.config
         self.geometry = make(cfg.geometry.name, cfg.geometry)
         self.texture = make(cfg.texture.name, cfg.texture)
-        if cfg.get("learned_background", False):
-            raise NotImplementedError("learned_background (forward_bg_) is not part of this build")
+        self.learned_background = bool(cfg.get("learned_background", False))
+        if self.learned_background:
+            self.geometry_bg = make(cfg.geometry_bg.name, cfg.geometry_bg)
+            self.texture_bg = make(cfg.texture_bg.name, cfg.texture_bg)
+            self.geometry_bg.contraction_type = "un_bounded_sphere"
+            self.near_plane_bg, self.far_plane_bg = 0.1, 1e3
+            self.cone_angle_bg = 10 ** (math.log10(self.far_plane_bg) / cfg.num_samples_per_ray_bg) - 1.0
+            self.render_step_size_bg = 0.01
+            self.max_samples_per_ray_bg = int(cfg.get("max_samples_per_ray_bg", 0))
         self.variance = VarianceNetwork(cfg.variance)
         r = float(cfg.radius)
         self.register_buffer("scene_aabb", torch.as_tensor([-r, -r, -r, r, r, r], dtype=torch.float32))
         if cfg.get("grid_prune", False):
             self.occupancy_grid = OccupancyGrid(self.scene_aabb, resolution=cfg.get("grid_resolution", 128))
+            if self.learned_background:
+                self.occupancy_grid_bg = OccupancyGrid(self.scene_aabb, resolution=256)
         self.randomized = cfg.randomized
         self.background_color = None
         step = cfg.get("render_step_size", None)
@@ -137,12 +161,19 @@ class NeuSModel(nn.Module):
     def update_step(self, epoch, global_step):
         update_module_step(self.geometry, epoch, global_step)
         update_module_step(self.texture, epoch, global_step)
+        if self.learned_background:
+            update_module_step(self.geometry_bg, epoch, global_step)
+            update_module_step(self.texture_bg, epoch, global_step)
         update_module_step(self.variance, epoch, global_step)
         cos_anneal_end = self.config.get("cos_anneal_end", 0)
         self.cos_anneal_ratio = 1.0 if cos_anneal_end == 0 else min(1.0, global_step / cos_anneal_end)
         if self.training and self.config.get("grid_prune", False):
             self.occupancy_grid.every_n_step(step=global_step, update_fn=self._occ_update,
                                              occ_thre=self.config.get("grid_prune_occ_thre", 0.01))
+            if self.learned_background:
+                self.occupancy_grid_bg.every_n_step(
+                    step=global_step, update_fn=self._occ_update_bg,
+                    occ_thre=self.config.get("grid_prune_occ_thre_bg", 0.01))
 
     def inv_s(self):
         return self.variance(torch.zeros([1, 3]))[:, :1].clip(1e-6, 1e6)
@@ -165,6 +196,30 @@ class NeuSModel(nn.Module):
                                     float(ema_decay), float(occ_thre), nat.ptr(og.occs),
                                     nat.ptr(og.binary), nat.ptr(ws), ws.numel(), nat.stream()),
                   "am_occ_update")
+
+    @torch.no_grad()
+    def _occ_update_bg(self, cells, jitter, m, occ_thre, ema_decay, chunk=1 << 22):
+        """occ_eval_fn_bg (models/neus.py:103-106: density * step) on the
+        unbounded-sphere grid, with nerfacc 0.3.3's update rule: jittered
+        cell points inside the contracted ball, mapped back to the world by
+        `contract_inv_unbounded`; occs = max(occs * decay, occ) at the
+        sampled cells; binary = occs > min(mean(occs), occ_thre)."""
+        og = self.occupancy_grid_bg
+        R = og.resolution
+        dev = og.occs.device
+        idx = torch.arange(og.num_cells, device=dev) if cells is None else cells.long()
+        coords = torch.stack([idx // (R * R), (idx // R) % R, idx % R], -1).float()
+        x = (coords + jitter) / R
+        inside = (x - 0.5).norm(dim=1) < 0.5
+        x, idx = x[inside], idx[inside]
+        pts = contract_inv_unbounded(x, og.roi_aabb)
+        occ = torch.empty(idx.shape[0], device=dev)
+        for a in range(0, idx.shape[0], chunk):
+            density, _ = self.geometry_bg(pts[a:a + chunk])
+            occ[a:a + chunk] = density * self.render_step_size_bg
+        og.occs[idx] = torch.maximum(og.occs[idx] * ema_decay, occ)
+        thre = torch.clamp(og.occs.mean(), max=occ_thre)
+        og.binary.copy_((og.occs > thre).view(R, R, R).to(torch.uint8))
 
     def get_alpha(self, sdf, normal, dirs, dists):
         """models/neus.py:117-139 (torch form, for callers outside the fused
@@ -190,10 +245,73 @@ class NeuSModel(nn.Module):
         return fn.ray_marching(rays, t_near, t_far, roi, grid, self.render_step_size, jitter,
                                self.max_samples_per_ray)
 
+    def march_bg(self, rays):
+        """Background sampling of forward_bg_ (models/neus.py:141-171): near =
+        the far AABB hit (0.1 on a miss), far 1e3, stratified jitter of one
+        step, cone-angle marching over the unbounded grid, then the samples
+        behind transmittance 1e-4 dropped (sigma_fn = the background
+        density)."""
+        _, t_max = fn.ray_aabb(rays, self.scene_aabb.tolist())
+        near = torch.where(t_max > 1e9, torch.full_like(t_max, self.near_plane_bg), t_max)
+        near = torch.clamp(near, min=0.0)
+        far = torch.full_like(near, self.far_plane_bg)
+        if self.randomized:
+            near = near + torch.rand_like(near) * self.render_step_size_bg
+        grid = self.occupancy_grid_bg.binary if self.config.get("grid_prune", False) else None
+
+        def sigma_fn(t0, t1, ridx):
+            pts, _, _ = fn.sample_points(rays, ridx, t0, t1)
+            density, _ = self.geometry_bg(pts)
+            return density
+
+        return fn.ray_marching_bg(rays, near, far, self.scene_aabb.tolist(), grid,
+                                  self.render_step_size_bg, self.cone_angle_bg,
+                                  self.max_samples_per_ray_bg, sigma_fn, 1e-4)
+
+    def forward_bg_(self, rays, samples=None):
+        """models/neus.py:141-203.  samples: optional (ray_indices int32,
+        t_starts, t_ends, packed) replacing the sampling."""
+        rays = rays.float().contiguous()
+        n_rays = rays.shape[0]
+        if samples is None:
+            with torch.no_grad():
+                ray_indices, t_starts, t_ends, packed = self.march_bg(rays)
+        else:
+            ray_indices, t_starts, t_ends, packed = samples
+        pts, mid, dists = fn.sample_points(rays, ray_indices, t_starts, t_ends)
+        density, feature = self.geometry_bg(pts)
+        t_dirs = rays[ray_indices.long(), 3:6]
+        rgb = self.texture_bg(feature, t_dirs)
+        weights, opacity, depth, comp_rgb = fn.composite_density(density, rgb, t_starts, t_ends,
+                                                                 packed)
+        opacity = opacity[:, None]
+        depth = depth[:, None]
+        bg = self.background_color
+        if bg is None:
+            bg = torch.ones(3, device=rays.device)
+        comp_rgb = comp_rgb + bg * (1.0 - opacity)
+        out = {
+            "comp_rgb": comp_rgb,
+            "opacity": opacity,
+            "depth": depth,
+            "rays_valid": opacity > 0,
+            "num_samples": torch.as_tensor([ray_indices.shape[0]], dtype=torch.int32,
+                                           device=rays.device),
+        }
+        if self.training:
+            out.update({
+                "weights": weights.view(-1),
+                "points": mid.view(-1),
+                "intervals": dists.view(-1),
+                "ray_indices": ray_indices.long().view(-1),
+            })
+        return out
+
     # -- forward ------------------------------------------------------------
-    def forward_(self, rays, samples=None, randn=None):
-        """models/neus.py:205-283.  samples: optional (ray_indices int32,
-        t_starts, t_ends, packed) replacing marching."""
+    def forward_(self, rays, samples=None, randn=None, samples_bg=None):
+        """models/neus.py:205-283.  samples / samples_bg: optional
+        (ray_indices int32, t_starts, t_ends, packed) replacing the
+        foreground / background sampling."""
         rays = rays.float().contiguous()
         n_rays = rays.shape[0]
         if samples is None:
@@ -232,14 +350,17 @@ class NeuSModel(nn.Module):
                 "ray_indices": ray_indices.long().view(-1),
                 "sdf_laplace_samples": lap.view(-1, 1),
             })
-        bg = self.background_color
-        if bg is None:
-            bg = torch.ones(3, device=rays.device)
-        out_bg = {
-            "comp_rgb": bg[None, :].expand(*comp_rgb.shape),
-            "num_samples": torch.zeros_like(out["num_samples"]),
-            "rays_valid": torch.zeros_like(out["rays_valid"]),
-        }
+        if self.learned_background:
+            out_bg = self.forward_bg_(rays, samples_bg)
+        else:
+            bg = self.background_color
+            if bg is None:
+                bg = torch.ones(3, device=rays.device)
+            out_bg = {
+                "comp_rgb": bg[None, :].expand(*comp_rgb.shape),
+                "num_samples": torch.zeros_like(out["num_samples"]),
+                "rays_valid": torch.zeros_like(out["rays_valid"]),
+            }
         out_full = {
             "comp_rgb": out["comp_rgb"] + out_bg["comp_rgb"] * (1.0 - out["opacity"]),
             "num_samples": out["num_samples"] + out_bg["num_samples"],
@@ -248,13 +369,13 @@ class NeuSModel(nn.Module):
         return {**out, **{k + "_bg": v for k, v in out_bg.items()},
                 **{k + "_full": v for k, v in out_full.items()}}
 
-    def forward(self, rays, samples=None, randn=None):
+    def forward(self, rays, samples=None, randn=None, samples_bg=None):
         """models/neus.py:285-293."""
         if self.training:
-            out = self.forward_(rays, samples, randn)
+            out = self.forward_(rays, samples, randn, samples_bg)
         else:
             out = chunk_batch(lambda r: self.forward_(r), self.config.ray_chunk, False, rays) \
-                if samples is None else self.forward_(rays, samples, randn)
+                if samples is None else self.forward_(rays, samples, randn, samples_bg)
         return {**out, "inv_s": self.variance.inv_s}
 
     def train(self, mode=True):

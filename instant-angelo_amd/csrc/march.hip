@@ -135,6 +135,125 @@ __device__ int march_one(const MarchArgs& a, const float* o, const float* d, flo
     return j;
 }
 
+// ---- background marching (nerfacc 0.3.3 ray_marching with
+// ContractionType.UN_BOUNDED_SPHERE and a cone angle, as called by
+// models/neus.py:141-171): steps of dt = clamp(t0 * cone, step, 1e10), a
+// sample kept when its midpoint's cell of the contracted grid is occupied;
+// contracted scenes take no voxel skip.  [near, far] per ray come from the
+// caller (its near/far planes and stratified jitter). ----
+struct MarchBgArgs {
+    MarchArgs m;
+    float cone;
+};
+
+// Unbounded-sphere contraction of a world point into [0, 1]^3: unit cube of
+// the roi -> [-1, 1]; |x| > 1 maps to (2 - 1/|x|) x/|x|; then x/4 + 1/2.
+__device__ __forceinline__ bool occupied_unbounded(const MarchArgs& a, const float xyz[3],
+                                                   const uint8_t* __restrict__ grid) {
+    if (!a.has_grid) return true;
+    float x[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) x[d] = div_range(a, xyz[d] - a.roi[d], d) * 2.0f - 1.0f;
+    const float mag = sqrtf((x[0] * x[0] + x[1] * x[1]) + x[2] * x[2]);
+    if (mag > 1.0f) {
+        const float k = 2.0f - 1.0f / mag;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) x[d] = k * (x[d] / mag);
+    }
+    int ix[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const float u = x[d] * 0.25f + 0.5f;
+        int v = (int)(u * (float)a.res);
+        v = v < 0 ? 0 : (v > a.res - 1 ? a.res - 1 : v);
+        ix[d] = v;
+    }
+    return grid[((int64_t)ix[0] * a.res + ix[1]) * a.res + ix[2]] != 0;
+}
+
+__device__ __forceinline__ float cone_dt(const MarchBgArgs& b, float t) {
+    return fminf(fmaxf(t * b.cone, b.m.step), 1e10f);
+}
+
+__device__ int march_one_bg(const MarchBgArgs& b, const float* o, const float* d, float near,
+                            float far, const uint8_t* __restrict__ grid, float* ts, float* te,
+                            int32_t* ri, int32_t ray) {
+    int j = 0;
+    float t0 = near;
+    float t1 = t0 + cone_dt(b, t0);
+    float t_mid = (t0 + t1) * 0.5f;
+    while (t_mid < far) {
+        if (b.m.cap > 0 && j >= b.m.cap) break;
+        float xyz[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) xyz[k] = __fmaf_rn(t_mid, d[k], o[k]);
+        if (occupied_unbounded(b.m, xyz, grid)) {
+            if (ts) {
+                ts[j] = t0;
+                te[j] = t1;
+                ri[j] = ray;
+            }
+            ++j;
+        }
+        const float prev = t_mid;
+        t0 = t1;
+        t1 = t0 + cone_dt(b, t0);
+        t_mid = (t0 + t1) * 0.5f;
+        if (!(t_mid > prev)) break;
+    }
+    return j;
+}
+
+__global__ void k_march_bg_count(MarchBgArgs b, const float* __restrict__ rays, int64_t n,
+                                 const float* __restrict__ t_near, const float* __restrict__ t_far,
+                                 const uint8_t* __restrict__ grid, int32_t* __restrict__ counts) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float o[3] = {rays[6 * i], rays[6 * i + 1], rays[6 * i + 2]};
+    const float d[3] = {rays[6 * i + 3], rays[6 * i + 4], rays[6 * i + 5]};
+    counts[i] = march_one_bg(b, o, d, t_near[i], t_far[i], grid, nullptr, nullptr, nullptr,
+                             (int32_t)i);
+}
+
+__global__ void k_march_bg_fill(MarchBgArgs b, const float* __restrict__ rays, int64_t n,
+                                const float* __restrict__ t_near, const float* __restrict__ t_far,
+                                const uint8_t* __restrict__ grid,
+                                const int32_t* __restrict__ packed, int32_t* __restrict__ ridx,
+                                float* __restrict__ ts, float* __restrict__ te) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t base = packed[2 * i];
+    const float o[3] = {rays[6 * i], rays[6 * i + 1], rays[6 * i + 2]};
+    const float d[3] = {rays[6 * i + 3], rays[6 * i + 4], rays[6 * i + 5]};
+    MarchBgArgs c = b;
+    c.m.cap = packed[2 * i + 1];  // never write past the counted samples
+    if (c.m.cap == 0) return;
+    march_one_bg(c, o, d, t_near[i], t_far[i], grid, ts + base, te + base, ridx + base,
+                 (int32_t)i);
+}
+
+// Early-stop visibility of the background samples (nerfacc 0.3.3
+// ray_marching with a sigma_fn, alpha_thre 0, as models/neus.py:157-167
+// calls it): alpha = 1 - exp(-sigma (t1 - t0)), T_j = prod_{i<j} (1 - alpha_i)
+// in sample order, a sample kept while T_j >= eps.  T never grows along a
+// ray, so the kept samples are a prefix of the ray's; one lane per ray walks
+// it sequentially (the product's rounding order is the sample order).
+__global__ void k_visible_prefix(const float* __restrict__ sigma, const float* __restrict__ ts,
+                                 const float* __restrict__ te, const int32_t* __restrict__ packed,
+                                 int64_t n_rays, float eps, int32_t* __restrict__ keep) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n_rays) return;
+    const int32_t b = packed[2 * r], c = packed[2 * r + 1];
+    float T = 1.0f;
+    int32_t k = 0;
+    for (; k < c; ++k) {
+        if (!(T >= eps)) break;
+        const float alpha = 1.0f - expf(-(sigma[b + k] * (te[b + k] - ts[b + k])));
+        T = T * (1.0f - alpha);
+    }
+    keep[r] = k;
+}
+
 __device__ __forceinline__ float ray_near(const float* t_near, const float* jitter, int64_t i,
                                           float step) {
     float nr = t_near[i];
@@ -317,6 +436,111 @@ extern "C" int am_march_fill(const float* rays, int64_t n_rays, const float* t_n
                            to_stream(stream), a, rays, n_rays, t_near, t_far, jitter, grid, packed,
                            ray_indices, t_starts, t_ends);
     }
+    AM_LAUNCH_CHECK();
+    return AM_OK;
+}
+
+extern "C" int am_march_bg_count(const float* rays, int64_t n_rays, const float* t_near,
+                                 const float* t_far, const float* roi6_host, const uint8_t* grid,
+                                 int32_t grid_res, float step, float cone_angle, int32_t cap,
+                                 int32_t* packed, int32_t* total, void* workspace,
+                                 size_t workspace_bytes, void* stream) {
+    if (int rc = check_march(n_rays, rays, t_near, t_far, roi6_host, grid, grid_res, step, cap))
+        return rc;
+    AM_CHECK_ARG(cone_angle >= 0.0f, "cone_angle %g < 0", (double)cone_angle);
+    AM_CHECK_ARG(packed && total && workspace, "NULL output");
+    hipStream_t s = to_stream(stream);
+    if (n_rays == 0) {
+        hipLaunchKernelGGL(k_march_pack, dim3(1), dim3(1), 0, s, (const int32_t*)nullptr,
+                           (const int32_t*)nullptr, (int64_t)0, packed, total);
+        AM_LAUNCH_CHECK();
+        return AM_OK;
+    }
+    WsCarver c(workspace, workspace_bytes);
+    int32_t* counts = c.take<int32_t>((size_t)n_rays + 1);
+    int32_t* starts = c.take<int32_t>((size_t)n_rays + 1);
+    size_t sb = scan_tmp_bytes(n_rays);
+    char* stmp = c.take<char>(sb + 256);
+    if (!c.ok()) return workspace_short(c.used, workspace_bytes);
+    MarchBgArgs b;
+    b.m = march_args(roi6_host, grid, grid_res, step, cap);
+    b.cone = cone_angle;
+    {
+        KernelTimer kt_("march_bg_count", s);
+        hipLaunchKernelGGL(k_march_bg_count, dim3((unsigned)ceil_div(n_rays, 64)), dim3(64), 0, s,
+                           b, rays, n_rays, t_near, t_far, grid, counts);
+    }
+    AM_LAUNCH_CHECK();
+    hipError_t e = rocprim::exclusive_scan(stmp, sb, counts, starts, 0, (size_t)n_rays,
+                                           rocprim::plus<int32_t>(), s, false);
+    if (e != hipSuccess) {
+        set_error("exclusive scan failed: %s", hipGetErrorString(e));
+        return AM_ERR_HIP;
+    }
+    hipLaunchKernelGGL(k_march_pack, dim3((unsigned)ceil_div(n_rays, 256)), dim3(256), 0, s, counts,
+                       starts, n_rays, packed, total);
+    AM_LAUNCH_CHECK();
+    return AM_OK;
+}
+
+extern "C" int am_march_bg_fill(const float* rays, int64_t n_rays, const float* t_near,
+                                const float* t_far, const float* roi6_host, const uint8_t* grid,
+                                int32_t grid_res, float step, float cone_angle, int32_t cap,
+                                const int32_t* packed, int32_t* ray_indices, float* t_starts,
+                                float* t_ends, void* stream) {
+    if (int rc = check_march(n_rays, rays, t_near, t_far, roi6_host, grid, grid_res, step, cap))
+        return rc;
+    AM_CHECK_ARG(cone_angle >= 0.0f, "cone_angle %g < 0", (double)cone_angle);
+    if (n_rays == 0) return AM_OK;
+    AM_CHECK_ARG(packed && ray_indices && t_starts && t_ends, "NULL output");
+    MarchBgArgs b;
+    b.m = march_args(roi6_host, grid, grid_res, step, cap);
+    b.cone = cone_angle;
+    {
+        KernelTimer kt_("march_bg_fill", to_stream(stream));
+        hipLaunchKernelGGL(k_march_bg_fill, dim3((unsigned)ceil_div(n_rays, 64)), dim3(64), 0,
+                           to_stream(stream), b, rays, n_rays, t_near, t_far, grid, packed,
+                           ray_indices, t_starts, t_ends);
+    }
+    AM_LAUNCH_CHECK();
+    return AM_OK;
+}
+
+extern "C" int am_march_bg_visible(const float* sigma, const float* t_starts, const float* t_ends,
+                                   const int32_t* packed, int64_t n_rays, float early_stop_eps,
+                                   int32_t* packed_out, int32_t* total, void* workspace,
+                                   size_t workspace_bytes, void* stream) {
+    AM_CHECK_ARG(n_rays >= 0 && n_rays < (1ll << 31), "n_rays %lld", (long long)n_rays);
+    AM_CHECK_ARG(packed_out && total, "NULL packed_out/total");
+    AM_CHECK_ARG(early_stop_eps >= 0.0f, "early_stop_eps %g < 0", (double)early_stop_eps);
+    hipStream_t s = to_stream(stream);
+    if (n_rays == 0) {
+        hipLaunchKernelGGL(k_march_pack, dim3(1), dim3(1), 0, s, (const int32_t*)nullptr,
+                           (const int32_t*)nullptr, (int64_t)0, packed_out, total);
+        AM_LAUNCH_CHECK();
+        return AM_OK;
+    }
+    AM_CHECK_ARG(sigma && t_starts && t_ends && packed && workspace, "NULL buffer");
+    WsCarver c(workspace, workspace_bytes);
+    int32_t* keep = c.take<int32_t>((size_t)n_rays + 1);
+    int32_t* starts = c.take<int32_t>((size_t)n_rays + 1);
+    size_t sb = scan_tmp_bytes(n_rays);
+    char* stmp = c.take<char>(sb + 256);
+    if (!c.ok()) return workspace_short(c.used, workspace_bytes);
+    {
+        KernelTimer kt_("march_bg_visible", s);
+        hipLaunchKernelGGL(k_visible_prefix, dim3((unsigned)ceil_div(n_rays, 64)), dim3(64), 0, s,
+                           sigma, t_starts, t_ends, packed, n_rays, early_stop_eps, keep);
+    }
+    AM_LAUNCH_CHECK();
+    hipError_t e = rocprim::exclusive_scan(stmp, sb, keep, starts, 0, (size_t)n_rays,
+                                           rocprim::plus<int32_t>(), s, false);
+    if (e != hipSuccess) {
+        set_error("exclusive scan failed: %s", hipGetErrorString(e));
+        return AM_ERR_HIP;
+    }
+    hipLaunchKernelGGL(k_march_pack, dim3((unsigned)ceil_div(n_rays, 256)), dim3(256), 0, s, keep,
+                       starts, n_rays, packed_out, total);
     AM_LAUNCH_CHECK();
     return AM_OK;
 }

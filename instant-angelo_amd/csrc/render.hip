@@ -310,6 +310,125 @@ __global__ void __launch_bounds__(256) k_composite_bwd(
     if (l == 0) g_inv_s_ray[r] = tot;
 }
 
+// ---- background compositing: nerfacc render_weight_from_density +
+// accumulate_along_rays as called by models/neus.py:174-178.  s_i =
+// sigma_i (t1_i - t0_i), T_i = exp(-sum_{j<i} s_j), w_i = T_i (1 - exp(-s_i));
+// opacity = sum w, depth = sum w mid, rgb = sum w c.  One wave per ray, the
+// exclusive sum of s as a segmented scan with a running carry. ----
+__device__ __forceinline__ float wave_incl_sum(float v) {
+    const int l = lane_id();
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const float u = __shfl_up(v, o, 64);
+        if (l >= o) v += u;
+    }
+    return v;
+}
+
+// Suffix (reverse inclusive) sum over lanes: lane l gets sum_{k >= l} v_k.
+__device__ __forceinline__ float wave_suffix_sum(float v) {
+    const int l = lane_id();
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const float u = __shfl_down(v, o, 64);
+        if (l + o < 64) v += u;
+    }
+    return v;
+}
+
+__global__ void __launch_bounds__(256) k_composite_density_fwd(
+    const float* __restrict__ density, const float* __restrict__ t0, const float* __restrict__ t1,
+    const float* __restrict__ rgb, const int32_t* __restrict__ packed, int64_t n_rays,
+    float* __restrict__ weights, float* __restrict__ trans, float* __restrict__ opacity,
+    float* __restrict__ depth, float* __restrict__ comp_rgb) {
+    const int64_t r = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (r >= n_rays) return;
+    const int l = lane_id();
+    const int start = packed[2 * r], count = packed[2 * r + 1];
+    float carry = 0.0f, s_op = 0.0f, s_dep = 0.0f, s_rgb[3] = {0.0f, 0.0f, 0.0f};
+    for (int c0 = 0; c0 < count; c0 += 64) {
+        const bool valid = c0 + l < count;
+        const int64_t j = (int64_t)start + c0 + l;
+        float sd = 0.0f, m = 0.0f, col[3] = {0, 0, 0};
+        if (valid) {
+            const float a = t0[j], b = t1[j];
+            sd = density[j] * (b - a);
+            m = (a + b) * 0.5f;
+            load3(rgb, j, col);
+        }
+        const float incl = wave_incl_sum(sd);
+        float excl = __shfl_up(incl, 1, 64);
+        if (l == 0) excl = 0.0f;
+        const float T = expf(-(carry + excl));
+        const float w = valid ? T * (1.0f - expf(-sd)) : 0.0f;
+        if (valid) {
+            weights[j] = w;
+            trans[j] = T;
+        }
+        s_op += wave_sum(w);
+        s_dep += wave_sum(w * m);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) s_rgb[d] += wave_sum(w * col[d]);
+        carry += __shfl(incl, 63, 64);
+    }
+    if (l == 0) {
+        opacity[r] = s_op;
+        depth[r] = s_dep;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) comp_rgb[3 * r + d] = s_rgb[d];
+    }
+}
+
+// dL/ds_i = g_i T_i exp(-s_i) - sum_{k>i} g_k w_k with g_i the total upstream
+// gradient of w_i; dL/dsigma_i = dL/ds_i (t1 - t0), dL/dc_i = w_i g_rgb.
+__global__ void __launch_bounds__(256) k_composite_density_bwd(
+    const float* __restrict__ density, const float* __restrict__ t0, const float* __restrict__ t1,
+    const float* __restrict__ rgb, const int32_t* __restrict__ packed, int64_t n_rays,
+    const float* __restrict__ weights, const float* __restrict__ trans,
+    const float* __restrict__ g_weights, const float* __restrict__ g_opacity,
+    const float* __restrict__ g_depth, const float* __restrict__ g_comp_rgb,
+    float* __restrict__ g_density, float* __restrict__ g_rgb) {
+    const int64_t r = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (r >= n_rays) return;
+    const int l = lane_id();
+    const int start = packed[2 * r], count = packed[2 * r + 1];
+    const float gop = g_opacity ? g_opacity[r] : 0.0f;
+    const float gdep = g_depth ? g_depth[r] : 0.0f;
+    float gcol[3] = {0, 0, 0};
+    if (g_comp_rgb) load3(g_comp_rgb, r, gcol);
+    float Rcarry = 0.0f;
+    const int nchunks = (count + 63) / 64;
+    for (int ch = nchunks - 1; ch >= 0; --ch) {
+        const int c0 = ch * 64;
+        const bool valid = c0 + l < count;
+        const int64_t j = (int64_t)start + c0 + l;
+        float gw = 0.0f, w = 0.0f, sd = 0.0f, dist = 0.0f, T = 0.0f;
+        if (valid) {
+            float col[3];
+            load3(rgb, j, col);
+            const float a = t0[j], b = t1[j];
+            dist = b - a;
+            sd = density[j] * dist;
+            w = weights[j];
+            T = trans[j];
+            gw = gop + gdep * ((a + b) * 0.5f) + (gcol[0] * col[0] + gcol[1] * col[1] + gcol[2] * col[2]);
+            if (g_weights) gw += g_weights[j];
+        }
+        const float v = gw * w;
+        const float suf = wave_suffix_sum(v);
+        float after = __shfl_down(suf, 1, 64);
+        if (l == 63) after = 0.0f;
+        const float R = Rcarry + after;
+        if (valid) {
+            const float gs = gw * T * expf(-sd) - R;
+            g_density[j] = gs * dist;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) g_rgb[3 * j + d] = w * gcol[d];
+        }
+        Rcarry += __shfl(suf, 0, 64);
+    }
+}
+
 }  // namespace am
 
 using namespace am;
@@ -402,4 +521,46 @@ extern "C" int am_composite_bwd(const am_composite_desc* c, const float* sdf, co
     }
     AM_LAUNCH_CHECK();
     return colsum_f32(per_ray, n_rays, 1, g_inv_s, tmp, s);
+}
+
+extern "C" int am_composite_density_fwd(const float* density, const float* t_starts,
+                                        const float* t_ends, const float* rgb,
+                                        const int32_t* packed, int64_t n_rays, int64_t n,
+                                        float* weights, float* trans, float* opacity,
+                                        float* depth, float* comp_rgb, void* stream) {
+    AM_CHECK_ARG(n_rays >= 0 && n >= 0, "negative size");
+    if (n_rays == 0) return AM_OK;
+    AM_CHECK_ARG(packed && opacity && depth && comp_rgb, "NULL output");
+    AM_CHECK_ARG(n == 0 || (density && t_starts && t_ends && rgb && weights && trans), "NULL input");
+    hipStream_t s = to_stream(stream);
+    {
+        KernelTimer kt_("composite_bg_fwd", s);
+        hipLaunchKernelGGL(k_composite_density_fwd, dim3((unsigned)ceil_div(n_rays, 4)), dim3(256),
+                           0, s, density, t_starts, t_ends, rgb, packed, n_rays, weights, trans,
+                           opacity, depth, comp_rgb);
+    }
+    AM_LAUNCH_CHECK();
+    return AM_OK;
+}
+
+extern "C" int am_composite_density_bwd(const float* density, const float* t_starts,
+                                        const float* t_ends, const float* rgb,
+                                        const int32_t* packed, int64_t n_rays, int64_t n,
+                                        const float* weights, const float* trans,
+                                        const float* g_weights, const float* g_opacity,
+                                        const float* g_depth, const float* g_comp_rgb,
+                                        float* g_density, float* g_rgb, void* stream) {
+    AM_CHECK_ARG(n_rays >= 0 && n >= 0, "negative size");
+    if (n_rays == 0 || n == 0) return AM_OK;
+    AM_CHECK_ARG(density && t_starts && t_ends && rgb && packed && weights && trans &&
+                     g_density && g_rgb, "NULL buffer");
+    hipStream_t s = to_stream(stream);
+    {
+        KernelTimer kt_("composite_bg_bwd", s);
+        hipLaunchKernelGGL(k_composite_density_bwd, dim3((unsigned)ceil_div(n_rays, 4)), dim3(256),
+                           0, s, density, t_starts, t_ends, rgb, packed, n_rays, weights, trans,
+                           g_weights, g_opacity, g_depth, g_comp_rgb, g_density, g_rgb);
+    }
+    AM_LAUNCH_CHECK();
+    return AM_OK;
 }

@@ -44,6 +44,13 @@ def load():
     lib.or_ray_marching_count.argtypes = [P, P, P, P, I64, P, I32, P, ctypes.c_float, I32, P]
     lib.or_ray_marching_fill.restype = None
     lib.or_ray_marching_fill.argtypes = [P, P, P, P, I64, P, I32, P, ctypes.c_float, I32, P, P, P]
+    F32 = ctypes.c_float
+    lib.or_march_bg_count.restype = ctypes.c_int64
+    lib.or_march_bg_count.argtypes = [P, P, P, P, I64, P, I32, P, F32, F32, I32, P]
+    lib.or_march_bg_fill.restype = None
+    lib.or_march_bg_fill.argtypes = [P, P, P, P, I64, P, I32, P, F32, F32, I32, P, P, P]
+    lib.or_visible_prefix.restype = None
+    lib.or_visible_prefix.argtypes = [P, P, P, P, I64, F32, P]
     _lib = lib
     return lib
 

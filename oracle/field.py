@@ -98,3 +98,24 @@ def field_from(encode, mlp):
     def f(x):
         return mlp(torch.cat([x * 2.0 - 1.0, encode(x)], dim=-1))
     return f
+
+
+def contract_unbounded(x, r, mode="f32"):
+    """models/geometry.py:22-28 (ContractionType.UN_BOUNDED_SPHERE):
+    [-r, r]^3 -> [0,1]^3 -> [-1,1]^3; norm m > 1 -> (2 - 1/m) x/m; x/4 + 1/2."""
+    x = contract(x, r, mode)
+    x = x * 2 - 1
+    mag = x.norm(dim=-1, keepdim=True)
+    mask = mag.squeeze(-1) > 1
+    x = x.clone()
+    x[mask] = (2 - 1 / mag[mask]) * (x[mask] / mag[mask])
+    return x / 4 + 0.5
+
+
+def volume_density(points, field, r, density_bias=-1.0, mode="f32"):
+    """VolumeDensity.forward (models/geometry.py:159-167) with
+    density_activation trunc_exp (exp in the forward, models/utils.py:54-69):
+    returns density [N], feature [N, n_out]."""
+    x = contract_unbounded(points, r, mode)
+    out = field(x.view(-1, 3))
+    return torch.exp(out[..., 0] + float(density_bias)), out

@@ -265,3 +265,103 @@ void or_ray_marching_fill(const float* rays_o, const float* rays_d, const float*
         base += c;
     }
 }
+
+/* ---- background marching: nerfacc 0.3.3 ray_marching with
+ * ContractionType.UN_BOUNDED_SPHERE and cone_angle > 0, as called by
+ * models/neus.py:141-171 (library absent from the reference tree; restated
+ * from its published kernel: contracted scenes step t0 -> t1 = t0 +
+ * clamp(t0 * cone, step, 1e10) and keep a sample when the contracted
+ * midpoint's cell is occupied; no voxel skip). */
+static int or_occupied_unbounded(const float xyz[3], const float* roi, int R,
+                                 const uint8_t* grid) {
+    if (!grid) return 1;
+    float x[3];
+    for (int d = 0; d < 3; ++d) x[d] = ((xyz[d] - roi[d]) / (roi[3 + d] - roi[d])) * 2.0f - 1.0f;
+    const float mag = sqrtf((x[0] * x[0] + x[1] * x[1]) + x[2] * x[2]);
+    if (mag > 1.0f) {
+        const float k = 2.0f - 1.0f / mag;
+        for (int d = 0; d < 3; ++d) x[d] = k * (x[d] / mag);
+    }
+    int ix[3];
+    for (int d = 0; d < 3; ++d) {
+        const float u = x[d] * 0.25f + 0.5f;
+        int v = (int)(u * (float)R);
+        if (v < 0) v = 0;
+        if (v > R - 1) v = R - 1;
+        ix[d] = v;
+    }
+    return grid[(int64_t)ix[0] * R * R + (int64_t)ix[1] * R + ix[2]] != 0;
+}
+
+static float or_cone_dt(float t, float cone, float step) {
+    return fminf(fmaxf(t * cone, step), 1e10f);
+}
+
+static int or_march_one_bg(const float* o, const float* d, float near, float far,
+                           const float* roi, int R, const uint8_t* grid, float step, float cone,
+                           int cap, float* t_starts, float* t_ends) {
+    int j = 0;
+    float t0 = near;
+    float t1 = t0 + or_cone_dt(t0, cone, step);
+    float t_mid = (t0 + t1) * 0.5f;
+    while (t_mid < far) {
+        if (cap > 0 && j >= cap) break;
+        float xyz[3];
+        for (int k = 0; k < 3; ++k) xyz[k] = fmaf(t_mid, d[k], o[k]);
+        if (or_occupied_unbounded(xyz, roi, R, grid)) {
+            if (t_starts) { t_starts[j] = t0; t_ends[j] = t1; }
+            ++j;
+        }
+        const float prev = t_mid;
+        t0 = t1;
+        t1 = t0 + or_cone_dt(t0, cone, step);
+        t_mid = (t0 + t1) * 0.5f;
+        if (!(t_mid > prev)) break;
+    }
+    return j;
+}
+
+int64_t or_march_bg_count(const float* rays_o, const float* rays_d, const float* t_near,
+                          const float* t_far, int64_t n, const float* roi, int R,
+                          const uint8_t* grid, float step, float cone, int cap, int32_t* counts) {
+    int64_t total = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        counts[i] = or_march_one_bg(rays_o + 3 * i, rays_d + 3 * i, t_near[i], t_far[i], roi, R,
+                                    grid, step, cone, cap, NULL, NULL);
+        total += counts[i];
+    }
+    return total;
+}
+
+void or_march_bg_fill(const float* rays_o, const float* rays_d, const float* t_near,
+                      const float* t_far, int64_t n, const float* roi, int R, const uint8_t* grid,
+                      float step, float cone, int cap, int32_t* ray_indices, float* t_starts,
+                      float* t_ends) {
+    int64_t base = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        int c = or_march_one_bg(rays_o + 3 * i, rays_d + 3 * i, t_near[i], t_far[i], roi, R, grid,
+                                step, cone, cap, t_starts + base, t_ends + base);
+        for (int k = 0; k < c; ++k) ray_indices[base + k] = (int32_t)i;
+        base += c;
+    }
+}
+
+/* Early-stop visibility of nerfacc 0.3.3 ray_marching with a sigma_fn
+ * (alpha_thre 0, early_stop_eps): alpha = 1 - exp(-sigma * (t1 - t0)),
+ * transmittance T_j = prod_{i<j} (1 - alpha_i) in sample order, a sample kept
+ * while T_j >= eps.  T never grows along a ray, so the kept samples are a
+ * prefix; writes each ray's prefix length. */
+void or_visible_prefix(const float* sigma, const float* t_starts, const float* t_ends,
+                       const int32_t* packed, int64_t n_rays, float eps, int32_t* keep) {
+    for (int64_t r = 0; r < n_rays; ++r) {
+        const int32_t b = packed[2 * r], c = packed[2 * r + 1];
+        float T = 1.0f;
+        int32_t k = 0;
+        for (; k < c; ++k) {
+            if (!(T >= eps)) break;
+            const float alpha = 1.0f - expf(-(sigma[b + k] * (t_ends[b + k] - t_starts[b + k])));
+            T = T * (1.0f - alpha);
+        }
+        keep[r] = k;
+    }
+}

@@ -13,10 +13,17 @@
   losses         systems/neus.py:130-159: 10*MSE(rgb_full[valid]),
                  lambda_eik*mean((|grad|-1)^2), lambda_curv*mean(|laplace|).
 
+  weights_from_density / forward_bg   nerfacc 0.3.3 render_weight_from_density
+                 (T_i = exp(-sum_{j<i} sigma_j dt_j), w_i = T_i (1 - exp(-s_i)))
+                 and the background pass NeuSModel.forward_bg_ (:172-203).
+
 Pins (tests/test_oracle_render.py): closed forms of the transmittance (one
 sample per ray, alpha = 1 stops the ray, alpha = 0 contributes nothing, the
 sum of weights equals 1 - prod(1 - a)); scalar hand cases of get_alpha; the
-per-ray sums against explicit Python loops; gradcheck in float64.
+per-ray sums against explicit Python loops; gradcheck in float64.  For the
+density weights: constant density gives T_i = exp(-sigma i dt) in closed
+form, sum w = 1 - exp(-sum s), a per-ray Python loop of the sequential
+product form, and gradcheck of forward_bg in float64.
 """
 import numpy as np
 import torch
@@ -105,6 +112,43 @@ def neus_forward(rays, ray_indices, t_starts, t_ends, n_rays, field, radiance, i
     return dict(c, sdf=geo["sdf"], grad=geo["grad"], normal=normal, feature=geo["feature"],
                 laplace=geo["laplace"], rgb=rgb, positions=positions, comp_rgb_full=full,
                 rays_valid=c["opacity"] > 0)
+
+
+def weights_from_density(sigma, t_starts, t_ends, ray_indices, n_rays):
+    """nerfacc 0.3.3 render_weight_from_density (models/neus.py:174):
+    s = sigma dt, T_i = exp(-sum_{j<i} s_j) per ray, w_i = T_i (1 - exp(-s_i))."""
+    s = sigma * (t_ends - t_starts)
+    packed = packed_from_indices(ray_indices, n_rays)
+    starts = packed[:, 0].long()[ray_indices.long()]
+    csum = torch.cumsum(s, 0)
+    # exclusive per-ray sum: global inclusive cumsum minus the ray's offset
+    offset = torch.where(starts > 0, csum[(starts - 1).clamp(min=0)], torch.zeros_like(csum))
+    excl = csum - s - offset
+    T = torch.exp(-excl)
+    return T * (1.0 - torch.exp(-s)), T
+
+
+def forward_bg(rays, ray_indices, t_starts, t_ends, n_rays, field, radiance, radius,
+               density_bias=-1.0, background=None, mode="f32"):
+    """NeuSModel.forward_bg_ (models/neus.py:172-203) from given samples.
+    field(x01) -> [M, n_out] (the background encoding + MLP);
+    radiance(feature, dirs) -> rgb."""
+    ridx = ray_indices.long()
+    rays_o, rays_d = rays[:, 0:3], rays[:, 3:6]
+    t_dirs = rays_d[ridx]
+    midpoints = (t_starts + t_ends) / 2.0
+    positions = rays_o[ridx] + t_dirs * midpoints[:, None]
+    density, feature = ofield.volume_density(positions, field, radius, density_bias, mode)
+    rgb = radiance(feature, t_dirs)
+    w, T = weights_from_density(density, t_starts, t_ends, ridx, n_rays)
+    opacity = accumulate(w, ridx, n_rays)
+    depth = accumulate(w, ridx, n_rays, midpoints[:, None])
+    comp_rgb = accumulate(w, ridx, n_rays, rgb)
+    bg = background if background is not None else torch.ones(3, dtype=rays.dtype,
+                                                                 device=rays.device)
+    comp_rgb = comp_rgb + bg[None, :] * (1.0 - opacity)
+    return dict(density=density, feature=feature, rgb=rgb, weights=w, trans=T, opacity=opacity,
+                depth=depth, comp_rgb=comp_rgb, positions=positions)
 
 
 def losses(out, gt_rgb, lambda_rgb=10.0, lambda_eik=0.1, lambda_curv=0.5):

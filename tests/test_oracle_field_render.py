@@ -2,6 +2,7 @@
 import math
 
 import torch
+import torch.nn.functional as F
 
 from oracle import field as ofield
 from oracle import render as orender
@@ -196,3 +197,64 @@ def test_composite_gradcheck():
         return o["opacity"].sum() + (o["comp_rgb"] * 1.3).sum() + o["depth"].sum()
 
     assert torch.autograd.gradcheck(fn, (sdf, nrm, rgb, inv_s), eps=1e-6, atol=1e-5)
+
+
+def test_density_weights_closed_forms():
+    n_rays, per = 5, 7
+    ridx = torch.arange(n_rays).repeat_interleave(per)
+    dt = 0.05
+    t0 = torch.arange(per, dtype=torch.float64).repeat(n_rays) * dt
+    t1 = t0 + dt
+    sig = torch.linspace(0.5, 20.0, n_rays, dtype=torch.float64).repeat_interleave(per)
+    w, T = orender.weights_from_density(sig, t0, t1, ridx, n_rays)
+    j = torch.arange(per, dtype=torch.float64).repeat(n_rays)
+    assert torch.allclose(T, torch.exp(-sig * j * dt), rtol=1e-12)
+    tot = orender.accumulate(w, ridx, n_rays)[:, 0]
+    s_ray = sig.view(n_rays, per)[:, 0] * dt * per
+    assert torch.allclose(tot, 1 - torch.exp(-s_ray), rtol=1e-12)
+    # sequential product form (nerfacc's per-ray loop), ragged rays
+    ridx2 = torch.tensor([0, 0, 0, 2, 2, 3])
+    a = torch.tensor([0.0, 0.1, 0.3, 0.0, 0.2, 1.0], dtype=torch.float64)
+    b = a + torch.tensor([0.1, 0.2, 0.1, 0.2, 0.5, 0.3], dtype=torch.float64)
+    sg = torch.tensor([1.0, 3.0, 0.5, 7.0, 2.0, 4.0], dtype=torch.float64)
+    w2, _ = orender.weights_from_density(sg, a, b, ridx2, 4)
+    ref = []
+    for r in range(4):
+        T_ = 1.0
+        for k in (ridx2 == r).nonzero()[:, 0].tolist():
+            alpha = 1 - math.exp(-float(sg[k]) * float(b[k] - a[k]))
+            ref.append(T_ * alpha)
+            T_ *= 1 - alpha
+    assert torch.allclose(w2, torch.tensor(ref, dtype=torch.float64), rtol=1e-12)
+
+
+def test_contract_unbounded_hand_cases_and_gradcheck():
+    x = torch.tensor([[0.0, 0.0, 0.0], [0.5, 0.0, 0.0], [3.0, 0.0, 0.0], [0.0, -100.0, 0.0]],
+                     dtype=torch.float64)
+    y = ofield.contract_unbounded(x, 1.0, mode="f64")
+    assert torch.allclose(y[0], torch.full((3,), 0.5, dtype=torch.float64))
+    assert torch.allclose(y[1], torch.tensor([0.625, 0.5, 0.5], dtype=torch.float64))
+    # |x| = 3 > 1: (2 - 1/3) / 4 + 1/2
+    assert torch.allclose(y[2, 0], torch.tensor((2 - 1 / 3) / 4 + 0.5, dtype=torch.float64))
+    # x = (0, -100, 0): (2 - 1/100) * (-1) / 4 + 1/2
+    assert torch.allclose(y[3], torch.tensor([0.5, 0.0025, 0.5], dtype=torch.float64))
+
+
+def test_forward_bg_gradcheck():
+    torch.manual_seed(0)
+    n_rays = 3
+    rays = torch.cat([torch.rand(n_rays, 3) - 0.5, F.normalize(torch.randn(n_rays, 3), dim=-1)],
+                     -1).double()
+    ridx = torch.tensor([0, 0, 1, 1, 1, 2])
+    t0 = torch.tensor([0.1, 0.3, 0.2, 0.5, 2.0, 0.4], dtype=torch.float64)
+    t1 = t0 + torch.tensor([0.2, 0.3, 0.3, 1.5, 3.0, 0.1], dtype=torch.float64)
+    W = torch.randn(4, 3, dtype=torch.float64, requires_grad=True)
+
+    def radiance(feature, dirs):
+        return torch.sigmoid(feature[:, 1:4] + dirs)
+
+    def f(Wv):
+        out = orender.forward_bg(rays, ridx, t0, t1, n_rays, lambda x: x @ Wv.t(), radiance, 1.0,
+                                 mode="f64")
+        return out["comp_rgb"], out["depth"], out["opacity"]
+    assert torch.autograd.gradcheck(f, (W,), eps=1e-6, atol=1e-6)
