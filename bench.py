@@ -114,7 +114,8 @@ def cpu_baseline(spec_name, workload, seconds=15.0):
     torch.set_num_threads(cores)
     model = workload.model
     spec = workload.spec
-    b = workload.batches[0]
+    b = workload.batches[0] if workload.uniform else \
+        workload.sampler.sample(spec["rays"], torch.Generator(device=workload.device).manual_seed(1))
     R = 128 if not workload.uniform else 64
     rays = b["rays"][:R].detach().cpu()
     gt = b["rgb"][:R].detach().cpu()

@@ -213,6 +213,22 @@ int am_composite_density_bwd(const float* density, const float* t_starts, const 
                              const float* g_comp_rgb, float* g_density, float* g_rgb,
                              void* stream);
 
+/* Training-ray sampling (systems/neus.py:35-118, train stage; get_rays at
+ * models/ray_utils.py:23-31): for ray k, view v = index[k], pixel (ys[k],
+ * xs[k]): rays[k] = (c2w[v][:, 3], normalize(c2w[v][:3, :3] d)) with d =
+ * directions[y, x] (or directions[v, y, x] when dirs_per_view), the rotation
+ * summed left to right and F.normalize's max(|r|, 1e-12); rgb[k] =
+ * images[v, y, x, 0:3] (C channels per pixel), fg_mask[k] = fg_masks[v, y, x]
+ * (1 when NULL); apply_mask blends rgb * fg + bg_color * (1 - fg) with the
+ * device colour bg_color[3].  c2w is [n_views, 3, 4].  Draws out of range are
+ * clamped and counted into *n_out_of_range (device int32, not cleared). */
+int am_sample_rays(const float* images, const float* fg_masks, const float* c2w,
+                   const float* directions, int32_t n_views, int32_t H, int32_t W, int32_t C,
+                   int32_t dirs_per_view, const float* bg_color, int32_t apply_mask,
+                   const int32_t* index, const int32_t* ys, const int32_t* xs, int64_t n,
+                   float* rays, float* rgb, float* fg_mask, int32_t* n_out_of_range,
+                   void* stream);
+
 /* Sample positions for models/neus.py:222-227: pts = o[idx] + d[idx]*mid,
  * mid = (t0+t1)/2, dists = t1-t0 (each a separate fp32 op, as torch). */
 int am_sample_points(const float* rays, const int32_t* ray_indices, const float* t_starts,

@@ -119,8 +119,8 @@ def uniform_samples(rays, n_samples, radius=1.0):
 
 
 def config0_batch(n_rays=1024, n_views=16, W=64, H=64, fov=40.0, seed=0):
-    """rays from seeded views that hit the unit sphere, targets from
-    the Lambertian radius-0.5 sphere."""
+    """Plain NeuS workload batch: rays from seeded views that hit the unit
+    sphere, targets from the Lambertian radius-0.5 sphere."""
     g = torch.Generator().manual_seed(seed + 1)
     cams = sphere_cameras(n_views, 2.5, seed)
     rays = torch.cat([camera_rays(c, W, H, fov) for c in cams])
@@ -138,7 +138,8 @@ def config1_pool(n_views=100, W=512, H=512, fov=40.0, seed=0):
 
 
 def config1_batch(cams, whf, n_rays=8192, gen=None, device="cpu"):
-    """random pixels of random views; targets by sphere tracing."""
+    """Neuralangelo workload batch: random pixels of random views (host draws);
+    targets by sphere tracing the sphere + torus union."""
     W, H, fov = whf
     v = torch.randint(0, cams.shape[0], (n_rays,), generator=gen)
     x = torch.randint(0, W, (n_rays,), generator=gen)
@@ -151,6 +152,25 @@ def config1_batch(cams, whf, n_rays=8192, gen=None, device="cpu"):
     rays = torch.cat([o, d], -1).to(device).contiguous()
     rgb, _ = shade(rays, sdf_union, albedo_union)
     return rays, rgb
+
+
+def config1_images(cams, whf, device, chunk_views=8):
+    """The Neuralangelo workload's image set rendered once: images
+    [N,H,W,3] (sphere + torus union, white where the rays miss), per-pixel
+    camera directions [H,W,3] (models/ray_utils.py:9-20) and c2w [N,3,4], all
+    on `device`, for on-device training-ray sampling."""
+    W, H, fov = whf
+    f = 0.5 * W / math.tan(0.5 * math.radians(fov))
+    dirs = get_ray_directions(W, H, f, f, W / 2, H / 2)
+    n = cams.shape[0]
+    images = torch.empty(n, H, W, 3, device=device)
+    for a in range(0, n, chunk_views):
+        c = cams[a:a + chunk_views].to(device)
+        o, d = get_rays(dirs.to(device), c)
+        d = torch.nn.functional.normalize(d, p=2, dim=-1)
+        rgb, _ = shade(torch.cat([o, d], -1).contiguous(), sdf_union, albedo_union)
+        images[a:a + c.shape[0]] = rgb.view(c.shape[0], H, W, 3)
+    return images, dirs.to(device), cams[:, :3, :4].to(device).contiguous()
 
 
 def prewarm_occupancy(sdf, res=128, radius=1.0, band_cells=1.5, device="cpu"):

@@ -11,11 +11,14 @@ synthetic data that feeds them.
                 SDF MLP 1x64, radiance 2x64; 8192 rays per step marched
                 through a 128^3 occupancy grid with step 2/1024 (scene
                 diameter 2), at most 512 samples per ray; finite-difference
-                normals, eikonal and curvature losses; sphere + torus scene.
+                normals, eikonal and curvature losses; sphere + torus scene
+                (100 views at 512x512, rendered once; each step samples its
+                rays on the device).
 """
 import torch
 
 from . import make, scenes
+from .sampling import TrainRaySampler
 from .system import DynamicLossScaler, NeuSTrainer
 from .utils import to_config
 
@@ -126,10 +129,13 @@ class Workload:
                 samples = scenes.uniform_samples(rays, spec["samples_per_ray"])
                 self.batches.append(dict(rays=rays, rgb=rgb, samples=samples))
         else:
+            # 100 seeded 512x512 views rendered once and kept on the device;
+            # each step draws its rays there (systems/neus.py:35-118)
             cams, whf = scenes.config1_pool(100, 512, 512, 40.0, seed)
-            for b in range(pool):
-                rays, rgb = scenes.config1_batch(cams, whf, spec["rays"], gen, device)
-                self.batches.append(dict(rays=rays, rgb=rgb, samples=None))
+            images, dirs, c2w = scenes.config1_images(cams, whf, device)
+            self.sampler = TrainRaySampler(images, c2w, dirs, None, device=device,
+                                           background_color=mcfg.background_color)
+            self.sample_gen = torch.Generator(device=device).manual_seed(seed + 17)
             og = self.model.occupancy_grid
             binary = scenes.prewarm_occupancy(scenes.sdf_union, og.resolution, 1.0, band_cells, device)
             og.binary.copy_(binary)
@@ -138,7 +144,11 @@ class Workload:
 
     def step(self, i):
         """One training step; returns the number of samples processed."""
-        b = self.batches[i % len(self.batches)]
+        if self.uniform:
+            b = self.batches[i % len(self.batches)]
+        else:
+            b = dict(self.sampler.sample(self.spec["rays"], self.sample_gen, self.model),
+                     samples=None)
         gs = self.global_step + i
         self.trainer.global_step = gs
         self.model.update_step(0, gs)
