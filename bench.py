@@ -28,7 +28,8 @@ HBM_PEAK_GBS = 8000.0
 FP32_MFMA_PEAK_TFLOPS = 157.3
 TIMED_KERNELS = ["geo_fwd", "geo_bwd", "pair_count", "pair_emit", "radix_sort", "seg_reduce",
                  "rad_fwd", "rad_bwd", "composite_fwd", "composite_bwd", "march_count",
-                 "march_fill", "occ_sdf"]
+                 "march_fill", "occ_sdf", "sample_rays", "hg_fwd", "hg_emit", "march_bg_count",
+                 "march_bg_fill", "march_bg_visible", "composite_bg_fwd", "composite_bg_bwd"]
 
 
 def log(*a):
@@ -242,6 +243,7 @@ def main():
     start = torch.cuda.Event(enable_timing=True)
     stop = torch.cuda.Event(enable_timing=True)
     total_samples = 0
+    bg_before = wl.bg_samples
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     start.record()
@@ -264,6 +266,8 @@ def main():
     nat.timer_enable(False)
     ms_per_step = gpu_ms / args.steps
     samples_per_step = total_samples / args.steps
+    bg_per_step = (wl.bg_samples - bg_before) / args.steps
+    fg_per_step = samples_per_step - bg_per_step   # the byte models count foreground samples
     rays = spec["rays"]
     value = total_samples / (gpu_ms / 1e3)
     log(f"[bench] {ms_per_step:.3f} ms/step, {samples_per_step:.0f} samples/step, wall {wall:.2f}s")
@@ -277,10 +281,14 @@ def main():
     # (launch averages: one launch per step for each of these kernels)
     roof, per_kernel = None, {}
     for k, v in kern.items():
-        nbytes = algorithmic_bytes(k, spec, samples_per_step, rays, pairs_per_step, wl.mixed)
+        if bg_per_step and k in ("radix_sort", "seg_reduce"):
+            continue   # shared with the background encode's backward: no per-step byte model
+        nbytes = algorithmic_bytes(k, spec, fg_per_step, rays, pairs_per_step, wl.mixed)
         if nbytes is None:
             continue
-        ach = nbytes / (v["avg_ms"] / 1e3) / 1e9
+        # the byte models are per step: divide by the kernel's time per step
+        # (its launch time where it runs once a step, as the fused kernels do)
+        ach = nbytes / (v["total_ms"] / args.steps / 1e3) / 1e9
         per_kernel[k] = {"achieved": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4),
                          "bytes": int(nbytes), "avg_ms": round(v["avg_ms"], 4)}
     # measured HBM traffic (profiled run of this command; per launch, scaled
@@ -292,8 +300,9 @@ def main():
             t = prof["kernels"].get(k)
             if t:
                 pk["traffic"] = int((t["fetch_bytes"] + t["write_bytes"]) * ratio)
-    if kern:
-        dom = max(kern, key=lambda k: kern[k]["total_ms"])
+    if per_kernel:
+        # the kernel with the most time per step among those with a byte model
+        dom = max(per_kernel, key=lambda k: kern[k]["total_ms"])
         if dom in per_kernel:
             pk = per_kernel[dom]
             roof = {"kernel": dom, "bound": "hbm", "achieved": pk["achieved"], "peak": HBM_PEAK_GBS,
@@ -322,9 +331,11 @@ def main():
         "dtype": "fp16 table / bf16 MFMA, fp32 accumulate" if wl.mixed else "fp32",
         "data": "synthetic (analytic scene, seeded cameras)",
         "config": {"workload": args.workload, "rays_per_step": rays,
-                   "samples_per_step": round(samples_per_step, 1), "levels": spec["levels"],
+                   "samples_per_step": round(samples_per_step, 1),
+                   "background_samples_per_step": round(bg_per_step, 1), "levels": spec["levels"],
                    "log2_hashmap_size": spec["log2T"], "evals_per_sample": evals_per_sample(spec),
                    "grad_type": spec.get("grad_type", "finite_difference"),
+                   "learned_background": bool(spec.get("learned_background", False)),
                    "optimizer": "AdamW (in the timed step)"},
         "rays_per_s": round(rays * args.steps / (gpu_ms / 1e3), 1),
         "loss_scaling": ({"final_scale": wl.scaler.scale, "skipped_steps": wl.skipped_steps}

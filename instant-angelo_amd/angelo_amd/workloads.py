@@ -27,8 +27,9 @@ NEURALANGELO_SCALE = 1.3195079107728942
 
 def model_config(levels, log2T, base, scale, rays, samples_per_ray, step=None, cap=0,
                  grid_prune=False, geo_hidden=1, feature_dim=16, radius=1.0, start_level=None,
-                 kahan=0, seed=0, precision="fp32", grad_type="finite_difference"):
-    return to_config({
+                 kahan=0, seed=0, precision="fp32", grad_type="finite_difference",
+                 learned_background=False):
+    cfg = to_config({
         "name": "neus", "radius": radius, "num_samples_per_ray": samples_per_ray,
         "train_num_rays": rays, "max_train_num_rays": rays, "dynamic_ray_sampling": False,
         "ray_chunk": 2048, "randomized": True, "grid_prune": grid_prune,
@@ -57,6 +58,32 @@ def model_config(levels, log2T, base, scale, rays, samples_per_ray, step=None, c
                                    "n_hidden_layers": 2},
             "color_activation": "sigmoid"},
     })
+    if learned_background:
+        # the background model of configs/neuralangelo-colmap_sparse.yaml:79-120
+        cfg.learned_background = True
+        cfg.background_color = "random"
+        cfg.num_samples_per_ray_bg = 256
+        cfg.grid_prune_occ_thre_bg = 0.01
+        cfg.geometry_bg = to_config({
+            "name": "volume-density", "radius": radius, "feature_dim": 8,
+            "density_activation": "trunc_exp", "density_bias": -1, "isosurface": None,
+            "seed": seed + 1,
+            "xyz_encoding_config": {
+                "otype": "ProgressiveBandHashGrid", "n_levels": 16, "n_features_per_level": 2,
+                "log2_hashmap_size": 19, "base_resolution": 32,
+                "per_level_scale": NEURALANGELO_SCALE, "include_xyz": True, "start_level": 4,
+                "start_step": 5000, "update_steps": 1000},
+            "mlp_network_config": {"otype": "VanillaMLP", "activation": "ReLU",
+                                   "output_activation": "none", "n_neurons": 64,
+                                   "n_hidden_layers": 1}})
+        cfg.texture_bg = to_config({
+            "name": "volume-dual-color", "input_feature_dim": 8,
+            "dir_encoding_config": {"otype": "SphericalHarmonics", "degree": 4},
+            "mlp_network_config": {"otype": "VanillaMLP", "activation": "ReLU",
+                                   "output_activation": "none", "n_neurons": 64,
+                                   "n_hidden_layers": 2},
+            "color_activation": "sigmoid"})
+    return cfg
 
 
 def system_config():
@@ -74,6 +101,9 @@ def make_optimizer(model, scfg):
     groups = [{"params": list(model.geometry.parameters()), "lr": o.lr},
               {"params": list(model.texture.parameters()), "lr": o.lr},
               {"params": list(model.variance.parameters()), "lr": o.variance_lr}]
+    if getattr(model, "learned_background", False):
+        groups += [{"params": list(model.geometry_bg.parameters()), "lr": o.lr},
+                   {"params": list(model.texture_bg.parameters()), "lr": o.lr}]
     try:
         return torch.optim.AdamW(groups, lr=o.lr, betas=tuple(o.betas), eps=o.eps, fused=True)
     except (RuntimeError, TypeError):
@@ -95,6 +125,11 @@ WORKLOADS["neuralangelo-mixed"] = dict(WORKLOADS["neuralangelo-fp32"], precision
 # one base evaluation with d sdf / d x instead of six central differences,
 # curvature still from six evaluations around the shifted point.
 WORKLOADS["neuralangelo-analytic"] = dict(WORKLOADS["neuralangelo-fp32"], grad_type="analytic")
+# With the learned background of the COLMAP configurations (learned_background
+# true, models/neus.py:141-203): unbounded-sphere cone-angle marching with 256
+# samples per ray, the early-stop filter, VolumeDensity + VolumeDualColor and
+# a 256^3 background occupancy grid.
+WORKLOADS["neuralangelo-bg"] = dict(WORKLOADS["neuralangelo-fp32"], learned_background=True)
 
 
 class Workload:
@@ -109,7 +144,8 @@ class Workload:
         mcfg = model_config(spec["levels"], spec["log2T"], spec["base"], spec["scale"], spec["rays"],
                             spec["samples_per_ray"], spec["step"], spec["cap"], spec["grid_prune"],
                             seed=seed, precision=spec.get("precision", "fp32"),
-                            grad_type=spec.get("grad_type", "finite_difference"))
+                            grad_type=spec.get("grad_type", "finite_difference"),
+                            learned_background=spec.get("learned_background", False))
         torch.manual_seed(seed)
         self.mcfg, self.scfg = mcfg, system_config()
         self.model = make("neus", mcfg).to(device)
@@ -119,6 +155,7 @@ class Workload:
         self.mixed = spec.get("precision", "fp32") == "mixed"
         self.scaler = DynamicLossScaler() if self.mixed else None
         self.skipped_steps = 0
+        self.bg_samples = 0   # background samples of the steps so far (in step()'s totals)
         self.global_step = global_step
         self.batches = []
         gen = torch.Generator().manual_seed(seed + 17)
@@ -162,4 +199,9 @@ class Workload:
         else:
             loss.backward()
             self.optimizer.step()
-        return out["ray_indices"].shape[0]
+        n = out["ray_indices"].shape[0]
+        if "ray_indices_bg" in out:
+            nb = out["ray_indices_bg"].shape[0]
+            self.bg_samples += nb
+            n += nb
+        return n

@@ -191,7 +191,7 @@ static int hg_bwd(const GridArgs& ga, uint32_t n_entries, const float* x, int64_
                                reinterpret_cast<Vals<F>*>(w.vals[0]));
         }
         AM_LAUNCH_CHECK();
-        timer_units("table_pairs", w.m);
+        timer_units("encode_table_pairs", w.m);
         return table_reduce_run(w, F, n_entries, dtable, s);
     }
     return AM_OK;
