@@ -133,6 +133,7 @@ class NeuSModel(nn.Module):
         self.variance = VarianceNetwork(cfg.variance)
         r = float(cfg.radius)
         self.register_buffer("scene_aabb", torch.as_tensor([-r, -r, -r, r, r, r], dtype=torch.float32))
+        self._aabb_host = [-r, -r, -r, r, r, r]   # launch arguments: no device read per step
         if cfg.get("grid_prune", False):
             self.occupancy_grid = OccupancyGrid(self.scene_aabb, resolution=cfg.get("grid_resolution", 128))
             if self.learned_background:
@@ -237,7 +238,7 @@ class NeuSModel(nn.Module):
     # -- sampling ---------------------------------------------------------
     def march(self, rays):
         """nerfacc ray_marching with the scene AABB and the occupancy grid."""
-        aabb = self.scene_aabb.tolist()
+        aabb = self._aabb_host
         t_near, t_far = fn.ray_aabb(rays, aabb)
         jitter = torch.rand_like(t_near) if self.randomized else None
         grid = self.occupancy_grid.binary if self.config.get("grid_prune", False) else None
@@ -251,7 +252,7 @@ class NeuSModel(nn.Module):
         step, cone-angle marching over the unbounded grid, then the samples
         behind transmittance 1e-4 dropped (sigma_fn = the background
         density)."""
-        _, t_max = fn.ray_aabb(rays, self.scene_aabb.tolist())
+        _, t_max = fn.ray_aabb(rays, self._aabb_host)
         near = torch.where(t_max > 1e9, torch.full_like(t_max, self.near_plane_bg), t_max)
         near = torch.clamp(near, min=0.0)
         far = torch.full_like(near, self.far_plane_bg)
@@ -264,7 +265,7 @@ class NeuSModel(nn.Module):
             density, _ = self.geometry_bg(pts)
             return density
 
-        return fn.ray_marching_bg(rays, near, far, self.scene_aabb.tolist(), grid,
+        return fn.ray_marching_bg(rays, near, far, self._aabb_host, grid,
                                   self.render_step_size_bg, self.cone_angle_bg,
                                   self.max_samples_per_ray_bg, sigma_fn, 1e-4)
 
@@ -295,8 +296,8 @@ class NeuSModel(nn.Module):
             "opacity": opacity,
             "depth": depth,
             "rays_valid": opacity > 0,
-            "num_samples": torch.as_tensor([ray_indices.shape[0]], dtype=torch.int32,
-                                           device=rays.device),
+            "num_samples": torch.full((1,), ray_indices.shape[0], dtype=torch.int32,
+                                      device=rays.device),
         }
         if self.training:
             out.update({
@@ -338,7 +339,7 @@ class NeuSModel(nn.Module):
             "opacity": opacity,
             "depth": depth,
             "rays_valid": opacity > 0,
-            "num_samples": torch.as_tensor([n], dtype=torch.int32, device=rays.device),
+            "num_samples": torch.full((1,), n, dtype=torch.int32, device=rays.device),
         }
         if self.training:
             out.update({

@@ -4,7 +4,6 @@ scheduled weights of BaseSystem.C (systems/base.py:28-45), the dynamic ray
 count (:126-128), and a dynamic loss scaler for the reduced-precision form.
 """
 import torch
-import torch.nn.functional as F
 
 from .utils import scheduled_value
 
@@ -45,16 +44,26 @@ class NeuSTrainer:
             return t.double().mean().float()
         return t.mean()
 
+    def _masked_mean(self, t, valid):
+        """mean of t[valid] (the rows of the valid rays) without the host
+        read a boolean index makes: sum(t * valid) / (count * columns); an
+        empty selection gives NaN as the indexed mean does."""
+        m = valid.to(t.dtype).unsqueeze(-1)
+        if getattr(self.model, "precision", "fp32") == "mixed":
+            return ((t * m).double().sum() / (m.double().sum() * t.shape[-1])).float()
+        return (t * m).sum() / (m.sum() * t.shape[-1])
+
     def losses(self, out, batch):
         """systems/neus.py:130-194; returns (loss, dict of terms)."""
         L = self.cfg.loss
         terms = {}
         loss = 0.0
         valid = out["rays_valid_full"][..., 0]
-        terms["rgb_mse"] = self._mean((out["comp_rgb_full"][valid] - batch["rgb"][valid]) ** 2)
+        diff = out["comp_rgb_full"] - batch["rgb"]
+        terms["rgb_mse"] = self._masked_mean(diff ** 2, valid)
         loss = loss + terms["rgb_mse"] * self.C(L.lambda_rgb_mse)
         if self.C(L.get("lambda_rgb_l1", 0.0)) > 0:
-            terms["rgb_l1"] = F.l1_loss(out["comp_rgb_full"][valid], batch["rgb"][valid])
+            terms["rgb_l1"] = self._masked_mean(diff.abs(), valid)
             loss = loss + terms["rgb_l1"] * self.C(L.lambda_rgb_l1)
         terms["eikonal"] = self._mean(
             (torch.linalg.norm(out["sdf_grad_samples"], ord=2, dim=-1) - 1.0) ** 2)
