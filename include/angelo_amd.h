@@ -168,6 +168,22 @@ int am_march_fill(const float* rays, int64_t n_rays, const float* t_near, const 
                   int32_t grid_res, float step, int32_t cap, const int32_t* packed,
                   int32_t* ray_indices, float* t_starts, float* t_ends, void* stream);
 
+/* Staged form of the two passes, for a per-ray sample cap > 0: the count
+ * pass also writes each ray's samples to a cap-strided area of the workspace
+ * (am_march_staged_workspace_size(n_rays, cap) bytes), and the fill pass
+ * copies them to the packed outputs instead of marching again.  Same
+ * arguments and outputs as am_march_count / am_march_fill, bit-identical
+ * samples; the fill call takes the workspace the count call filled. */
+size_t am_march_staged_workspace_size(int64_t n_rays, int32_t cap);
+int am_march_count_staged(const float* rays, int64_t n_rays, const float* t_near,
+                          const float* t_far, const float* jitter, const float* roi6_host,
+                          const uint8_t* grid, int32_t grid_res, float step, int32_t cap,
+                          int32_t* packed, int32_t* total, void* workspace,
+                          size_t workspace_bytes, void* stream);
+int am_march_fill_staged(int64_t n_rays, int32_t cap, const int32_t* packed,
+                         const void* workspace, size_t workspace_bytes, int32_t* ray_indices,
+                         float* t_starts, float* t_ends, void* stream);
+
 /* Background marching: nerfacc ray_marching with ContractionType.
  * UN_BOUNDED_SPHERE and cone angle as called by forward_bg_ (models/neus.py:
  * 141-171, scene_aabb None): per ray [t_near, t_far] are given (near plane,

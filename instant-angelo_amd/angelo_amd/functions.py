@@ -319,6 +319,11 @@ def ray_aabb(rays, aabb):
     return t_near, t_far
 
 
+# Marching with a per-ray cap stages the count pass's samples (one march per
+# ray); off: count and fill both march (the form without a cap).
+MARCH_STAGED = {"on": True}
+
+
 def ray_marching(rays, t_near, t_far, roi, grid_binary, step, jitter=None, cap=0):
     """nerfacc ray_marching (models/neus.py:209-220): count -> scan -> fill.
     Returns (ray_indices int32 [N], t_starts [N], t_ends [N], packed [R,2]);
@@ -337,8 +342,24 @@ def ray_marching(rays, t_near, t_far, roi, grid_binary, step, jitter=None, cap=0
         gptr = nat.ptr(grid_binary)
     packed = torch.empty(R, 2, device=dev, dtype=torch.int32)
     total = torch.empty(1, device=dev, dtype=torch.int32)
-    ws = nat.workspace(lib.am_march_workspace_size(R), dev)
     jit = _f32c(jitter) if jitter is not None else None
+    if cap > 0 and MARCH_STAGED["on"]:
+        # one march per ray: the count pass stages the samples, the fill copies
+        ws = nat.workspace(lib.am_march_staged_workspace_size(R, int(cap)), dev)
+        nat.check(lib.am_march_count_staged(nat.ptr(rays), R, nat.ptr(t_near), nat.ptr(t_far),
+                                            nat.ptr(jit), roi_p, gptr, res, float(step), int(cap),
+                                            nat.ptr(packed), nat.ptr(total), nat.ptr(ws),
+                                            ws.numel(), nat.stream()), "am_march_count_staged")
+        N = int(total.item())
+        ridx = torch.empty(N, device=dev, dtype=torch.int32)
+        t0 = torch.empty(N, device=dev, dtype=torch.float32)
+        t1 = torch.empty(N, device=dev, dtype=torch.float32)
+        if N > 0:
+            nat.check(lib.am_march_fill_staged(R, int(cap), nat.ptr(packed), nat.ptr(ws),
+                                               ws.numel(), nat.ptr(ridx), nat.ptr(t0),
+                                               nat.ptr(t1), nat.stream()), "am_march_fill_staged")
+        return ridx, t0, t1, packed
+    ws = nat.workspace(lib.am_march_workspace_size(R), dev)
     nat.check(lib.am_march_count(nat.ptr(rays), R, nat.ptr(t_near), nat.ptr(t_far), nat.ptr(jit),
                                  roi_p, gptr, res, float(step), int(cap), nat.ptr(packed),
                                  nat.ptr(total), nat.ptr(ws), ws.numel(), nat.stream()),

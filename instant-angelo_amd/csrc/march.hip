@@ -109,7 +109,7 @@ __device__ int march_one(const MarchArgs& a, const float* o, const float* d, flo
             if (ts) {
                 ts[j] = t0;
                 te[j] = t1;
-                ri[j] = ray;
+                if (ri) ri[j] = ray;
             }
             ++j;
             const float prev = t_mid;
@@ -312,6 +312,44 @@ __global__ void k_march_fill(MarchArgs a, const float* __restrict__ rays, int64_
               ridx + base, (int32_t)i);
 }
 
+// Staged marching (a per-ray sample cap is set): the count pass also writes
+// each ray's samples into a cap-strided staging area, and the fill pass
+// copies them to their packed positions instead of marching again.
+__global__ void k_march_count_staged(MarchArgs a, const float* __restrict__ rays, int64_t n,
+                                     const float* __restrict__ t_near,
+                                     const float* __restrict__ t_far,
+                                     const float* __restrict__ jitter,
+                                     const uint8_t* __restrict__ grid, int32_t* __restrict__ counts,
+                                     float* __restrict__ st0, float* __restrict__ st1) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float o[3] = {rays[6 * i], rays[6 * i + 1], rays[6 * i + 2]};
+    const float d[3] = {rays[6 * i + 3], rays[6 * i + 4], rays[6 * i + 5]};
+    const int64_t base = i * (int64_t)a.cap;   // a.cap > 0: march_one writes j < cap
+    counts[i] = march_one(a, o, d, ray_near(t_near, jitter, i, a.step), t_far[i], grid,
+                          st0 + base, st1 + base, nullptr, (int32_t)i);
+}
+
+// One wave per ray: its staged samples to the packed outputs, coalesced.
+__global__ void __launch_bounds__(256) k_march_fill_staged(const int32_t* __restrict__ packed,
+                                                           int64_t n, int32_t cap,
+                                                           const float* __restrict__ st0,
+                                                           const float* __restrict__ st1,
+                                                           int32_t* __restrict__ ridx,
+                                                           float* __restrict__ ts,
+                                                           float* __restrict__ te) {
+    const int64_t r = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (r >= n) return;
+    const int l = threadIdx.x & 63;
+    const int32_t base = packed[2 * r], cnt = packed[2 * r + 1];
+    const int64_t src = r * (int64_t)cap;
+    for (int j = l; j < cnt; j += 64) {
+        ts[base + j] = st0[src + j];
+        te[base + j] = st1[src + j];
+        ridx[base + j] = (int32_t)r;
+    }
+}
+
 static MarchArgs march_args(const float* roi6, const uint8_t* grid, int res, float step, int cap) {
     MarchArgs a;
     for (int d = 0; d < 6; ++d) a.roi[d] = roi6[d];
@@ -435,6 +473,86 @@ extern "C" int am_march_fill(const float* rays, int64_t n_rays, const float* t_n
         hipLaunchKernelGGL(k_march_fill, dim3((unsigned)ceil_div(n_rays, 128)), dim3(128), 0,
                            to_stream(stream), a, rays, n_rays, t_near, t_far, jitter, grid, packed,
                            ray_indices, t_starts, t_ends);
+    }
+    AM_LAUNCH_CHECK();
+    return AM_OK;
+}
+
+extern "C" size_t am_march_staged_workspace_size(int64_t n_rays, int32_t cap) {
+    if (n_rays < 0 || cap <= 0) return 0;
+    WsCarver c(nullptr, 0);
+    c.take<int32_t>((size_t)n_rays + 1);
+    c.take<int32_t>((size_t)n_rays + 1);
+    c.take<char>(scan_tmp_bytes(n_rays > 0 ? n_rays : 1) + 256);
+    c.take<float>((size_t)n_rays * (size_t)cap);
+    c.take<float>((size_t)n_rays * (size_t)cap);
+    return c.used + 256;
+}
+
+extern "C" int am_march_count_staged(const float* rays, int64_t n_rays, const float* t_near,
+                                     const float* t_far, const float* jitter,
+                                     const float* roi6_host, const uint8_t* grid, int32_t grid_res,
+                                     float step, int32_t cap, int32_t* packed, int32_t* total,
+                                     void* workspace, size_t workspace_bytes, void* stream) {
+    if (int rc = check_march(n_rays, rays, t_near, t_far, roi6_host, grid, grid_res, step, cap))
+        return rc;
+    AM_CHECK_ARG(cap > 0, "staged marching needs a per-ray sample cap (got %d)", cap);
+    AM_CHECK_ARG(packed && total && workspace, "NULL packed/total/workspace");
+    hipStream_t s = to_stream(stream);
+    if (n_rays == 0) {
+        hipLaunchKernelGGL(k_march_pack, dim3(1), dim3(1), 0, s, (const int32_t*)nullptr,
+                           (const int32_t*)nullptr, (int64_t)0, packed, total);
+        AM_LAUNCH_CHECK();
+        return AM_OK;
+    }
+    WsCarver c(workspace, workspace_bytes);
+    int32_t* counts = c.take<int32_t>((size_t)n_rays + 1);
+    int32_t* starts = c.take<int32_t>((size_t)n_rays + 1);
+    size_t sb = scan_tmp_bytes(n_rays);
+    char* stmp = c.take<char>(sb + 256);
+    float* st0 = c.take<float>((size_t)n_rays * (size_t)cap);
+    float* st1 = c.take<float>((size_t)n_rays * (size_t)cap);
+    if (!c.ok()) return workspace_short(c.used, workspace_bytes);
+    const MarchArgs a = march_args(roi6_host, grid, grid_res, step, cap);
+    {
+        KernelTimer kt_("march_count", s);
+        hipLaunchKernelGGL(k_march_count_staged, dim3((unsigned)ceil_div(n_rays, 128)), dim3(128),
+                           0, s, a, rays, n_rays, t_near, t_far, jitter, grid, counts, st0, st1);
+    }
+    AM_LAUNCH_CHECK();
+    hipError_t e = rocprim::exclusive_scan(stmp, sb, counts, starts, 0, (size_t)n_rays,
+                                           rocprim::plus<int32_t>(), s, false);
+    if (e != hipSuccess) {
+        set_error("exclusive scan failed: %s", hipGetErrorString(e));
+        return AM_ERR_HIP;
+    }
+    hipLaunchKernelGGL(k_march_pack, dim3((unsigned)ceil_div(n_rays, 256)), dim3(256), 0, s, counts,
+                       starts, n_rays, packed, total);
+    AM_LAUNCH_CHECK();
+    return AM_OK;
+}
+
+extern "C" int am_march_fill_staged(int64_t n_rays, int32_t cap, const int32_t* packed,
+                                    const void* workspace, size_t workspace_bytes,
+                                    int32_t* ray_indices, float* t_starts, float* t_ends,
+                                    void* stream) {
+    AM_CHECK_ARG(n_rays >= 0 && n_rays < (1ll << 31), "n_rays %lld", (long long)n_rays);
+    AM_CHECK_ARG(cap > 0, "staged marching needs a per-ray sample cap (got %d)", cap);
+    if (n_rays == 0) return AM_OK;
+    AM_CHECK_ARG(packed && workspace && ray_indices && t_starts && t_ends, "NULL buffer");
+    // the same carve as am_march_count_staged
+    WsCarver c(const_cast<void*>(workspace), workspace_bytes);
+    c.take<int32_t>((size_t)n_rays + 1);
+    c.take<int32_t>((size_t)n_rays + 1);
+    c.take<char>(scan_tmp_bytes(n_rays) + 256);
+    const float* st0 = c.take<float>((size_t)n_rays * (size_t)cap);
+    const float* st1 = c.take<float>((size_t)n_rays * (size_t)cap);
+    if (!c.ok()) return workspace_short(c.used, workspace_bytes);
+    {
+        KernelTimer kt_("march_fill", to_stream(stream));
+        hipLaunchKernelGGL(k_march_fill_staged, dim3((unsigned)ceil_div(n_rays, 4)), dim3(256), 0,
+                           to_stream(stream), packed, n_rays, cap, st0, st1, ray_indices, t_starts,
+                           t_ends);
     }
     AM_LAUNCH_CHECK();
     return AM_OK;

@@ -39,10 +39,14 @@ def test_ray_aabb_bitexact(dev):
     assert on[-1] == 0.0
 
 
-@pytest.mark.parametrize("use_grid,jit,cap", [(True, False, 0), (True, True, 0), (False, True, 0),
-                                              (True, True, 40)])
-def test_march_bitexact(dev, use_grid, jit, cap):
+@pytest.mark.parametrize("use_grid,jit,cap,staged", [
+    (True, False, 0, True), (True, True, 0, True), (False, True, 0, True), (True, True, 40, True),
+    (True, True, 40, False), (False, False, 7, True), (True, False, 512, True)])
+def test_march_bitexact(dev, use_grid, jit, cap, staged, monkeypatch):
+    """cap > 0 runs the staged form (one march per ray) unless staged is
+    off; every form is bit-exact with the oracle."""
     from angelo_amd import functions as fn, scenes
+    monkeypatch.setitem(fn.MARCH_STAGED, "on", staged)
     rays = _rays(1500, seed=3)
     aabb = [-1.0, -1.0, -1.0, 1.0, 1.0, 1.0]
     step = 2.0 / 256
