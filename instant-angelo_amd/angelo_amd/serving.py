@@ -179,7 +179,18 @@ class FieldServer:
         cd.kahan = int(m.kahan)
         _, opacity, depth, comp_rgb, _ = fn.composite(sdf, normal, rgb, m.inv_s().reshape(1), rays,
                                                       ridx, t0, t1, packed, cd)
-        full = comp_rgb + self.bg[None, :] * (1.0 - opacity[:, None])
+        if getattr(m, "learned_background", False):
+            # the background pass of the eval forward (models/neus.py:141-203,
+            # 263-270) with this server's background colour behind it
+            saved = m.background_color
+            m.background_color = self.bg
+            try:
+                bg_rgb = m.forward_bg_(rays)["comp_rgb"]
+            finally:
+                m.background_color = saved
+        else:
+            bg_rgb = self.bg[None, :]
+        full = comp_rgb + bg_rgb * (1.0 - opacity[:, None])
         return full, depth, opacity
 
     def _render_batch(self, rays):
@@ -192,7 +203,7 @@ class FieldServer:
         nb = self.bucket
         while nb < n:   # power-of-two buckets: few captured graphs
             nb *= 2
-        if not self.use_graphs:
+        if not self.use_graphs or getattr(m, "learned_background", False):
             return self._render_stage(rays, ridx, t0, t1, packed)
         g = self._render_graphs.get(nb)
         if g is None:

@@ -134,6 +134,30 @@ def test_served_render_and_lattice_match_model(dev):
     assert torch.equal(g.result["sdf"].view_as(ref), ref)
 
 
+def test_served_render_with_learned_background(dev):
+    """A model with the learned background: the served render equals the
+    eval forward's comp_rgb_full (foreground over the background pass)."""
+    import angelo_amd
+    from angelo_amd import serving
+    import test_gpu_background as tb
+    cfg = tb._bg_model_config()
+    model = helpers.randomize_(angelo_amd.make("neus", cfg), seed=83)
+    model.update_step(0, 30000)
+    model.eval()
+    model = model.to(dev)
+    reqs = _clone_reqs(_requests(86))[:2]
+    _serve(model, reqs, use_graphs=False)
+    for r in reqs:
+        srv = serving.FieldServer(model, rays_per_step=1500, sdf_chunks_per_step=1,
+                                  sample_bucket=1 << 14, use_graphs=False)
+        srv._start(serving.Request("render", 0.0, **r.params), 0.0)
+        rays = srv.active[0].rays
+        with torch.no_grad():
+            out = model(rays)
+        assert int(out["num_samples_bg"]) > 0
+        torch.testing.assert_close(r.result["rgb"], out["comp_rgb_full"], rtol=1e-5, atol=1e-5)
+
+
 def test_latency_stats_and_poisson_mix():
     from angelo_amd import serving
     reqs = serving.poisson_requests(40, 10.0, seed=1)
