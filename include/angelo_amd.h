@@ -245,6 +245,19 @@ int am_sample_rays(const float* images, const float* fg_masks, const float* c2w,
                    float* rays, float* rgb, float* fg_mask, int32_t* n_out_of_range,
                    void* stream);
 
+/* Spatial (Morton) order key of each point of [-radius, radius]^3: 10 bits
+ * per axis, interleaved, x highest (points outside are clamped to the box).
+ * No counterpart in the reference: the fused SDF field processes its samples
+ * in this order (a stable sort of the keys) so that concurrently running
+ * waves gather nearby table rows; outputs are returned in the caller's order. */
+int am_morton_keys(const float* pts, int64_t n, float radius, int32_t* keys, void* stream);
+/* The same order in one call: perm[n] = the points sorted by key (stable),
+ * inv[n] its inverse (inv[perm[i]] = i).  Workspace:
+ * am_morton_order_workspace_size(n). */
+size_t am_morton_order_workspace_size(int64_t n);
+int am_morton_order(const float* pts, int64_t n, float radius, int32_t* perm, int32_t* inv,
+                    void* workspace, size_t workspace_bytes, void* stream);
+
 /* Sample positions for models/neus.py:222-227: pts = o[idx] + d[idx]*mid,
  * mid = (t0+t1)/2, dists = t1-t0 (each a separate fp32 op, as torch). */
 int am_sample_points(const float* rays, const int32_t* ray_indices, const float* t_starts,

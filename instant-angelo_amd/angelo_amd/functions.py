@@ -133,6 +133,39 @@ def _sdf_store_floats(lib, grid, mdesc, sdesc, n):
     return floats
 
 
+# The fused SDF field runs its samples in spatial (Morton) order when there
+# are at least min_samples of them: the caller's order is restored on the
+# outputs (and the backward's upstream gradients follow the same permutation).
+SPATIAL_ORDER = {"on": True, "min_samples": 1 << 16}
+
+
+def spatial_order(pts, radius):
+    """Stable sort of the samples by their Morton key: (perm, inverse), int32."""
+    lib = nat.load()
+    n = pts.shape[0]
+    perm = torch.empty(n, device=pts.device, dtype=torch.int32)
+    inv = torch.empty_like(perm)
+    ws = nat.workspace(lib.am_morton_order_workspace_size(n), pts.device)
+    nat.check(lib.am_morton_order(nat.ptr(pts), n, float(radius), nat.ptr(perm), nat.ptr(inv),
+                                  nat.ptr(ws), ws.numel(), nat.stream()), "am_morton_order")
+    return perm, inv
+
+
+class Permute(torch.autograd.Function):
+    """Rows of x in the order idx; the backward gathers with the inverse
+    permutation (no scatter-add)."""
+
+    @staticmethod
+    def forward(ctx, x, idx, inv):
+        ctx.save_for_backward(inv)
+        return x.index_select(0, idx)
+
+    @staticmethod
+    def backward(ctx, g):
+        (inv,) = ctx.saved_tensors
+        return g.index_select(0, inv), None, None
+
+
 class SDFFieldFn(torch.autograd.Function):
     """Fused VolumeSDF.forward (models/geometry.py:195-287): outputs sdf [n],
     grad [n,3], normal [n,3] = F.normalize(grad), feature [n, n_out+3],

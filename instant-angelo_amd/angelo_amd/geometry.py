@@ -131,9 +131,16 @@ class VolumeSDF(BaseImplicitGeometry):
         if with_laplace and randn is None:
             randn = torch.randn_like(pts)
         g = self.grid_module
-        return fn.SDFFieldFn.apply(pts, randn, g.table, self.network.flat_params(), g.grid,
-                                   self.network.desc, self._sdf_desc(with_grad, with_laplace),
-                                   g.kernel_table())
+        args = (g.table, self.network.flat_params(), g.grid, self.network.desc,
+                self._sdf_desc(with_grad, with_laplace), g.kernel_table())
+        so = fn.SPATIAL_ORDER
+        if not (so["on"] and pts.shape[0] >= so["min_samples"]):
+            return fn.SDFFieldFn.apply(pts, randn, *args)
+        pts = pts.float().contiguous()
+        perm, inv = fn.spatial_order(pts, self.radius)
+        outs = fn.SDFFieldFn.apply(pts.index_select(0, perm),
+                                   randn.index_select(0, perm) if randn is not None else None, *args)
+        return tuple(fn.Permute.apply(o, inv, perm) if o.shape[0] else o for o in outs)
 
     # -- reference API ------------------------------------------------------
     def forward(self, points, with_grad=True, with_feature=True, with_laplace=False,

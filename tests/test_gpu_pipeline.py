@@ -306,6 +306,62 @@ def test_sdf_bwd_workspace_sized_by_pairs(dev):
         functions.SDF_BWD_WORKSPACE = held
 
 
+def test_morton_order(dev):
+    """am_morton_order: keys = 10-bit quantised [-r, r] coordinates, bits
+    interleaved x-major (checked on the host), perm = their stable sort,
+    inv its inverse."""
+    from angelo_amd import functions
+    g = torch.Generator().manual_seed(5)
+    n = 70001
+    pts = (torch.rand(n, 3, generator=g) * 2.4 - 1.2)   # some points outside the box
+    pts[:5000] = pts[5000:10000]                        # equal keys: stability
+    perm, inv = functions.spatial_order(pts.to(dev), 1.0)
+    q = ((pts.double() + 1.0) * 0.5).clamp(0, 1).mul(1024).floor().clamp(max=1023).long().numpy()
+    key = np.zeros(n, np.int64)
+    for b in range(10):
+        for d in range(3):
+            key |= ((q[:, d] >> b) & 1) << (3 * b + (2 - d))
+    ref = np.argsort(key, kind="stable")
+    assert np.array_equal(perm.cpu().numpy(), ref)
+    assert np.array_equal(inv.cpu().numpy()[ref], np.arange(n))
+
+
+@pytest.mark.parametrize("grad_type", ["finite_difference", "analytic"])
+def test_spatial_order_same_results(dev, monkeypatch, grad_type):
+    """The fused field in Morton order: every per-sample output is bit-identical
+    with the caller's order (each sample is evaluated on its own); the
+    parameter gradients differ only by the order of their fp32 sums; two
+    ordered runs are bit-identical."""
+    from angelo_amd import functions
+    model = _field_model(dev, grad_type=grad_type)
+    geo = model.geometry
+    g = torch.Generator().manual_seed(17)
+    n = 20000
+    pts = ((torch.rand(n, 3, generator=g) * 2 - 1) * 0.9).to(dev)
+    randn = torch.randn(n, 3, generator=g).to(dev)
+    w = [torch.randn(n, generator=g).to(dev), torch.randn(n, 3, generator=g).to(dev),
+         torch.randn(n, generator=g).to(dev)]
+
+    def run(on):
+        monkeypatch.setitem(functions.SPATIAL_ORDER, "on", on)
+        monkeypatch.setitem(functions.SPATIAL_ORDER, "min_samples", 1)
+        model.zero_grad()
+        sdf, grad, normal, feature, lap = geo.fused(pts, True, True, randn)
+        ((sdf * w[0]).sum() + (grad * w[1]).sum() + (lap * w[2]).sum() + feature.sum()).backward()
+        return [sdf, grad, normal, feature, lap], [p.grad.clone() for p in geo.parameters()]
+
+    o0, g0 = run(False)
+    o1, g1 = run(True)
+    o2, g2 = run(True)
+    for a, b in zip(o0, o1):
+        assert torch.equal(a, b)
+    for a, b in zip(g1, g2):
+        assert torch.equal(a, b)
+    for a, b in zip(g0, g1):
+        helpers.assert_nontrivial(a, "gradient")
+        assert helpers.rel_err(b, a) < 1e-5
+
+
 @pytest.mark.parametrize("rad_hidden,dual", [(1, False), (2, False), (2, True)])
 def test_radiance_fused_fwd_bwd(dev, rad_hidden, dual):
     """dual: VolumeDualColor (models/texture.py:38-64), + sigmoid(feature[:, 1:4])."""
