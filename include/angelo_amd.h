@@ -251,12 +251,13 @@ int am_sample_rays(const float* images, const float* fg_masks, const float* c2w,
  * in this order (a stable sort of the keys) so that concurrently running
  * waves gather nearby table rows; outputs are returned in the caller's order. */
 int am_morton_keys(const float* pts, int64_t n, float radius, int32_t* keys, void* stream);
-/* The same order in one call: perm[n] = the points sorted by key (stable),
+/* The order in one call, with `bits` (1..10) bits per axis: perm[n] = the
+ * points sorted by key (stable: points of one cell keep the caller's order),
  * inv[n] its inverse (inv[perm[i]] = i).  Workspace:
- * am_morton_order_workspace_size(n). */
-size_t am_morton_order_workspace_size(int64_t n);
-int am_morton_order(const float* pts, int64_t n, float radius, int32_t* perm, int32_t* inv,
-                    void* workspace, size_t workspace_bytes, void* stream);
+ * am_morton_order_workspace_size(n, bits). */
+size_t am_morton_order_workspace_size(int64_t n, int32_t bits);
+int am_morton_order(const float* pts, int64_t n, float radius, int32_t bits, int32_t* perm,
+                    int32_t* inv, void* workspace, size_t workspace_bytes, void* stream);
 
 /* Sample positions for models/neus.py:222-227: pts = o[idx] + d[idx]*mid,
  * mid = (t0+t1)/2, dists = t1-t0 (each a separate fp32 op, as torch). */

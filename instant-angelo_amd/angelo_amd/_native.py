@@ -82,8 +82,8 @@ _SIGS = {
     "am_march_bg_visible": (ctypes.c_int, [_P, _P, _P, _P, _I64, _F, _P, _P, _P, _SZ, _P]),
     "am_march_staged_workspace_size": (ctypes.c_size_t, [_I64, _I32]),
     "am_morton_keys": (ctypes.c_int, [_P, _I64, _F, _P, _P]),
-    "am_morton_order_workspace_size": (ctypes.c_size_t, [_I64]),
-    "am_morton_order": (ctypes.c_int, [_P, _I64, _F, _P, _P, _P, _SZ, _P]),
+    "am_morton_order_workspace_size": (ctypes.c_size_t, [_I64, _I32]),
+    "am_morton_order": (ctypes.c_int, [_P, _I64, _F, _I32, _P, _P, _P, _SZ, _P]),
     "am_march_count_staged": (ctypes.c_int, [_P, _I64, _P, _P, _P, _P, _P, _I32, _F, _I32, _P, _P, _P, _SZ, _P]),
     "am_march_fill_staged": (ctypes.c_int, [_I64, _I32, _P, _P, _SZ, _P, _P, _P, _P]),
     "am_sample_rays": (ctypes.c_int, [_P, _P, _P, _P, _I32, _I32, _I32, _I32, _I32, _P, _I32, _P, _P, _P, _I64, _P, _P, _P, _P, _P]),

@@ -136,18 +136,25 @@ def _sdf_store_floats(lib, grid, mdesc, sdesc, n):
 # The fused SDF field runs its samples in spatial (Morton) order when there
 # are at least min_samples of them: the caller's order is restored on the
 # outputs (and the backward's upstream gradients follow the same permutation).
-SPATIAL_ORDER = {"on": True, "min_samples": 1 << 16}
+# Cells of 2^-bits of the box per axis; within a cell the caller's (ray)
+# order is kept, so neighbouring samples of a ray still merge their table-
+# gradient pairs.  5 bits measured best on the Neuralangelo step (10 bits:
+# the same encode gain, 5 % more pairs).
+SPATIAL_ORDER = {"on": True, "min_samples": 1 << 16, "bits": 5}
 
 
-def spatial_order(pts, radius):
-    """Stable sort of the samples by their Morton key: (perm, inverse), int32."""
+def spatial_order(pts, radius, bits=None):
+    """Stable sort of the samples by their Morton key (`bits` per axis):
+    (perm, inverse), int32."""
     lib = nat.load()
     n = pts.shape[0]
+    bits = int(SPATIAL_ORDER["bits"] if bits is None else bits)
     perm = torch.empty(n, device=pts.device, dtype=torch.int32)
     inv = torch.empty_like(perm)
-    ws = nat.workspace(lib.am_morton_order_workspace_size(n), pts.device)
-    nat.check(lib.am_morton_order(nat.ptr(pts), n, float(radius), nat.ptr(perm), nat.ptr(inv),
-                                  nat.ptr(ws), ws.numel(), nat.stream()), "am_morton_order")
+    ws = nat.workspace(lib.am_morton_order_workspace_size(n, bits), pts.device)
+    nat.check(lib.am_morton_order(nat.ptr(pts), n, float(radius), bits, nat.ptr(perm),
+                                  nat.ptr(inv), nat.ptr(ws), ws.numel(), nat.stream()),
+              "am_morton_order")
     return perm, inv
 
 

@@ -65,7 +65,7 @@ __global__ void k_sample_rays(RaySampleArgs a, const int32_t* __restrict__ index
 }
 
 // Morton (Z-order) key of a sample: its position in [-r, r]^3 quantised to
-// 10 bits per axis and the bits interleaved (x highest).  Sorting samples by
+// `bits` (<= 10) bits per axis and the bits interleaved (x highest).  Sorting samples by
 // it makes the waves that run together gather nearby table rows.
 __device__ __forceinline__ uint32_t spread10(uint32_t v) {
     v &= 0x3FFu;
@@ -77,16 +77,17 @@ __device__ __forceinline__ uint32_t spread10(uint32_t v) {
 }
 
 __global__ void k_morton_keys(const float* __restrict__ pts, int64_t n, float r, float inv2r,
-                              int32_t* __restrict__ keys) {
+                              int bits, int32_t* __restrict__ keys) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    const uint32_t cells = 1u << bits;
     uint32_t q[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
         float u = (pts[3 * i + d] + r) * inv2r;
         u = fminf(fmaxf(u, 0.0f), 1.0f);   // NaN -> 0
-        const uint32_t v = (uint32_t)(u * 1024.0f);
-        q[d] = v > 1023u ? 1023u : v;
+        const uint32_t v = (uint32_t)(u * (float)cells);
+        q[d] = v > cells - 1 ? cells - 1 : v;
     }
     keys[i] = (int32_t)((spread10(q[0]) << 2) | (spread10(q[1]) << 1) | spread10(q[2]));
 }
@@ -101,11 +102,11 @@ __global__ void k_invert(const int32_t* __restrict__ perm, int64_t n, int32_t* _
     if (i < n) inv[perm[i]] = (int32_t)i;
 }
 
-static size_t order_sort_bytes(int64_t n) {
+static size_t order_sort_bytes(int64_t n, int bits) {
     size_t bytes = 0;
     rocprim::radix_sort_pairs(nullptr, bytes, (const int32_t*)nullptr, (int32_t*)nullptr,
-                              (const int32_t*)nullptr, (int32_t*)nullptr, (size_t)n, 0u, 30u,
-                              nullptr, false);
+                              (const int32_t*)nullptr, (int32_t*)nullptr, (size_t)n, 0u,
+                              (unsigned)(3 * bits), nullptr, false);
     return bytes;
 }
 
@@ -113,19 +114,20 @@ static size_t order_sort_bytes(int64_t n) {
 
 using namespace am;
 
-extern "C" size_t am_morton_order_workspace_size(int64_t n) {
-    if (n < 0) return 0;
+extern "C" size_t am_morton_order_workspace_size(int64_t n, int32_t bits) {
+    if (n < 0 || bits < 1 || bits > 10) return 0;
     WsCarver c(nullptr, 0);
     for (int b = 0; b < 3; ++b) c.take<int32_t>((size_t)n + 1);
-    c.take<char>(order_sort_bytes(n > 0 ? n : 1) + 256);
+    c.take<char>(order_sort_bytes(n > 0 ? n : 1, bits) + 256);
     return c.used + 256;
 }
 
-extern "C" int am_morton_order(const float* pts, int64_t n, float radius, int32_t* perm,
-                               int32_t* inv, void* workspace, size_t workspace_bytes,
-                               void* stream) {
+extern "C" int am_morton_order(const float* pts, int64_t n, float radius, int32_t bits,
+                               int32_t* perm, int32_t* inv, void* workspace,
+                               size_t workspace_bytes, void* stream) {
     AM_CHECK_ARG(n >= 0 && n < (1ll << 31), "n %lld", (long long)n);
     AM_CHECK_ARG(radius > 0.0f, "radius %g <= 0", (double)radius);
+    AM_CHECK_ARG(bits >= 1 && bits <= 10, "bits %d out of [1, 10]", bits);
     if (n == 0) return AM_OK;
     AM_CHECK_ARG(pts && perm && inv && workspace, "NULL buffer");
     hipStream_t s = to_stream(stream);
@@ -133,18 +135,18 @@ extern "C" int am_morton_order(const float* pts, int64_t n, float radius, int32_
     int32_t* keys = c.take<int32_t>((size_t)n + 1);
     int32_t* keys2 = c.take<int32_t>((size_t)n + 1);
     int32_t* vals = c.take<int32_t>((size_t)n + 1);
-    size_t sb = order_sort_bytes(n);
+    size_t sb = order_sort_bytes(n, bits);
     char* tmp = c.take<char>(sb + 256);
     if (!c.ok()) return workspace_short(c.used, workspace_bytes);
     const unsigned nb = (unsigned)ceil_div(n, 256);
     hipLaunchKernelGGL(k_morton_keys, dim3(nb), dim3(256), 0, s, pts, n, radius, 0.5f / radius,
-                       keys);
+                       (int)bits, keys);
     hipLaunchKernelGGL(k_iota, dim3(nb), dim3(256), 0, s, n, vals);
     AM_LAUNCH_CHECK();
     // a radix sort is stable: equal keys keep the caller's order
     hipError_t e = rocprim::radix_sort_pairs(tmp, sb, (const int32_t*)keys, keys2,
-                                             (const int32_t*)vals, perm, (size_t)n, 0u, 30u, s,
-                                             false);
+                                             (const int32_t*)vals, perm, (size_t)n, 0u,
+                                             (unsigned)(3 * bits), s, false);
     if (e != hipSuccess) {
         set_error("radix sort failed: %s", hipGetErrorString(e));
         return AM_ERR_HIP;
@@ -161,7 +163,7 @@ extern "C" int am_morton_keys(const float* pts, int64_t n, float radius, int32_t
     if (n == 0) return AM_OK;
     AM_CHECK_ARG(pts && keys, "NULL buffer");
     hipLaunchKernelGGL(k_morton_keys, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0,
-                       to_stream(stream), pts, n, radius, 0.5f / radius, keys);
+                       to_stream(stream), pts, n, radius, 0.5f / radius, 10, keys);
     AM_LAUNCH_CHECK();
     return AM_OK;
 }

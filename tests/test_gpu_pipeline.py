@@ -315,15 +315,18 @@ def test_morton_order(dev):
     n = 70001
     pts = (torch.rand(n, 3, generator=g) * 2.4 - 1.2)   # some points outside the box
     pts[:5000] = pts[5000:10000]                        # equal keys: stability
-    perm, inv = functions.spatial_order(pts.to(dev), 1.0)
-    q = ((pts.double() + 1.0) * 0.5).clamp(0, 1).mul(1024).floor().clamp(max=1023).long().numpy()
-    key = np.zeros(n, np.int64)
-    for b in range(10):
-        for d in range(3):
-            key |= ((q[:, d] >> b) & 1) << (3 * b + (2 - d))
-    ref = np.argsort(key, kind="stable")
-    assert np.array_equal(perm.cpu().numpy(), ref)
-    assert np.array_equal(inv.cpu().numpy()[ref], np.arange(n))
+    for bits in (10, 5):
+        perm, inv = functions.spatial_order(pts.to(dev), 1.0, bits)
+        cells = 1 << bits
+        q = ((pts.double() + 1.0) * 0.5).clamp(0, 1).mul(cells).floor().clamp(max=cells - 1)
+        q = q.long().numpy()
+        key = np.zeros(n, np.int64)
+        for b in range(bits):
+            for d in range(3):
+                key |= ((q[:, d] >> b) & 1) << (3 * b + (2 - d))
+        ref = np.argsort(key, kind="stable")
+        assert np.array_equal(perm.cpu().numpy(), ref)
+        assert np.array_equal(inv.cpu().numpy()[ref], np.arange(n))
 
 
 @pytest.mark.parametrize("grad_type", ["finite_difference", "analytic"])
