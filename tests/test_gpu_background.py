@@ -294,3 +294,24 @@ def test_neus_steps_with_background(dev):
     assert og.binary.shape == (256, 256, 256)
     assert 0 < int(og.binary.sum()) <= og.num_cells
     assert bool(torch.isfinite(og.occs).all())
+
+
+def test_bg_empty_cases(dev):
+    """No background sample at all (far plane before the near plane): empty
+    marching outputs, zero opacity / depth / colour, zero-size gradients."""
+    from angelo_amd import functions as fn
+    R = 7
+    rays = _rays(R, seed=8).to(dev)
+    near = torch.full((R,), 5.0, device=dev)
+    far = torch.full((R,), 1.0, device=dev)
+    ridx, t0, t1, packed = fn.ray_marching_bg(rays, near, far, ROI, None, 0.01, _cone(64), 0,
+                                             lambda a, b, c: torch.ones_like(a), 1e-4)
+    assert ridx.shape[0] == 0 and int(packed[:, 1].sum()) == 0
+    density = torch.zeros(0, device=dev, requires_grad=True)
+    rgb = torch.zeros(0, 3, device=dev, requires_grad=True)
+    w, op, dep, crgb = fn.composite_density(density, rgb, t0, t1, packed)
+    assert w.shape[0] == 0 and torch.equal(op, torch.zeros(R, device=dev))
+    assert torch.equal(dep, torch.zeros(R, device=dev))
+    assert torch.equal(crgb, torch.zeros(R, 3, device=dev))
+    (op.sum() + crgb.sum()).backward()
+    assert density.grad.shape == (0,) and rgb.grad.shape == (0, 3)
