@@ -306,6 +306,28 @@ def test_sdf_bwd_workspace_sized_by_pairs(dev):
         functions.SDF_BWD_WORKSPACE = held
 
 
+@pytest.mark.parametrize("radius,res", [(1.0, 24), (0.5, 40), (1.0, 48)])
+def test_sdf_grid_matches_oracle(dev, radius, res):
+    """The marching-cubes lattice (models/geometry.py:36-113): torch.arange
+    axes over [-r, r) with intv = 2/resolution whatever the radius (so radius
+    0.5 gives res/2 points per axis), SDF = forward_level at every point (in
+    64^3 chunks here, 257^3 blocks in the reference: values are per point);
+    res 48 puts more than 2^16 points through the spatial order."""
+    model = _field_model(dev, radius=radius)
+    geo = model.geometry
+    grid = geo.sdf_grid(res)
+    axis = torch.arange(-radius, radius, 2.0 / res)
+    assert grid.shape == (len(axis),) * 3
+    assert len(axis) == int(round(res * radius))
+    X, Y, Z = torch.meshgrid(axis, axis, axis, indexing="ij")
+    pts = torch.stack([X, Y, Z], -1).reshape(-1, 3)
+    field, _, _ = helpers.oracle_fns(model, torch.float64, "f64")
+    with torch.no_grad():
+        ref = ofield.volume_sdf(pts.double(), field, radius, 1e-3, with_grad=False,
+                                mode="f64")["sdf"]
+    assert helpers.rel_err(grid.reshape(-1), ref) < 1e-5
+
+
 def test_morton_order(dev):
     """am_morton_order: keys = 10-bit quantised [-r, r] coordinates, bits
     interleaved x-major (checked on the host), perm = their stable sort,
