@@ -315,3 +315,80 @@ def test_bg_empty_cases(dev):
     assert torch.equal(crgb, torch.zeros(R, 3, device=dev))
     (op.sum() + crgb.sum()).backward()
     assert density.grad.shape == (0,) and rgb.grad.shape == (0, 3)
+
+
+def test_neus_step_with_background_matches_oracle(dev):
+    """NeuSModel.forward with learned_background on given foreground and
+    background samples: comp_rgb_full = fg + bg (1 - opacity_fg)
+    (models/neus.py:263-283), the colour loss and the gradients of the
+    foreground table and of every background parameter against the FP64
+    oracle (foreground + background passes composed as the reference does)."""
+    import angelo_amd
+    from angelo_amd import scenes
+    cfg = _bg_model_config()
+    model = helpers.randomize_(angelo_amd.make("neus", cfg), seed=3)
+    g = torch.Generator().manual_seed(12)
+    with torch.no_grad():
+        gm = model.geometry_bg.encoding_with_network.encoding.encoding.encoding
+        gm.params.copy_((torch.rand(gm.params.shape, generator=g) * 2 - 1) * 0.5)
+        for net in (model.geometry_bg.encoding_with_network.network, model.texture_bg.network):
+            for lin in net.linears():
+                lin.weight.copy_(torch.randn(lin.weight.shape, generator=g) / lin.weight.shape[1] ** 0.5)
+                lin.bias.copy_(torch.randn(lin.bias.shape, generator=g) * 0.1)
+    model.train()
+    model.update_step(0, 30000)
+    model = model.to(dev)
+    bgc = torch.tensor([0.3, 0.6, 0.1])
+    model.background_color = bgc.to(dev)
+    n_rays = 48
+    rays, gt = scenes.config0_batch(n_rays=n_rays, n_views=4, seed=5)
+    ridx, t0, t1, packed = scenes.uniform_samples(rays, 32)
+    near = torch.full((n_rays,), 1.5)
+    far = torch.full((n_rays,), 4.0)
+    b_r, b_0, b_1, b_c = omarch.march_bg(rays.numpy(), near.numpy(), far.numpy(), ROI, None, 0.01,
+                                        _cone(32))
+    bcnt = torch.from_numpy(b_c).long()
+    bpacked = torch.stack([torch.cumsum(bcnt, 0) - bcnt, bcnt], -1).to(torch.int32)
+    bridx, bt0, bt1 = torch.from_numpy(b_r), torch.from_numpy(b_0), torch.from_numpy(b_1)
+    randn = torch.randn(ridx.shape[0], 3, generator=g)
+
+    # oracle, FP64
+    field, radiance, ol = helpers.oracle_fns(model, torch.float64, "f64")
+    bfield, bradiance, bol, _, _ = _bg_oracle(model)
+    inv_s = torch.exp(model.variance.variance.detach().cpu().double() * 10.0).clip(1e-6, 1e6)
+    eps = float(model.geometry._finite_difference_eps)
+    o = orender.neus_forward(rays.double(), ridx, t0.double(), t1.double(), n_rays, field,
+                             radiance, inv_s, 1.0, eps, model.cos_anneal_ratio, randn.double())
+    ob = orender.forward_bg(rays.double(), bridx, bt0.double(), bt1.double(), n_rays, bfield,
+                            bradiance, 1.0, -1.0, bgc.double(), mode="f64")
+    o_full = o["comp_rgb"] + ob["comp_rgb"] * (1.0 - o["opacity"])
+    o_loss = ((o_full - gt.double()) ** 2).mean()
+    o_loss.backward()
+
+    # HIP path
+    leaves = {}
+    for name, net in (("bgeo", model.geometry_bg.encoding_with_network.network),
+                      ("brad", model.texture_bg.network)):
+        leaf = net.flat_params().detach().clone().requires_grad_(True)
+        net.flat_params = (lambda l=leaf: l)
+        leaves[name] = leaf
+    out = model(rays.to(dev), samples=(ridx.to(dev), t0.to(dev), t1.to(dev), packed.to(dev)),
+                randn=randn.to(dev),
+                samples_bg=(bridx.to(dev), bt0.to(dev), bt1.to(dev), bpacked.to(dev)))
+    loss = ((out["comp_rgb_full"] - gt.to(dev)) ** 2).mean()
+    loss.backward()
+    assert helpers.rel_err(out["comp_rgb_full"], o_full) < 1e-4
+    assert helpers.rel_err(out["comp_rgb_bg"], ob["comp_rgb"]) < 1e-4
+    assert abs(loss.item() - o_loss.item()) / o_loss.item() < 1e-4
+    assert int(out["num_samples_bg"]) == bridx.shape[0]
+    assert int(out["num_samples_full"]) == bridx.shape[0] + ridx.shape[0]
+    assert set(["comp_rgb_bg", "opacity_bg", "depth_bg", "weights_bg", "points_bg",
+                "intervals_bg", "ray_indices_bg", "rays_valid_full"]) <= set(out.keys())
+    gm = model.geometry_bg.encoding_with_network.encoding.encoding.encoding
+    helpers.assert_nontrivial(bol["table"].grad, "oracle bg table gradient")
+    assert helpers.rel_err(gm.params.grad.view(-1, 2), bol["table"].grad) < 1e-3
+    assert helpers.rel_err(leaves["bgeo"].grad, bol["geo"].grad) < 1e-3
+    assert helpers.rel_err(leaves["brad"].grad, bol["rad"].grad) < 1e-3
+    helpers.assert_nontrivial(ol["table"].grad, "oracle fg table gradient")
+    assert helpers.rel_err(model.geometry.grid_module.params.grad.view(-1, 2),
+                           ol["table"].grad) < 2e-3
