@@ -332,6 +332,25 @@ int am_sdf_bwd(const am_grid_desc* g, const am_mlp_desc* m, const am_sdf_desc* s
                const float* g_sdf, const float* g_grad, const float* g_normal,
                const float* g_feature, const float* g_laplace, float* dtable, float* dparams,
                void* workspace, size_t workspace_bytes, void* stream);
+/* The same two calls evaluating the samples in the order `order` (a
+ * permutation of [0, n): kernel sample i is the caller's sample order[i],
+ * for instance from am_morton_order).  Positions, directions, outputs and
+ * upstream gradients stay in the caller's order; fd_values and enc_store
+ * are in the evaluation order, so the backward takes the forward's order.
+ * No counterpart in the reference (it evaluates in ray order); per-sample
+ * results are identical, only the gradients' fp32 summation order differs. */
+int am_sdf_fwd_ordered(const am_grid_desc* g, const am_mlp_desc* m, const am_sdf_desc* s,
+                       const int32_t* order, const float* pts, const float* randn, int64_t n,
+                       const void* table, const float* params, float* sdf, float* grad,
+                       float* normal, float* feature, float* laplace, float* fd_values,
+                       float* enc_store, void* stream);
+int am_sdf_bwd_ordered(const am_grid_desc* g, const am_mlp_desc* m, const am_sdf_desc* s,
+                       const int32_t* order, const float* pts, const float* randn, int64_t n,
+                       const void* table, const float* params, const float* fd_values,
+                       const float* enc_store, const float* g_sdf, const float* g_grad,
+                       const float* g_normal, const float* g_feature, const float* g_laplace,
+                       float* dtable, float* dparams, void* workspace, size_t workspace_bytes,
+                       void* stream);
 
 /* ------------------------------------------------------------------------ */
 /* Fused VolumeRadiance.forward (models/texture.py:23-30): SH(deg) of        */

@@ -111,6 +111,8 @@ _SIGS = {
     "am_sdf_store_floats": (_I64, [_P, _P, _P, _I64]),
     "am_sdf_bwd_workspace_size": (_SZ, [_P, _P, _P, _I64]),
     "am_sdf_bwd": (ctypes.c_int, [_P] * 5 + [_I64] + [_P] * 11 + [_P, _SZ, _P]),
+    "am_sdf_fwd_ordered": (ctypes.c_int, [_P] * 6 + [_I64] + [_P] * 9 + [_P]),
+    "am_sdf_bwd_ordered": (ctypes.c_int, [_P] * 6 + [_I64] + [_P] * 11 + [_P, _SZ, _P]),
     "am_radiance_fwd": (ctypes.c_int, [_P, _I32, _I32, _P, _I32, _P, _P, _P, _I64, _P, _P, _P]),
     "am_radiance_bwd_workspace_size": (_SZ, [_P, _I64]),
     "am_radiance_bwd": (ctypes.c_int, [_P, _I32, _I32, _P, _I32, _P, _P, _P, _I64, _P, _P, _P, _P, _P, _P, _SZ, _P]),

@@ -134,8 +134,8 @@ def _sdf_store_floats(lib, grid, mdesc, sdesc, n):
 
 
 # The fused SDF field runs its samples in spatial (Morton) order when there
-# are at least min_samples of them: the caller's order is restored on the
-# outputs (and the backward's upstream gradients follow the same permutation).
+# are at least min_samples of them; the kernels read inputs and write outputs
+# (and read upstream gradients) in the caller's order through the permutation.
 # Cells of 2^-bits of the box per axis; within a cell the caller's (ray)
 # order is kept, so neighbouring samples of a ray still merge their table-
 # gradient pairs.  5 bits measured best on the Neuralangelo step (10 bits:
@@ -158,31 +158,17 @@ def spatial_order(pts, radius, bits=None):
     return perm, inv
 
 
-class Permute(torch.autograd.Function):
-    """Rows of x in the order idx; the backward gathers with the inverse
-    permutation (no scatter-add)."""
-
-    @staticmethod
-    def forward(ctx, x, idx, inv):
-        ctx.save_for_backward(inv)
-        return x.index_select(0, idx)
-
-    @staticmethod
-    def backward(ctx, g):
-        (inv,) = ctx.saved_tensors
-        return g.index_select(0, inv), None, None
-
-
 class SDFFieldFn(torch.autograd.Function):
     """Fused VolumeSDF.forward (models/geometry.py:195-287): outputs sdf [n],
     grad [n,3], normal [n,3] = F.normalize(grad), feature [n, n_out+3],
     laplace [n] (grad/normal/laplace empty when not requested)."""
 
     @staticmethod
-    def forward(ctx, pts, randn, table, params, grid, mdesc, sdesc, table_k=None):
+    def forward(ctx, pts, randn, table, params, grid, mdesc, sdesc, table_k=None, order=None):
         """table: the fp32 master (its gradient is returned); table_k: the
         rows the kernels read (the master, or its fp16 copy in the mixed
-        form)."""
+        form); order: int32 evaluation order of the samples (None: theirs),
+        inputs and outputs stay in the caller's order."""
         if table_k is None:
             table_k = table
         nat.require_cuda(pts, table, params, table_k)
@@ -201,22 +187,29 @@ class SDFFieldFn(torch.autograd.Function):
         fdv = torch.empty(n, 12, **f32) if (sdesc.with_grad and keep) else None
         st_floats = _sdf_store_floats(lib, grid, mdesc, sdesc, n) if (keep and n > 0) else 0
         store = torch.empty(st_floats, **f32) if st_floats > 0 else None
-        nat.check(lib.am_sdf_fwd(nat.sptr(grid), nat.sptr(mdesc), nat.sptr(sdesc), nat.ptr(pts),
-                                 nat.ptr(rnd), n, nat.ptr(table_k), nat.ptr(params), nat.ptr(sdf),
-                                 nat.ptr(grad) if sdesc.with_grad else None,
-                                 nat.ptr(normal) if sdesc.with_grad else None, nat.ptr(feature),
-                                 nat.ptr(lap) if sdesc.with_laplace else None, nat.ptr(fdv),
-                                 nat.ptr(store), nat.stream()), "am_sdf_fwd")
+        outs = (nat.ptr(table_k), nat.ptr(params), nat.ptr(sdf),
+                nat.ptr(grad) if sdesc.with_grad else None,
+                nat.ptr(normal) if sdesc.with_grad else None, nat.ptr(feature),
+                nat.ptr(lap) if sdesc.with_laplace else None, nat.ptr(fdv), nat.ptr(store),
+                nat.stream())
+        if order is None:
+            nat.check(lib.am_sdf_fwd(nat.sptr(grid), nat.sptr(mdesc), nat.sptr(sdesc),
+                                     nat.ptr(pts), nat.ptr(rnd), n, *outs), "am_sdf_fwd")
+        else:
+            nat.check(lib.am_sdf_fwd_ordered(nat.sptr(grid), nat.sptr(mdesc), nat.sptr(sdesc),
+                                             nat.ptr(order), nat.ptr(pts), nat.ptr(rnd), n,
+                                             *outs), "am_sdf_fwd_ordered")
         empty = pts.new_empty(0)
         ctx.save_for_backward(pts, rnd if rnd is not None else empty, table, params,
                               fdv if fdv is not None else empty,
-                              store if store is not None else empty, table_k)
+                              store if store is not None else empty, table_k,
+                              order if order is not None else empty)
         ctx.descs = (grid, mdesc, sdesc)
         return sdf, grad, normal, feature, lap
 
     @staticmethod
     def backward(ctx, g_sdf, g_grad, g_normal, g_feature, g_lap):
-        pts, rnd, table, params, fdv, store, table_k = ctx.saved_tensors
+        pts, rnd, table, params, fdv, store, table_k, order = ctx.saved_tensors
         grid, mdesc, sdesc = ctx.descs
         lib = nat.load()
         n = pts.shape[0]
@@ -231,13 +224,19 @@ class SDFFieldFn(torch.autograd.Function):
         gl = opt(g_lap, n) if sdesc.with_laplace else None
         dtable = torch.empty_like(table)
         dparams = torch.empty_like(params)
-        SDF_BWD_WORKSPACE.run(pts.device, lambda wp, wb: lib.am_sdf_bwd(
-            nat.sptr(grid), nat.sptr(mdesc), nat.sptr(sdesc), nat.ptr(pts),
-            nat.ptr(rnd) if rnd.numel() else None, n, nat.ptr(table_k), nat.ptr(params),
-            nat.ptr(fdv) if fdv.numel() else None, nat.ptr(store) if store.numel() else None,
-            nat.ptr(gs), nat.ptr(gg), nat.ptr(gn), nat.ptr(gf), nat.ptr(gl), nat.ptr(dtable),
-            nat.ptr(dparams), wp, wb, nat.stream()), "am_sdf_bwd")
-        return None, None, dtable, dparams, None, None, None, None
+        rest = (nat.ptr(rnd) if rnd.numel() else None, n, nat.ptr(table_k), nat.ptr(params),
+                nat.ptr(fdv) if fdv.numel() else None, nat.ptr(store) if store.numel() else None,
+                nat.ptr(gs), nat.ptr(gg), nat.ptr(gn), nat.ptr(gf), nat.ptr(gl), nat.ptr(dtable),
+                nat.ptr(dparams))
+        ds = (nat.sptr(grid), nat.sptr(mdesc), nat.sptr(sdesc))
+        if order.numel():
+            SDF_BWD_WORKSPACE.run(pts.device, lambda wp, wb: lib.am_sdf_bwd_ordered(
+                *ds, nat.ptr(order), nat.ptr(pts), *rest, wp, wb, nat.stream()),
+                "am_sdf_bwd_ordered")
+        else:
+            SDF_BWD_WORKSPACE.run(pts.device, lambda wp, wb: lib.am_sdf_bwd(
+                *ds, nat.ptr(pts), *rest, wp, wb, nat.stream()), "am_sdf_bwd")
+        return None, None, dtable, dparams, None, None, None, None, None
 
 
 class RadianceFn(torch.autograd.Function):

@@ -137,10 +137,8 @@ class VolumeSDF(BaseImplicitGeometry):
         if not (so["on"] and pts.shape[0] >= so["min_samples"]):
             return fn.SDFFieldFn.apply(pts, randn, *args)
         pts = pts.float().contiguous()
-        perm, inv = fn.spatial_order(pts, self.radius)
-        outs = fn.SDFFieldFn.apply(pts.index_select(0, perm),
-                                   randn.index_select(0, perm) if randn is not None else None, *args)
-        return tuple(fn.Permute.apply(o, inv, perm) if o.shape[0] else o for o in outs)
+        perm, _ = fn.spatial_order(pts, self.radius)
+        return fn.SDFFieldFn.apply(pts, randn, *args, perm)
 
     # -- reference API ------------------------------------------------------
     def forward(self, points, with_grad=True, with_feature=True, with_laplace=False,

@@ -59,7 +59,17 @@ struct GeoArgs {
     // evaluations), genc_slots = encode-gradient slots per tile (n_evals, and
     // 3 more in analytic mode: u * scale * g_x along each axis).
     int analytic, curv0, n_dx, dx_curv0, genc_slots;
+    // Evaluation order: kernel sample n is the caller's sample order[n]
+    // (nullptr: the caller's order).  Inputs (positions, curvature
+    // directions, upstream gradients) and outputs are addressed in the
+    // caller's order; the forward's saved values (FD values, stored
+    // encodings) and the encode gradients in the kernels' order.
+    const int32_t* order;
 };
+
+__device__ __forceinline__ int64_t caller_row(const GeoArgs& a, int64_t n, bool valid) {
+    return (a.order && valid) ? (int64_t)a.order[n] : n;
+}
 
 // Row-space column of the SDF MLP first layer for lane-group level slots:
 // K-step s < 2LJ -> level k + 4(s/2), feature s%2; s == 2LJ -> xyz[k].
@@ -449,16 +459,17 @@ __global__ void __launch_bounds__(256) k_geo_fwd(GeoArgs a, const float* __restr
          tile += (int64_t)gridDim.x * nw) {
         const int64_t n = tile * 16 + p;
         const bool valid = n < N;
+        const int64_t m = caller_row(a, n, valid);
         GeoSample s;
 #pragma unroll
-        for (int d = 0; d < 3; ++d) s.P[d] = valid ? pts[3 * n + d] : 0.0f;
+        for (int d = 0; d < 3; ++d) s.P[d] = valid ? pts[3 * m + d] : 0.0f;
 #pragma unroll
         for (int d = 0; d < 3; ++d) s.x0[d] = contract1(s.P[d], a);
         if constexpr (AN) {
             GeoEval<LJ, NH, true> e;   // with the position derivatives
             geo_eval<LJ, NH, true, mlp::kF32>(a, sm, lv, table, s.x0, e);
             if constexpr (ST) geo_store<LJ, NH, true>(store, a, tile, n_tiles, 0, 0, e);
-            write_base(a, e, s, n, valid, k, fdim, sdf, feature);
+            write_base(a, e, s, m, valid, k, fdim, sdf, feature);
             if (!a.with_grad) continue;
             float gp[3], u[GeoDims<LJ>::KS1];
             geo_input_grad<LJ>(a, sm, e, gp, u);
@@ -472,7 +483,7 @@ __global__ void __launch_bounds__(256) k_geo_fwd(GeoArgs a, const float* __restr
                 GeoEval<LJ, NH, false> e;
                 geo_eval<LJ, NH, false, mlp::kF32>(a, sm, lv, table, s.x0, e);
                 if constexpr (ST) geo_store<LJ, NH, false>(store, a, tile, n_tiles, 0, 0, e);
-                write_base(a, e, s, n, valid, k, fdim, sdf, feature);
+                write_base(a, e, s, m, valid, k, fdim, sdf, feature);
             }
             if (!a.with_grad) continue;
             float g[3] = {0.0f, 0.0f, 0.0f};
@@ -496,13 +507,13 @@ __global__ void __launch_bounds__(256) k_geo_fwd(GeoArgs a, const float* __restr
         if (valid && k == 0) {
             if (grad)
 #pragma unroll
-                for (int d = 0; d < 3; ++d) grad[3 * n + d] = s.grad[d];
+                for (int d = 0; d < 3; ++d) grad[3 * m + d] = s.grad[d];
             if (normal)
 #pragma unroll
-                for (int d = 0; d < 3; ++d) normal[3 * n + d] = s.n[d];
+                for (int d = 0; d < 3; ++d) normal[3 * m + d] = s.n[d];
         }
         if (!a.with_laplace) continue;
-        geo_tangent(a, randn, n, valid, s);
+        geo_tangent(a, randn, m, valid, s);
         float gs[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll 1
         for (int j = 0; j < 6; ++j) {
@@ -523,7 +534,7 @@ __global__ void __launch_bounds__(256) k_geo_fwd(GeoArgs a, const float* __restr
         const float dot = (s.n[0] * ns[0] + s.n[1] * ns[1]) + s.n[2] * ns[2];
         if (valid && k == 0 && laplace) {
             const float lo = -1.0f + 1e-6f, hi = 1.0f - 1e-6f;
-            laplace[n] = acosf(fminf(fmaxf(dot, lo), hi)) * a.inv_pi;
+            laplace[m] = acosf(fminf(fmaxf(dot, lo), hi)) * a.inv_pi;
         }
     }
 }
@@ -974,13 +985,14 @@ __device__ __forceinline__ void flush_cluster(const GeoArgs& a, const LevelParam
 
 // Rebuilds the per-sample state (grad, normal, tangent, curvature terms) of
 // the lane's sample from the 12 stored values (6 FD values, or the analytic
-// grad in the first 3, then the 6 curvature values).
+// grad in the first 3, then the 6 curvature values).  n: the kernels' index
+// (stored values), m: the caller's index (positions, directions).
 __device__ __forceinline__ void geo_rebuild(const GeoArgs& a, const float* __restrict__ pts,
                                             const float* __restrict__ randn,
-                                            const float* __restrict__ fdv, int64_t n, bool valid,
-                                            GeoSample& s) {
+                                            const float* __restrict__ fdv, int64_t n, int64_t m,
+                                            bool valid, GeoSample& s) {
 #pragma unroll
-    for (int d = 0; d < 3; ++d) s.P[d] = valid ? pts[3 * n + d] : 0.0f;
+    for (int d = 0; d < 3; ++d) s.P[d] = valid ? pts[3 * m + d] : 0.0f;
 #pragma unroll
     for (int d = 0; d < 3; ++d) s.x0[d] = contract1(s.P[d], a);
     if (a.with_grad) {
@@ -998,7 +1010,7 @@ __device__ __forceinline__ void geo_rebuild(const GeoArgs& a, const float* __res
         normalize3(s.grad, s.n);
     }
     if (a.with_laplace) {
-        geo_tangent(a, randn, n, valid, s);
+        geo_tangent(a, randn, m, valid, s);
         float gs[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll
         for (int j = 0; j < 6; ++j) diff_pair(gs, j, valid ? fdv[n * 12 + 6 + j] : 0.0f);
@@ -1040,7 +1052,7 @@ __global__ void __launch_bounds__(256) k_geo_pairs(
         const int64_t n = tile * 16 + p;
         const bool valid = n < N;
         GeoSample s;
-        geo_rebuild(a, pts, randn, fdv, n, valid, s);
+        geo_rebuild(a, pts, randn, fdv, n, caller_row(a, n, valid), valid, s);
         EmitTile<VT> et{nullptr, nullptr, 0, 0};
         const float* gt = nullptr;
         if constexpr (!COUNT) {
@@ -1097,19 +1109,20 @@ __device__ __forceinline__ void geo_bwd_body(
          tile += (int64_t)gridDim.x * nw) {
         const int64_t n = tile * 16 + p;
         const bool valid = n < N;
+        const int64_t m = caller_row(a, n, valid);
         // this tile's encode-gradient slots (0 base, then FD and curvature, or
         // curvature and the three analytic extra slots)
         float* genc = genc_g + tile * a.genc_slots * LJ * 128;
         GeoSample s;
-        geo_rebuild(a, pts, randn, fdv, n, valid, s);
+        geo_rebuild(a, pts, randn, fdv, n, m, valid, s);
         // upstream gradients of this sample
         float gg[3] = {0, 0, 0}, gnrm[3] = {0, 0, 0};
         if (valid && g_grad)
 #pragma unroll
-            for (int d = 0; d < 3; ++d) gg[d] = g_grad[3 * n + d];
+            for (int d = 0; d < 3; ++d) gg[d] = g_grad[3 * m + d];
         if (valid && g_normal)
 #pragma unroll
-            for (int d = 0; d < 3; ++d) gnrm[d] = g_normal[3 * n + d];
+            for (int d = 0; d < 3; ++d) gnrm[d] = g_normal[3 * m + d];
         // output gradient of the base evaluation
         v4f go;
 #pragma unroll
@@ -1117,8 +1130,8 @@ __device__ __forceinline__ void geo_bwd_body(
             const int row = 4 * k + r;
             float v = 0.0f;
             if (valid && row < a.n_out) {
-                if (g_feature) v = g_feature[n * fdim + row];
-                if (row == 0 && g_sdf) v += g_sdf[n];
+                if (g_feature) v = g_feature[m * fdim + row];
+                if (row == 0 && g_sdf) v += g_sdf[m];
             }
             go[r] = v;
         }
@@ -1129,7 +1142,7 @@ __device__ __forceinline__ void geo_bwd_body(
             geo_eval_bwd<LJ, NH, false, mlp::kF32>(a, sm, e, go, scratch, G, genc, nullptr);
         }
         if (a.with_laplace) {
-            const float gl = (valid && g_laplace) ? g_laplace[n] : 0.0f;
+            const float gl = (valid && g_laplace) ? g_laplace[m] : 0.0f;
             const float lo = -1.0f + 1e-6f, hi = 1.0f - 1e-6f;
             const float dc = fminf(fmaxf(s.dot, lo), hi);
             const bool pass = s.dot >= lo && s.dot <= hi;
@@ -1646,14 +1659,15 @@ extern "C" int64_t am_sdf_store_floats(const am_grid_desc* g, const am_mlp_desc*
     return geo_enc_floats(tiles, a.n_evals, LJ) + geo_dx_floats(tiles, a.n_dx, LJ);
 }
 
-extern "C" int am_sdf_fwd(const am_grid_desc* g, const am_mlp_desc* m, const am_sdf_desc* sd,
-                          const float* pts, const float* randn, int64_t n, const void* table,
-                          const float* params, float* sdf, float* grad, float* normal,
-                          float* feature, float* laplace, float* fd_values, float* enc_store,
-                          void* stream) {
+static int sdf_fwd(const am_grid_desc* g, const am_mlp_desc* m, const am_sdf_desc* sd,
+                   const int32_t* order, const float* pts, const float* randn, int64_t n,
+                   const void* table, const float* params, float* sdf, float* grad,
+                   float* normal, float* feature, float* laplace, float* fd_values,
+                   float* enc_store, void* stream) {
     GeoArgs a;
     int LJ;
     if (int rc = make_geo_args(g, m, sd, &a, &LJ)) return rc;
+    a.order = order;
     AM_CHECK_ARG(n >= 0, "n < 0");
     if (n == 0) return AM_OK;
     AM_CHECK_ARG(pts && table && params && sdf, "NULL buffer");
@@ -1661,6 +1675,26 @@ extern "C" int am_sdf_fwd(const am_grid_desc* g, const am_mlp_desc* m, const am_
     hipStream_t s = to_stream(stream);
     AM_GEO_DISPATCH(geo_fwd_launch, a, pts, randn, n, table, params, sdf, grad, normal, feature,
                     laplace, fd_values, enc_store, s);
+}
+
+extern "C" int am_sdf_fwd(const am_grid_desc* g, const am_mlp_desc* m, const am_sdf_desc* sd,
+                          const float* pts, const float* randn, int64_t n, const void* table,
+                          const float* params, float* sdf, float* grad, float* normal,
+                          float* feature, float* laplace, float* fd_values, float* enc_store,
+                          void* stream) {
+    return sdf_fwd(g, m, sd, nullptr, pts, randn, n, table, params, sdf, grad, normal, feature,
+                   laplace, fd_values, enc_store, stream);
+}
+
+extern "C" int am_sdf_fwd_ordered(const am_grid_desc* g, const am_mlp_desc* m,
+                                  const am_sdf_desc* sd, const int32_t* order, const float* pts,
+                                  const float* randn, int64_t n, const void* table,
+                                  const float* params, float* sdf, float* grad, float* normal,
+                                  float* feature, float* laplace, float* fd_values,
+                                  float* enc_store, void* stream) {
+    AM_CHECK_ARG(order || n == 0, "NULL order");
+    return sdf_fwd(g, m, sd, order, pts, randn, n, table, params, sdf, grad, normal, feature,
+                   laplace, fd_values, enc_store, stream);
 }
 
 extern "C" size_t am_sdf_bwd_workspace_size(const am_grid_desc* g, const am_mlp_desc* m,
@@ -1678,15 +1712,17 @@ extern "C" size_t am_sdf_bwd_workspace_size(const am_grid_desc* g, const am_mlp_
 
 extern "C" int am_sdf_bwd_check(void) { return collect_flags(true); }
 
-extern "C" int am_sdf_bwd(const am_grid_desc* g, const am_mlp_desc* m, const am_sdf_desc* sd,
-                          const float* pts, const float* randn, int64_t n, const void* table,
-                          const float* params, const float* fd_values, const float* enc_store,
-                          const float* g_sdf, const float* g_grad, const float* g_normal,
-                          const float* g_feature, const float* g_laplace, float* dtable,
-                          float* dparams, void* workspace, size_t workspace_bytes, void* stream) {
+static int sdf_bwd(const am_grid_desc* g, const am_mlp_desc* m, const am_sdf_desc* sd,
+                   const int32_t* order, const float* pts, const float* randn, int64_t n,
+                   const void* table, const float* params, const float* fd_values,
+                   const float* enc_store, const float* g_sdf, const float* g_grad,
+                   const float* g_normal, const float* g_feature, const float* g_laplace,
+                   float* dtable, float* dparams, void* workspace, size_t workspace_bytes,
+                   void* stream) {
     GeoArgs a;
     int LJ;
     if (int rc = make_geo_args(g, m, sd, &a, &LJ)) return rc;
+    a.order = order;
     AM_CHECK_ARG(n >= 0, "n < 0");
     AM_CHECK_ARG(table && params && dtable && dparams && workspace, "NULL buffer");
     hipStream_t s = to_stream(stream);
@@ -1699,4 +1735,29 @@ extern "C" int am_sdf_bwd(const am_grid_desc* g, const am_mlp_desc* m, const am_
     AM_GEO_DISPATCH(geo_bwd_launch, a, g, m, sd, pts, randn, n, table, params, fd_values,
                     enc_store, g_sdf, g_grad, g_normal, g_feature, g_laplace, dtable, dparams,
                     workspace, workspace_bytes, s);
+}
+
+extern "C" int am_sdf_bwd(const am_grid_desc* g, const am_mlp_desc* m, const am_sdf_desc* sd,
+                          const float* pts, const float* randn, int64_t n, const void* table,
+                          const float* params, const float* fd_values, const float* enc_store,
+                          const float* g_sdf, const float* g_grad, const float* g_normal,
+                          const float* g_feature, const float* g_laplace, float* dtable,
+                          float* dparams, void* workspace, size_t workspace_bytes, void* stream) {
+    return sdf_bwd(g, m, sd, nullptr, pts, randn, n, table, params, fd_values, enc_store, g_sdf,
+                   g_grad, g_normal, g_feature, g_laplace, dtable, dparams, workspace,
+                   workspace_bytes, stream);
+}
+
+extern "C" int am_sdf_bwd_ordered(const am_grid_desc* g, const am_mlp_desc* m,
+                                  const am_sdf_desc* sd, const int32_t* order, const float* pts,
+                                  const float* randn, int64_t n, const void* table,
+                                  const float* params, const float* fd_values,
+                                  const float* enc_store, const float* g_sdf, const float* g_grad,
+                                  const float* g_normal, const float* g_feature,
+                                  const float* g_laplace, float* dtable, float* dparams,
+                                  void* workspace, size_t workspace_bytes, void* stream) {
+    AM_CHECK_ARG(order || n == 0, "NULL order");
+    return sdf_bwd(g, m, sd, order, pts, randn, n, table, params, fd_values, enc_store, g_sdf,
+                   g_grad, g_normal, g_feature, g_laplace, dtable, dparams, workspace,
+                   workspace_bytes, stream);
 }
