@@ -134,7 +134,10 @@ class VolumeSDF(BaseImplicitGeometry):
         args = (g.table, self.network.flat_params(), g.grid, self.network.desc,
                 self._sdf_desc(with_grad, with_laplace), g.kernel_table())
         so = fn.SPATIAL_ORDER
-        if not (so["on"] and pts.shape[0] >= so["min_samples"]):
+        # the analytic-gradient form (7 evaluations, more arithmetic per gather)
+        # measured no gain from the spatial order
+        analytic = with_grad and self.grad_type == "analytic"
+        if not (so["on"] and pts.shape[0] >= so["min_samples"] and not analytic):
             return fn.SDFFieldFn.apply(pts, randn, *args)
         pts = pts.float().contiguous()
         perm, _ = fn.spatial_order(pts, self.radius)

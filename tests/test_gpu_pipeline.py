@@ -368,10 +368,13 @@ def test_spatial_order_same_results(dev, monkeypatch, grad_type):
          torch.randn(n, generator=g).to(dev)]
 
     def run(on):
-        monkeypatch.setitem(functions.SPATIAL_ORDER, "on", on)
-        monkeypatch.setitem(functions.SPATIAL_ORDER, "min_samples", 1)
+        # the fused launch with and without an evaluation order
         model.zero_grad()
-        sdf, grad, normal, feature, lap = geo.fused(pts, True, True, randn)
+        gm = geo.grid_module
+        args = (gm.table, geo.network.flat_params(), gm.grid, geo.network.desc,
+                geo._sdf_desc(True, True), gm.kernel_table())
+        order = functions.spatial_order(pts, geo.radius)[0] if on else None
+        sdf, grad, normal, feature, lap = functions.SDFFieldFn.apply(pts, randn, *args, order)
         ((sdf * w[0]).sum() + (grad * w[1]).sum() + (lap * w[2]).sum() + feature.sum()).backward()
         return [sdf, grad, normal, feature, lap], [p.grad.clone() for p in geo.parameters()]
 
