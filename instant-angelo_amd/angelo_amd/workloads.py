@@ -133,7 +133,10 @@ WORKLOADS["neuralangelo-bg"] = dict(WORKLOADS["neuralangelo-fp32"], learned_back
 
 
 class Workload:
-    """Model, trainer, optimizer and a pool of device-resident batches."""
+    """Model, trainer, optimizer and the step's data: a pool of
+    device-resident batches with uniform samples (sphere workload), or rays
+    drawn on the device each step from a resident image set (Neuralangelo
+    workloads)."""
 
     def __init__(self, name, device, pool=8, seed=0, rays=None, global_step=20000, band_cells=1.5):
         spec = dict(WORKLOADS[name])
