@@ -97,6 +97,19 @@ size_t am_hashgrid_bwd_workspace_size(const am_grid_desc* g, int64_t n, int32_t 
 int am_hashgrid_bwd(const am_grid_desc* g, const float* x, int64_t n, const float* table,
                     const float* dy, int32_t active_levels, float* dtable, float* dx,
                     void* workspace, size_t workspace_bytes, void* stream);
+/* am_hashgrid_fwd / am_hashgrid_bwd evaluating the points in the order
+ * `order` (a permutation of [0, n); point k of the launch is the caller's
+ * point order[k]); x, out, dy and dx stay in the caller's order.  No
+ * counterpart in the reference: the binding passes the Morton order of large
+ * batches so that nearby points gather nearby rows.  Outputs are identical;
+ * the table gradient's fp32 sums run in another fixed order. */
+int am_hashgrid_fwd_ordered(const am_grid_desc* g, const int32_t* order, const float* x,
+                            int64_t n, const float* table, int32_t active_levels, float* out,
+                            void* stream);
+int am_hashgrid_bwd_ordered(const am_grid_desc* g, const int32_t* order, const float* x,
+                            int64_t n, const float* table, const float* dy,
+                            int32_t active_levels, float* dtable, float* dx, void* workspace,
+                            size_t workspace_bytes, void* stream);
 
 /* ------------------------------------------------------------------------ */
 /* VanillaMLP (models/network_utils.py:96-140, get_mlp :177-185).            */

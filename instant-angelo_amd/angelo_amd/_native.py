@@ -97,6 +97,8 @@ _SIGS = {
     "am_hashgrid_fwd": (ctypes.c_int, [_P, _P, _I64, _P, _I32, _P, _P]),
     "am_hashgrid_bwd_workspace_size": (_SZ, [_P, _I64, _I32]),
     "am_hashgrid_bwd": (ctypes.c_int, [_P, _P, _I64, _P, _P, _I32, _P, _P, _P, _SZ, _P]),
+    "am_hashgrid_fwd_ordered": (ctypes.c_int, [_P, _P, _P, _I64, _P, _I32, _P, _P]),
+    "am_hashgrid_bwd_ordered": (ctypes.c_int, [_P, _P, _P, _I64, _P, _P, _I32, _P, _P, _P, _SZ, _P]),
     "am_mlp_n_params": (_I64, [_P]),
     "am_mlp_fwd": (ctypes.c_int, [_P, _P, _I64, _P, _P, _P]),
     "am_mlp_bwd_workspace_size": (_SZ, [_P, _I64]),

@@ -64,9 +64,11 @@ __device__ __forceinline__ FVec<F> load_row(const float* __restrict__ t, uint32_
 template <int F>
 __global__ void __launch_bounds__(256) k_hg_fwd(GridArgs g, const float* __restrict__ x, int64_t n,
                                                 const float* __restrict__ table, int active,
-                                                float* __restrict__ out) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+                                                float* __restrict__ out,
+                                                const int32_t* __restrict__ order) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int64_t i = order ? (int64_t)order[k] : k;   // the caller's row
     const float xi[3] = {x[3 * i], x[3 * i + 1], x[3 * i + 2]};
     float* o = out + i * (int64_t)g.L * F;
     for (int l = 0; l < g.L; ++l) {
@@ -96,11 +98,13 @@ template <int F>
 __global__ void __launch_bounds__(256) k_hg_emit(GridArgs g, const float* __restrict__ x, int64_t n,
                                                  const float* __restrict__ dy, int active,
                                                  uint32_t* __restrict__ keys,
-                                                 Vals<F>* __restrict__ vals) {
+                                                 Vals<F>* __restrict__ vals,
+                                                 const int32_t* __restrict__ order) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n * active) return;
     const int l = (int)(t / n);
-    const int64_t i = t - (int64_t)l * n;
+    const int64_t k = t - (int64_t)l * n;
+    const int64_t i = order ? (int64_t)order[k] : k;   // the caller's row
     const float xi[3] = {x[3 * i], x[3 * i + 1], x[3 * i + 2]};
     const LevelParams lp = g.lv[l];
     const LevelPos p = level_pos(lp, xi);
@@ -123,9 +127,11 @@ template <int F>
 __global__ void __launch_bounds__(256) k_hg_dx(GridArgs g, const float* __restrict__ x, int64_t n,
                                                const float* __restrict__ table,
                                                const float* __restrict__ dy, int active,
-                                               float* __restrict__ dx) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+                                               float* __restrict__ dx,
+                                               const int32_t* __restrict__ order) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int64_t i = order ? (int64_t)order[k] : k;   // the caller's row
     const float xi[3] = {x[3 * i], x[3 * i + 1], x[3 * i + 2]};
     float acc[3] = {0.0f, 0.0f, 0.0f};
     for (int l = 0; l < active; ++l) {
@@ -157,11 +163,11 @@ __global__ void __launch_bounds__(256) k_hg_dx(GridArgs g, const float* __restri
 
 template <int F>
 static int hg_fwd(const GridArgs& ga, const float* x, int64_t n, const float* table, int active,
-                  float* out, hipStream_t s) {
+                  float* out, const int32_t* order, hipStream_t s) {
     {
         KernelTimer kt_("hg_fwd", s);
         hipLaunchKernelGGL(k_hg_fwd<F>, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, ga, x, n,
-                           table, active, out);
+                           table, active, out, order);
     }
     AM_LAUNCH_CHECK();
     return AM_OK;
@@ -170,10 +176,10 @@ static int hg_fwd(const GridArgs& ga, const float* x, int64_t n, const float* ta
 template <int F>
 static int hg_bwd(const GridArgs& ga, uint32_t n_entries, const float* x, int64_t n,
                   const float* table, const float* dy, int active, float* dtable, float* dx,
-                  void* ws, size_t ws_bytes, hipStream_t s) {
+                  void* ws, size_t ws_bytes, const int32_t* order, hipStream_t s) {
     if (dx) {
         hipLaunchKernelGGL(k_hg_dx<F>, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, ga, x,
-                           n, table, dy, active, dx);
+                           n, table, dy, active, dx, order);
         AM_LAUNCH_CHECK();
     }
     if (dtable) {
@@ -188,7 +194,7 @@ static int hg_bwd(const GridArgs& ga, uint32_t n_entries, const float* x, int64_
             KernelTimer kt_("hg_emit", s);
             hipLaunchKernelGGL(k_hg_emit<F>, dim3((unsigned)ceil_div(n * active, 256)), dim3(256), 0,
                                s, ga, x, n, dy, active, w.keys[0],
-                               reinterpret_cast<Vals<F>*>(w.vals[0]));
+                               reinterpret_cast<Vals<F>*>(w.vals[0]), order);
         }
         AM_LAUNCH_CHECK();
         timer_units("encode_table_pairs", w.m);
@@ -252,9 +258,8 @@ extern "C" int am_grid_init(am_grid_desc* g, int32_t n_levels, int32_t n_feature
     return AM_OK;
 }
 
-extern "C" int am_hashgrid_fwd(const am_grid_desc* g, const float* x, int64_t n,
-                               const float* table, int32_t active_levels, float* out,
-                               void* stream) {
+static int hashgrid_fwd(const am_grid_desc* g, const float* x, int64_t n, const float* table,
+                        int32_t active_levels, float* out, const int32_t* order, void* stream) {
     if (int rc = check_grid(g)) return rc;
     AM_CHECK_ARG(n >= 0, "n < 0");
     if (n == 0) return AM_OK;
@@ -264,11 +269,24 @@ extern "C" int am_hashgrid_fwd(const am_grid_desc* g, const float* x, int64_t n,
     const GridArgs ga = make_grid_args(g);
     hipStream_t s = to_stream(stream);
     switch (g->n_features) {
-        case 1: return hg_fwd<1>(ga, x, n, table, active_levels, out, s);
-        case 2: return hg_fwd<2>(ga, x, n, table, active_levels, out, s);
-        case 4: return hg_fwd<4>(ga, x, n, table, active_levels, out, s);
-        default: return hg_fwd<8>(ga, x, n, table, active_levels, out, s);
+        case 1: return hg_fwd<1>(ga, x, n, table, active_levels, out, order, s);
+        case 2: return hg_fwd<2>(ga, x, n, table, active_levels, out, order, s);
+        case 4: return hg_fwd<4>(ga, x, n, table, active_levels, out, order, s);
+        default: return hg_fwd<8>(ga, x, n, table, active_levels, out, order, s);
     }
+}
+
+extern "C" int am_hashgrid_fwd(const am_grid_desc* g, const float* x, int64_t n,
+                               const float* table, int32_t active_levels, float* out,
+                               void* stream) {
+    return hashgrid_fwd(g, x, n, table, active_levels, out, nullptr, stream);
+}
+
+extern "C" int am_hashgrid_fwd_ordered(const am_grid_desc* g, const int32_t* order,
+                                       const float* x, int64_t n, const float* table,
+                                       int32_t active_levels, float* out, void* stream) {
+    AM_CHECK_ARG(order || n == 0, "NULL order");
+    return hashgrid_fwd(g, x, n, table, active_levels, out, order, stream);
 }
 
 extern "C" size_t am_hashgrid_bwd_workspace_size(const am_grid_desc* g, int64_t n,
@@ -280,10 +298,10 @@ extern "C" size_t am_hashgrid_bwd_workspace_size(const am_grid_desc* g, int64_t 
     return c.used + 256;
 }
 
-extern "C" int am_hashgrid_bwd(const am_grid_desc* g, const float* x, int64_t n,
-                               const float* table, const float* dy, int32_t active_levels,
-                               float* dtable, float* dx, void* workspace, size_t workspace_bytes,
-                               void* stream) {
+static int hashgrid_bwd(const am_grid_desc* g, const float* x, int64_t n, const float* table,
+                        const float* dy, int32_t active_levels, float* dtable, float* dx,
+                        void* workspace, size_t workspace_bytes, const int32_t* order,
+                        void* stream) {
     if (int rc = check_grid(g)) return rc;
     AM_CHECK_ARG(n >= 0, "n < 0");
     AM_CHECK_ARG(active_levels >= 0 && active_levels <= g->n_levels, "active_levels %d",
@@ -295,9 +313,27 @@ extern "C" int am_hashgrid_bwd(const am_grid_desc* g, const float* x, int64_t n,
     const GridArgs ga = make_grid_args(g);
     hipStream_t s = to_stream(stream);
     switch (g->n_features) {
-        case 1: return hg_bwd<1>(ga, g->n_entries, x, n, table, dy, active_levels, dtable, dx, workspace, workspace_bytes, s);
-        case 2: return hg_bwd<2>(ga, g->n_entries, x, n, table, dy, active_levels, dtable, dx, workspace, workspace_bytes, s);
-        case 4: return hg_bwd<4>(ga, g->n_entries, x, n, table, dy, active_levels, dtable, dx, workspace, workspace_bytes, s);
-        default: return hg_bwd<8>(ga, g->n_entries, x, n, table, dy, active_levels, dtable, dx, workspace, workspace_bytes, s);
+        case 1: return hg_bwd<1>(ga, g->n_entries, x, n, table, dy, active_levels, dtable, dx, workspace, workspace_bytes, order, s);
+        case 2: return hg_bwd<2>(ga, g->n_entries, x, n, table, dy, active_levels, dtable, dx, workspace, workspace_bytes, order, s);
+        case 4: return hg_bwd<4>(ga, g->n_entries, x, n, table, dy, active_levels, dtable, dx, workspace, workspace_bytes, order, s);
+        default: return hg_bwd<8>(ga, g->n_entries, x, n, table, dy, active_levels, dtable, dx, workspace, workspace_bytes, order, s);
     }
+}
+
+extern "C" int am_hashgrid_bwd(const am_grid_desc* g, const float* x, int64_t n,
+                               const float* table, const float* dy, int32_t active_levels,
+                               float* dtable, float* dx, void* workspace, size_t workspace_bytes,
+                               void* stream) {
+    return hashgrid_bwd(g, x, n, table, dy, active_levels, dtable, dx, workspace,
+                        workspace_bytes, nullptr, stream);
+}
+
+extern "C" int am_hashgrid_bwd_ordered(const am_grid_desc* g, const int32_t* order,
+                                       const float* x, int64_t n, const float* table,
+                                       const float* dy, int32_t active_levels, float* dtable,
+                                       float* dx, void* workspace, size_t workspace_bytes,
+                                       void* stream) {
+    AM_CHECK_ARG(order || n == 0, "NULL order");
+    return hashgrid_bwd(g, x, n, table, dy, active_levels, dtable, dx, workspace,
+                        workspace_bytes, order, stream);
 }

@@ -6,6 +6,8 @@ import torch
 from oracle import hashgrid as ohg
 from oracle import nets as onets
 
+import helpers
+
 pytestmark = pytest.mark.gpu
 
 CONFIGS = {
@@ -124,3 +126,34 @@ def test_mlp_fwd_bwd(dev, case):
     torch.testing.assert_close(y.detach().cpu().double(), y64.detach(), rtol=1e-4, atol=1e-5)
     torch.testing.assert_close(xd.grad.cpu().double(), x64.grad, rtol=1e-4, atol=1e-4)
     torch.testing.assert_close(fd.grad.cpu().double(), f64.grad, rtol=1e-4, atol=1e-3)
+
+
+@pytest.mark.parametrize("name", ["c0", "c1_full"])
+def test_hashgrid_spatial_order(dev, monkeypatch, name):
+    """The encode of a large batch runs in Morton order: outputs and dL/dx
+    bit-identical with the caller's order, the table gradient equal up to
+    the order of its fp32 sums, and repeatable bit for bit."""
+    from angelo_amd import descs, functions
+    from angelo_amd.functions import hashgrid_encode
+    L, F, log2T, base, scale = CONFIGS[name]
+    g = descs.grid_desc(L, F, log2T, base, scale)
+    gen = torch.Generator().manual_seed(21)
+    table = ((torch.rand(int(g.n_entries), F, generator=gen) * 2 - 1) * 0.1).to(dev)
+    x = _points(100000, 22).to(dev)
+    dy = torch.randn(x.shape[0], L * F, generator=gen).to(dev)
+
+    def run(on):
+        monkeypatch.setitem(functions.SPATIAL_ORDER, "encode", on)
+        xd = x.clone().requires_grad_(True)
+        td = table.clone().requires_grad_(True)
+        out = hashgrid_encode(xd, td, g, L)
+        (out * dy).sum().backward()
+        return out.detach(), xd.grad, td.grad
+
+    o0, dx0, dt0 = run(False)
+    o1, dx1, dt1 = run(True)
+    o2, dx2, dt2 = run(True)
+    assert torch.equal(o0, o1) and torch.equal(dx0, dx1)
+    assert torch.equal(dt1, dt2)
+    helpers.assert_nontrivial(dt0, "table gradient")
+    assert helpers.rel_err(dt1, dt0) < 1e-5
