@@ -56,14 +56,16 @@ def algorithmic_bytes(kernel, spec, n_samples, n_rays, n_pairs, mixed=False):
     composite: 40 B per sample + 32 B per ray.
     Analytic-gradient mode: 7 evaluations (base with position derivatives +
     6 curvature), 7 derivative slots stored, 3 more encode-gradient slots.
+    A workload held at its first band (start_level) gathers that many levels.
     """
     L, F = spec["levels"], 2
+    La = spec.get("start_level") or L   # levels gathered (the band's active levels)
     bt = 2 if mixed else 4
     pair = 8 if mixed else 12
     n_ev, n_dx, n_genc = evals_per_sample(spec), 6, evals_per_sample(spec)
     if spec.get("grad_type") == "analytic":
         n_dx, n_genc = 7, n_ev + 3
-    per_eval = 12 + 8 * L * F * bt
+    per_eval = 12 + 8 * La * F * bt
     store = (n_ev * L * F + n_dx * 3 * L * F) * 4
     genc = n_genc * L * F * 4
     if kernel == "geo_fwd":
@@ -336,6 +338,7 @@ def main():
                    "log2_hashmap_size": spec["log2T"], "evals_per_sample": evals_per_sample(spec),
                    "grad_type": spec.get("grad_type", "finite_difference"),
                    "learned_background": bool(spec.get("learned_background", False)),
+                   "active_levels": int(spec.get("start_level") or spec["levels"]),
                    "optimizer": "AdamW (in the timed step)"},
         "rays_per_s": round(rays * args.steps / (gpu_ms / 1e3), 1),
         "loss_scaling": ({"final_scale": wl.scaler.scale, "skipped_steps": wl.skipped_steps}

@@ -130,6 +130,11 @@ WORKLOADS["neuralangelo-analytic"] = dict(WORKLOADS["neuralangelo-fp32"], grad_t
 # samples per ray, the early-stop filter, VolumeDensity + VolumeDualColor and
 # a 256^3 background occupancy grid.
 WORKLOADS["neuralangelo-bg"] = dict(WORKLOADS["neuralangelo-fp32"], learned_background=True)
+# The start of training: the progressive band at its first 4 active levels
+# (start_level 4; the timed steps stay below the first band update) and the
+# finite-difference step of that band.
+WORKLOADS["neuralangelo-fp32-l4"] = dict(WORKLOADS["neuralangelo-fp32"], start_level=4,
+                                         global_step=0)
 
 
 class Workload:
@@ -148,7 +153,8 @@ class Workload:
                             spec["samples_per_ray"], spec["step"], spec["cap"], spec["grid_prune"],
                             seed=seed, precision=spec.get("precision", "fp32"),
                             grad_type=spec.get("grad_type", "finite_difference"),
-                            learned_background=spec.get("learned_background", False))
+                            learned_background=spec.get("learned_background", False),
+                            start_level=spec.get("start_level"))
         torch.manual_seed(seed)
         self.mcfg, self.scfg = mcfg, system_config()
         self.model = make("neus", mcfg).to(device)
@@ -159,6 +165,7 @@ class Workload:
         self.scaler = DynamicLossScaler() if self.mixed else None
         self.skipped_steps = 0
         self.bg_samples = 0   # background samples of the steps so far (in step()'s totals)
+        global_step = spec.get("global_step", global_step)
         self.global_step = global_step
         self.batches = []
         gen = torch.Generator().manual_seed(seed + 17)
