@@ -179,11 +179,16 @@ class SDFFieldFn(torch.autograd.Function):
     laplace [n] (grad/normal/laplace empty when not requested)."""
 
     @staticmethod
-    def forward(ctx, pts, randn, table, params, grid, mdesc, sdesc, table_k=None, order=None):
+    def forward(ctx, pts, randn, table, params, grid, mdesc, sdesc, table_k=None, order=None,
+                keep=True):
         """table: the fp32 master (its gradient is returned); table_k: the
         rows the kernels read (the master, or its fp16 copy in the mixed
         form); order: int32 evaluation order of the samples (None: theirs),
-        inputs and outputs stay in the caller's order."""
+        inputs and outputs stay in the caller's order; keep: the caller will
+        run the backward, so the forward writes the FD values and stored
+        encodings it reads (decided by the caller before apply: inside
+        forward grad mode is always off, and needs_input_grad is set for a
+        Parameter table even under no_grad)."""
         if table_k is None:
             table_k = table
         nat.require_cuda(pts, table, params, table_k)
@@ -198,7 +203,7 @@ class SDFFieldFn(torch.autograd.Function):
         normal = torch.empty(n, 3, **f32) if sdesc.with_grad else torch.empty(0, 3, **f32)
         lap = torch.empty(n, **f32) if sdesc.with_laplace else torch.empty(0, **f32)
         rnd = _f32c(randn) if (sdesc.with_laplace and randn is not None) else None
-        keep = torch.is_grad_enabled() or any(ctx.needs_input_grad)
+        keep = bool(keep)
         fdv = torch.empty(n, 12, **f32) if (sdesc.with_grad and keep) else None
         st_floats = _sdf_store_floats(lib, grid, mdesc, sdesc, n) if (keep and n > 0) else 0
         store = torch.empty(st_floats, **f32) if st_floats > 0 else None
@@ -251,7 +256,7 @@ class SDFFieldFn(torch.autograd.Function):
         else:
             SDF_BWD_WORKSPACE.run(pts.device, lambda wp, wb: lib.am_sdf_bwd(
                 *ds, nat.ptr(pts), *rest, wp, wb, nat.stream()), "am_sdf_bwd")
-        return None, None, dtable, dparams, None, None, None, None, None
+        return None, None, dtable, dparams, None, None, None, None, None, None
 
 
 class RadianceFn(torch.autograd.Function):

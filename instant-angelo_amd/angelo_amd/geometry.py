@@ -119,9 +119,12 @@ class VolumeSDF(BaseImplicitGeometry):
                 and self.n_output_dims <= 16
                 and "sdf_activation" not in self.config and "feature_activation" not in self.config)
 
-    def fused(self, points, with_grad=True, with_laplace=False, randn=None):
+    def fused(self, points, with_grad=True, with_laplace=False, randn=None, order=None):
         """All outputs of the fused launch: sdf [n], grad [n,3], normal [n,3],
-        feature [n, feature_dim+3], laplace [n]."""
+        feature [n, feature_dim+3], laplace [n].  order: the samples'
+        evaluation order (int32 permutation; None: the Morton order of a
+        large batch, computed here, or the caller's order; False: the
+        caller's order)."""
         if not self._fused_ok():
             raise NotImplementedError(
                 "fused VolumeSDF needs grad_type=finite_difference (or analytic with one hidden "
@@ -131,17 +134,22 @@ class VolumeSDF(BaseImplicitGeometry):
         if with_laplace and randn is None:
             randn = torch.randn_like(pts)
         g = self.grid_module
-        args = (g.table, self.network.flat_params(), g.grid, self.network.desc,
+        params = self.network.flat_params()
+        # the backward's saved buffers (FD values, stored encodings) only when
+        # a backward can follow: grad mode on and a parameter that takes one
+        keep = torch.is_grad_enabled() and (g.table.requires_grad or params.requires_grad)
+        args = (g.table, params, g.grid, self.network.desc,
                 self._sdf_desc(with_grad, with_laplace), g.kernel_table())
         so = fn.SPATIAL_ORDER
         # the analytic-gradient form (7 evaluations, more arithmetic per gather)
         # measured no gain from the spatial order
         analytic = with_grad and self.grad_type == "analytic"
-        if not (so["on"] and pts.shape[0] >= so["min_samples"] and not analytic):
-            return fn.SDFFieldFn.apply(pts, randn, *args)
-        pts = pts.float().contiguous()
-        perm, _ = fn.spatial_order(pts, self.radius)
-        return fn.SDFFieldFn.apply(pts, randn, *args, perm)
+        if order is False:
+            order = None
+        elif order is None and so["on"] and pts.shape[0] >= so["min_samples"] and not analytic:
+            pts = pts.float().contiguous()
+            order, _ = fn.spatial_order(pts, self.radius)
+        return fn.SDFFieldFn.apply(pts, randn, *args, order, keep)
 
     # -- reference API ------------------------------------------------------
     def forward(self, points, with_grad=True, with_feature=True, with_laplace=False,

@@ -24,7 +24,8 @@ class TrainRaySampler:
 
     def __init__(self, images, c2w, directions, fg_masks=None, device="cuda", apply_mask=False,
                  background_color="white", batch_image_sampling=True,
-                 sample_foreground_ratio=1.0, fg_indices=None, bg_indices=None):
+                 sample_foreground_ratio=1.0, fg_indices=None, bg_indices=None,
+                 points=None, points_confidence=None, points_normal=None):
         if background_color not in ("white", "random"):
             raise NotImplementedError(f"background_color {background_color!r}")
         self.device = torch.device(device)
@@ -45,6 +46,18 @@ class TrainRaySampler:
         self.ratio = float(sample_foreground_ratio)
         self.fg_indices = None if fg_indices is None else fg_indices.to(self.device).long()
         self.bg_indices = None if bg_indices is None else bg_indices.to(self.device).long()
+        # sparse scene points of the dataset (all_points, all_points_confidence,
+        # pts3d_normal), drawn one per ray for the sparse-point SDF loss
+        self.points = None if points is None else points.to(self.device, torch.float32)
+        self.points_confidence = None
+        self.points_normal = None
+        if self.points is not None:
+            m = self.points.shape[0]
+            self.points_confidence = (torch.ones(m, device=self.device) if points_confidence is None
+                                      else points_confidence.to(self.device, torch.float32))
+            if points_normal is not None:
+                self.points_normal = points_normal.to(self.device, torch.float32)
+        self.has_mask = fg_masks is not None
         self._bad = torch.zeros(1, dtype=torch.int32, device=self.device)
         self.last_draw = None
 
@@ -87,8 +100,9 @@ class TrainRaySampler:
         return rays, rgb, fg
 
     def sample(self, n, generator=None, model=None):
-        """One training batch: dict(rays, rgb, fg_mask); the background colour
-        of this step is set on `model` (systems/neus.py:99-107)."""
+        """One training batch: dict(rays, rgb, fg_mask, has_mask, and with
+        sparse points pts, pts_weights, pts_normal); the background colour of
+        this step is set on `model` (systems/neus.py:99-107)."""
         if self.background_color == "white":
             bg = torch.ones(3, device=self.device)
         else:
@@ -98,7 +112,15 @@ class TrainRaySampler:
         rays, rgb, fg = self.gather(index, y, x, bg)
         if model is not None:
             model.background_color = bg
-        return dict(rays=rays, rgb=rgb, fg_mask=fg)
+        batch = dict(rays=rays, rgb=rgb, fg_mask=fg, has_mask=self.has_mask)
+        if self.points is not None:
+            # systems/neus.py:67-74: as many points as rays
+            k = torch.randint(0, self.points.shape[0], (n,), device=self.device, generator=generator)
+            batch["pts"] = self.points[k]
+            batch["pts_weights"] = self.points_confidence[k]
+            batch["pts_normal"] = (self.points_normal[k] if self.points_normal is not None
+                                   else self.points.new_empty(0))
+        return batch
 
     def out_of_range(self):
         """Draws clamped so far (synchronises; 0 unless an index list is bad)."""

@@ -16,8 +16,12 @@ def binary_cross_entropy(input, target):
 class NeuSTrainer:
     """Holds the per-step state of NeuSSystem that the hot path needs."""
 
-    def __init__(self, model, system_config, model_config):
+    def __init__(self, model, system_config, model_config, has_mask=False):
         self.model = model
+        # dataset.has_mask of the reference: the mask loss counts only when
+        # the dataset has foreground masks (systems/neus.py:143-145); a batch
+        # may say so itself ("has_mask", TrainRaySampler.sample does)
+        self.has_mask = bool(has_mask)
         self.cfg = system_config
         self.mcfg = model_config
         self.train_num_rays = int(model_config.train_num_rays)
@@ -69,7 +73,8 @@ class NeuSTrainer:
             (torch.linalg.norm(out["sdf_grad_samples"], ord=2, dim=-1) - 1.0) ** 2)
         loss = loss + terms["eikonal"] * self.C(L.lambda_eikonal)
         opacity = torch.clamp(out["opacity"].squeeze(-1), 1e-3, 1.0 - 1e-3)
-        if self.C(L.get("lambda_mask", 0.0)) > 0 and "fg_mask" in batch:
+        has_mask = bool(batch.get("has_mask", self.has_mask))
+        if self.C(L.get("lambda_mask", 0.0)) > 0 and has_mask:
             terms["mask"] = binary_cross_entropy(opacity, batch["fg_mask"].float())
             loss = loss + terms["mask"] * self.C(L.lambda_mask)
         if self.C(L.get("lambda_opaque", 0.0)) > 0:
@@ -81,10 +86,35 @@ class NeuSTrainer:
         if self.C(L.get("lambda_curvature", 0.0)) > 0:
             terms["curvature"] = self._mean(out["sdf_laplace_samples"].abs())
             loss = loss + terms["curvature"] * self.C(L.lambda_curvature)
-        for k in ("lambda_distortion", "lambda_sdf_l1"):
+        distortion = ["lambda_distortion"]
+        if self.mcfg.get("learned_background", False):
+            distortion.append("lambda_distortion_bg")   # counted only then (:172)
+        for k in distortion:
             if self.C(L.get(k, 0.0)) > 0:
                 raise NotImplementedError(f"{k} > 0 is not part of this build")
+        if self.C(L.get("lambda_sdf_l1", 0.0)) > 0:
+            sdf_l1, normal_cos = self.sparse_point_losses(batch)
+            terms["sdf_l1"], terms["normal_cos"] = sdf_l1, normal_cos
+            loss = loss + sdf_l1 * self.C(L.lambda_sdf_l1)
+            loss = loss + normal_cos * self.C(L.get("lambda_normal", L.lambda_sdf_l1))
         return loss, terms
+
+    def sparse_point_losses(self, batch):
+        """systems/neus.py:180-194: the SDF at the dataset's sparse points
+        (l1 towards 0, scaled by their confidence) and the cosine distance of
+        its gradient to the points' normals.  As there, the l1 term is the
+        mean |sdf| (F.l1_loss) times the weights, averaged over dim 0."""
+        pts, normal, weights = batch["pts"], batch["pts_normal"], batch["pts_weights"]
+        if pts.numel() == 0:
+            raise ValueError("lambda_sdf_l1 > 0 needs the batch's sparse points ('pts')")
+        if normal.numel() == 0:
+            raise ValueError("lambda_sdf_l1 > 0 needs the sparse points' normals ('pts_normal')")
+        sdf, sdf_grad = self.model.geometry(pts, with_grad=True, with_feature=False)
+        loss_sdf = (sdf.abs().mean() * weights).mean(dim=0)
+        n_gt = torch.nn.functional.normalize(normal, p=2, dim=-1)
+        n_pred = torch.nn.functional.normalize(sdf_grad, p=2, dim=-1)
+        loss_normal = (1.0 - torch.sum(n_pred * n_gt, dim=-1)).mean()
+        return loss_sdf, loss_normal
 
 
 class DynamicLossScaler:

@@ -490,8 +490,8 @@ __global__ void __launch_bounds__(256) k_geo_fwd(GeoArgs a, const float* __restr
 #pragma unroll 1
             for (int j = 0; j < 6; ++j) {
                 float x[3];
-                bool m[3];
-                offset_point(a, s.P, j, x, m);
+                bool inb[3];
+                offset_point(a, s.P, j, x, inb);
                 GeoEval<LJ, NH, false> e;
                 geo_eval<LJ, NH, false, mlp::kF32>(a, sm, lv, table, x, e);
                 if constexpr (ST) geo_store<LJ, NH, false>(store, a, tile, n_tiles, 1 + j, 0, e);
@@ -518,8 +518,8 @@ __global__ void __launch_bounds__(256) k_geo_fwd(GeoArgs a, const float* __restr
 #pragma unroll 1
         for (int j = 0; j < 6; ++j) {
             float x[3];
-            bool m[3];
-            offset_point(a, s.ps, j, x, m);
+            bool inb[3];
+            offset_point(a, s.ps, j, x, inb);
             GeoEval<LJ, NH, ST> e;  // the store keeps the position derivatives
             geo_eval<LJ, NH, ST, mlp::kF32>(a, sm, lv, table, x, e);
             if constexpr (ST)
@@ -775,16 +775,16 @@ __device__ __forceinline__ int face_corner(int i, int ax, int b) {
 template <int CL>
 __device__ __forceinline__ void cluster_point(const GeoArgs& a, const GeoSample& s, int q,
                                               float x[3]) {
-    bool m[3];
+    bool inb[3];
     if (CL == 0) {  // base, then FD 0..5
         if (q == 0) {
 #pragma unroll
             for (int d = 0; d < 3; ++d) x[d] = s.x0[d];
         } else {
-            offset_point(a, s.P, q - 1, x, m);
+            offset_point(a, s.P, q - 1, x, inb);
         }
     } else {        // curvature 0..5
-        offset_point(a, s.ps, q, x, m);
+        offset_point(a, s.ps, q, x, inb);
     }
 }
 
@@ -1161,8 +1161,8 @@ __device__ __forceinline__ void geo_bwd_body(
                 const float ggs = pick3(g_gs, j >> 1);
                 const float gf = (j & 1 ? -0.5f : 0.5f) * a.inv_eps * ggs;
                 float x[3];
-                bool m[3];
-                offset_point(a, s.ps, j, x, m);
+                bool inb[3];
+                offset_point(a, s.ps, j, x, inb);
                 GeoEval<LJ, NH, true> e;
                 geo_eval_b<LJ, NH, true, ST, mlp::kF32>(a, sm, lv, table, store, tile, n_tiles,
                                                         a.curv0 + j, a.dx_curv0 + j, x, e);
@@ -1171,7 +1171,7 @@ __device__ __forceinline__ void geo_bwd_body(
                                                       genc + (a.curv0 + j) * LJ * 128, gpos);
 #pragma unroll
                 for (int d = 0; d < 3; ++d)
-                    if (m[d]) g_ps[d] += gpos[d] * a.inv2r;
+                    if (inb[d]) g_ps[d] += gpos[d] * a.inv2r;
             }
             float g_t[3], gcr[3];
 #pragma unroll
@@ -1202,8 +1202,8 @@ __device__ __forceinline__ void geo_bwd_body(
             for (int j = 0; j < 6; ++j) {
                 const float gf = (j & 1 ? -0.5f : 0.5f) * a.inv_eps * pick3(gg, j >> 1);
                 float x[3];
-                bool m[3];
-                offset_point(a, s.P, j, x, m);
+                bool inb[3];
+                offset_point(a, s.P, j, x, inb);
                 GeoEval<LJ, NH, false> e;
                 geo_eval_b<LJ, NH, false, ST, mlp::kF32>(a, sm, lv, table, store, tile, n_tiles,
                                                          1 + j, 0, x, e);

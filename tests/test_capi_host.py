@@ -202,3 +202,106 @@ def test_losses_on_cpu_tensors():
     assert abs(terms["eikonal"].item() - 1.0) < 1e-7
     assert abs(terms["curvature"].item() - 0.3) < 1e-7
     assert abs(loss.item() - (2.5 + 0.1 + 0.15)) < 1e-6
+
+
+def _trainer(**model_keys):
+    from angelo_amd.system import NeuSTrainer
+    from angelo_amd.workloads import system_config
+    from angelo_amd.utils import to_config
+    mc = {"train_num_rays": 4, "num_samples_per_ray": 8}
+    mc.update(model_keys)
+    tr = NeuSTrainer(None, system_config(), to_config(mc))
+    tr.global_step = 10000
+    return tr
+
+
+def _loss_out():
+    return {"rays_valid_full": torch.tensor([[True], [True]]),
+            "comp_rgb_full": torch.full((2, 3), 0.5), "sdf_grad_samples": torch.tensor([[0.0, 1.0, 0.0]]),
+            "opacity": torch.tensor([[0.25], [0.75]]), "sdf_samples": torch.zeros(1),
+            "sdf_laplace_samples": torch.zeros(1, 1)}
+
+
+def test_mask_loss_counts_only_with_dataset_masks():
+    """systems/neus.py:143-145: the BCE mask term is weighted by lambda_mask
+    only when the dataset has masks; the sampler's all-ones fg_mask of a
+    mask-less dataset adds nothing."""
+    import math
+    tr = _trainer()
+    tr.cfg.loss["lambda_mask"] = 0.5
+    batch = {"rgb": torch.full((2, 3), 0.5), "fg_mask": torch.ones(2)}
+    base, terms = tr.losses(_loss_out(), batch)
+    assert "mask" not in terms
+    with_mask, terms = tr.losses(_loss_out(), dict(batch, has_mask=True))
+    bce = -(math.log(0.25) + math.log(0.75)) / 2     # BCE(opacity, 1)
+    assert abs(terms["mask"].item() - bce) < 1e-6
+    assert abs((with_mask - base).item() - 0.5 * bce) < 1e-5
+    tr.has_mask = True
+    assert "mask" in tr.losses(_loss_out(), batch)[1]
+
+
+def test_distortion_losses_raise_when_weighted():
+    """A non-zero distortion weight raises (the efficient distortion loss is
+    not built); the background's counts only with the learned background,
+    as at systems/neus.py:172."""
+    import pytest
+    batch = {"rgb": torch.zeros(2, 3)}
+    tr = _trainer()
+    tr.cfg.loss["lambda_distortion_bg"] = 0.1
+    tr.losses(_loss_out(), batch)          # no learned background: not counted
+    tr = _trainer(learned_background=True)
+    tr.cfg.loss["lambda_distortion_bg"] = 0.1
+    with pytest.raises(NotImplementedError):
+        tr.losses(_loss_out(), batch)
+    tr = _trainer()
+    tr.cfg.loss["lambda_distortion"] = 0.1
+    with pytest.raises(NotImplementedError):
+        tr.losses(_loss_out(), batch)
+
+
+def test_sparse_point_losses_closed_form():
+    """systems/neus.py:180-194 on a sphere SDF |p| - 0.5 (gradient p/|p|):
+    the l1 term is mean|sdf| times the mean confidence, the normal term 0
+    for the true normals and 1 for perpendicular ones; lambda_normal
+    defaults to lambda_sdf_l1."""
+    class Sphere:
+        def __call__(self, pts, with_grad=True, with_feature=False):
+            r = pts.norm(dim=-1)
+            return r - 0.5, pts / r[:, None]
+
+    class M:
+        geometry = Sphere()
+
+    tr = _trainer()
+    tr.model = M()
+    pts = torch.tensor([[1.0, 0.0, 0.0], [0.0, 0.25, 0.0], [0.0, 0.0, -0.5]])
+    w = torch.tensor([1.0, 0.5, 0.0])
+    batch = {"rgb": torch.full((2, 3), 0.5), "pts": pts, "pts_weights": w,
+             "pts_normal": pts * 3.0}
+    base = tr.losses(_loss_out(), {"rgb": batch["rgb"]})[0]
+    tr.cfg.loss["lambda_sdf_l1"] = 2.0
+    loss, terms = tr.losses(_loss_out(), batch)
+    l1 = (0.5 + 0.25 + 0.0) / 3 * (1.5 / 3)
+    assert abs(terms["sdf_l1"].item() - l1) < 1e-7
+    assert abs(terms["normal_cos"].item()) < 1e-7
+    assert abs((loss - base).item() - 2.0 * l1) < 1e-6
+    perp = torch.tensor([[0.0, 1.0, 0.0], [1.0, 0.0, 0.0], [0.0, 1.0, 0.0]])
+    _, terms = tr.losses(_loss_out(), dict(batch, pts_normal=perp))
+    assert abs(terms["normal_cos"].item() - 1.0) < 1e-7
+
+
+def test_oracle_losses_closed_form():
+    """The oracle's loss (oracle/render.py, systems/neus.py:130-159 with the
+    Neuralangelo weights 10 / 0.1 / 0.5) on hand-computed inputs: MSE over
+    the valid rays only, eikonal mean((|g| - 1)^2), curvature mean|laplace|."""
+    from oracle import render as orender
+    out = {"rays_valid": torch.tensor([[True], [False], [True]]),
+           "comp_rgb_full": torch.tensor([[0.5, 0.5, 0.5], [9.0, 9.0, 9.0], [1.0, 0.0, 0.0]]),
+           "grad": torch.tensor([[0.0, 3.0, 0.0], [0.6, 0.8, 0.0]]),
+           "laplace": torch.tensor([[0.1], [-0.3]])}
+    total, t = orender.losses(out, torch.zeros(3, 3))
+    mse = (3 * 0.25 + 1.0) / 6
+    assert abs(t["rgb_mse"].item() - mse) < 1e-7
+    assert abs(t["eikonal"].item() - 2.0) < 1e-6          # ((3-1)^2 + 0) / 2
+    assert abs(t["curvature"].item() - 0.2) < 1e-7
+    assert abs(total.item() - (10 * mse + 0.1 * 2.0 + 0.5 * 0.2)) < 1e-5
