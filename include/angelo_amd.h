@@ -417,6 +417,25 @@ int am_composite_bwd(const am_composite_desc* c, const float* sdf, const float* 
                      float* g_normal, float* g_rgb, float* g_inv_s, void* workspace,
                      size_t workspace_bytes, void* stream);
 
+/* am_composite_bwd without a workspace argument, for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  Its
+ * scratch is a static device array of the library (about 4.1 MiB per device,
+ * held until the process ends): n_rays <= 1,048,576, larger calls return
+ * AM_ERR_INVALID before any launch.  Same arguments (less the workspace) and
+ * bit-identical outputs.  The array is shared by every call on a device:
+ * calls that overlap on one device race; the stream must belong to the
+ * current device. */
+int am_composite_bwd_noalloc(const am_composite_desc* c, const float* sdf, const float* normal,
+                             const float* rays, const int32_t* ray_indices,
+                             const float* t_starts, const float* t_ends, const float* rgb,
+                             const float* inv_s, const int32_t* packed, int64_t n_rays,
+                             int64_t n, const float* trans, const float* opacity,
+                             const float* comp_normal_raw, const float* g_weights,
+                             const float* g_opacity, const float* g_depth,
+                             const float* g_comp_rgb, const float* g_comp_normal, float* g_sdf,
+                             float* g_normal, float* g_rgb, float* g_inv_s, void* stream);
+
 /* ------------------------------------------------------------------------ */
 /* Occupancy grid (nerfacc 0.3.3 OccupancyGrid, models/neus.py:63-68,90-111). */
 /* ------------------------------------------------------------------------ */

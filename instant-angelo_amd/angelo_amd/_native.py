@@ -121,6 +121,7 @@ _SIGS = {
     "am_composite_fwd": (ctypes.c_int, [_P] * 10 + [_I64, _I64] + [_P] * 7 + [_P]),
     "am_composite_bwd_workspace_size": (_SZ, [_I64, _I64]),
     "am_composite_bwd": (ctypes.c_int, [_P] * 10 + [_I64, _I64] + [_P] * 12 + [_P, _SZ, _P]),
+    "am_composite_bwd_noalloc": (ctypes.c_int, [_P] * 10 + [_I64, _I64] + [_P] * 12 + [_P]),
     "am_occ_update_workspace_size": (_SZ, [_I32, _I64]),
     "am_occ_update": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _F, _P, _P, _I64, _I32, _F, _F, _P, _P, _P, _SZ, _P]),
 }

@@ -9,6 +9,7 @@
 // R <- (1-a_j) R + g_j a_j, so no division by (1 - a_i) is needed.
 #include "am_common.h"
 #include "table_grad.h"
+#include <mutex>
 
 namespace am {
 
@@ -521,6 +522,79 @@ extern "C" int am_composite_bwd(const am_composite_desc* c, const float* sdf, co
     }
     AM_LAUNCH_CHECK();
     return colsum_f32(per_ray, n_rays, 1, g_inv_s, tmp, s);
+}
+
+// ---- am_composite_bwd without a caller workspace -------------------------
+// Its scratch (the per-ray partials of d loss / d inv_s and the slices of
+// their fixed-order sum) lives in a file-scope device array of
+// kStaticScratchBytes; each device holds its own copy, whose address
+// hipGetSymbolAddress gives once per device (kept in g_scratch_dev).  Every
+// partial and slice is written before it is read, so nothing is cleared;
+// calls that overlap on one device share the array and race.
+namespace am {
+
+static constexpr int64_t kStaticMaxRays = int64_t(1) << 20;
+static constexpr size_t kStaticScratchBytes = (size_t(4) << 20) + (size_t(80) << 10);
+static constexpr int kMaxDevices = 64;
+
+}  // namespace am
+
+static __device__ __align__(256) unsigned char scratch[am::kStaticScratchBytes];
+
+namespace am {
+
+static std::mutex g_scratch_mu;
+static void* g_scratch_dev[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device.
+static int static_scratch(void** out) {
+    int dev = -1;
+    AM_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) {
+        set_error("device %d beyond the %d the static scratch tracks", dev, kMaxDevices);
+        return AM_ERR_UNSUPPORTED;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_error("hipGetSymbolAddress(scratch) failed on device %d: %s", dev,
+                      hipGetErrorString(e));
+            return AM_ERR_HIP;
+        }
+        g_scratch_dev[dev] = p;
+    }
+    *out = g_scratch_dev[dev];
+    return AM_OK;
+}
+
+}  // namespace am
+
+extern "C" int am_composite_bwd_noalloc(
+    const am_composite_desc* c, const float* sdf, const float* normal, const float* rays,
+    const int32_t* ray_indices, const float* t_starts, const float* t_ends, const float* rgb,
+    const float* inv_s, const int32_t* packed, int64_t n_rays, int64_t n, const float* trans,
+    const float* opacity, const float* comp_normal_raw, const float* g_weights,
+    const float* g_opacity, const float* g_depth, const float* g_comp_rgb,
+    const float* g_comp_normal, float* g_sdf, float* g_normal, float* g_rgb, float* g_inv_s,
+    void* stream) {
+    AM_CHECK_ARG(n_rays >= 0 && n >= 0, "negative size");
+    const size_t need = am_composite_bwd_workspace_size(n_rays, n);
+    if (n_rays > am::kStaticMaxRays || need > am::kStaticScratchBytes) {
+        am::set_error("am_composite_bwd_noalloc takes at most %lld rays (got %lld): its static "
+                      "scratch holds %zu bytes, the call needs %zu",
+                      (long long)am::kStaticMaxRays, (long long)n_rays, am::kStaticScratchBytes,
+                      need);
+        return AM_ERR_INVALID;
+    }
+    void* ws = nullptr;
+    const int rc = am::static_scratch(&ws);
+    if (rc != AM_OK) return rc;
+    return am_composite_bwd(c, sdf, normal, rays, ray_indices, t_starts, t_ends, rgb, inv_s,
+                            packed, n_rays, n, trans, opacity, comp_normal_raw, g_weights,
+                            g_opacity, g_depth, g_comp_rgb, g_comp_normal, g_sdf, g_normal, g_rgb,
+                            g_inv_s, ws, am::kStaticScratchBytes, stream);
 }
 
 extern "C" int am_composite_density_fwd(const float* density, const float* t_starts,
