@@ -166,9 +166,14 @@ def check(rc, what):
 
 
 def ptr(t):
-    """Device/host pointer of a tensor (None -> NULL)."""
+    """Device pointer of a tensor (None -> NULL).  Every buffer the kernels
+    read or write must be device memory: a host tensor's address handed to
+    a kernel faults the GPU, so it is refused here."""
     if t is None:
         return None
+    if not t.is_cuda:
+        raise NativeError("angelo_amd kernels take device tensors (got a CPU tensor of shape "
+                          f"{tuple(t.shape)}); move the module or data to the GPU first")
     return ctypes.c_void_p(t.data_ptr())
 
 

@@ -10,15 +10,18 @@ chunks, runs them, and scatters the results back.  A request is admitted when
 its result buffers fit the HBM budget next to those already admitted
 (admission by footprint); a finished request frees its share.
 
-Graph replay (use_graphs=True) is switched off: a run of the captured form
-left the device with a memory fault whose cause is not established, so the
-option raises until it is.  Its design: the SDF batch is one captured graph
-over a fixed point buffer; the render batch marches eagerly (the sample count
-is data-dependent), then replays the graph captured for the sample-count
-bucket (`sample_bucket` times the smallest power of two that holds the
-count) over sample positions, fused SDF field (base + FD normal), fused
-radiance and compositing, with padded samples outside every ray's range.
-Steps run eagerly on the device-resident buffers.
+Graph replay (use_graphs=True): the SDF batch is one captured graph over a
+fixed point buffer; the render batch marches eagerly (the sample count is
+data-dependent), then replays the graph captured for the sample-count bucket
+(`sample_bucket` times the smallest power of two that holds the count) over
+sample positions, fused SDF field (base + FD normal), fused radiance and
+compositing, with padded samples outside every ray's range.  The captured
+regions hold only kernel launches on buffers that live as long as the
+graph: the fused field runs in the caller's order there (the spatial
+order's library sort clears its state with memset nodes, which must not be
+captured), the forward keeps no backward buffers (no grad mode), and the
+table / weights are those of the warm-up run before capture.  Models with
+the learned background render eagerly.
 Per-ray and per-point results do not depend on what else is in the batch
 (every kernel computes a ray or a point from its own inputs in a fixed
 order), so a request renders the same alone and in a mix.
@@ -92,10 +95,6 @@ class FieldServer:
     def __init__(self, model, rays_per_step=1 << 16, sdf_chunks_per_step=4,
                  hbm_budget_bytes=32 << 30, sample_bucket=1 << 18, use_graphs=False,
                  background=(1.0, 1.0, 1.0)):
-        if use_graphs:
-            raise NotImplementedError(
-                "graph replay of the serving steps is switched off: a captured run left a "
-                "device memory fault whose cause is not established")
         self.model = model
         self.dev = model.geometry.grid_module.params.device
         self.R = int(rays_per_step)
@@ -172,7 +171,9 @@ class FieldServer:
         """Post-march render of the batch (captured per sample bucket)."""
         m = self.model
         pts, mid, dists = fn.sample_points(rays, ridx, t0, t1)
-        sdf, _, normal, feature, _ = m.geometry.fused(pts, with_grad=True, with_laplace=False)
+        # the caller's order: the spatial order's sort may not run in a graph
+        sdf, _, normal, feature, _ = m.geometry.fused(pts, with_grad=True, with_laplace=False,
+                                                      order=False if self.use_graphs else None)
         rgb = m.texture.fused(feature, normal, rays, ridx)
         cd = nat.CompositeDesc()
         cd.cos_anneal_ratio = float(m.cos_anneal_ratio)
@@ -209,6 +210,9 @@ class FieldServer:
         if g is None:
             g = self._capture_render(nb)
         s = g["static"]
+        # the static buffers hold the batch: n samples of R rays
+        assert rays.shape == s["rays"].shape and packed.shape == s["packed"].shape
+        assert n <= s["ridx"].shape[0] and ridx.dtype == s["ridx"].dtype == torch.int32
         s["rays"].copy_(rays)
         s["ridx"].zero_()
         s["t0"].zero_()
@@ -240,7 +244,9 @@ class FieldServer:
 
     # ---- SDF batch --------------------------------------------------------
     def _sdf_stage(self):
-        return self.model.geometry.fused(self.s_pts, with_grad=False)[0]
+        # lattice points are in lattice order already (coherent gathers), and
+        # the spatial order's sort may not run in a graph
+        return self.model.geometry.fused(self.s_pts, with_grad=False, order=False)[0]
 
     def _sdf_batch(self, n):
         if not self.use_graphs:

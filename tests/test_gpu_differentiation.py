@@ -237,8 +237,8 @@ def test_store_and_recompute_backward_identical_at_2pow20_samples(dev):
     cfg = helpers.model_config(L=16, log2T=16, base=32, scale=GRID["scale"])
     model = helpers.randomize_(angelo_amd.make("neus", cfg), seed=71)
     model.train()
-    model.update_step(0, 30000)
     model = model.to(dev)
+    model.update_step(0, 30000)
     geo = model.geometry
     n = 1 << 20
     g = torch.Generator().manual_seed(72)

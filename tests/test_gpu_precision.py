@@ -46,8 +46,9 @@ def _model(dev):
     cfg = helpers.model_config(**SHAPES)
     model = helpers.randomize_(angelo_amd.make("neus", cfg), seed=11)
     model.train()
+    model = model.to(dev)
     model.update_step(0, 30000)
-    return model.to(dev), cfg
+    return model, cfg
 
 
 def _err(a, b):

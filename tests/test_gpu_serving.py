@@ -2,8 +2,7 @@
 
 * Every request's result is bit-identical alone and in a mix (per-ray and
   per-point computations do not depend on the rest of the batch).
-* Graph replay gives the same results as eager execution (skipped: graph
-  replay is switched off, see FieldServer).
+* Graph replay gives the same results as eager execution, bit for bit.
 * Admission by footprint never holds more than the HBM budget (beyond a
   single request that exceeds it on its own) and serves every request.
 * A served render equals the eval-mode NeuSModel.forward of the same rays
@@ -23,9 +22,10 @@ def _model(dev, grid_prune=False):
     cfg = helpers.model_config(L=8, log2T=15, base=16, scale=1.5, grid_prune=grid_prune,
                                step=2.0 / 256 if grid_prune else None)
     model = helpers.randomize_(angelo_amd.make("neus", cfg), seed=81)
+    model = model.to(dev)
     model.update_step(0, 30000)
     model.eval()
-    return model.to(dev)
+    return model
 
 
 def _requests(seed=82):
@@ -56,12 +56,7 @@ def _serve(model, reqs, **kw):
     return srv
 
 
-GRAPHS_OFF = ("graph replay of the serving steps is switched off: a captured run left a device "
-              "memory fault whose cause is not established")
-
-
-@pytest.mark.parametrize("use_graphs", [False, pytest.param(True, marks=pytest.mark.skip(
-    reason=GRAPHS_OFF))])
+@pytest.mark.parametrize("use_graphs", [False, True])
 def test_results_identical_alone_and_in_mix(dev, use_graphs):
     model = _model(dev)
     mix = _clone_reqs(_requests())
@@ -74,7 +69,6 @@ def test_results_identical_alone_and_in_mix(dev, use_graphs):
             assert torch.equal(r.result[k], m.result[k]), (r.kind, k)
 
 
-@pytest.mark.skip(reason=GRAPHS_OFF)
 def test_graph_replay_matches_eager(dev):
     model = _model(dev, grid_prune=True)
     og = model.occupancy_grid
@@ -141,10 +135,9 @@ def test_served_render_with_learned_background(dev):
     from angelo_amd import serving
     import test_gpu_background as tb
     cfg = tb._bg_model_config()
-    model = helpers.randomize_(angelo_amd.make("neus", cfg), seed=83)
+    model = helpers.randomize_(angelo_amd.make("neus", cfg), seed=83).to(dev)
     model.update_step(0, 30000)
     model.eval()
-    model = model.to(dev)
     reqs = _clone_reqs(_requests(86))[:2]
     _serve(model, reqs, use_graphs=False)
     for r in reqs:
@@ -168,3 +161,50 @@ def test_latency_stats_and_poisson_mix():
         r.finished = r.arrival + 0.01 * (i + 1)
     st = serving.latency_stats(reqs)
     assert st["n"] == 40 and st["p50_s"] <= st["p99_s"]
+
+
+def test_full_size_render_and_lattice_alone_and_in_mix_with_graphs(dev):
+    """The serving workload's sizes on a field of the Neuralangelo training
+    workload's shape (16 levels, 2^21 rows per level, base 32, occupancy
+    grid 128^3 pre-warmed from the analytic scene, step 2/1024, at most 512
+    samples per ray): one 1024^2 render and one 256^3 SDF lattice, served
+    with graph replay alone and in a mix with other requests, give
+    bit-identical results; both match the eager server."""
+    import angelo_amd
+    from angelo_amd import scenes, serving
+    from angelo_amd.scenes import look_at_c2w
+    from angelo_amd.workloads import NEURALANGELO_SCALE, model_config
+    print("serving at full size: 1024^2 render + 256^3 lattice, graphs", flush=True)
+    cfg = model_config(16, 21, 32, NEURALANGELO_SCALE, rays=8192, samples_per_ray=512,
+                       step=2.0 / 1024, cap=512, grid_prune=True)
+    torch.manual_seed(31)
+    model = helpers.randomize_(angelo_amd.make("neus", cfg), seed=31).to(dev)
+    model.eval()
+    og = model.occupancy_grid
+    og.binary.copy_(scenes.prewarm_occupancy(scenes.sdf_union, og.resolution, 1.0, 1.5, dev))
+    eye = torch.tensor([1.3, -1.7, 1.2])
+    big = [serving.Request("render", 0.0, c2w=look_at_c2w(eye / eye.norm() * 2.5), W=1024,
+                           H=1024, fov=40.0),
+           serving.Request("sdf_grid", 0.0, resolution=256)]
+    others = _clone_reqs(_requests(87))
+
+    def serve(reqs, use_graphs):
+        srv = serving.FieldServer(model, rays_per_step=1 << 17, sdf_chunks_per_step=8,
+                                  use_graphs=use_graphs)
+        srv.run(reqs, realtime=False)
+        assert all(r.finished is not None for r in reqs)
+        return srv
+
+    alone = _clone_reqs(big)
+    srv = serve(alone, True)
+    assert srv._render_graphs and srv._sdf_graph is not None
+    mix = _clone_reqs(others[:3]) + _clone_reqs(big) + _clone_reqs(others[3:])
+    serve(mix, True)
+    eager = _clone_reqs(big)
+    serve(eager, False)
+    for a, m, e in zip(alone, mix[3:5], eager):
+        assert a.kind == m.kind == e.kind
+        for k in a.result:
+            assert torch.equal(a.result[k], m.result[k]), (a.kind, k)
+            assert torch.equal(a.result[k], e.result[k]), (a.kind, k)
+    assert float(alone[0].result["opacity"].max()) > 0.5     # the render sees the scene
