@@ -128,12 +128,16 @@ int am_hashgrid_bwd_ordered(const am_grid_desc* g, const int32_t* order, const f
 /* VolumeSDF grad_type (models/geometry.py:214-234) */
 #define AM_GRAD_FD 0       /* 'finite_difference': 6 central differences */
 #define AM_GRAD_ANALYTIC 1 /* 'analytic': d sdf / d x through the encoding and
-                              the MLP (one hidden layer), backward with the
-                              second-order terms of the gradient output */
+                              the MLP (one or two hidden layers), backward
+                              with the second-order terms of the gradient
+                              output */
 
 typedef struct am_mlp_desc {
-    int32_t n_in;            /* <= 64 */
-    int32_t n_out;           /* <= 64 */
+    int32_t n_in;            /* <= 64 (stand-alone MLP); the fused SDF MLP takes
+                                3 + 2 L, the fused radiance MLP <= 96 */
+    int32_t n_out;           /* <= 64 (stand-alone MLP); the fused SDF MLP takes
+                                <= 16, or <= 80 with two hidden layers (the 65
+                                outputs of the Neuralangelo configurations) */
     int32_t n_hidden_layers; /* 1..4 */
     int32_t n_neurons;       /* 64 */
     int32_t hidden_act;      /* AM_ACT_RELU or AM_ACT_SOFTPLUS */

@@ -107,16 +107,17 @@ class VolumeSDF(BaseImplicitGeometry):
 
     def _fused_ok(self):
         enc = self.encoding
-        fd_or_analytic = (self.grad_type == "finite_difference"
-                          or (self.grad_type == "analytic" and self.network.n_hidden_layers == 1))
-        return (fd_or_analytic and enc.include_xyz
+        # up to 16 outputs, or up to 80 with two hidden layers (the 65 of
+        # the Neuralangelo configurations)
+        max_out = 80 if self.network.n_hidden_layers == 2 else 16
+        return (self.grad_type in ("finite_difference", "analytic") and enc.include_xyz
                 and isinstance(self.grid_module, HashGrid)
                 and self.grid_module.n_features_per_level == 2
                 and self.grid_module.n_levels <= 16
                 and self.network.n_hidden_layers in (1, 2)
                 and self.network._kernel_out_act == "none"
                 and self.network.output_activation_name == "none"
-                and self.n_output_dims <= 16
+                and self.n_output_dims <= max_out
                 and "sdf_activation" not in self.config and "feature_activation" not in self.config)
 
     def fused(self, points, with_grad=True, with_laplace=False, randn=None, order=None):
@@ -127,9 +128,10 @@ class VolumeSDF(BaseImplicitGeometry):
         caller's order)."""
         if not self._fused_ok():
             raise NotImplementedError(
-                "fused VolumeSDF needs grad_type=finite_difference (or analytic with one hidden "
-                "layer), include_xyz, a 2-feature hash grid with <= 16 levels and a 1-2 "
-                "hidden-layer VanillaMLP without output/sdf/feature activations")
+                "fused VolumeSDF needs grad_type finite_difference or analytic, include_xyz, a "
+                "2-feature hash grid with <= 16 levels and a 1-2 hidden-layer VanillaMLP "
+                "(<= 16 outputs, <= 80 with two hidden layers) without output/sdf/feature "
+                "activations")
         pts = points.reshape(-1, 3)
         if with_laplace and randn is None:
             randn = torch.randn_like(pts)

@@ -93,24 +93,32 @@ class NeuSTrainer:
             if self.C(L.get(k, 0.0)) > 0:
                 raise NotImplementedError(f"{k} > 0 is not part of this build")
         if self.C(L.get("lambda_sdf_l1", 0.0)) > 0:
-            sdf_l1, normal_cos = self.sparse_point_losses(batch)
-            terms["sdf_l1"], terms["normal_cos"] = sdf_l1, normal_cos
+            w_normal = self.C(L.get("lambda_normal", L.lambda_sdf_l1))
+            sdf_l1, normal_cos = self.sparse_point_losses(batch, with_normal=w_normal > 0)
+            terms["sdf_l1"] = sdf_l1
             loss = loss + sdf_l1 * self.C(L.lambda_sdf_l1)
-            loss = loss + normal_cos * self.C(L.get("lambda_normal", L.lambda_sdf_l1))
+            if normal_cos is not None:
+                terms["normal_cos"] = normal_cos
+                loss = loss + normal_cos * w_normal
         return loss, terms
 
-    def sparse_point_losses(self, batch):
+    def sparse_point_losses(self, batch, with_normal=True):
         """systems/neus.py:180-194: the SDF at the dataset's sparse points
         (l1 towards 0, scaled by their confidence) and the cosine distance of
-        its gradient to the points' normals.  As there, the l1 term is the
-        mean |sdf| (F.l1_loss) times the weights, averaged over dim 0."""
-        pts, normal, weights = batch["pts"], batch["pts_normal"], batch["pts_weights"]
-        if pts.numel() == 0:
+        its gradient to the points' normals (None when not asked for: the
+        Neuralangelo configurations weight it 0, lambda_normal).  As there,
+        the l1 term is the mean |sdf| (F.l1_loss) times the weights, averaged
+        over dim 0."""
+        pts, weights = batch.get("pts"), batch.get("pts_weights")
+        normal = batch.get("pts_normal")
+        if pts is None or pts.numel() == 0:
             raise ValueError("lambda_sdf_l1 > 0 needs the batch's sparse points ('pts')")
-        if normal.numel() == 0:
-            raise ValueError("lambda_sdf_l1 > 0 needs the sparse points' normals ('pts_normal')")
+        if with_normal and (normal is None or normal.numel() == 0):
+            raise ValueError("lambda_normal > 0 needs the sparse points' normals ('pts_normal')")
         sdf, sdf_grad = self.model.geometry(pts, with_grad=True, with_feature=False)
         loss_sdf = (sdf.abs().mean() * weights).mean(dim=0)
+        if not with_normal:
+            return loss_sdf, None
         n_gt = torch.nn.functional.normalize(normal, p=2, dim=-1)
         n_pred = torch.nn.functional.normalize(sdf_grad, p=2, dim=-1)
         loss_normal = (1.0 - torch.sum(n_pred * n_gt, dim=-1)).mean()

@@ -45,7 +45,7 @@ class VolumeRadiance(nn.Module):
                 and (self.color_activation or "").lower() == "sigmoid"
                 and self.network.output_activation_name == "none"
                 and self.network.n_hidden_layers in (1, 2)
-                and self.n_input_dims <= 48
+                and self.n_input_dims <= 96
                 and features.shape[-1] + enc.n_output_dims + 3 == self.network.dim_in)
 
     base_color = 0

@@ -172,3 +172,117 @@ def scheduled_value(value, global_step, current_epoch=0):
     cur = global_step if isinstance(end_step, int) else current_epoch
     return start_value + (end_value - start_value) * max(
         min(1.0, (cur - start_step) / (end_step - start_step)), 0.0)
+
+
+# ---- config files (utils/misc.py:6-32) ------------------------------------
+# The resolvers the reference registers with OmegaConf (utils/misc.py:7-13).
+_RESOLVERS = {
+    "calc_exp_lr_decay_rate": lambda factor, n: factor ** (1.0 / n),
+    "add": lambda a, b: a + b,
+    "sub": lambda a, b: a - b,
+    "mul": lambda a, b: a * b,
+    "div": lambda a, b: a / b,
+    "idiv": lambda a, b: a // b,
+    "basename": lambda p: __import__("os").path.basename(p),
+}
+
+
+def _merge(dst, src):
+    for k, v in src.items():
+        if isinstance(v, dict) and isinstance(dst.get(k), dict):
+            _merge(dst[k], v)
+        else:
+            dst[k] = v
+    return dst
+
+
+def _scalar(text):
+    """A YAML scalar written inside an interpolation or on the command line."""
+    import yaml
+    return yaml.safe_load(text) if text.strip() else text
+
+
+def _resolve(root, value, depth=0):
+    """OmegaConf interpolation of a string: ${a.b.c} (absolute key paths) and
+    ${resolver:arg, ...} with the resolvers above; a string that is exactly
+    one interpolation takes the referenced value's type."""
+    if not isinstance(value, str) or "${" not in value:
+        return value
+    if depth > 32:
+        raise ValueError(f"interpolation too deep in {value!r}")
+
+    def find_close(s, i):
+        level = 0
+        for j in range(i, len(s)):
+            if s.startswith("${", j):
+                level += 1
+            elif s[j] == "}":
+                level -= 1
+                if level == 0:
+                    return j
+        raise ValueError(f"unbalanced interpolation in {s!r}")
+
+    def split_args(s):
+        out, level, cur = [], 0, ""
+        for ch in s:
+            if ch == "{":
+                level += 1
+            elif ch == "}":
+                level -= 1
+            if ch == "," and level == 0:
+                out.append(cur)
+                cur = ""
+            else:
+                cur += ch
+        out.append(cur)
+        return out
+
+    def evaluate(body):
+        head, sep, rest = body.partition(":")
+        if sep and head in _RESOLVERS:
+            args = [_resolve(root, _scalar(a.strip()) if "${" not in a else a.strip(), depth + 1)
+                    for a in split_args(rest)]
+            return _RESOLVERS[head](*args)
+        node = root
+        for part in body.split("."):
+            node = node[part]
+        return _resolve(root, node, depth + 1)
+
+    i = value.index("${")
+    j = find_close(value, i)
+    if i == 0 and j == len(value) - 1:
+        return evaluate(value[2:-1])
+    out = value[:i] + str(evaluate(value[i + 2:j])) + value[j + 1:]
+    return _resolve(root, out, depth + 1)
+
+
+def _resolve_all(root, node):
+    if isinstance(node, dict):
+        for k in list(node.keys()):
+            node[k] = _resolve_all(root, node[k])
+        return node
+    if isinstance(node, list):
+        return [_resolve_all(root, v) for v in node]
+    return _resolve(root, node)
+
+
+def load_config(*yaml_files, cli_args=()):
+    """utils/misc.py:27-32 without OmegaConf: the YAML files merged in order,
+    then `key.path=value` overrides, then every interpolation resolved (the
+    reference's resolvers add, sub, mul, div, idiv, basename,
+    calc_exp_lr_decay_rate).  Mandatory values left as '???' stay strings.
+    Returns a Config (attribute access and .get, as the modules use)."""
+    import yaml
+    conf = {}
+    for f in yaml_files:
+        with open(f) as fh:
+            _merge(conf, yaml.safe_load(fh) or {})
+    for arg in cli_args:
+        key, _, val = arg.partition("=")
+        node = conf
+        parts = key.split(".")
+        for p in parts[:-1]:
+            node = node.setdefault(p, {})
+        node[parts[-1]] = _scalar(val)
+    _resolve_all(conf, conf)
+    return to_config(conf)

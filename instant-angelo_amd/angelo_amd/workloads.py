@@ -86,6 +86,62 @@ def model_config(levels, log2T, base, scale, rays, samples_per_ray, step=None, c
     return cfg
 
 
+def neuralangelo_configs(radius=1.5, train_num_rays=256, log2_hashmap_size=19):
+    """The model and system sections of configs/neuralangelo-colmap_sparse.yaml
+    (:16-150) with its interpolations resolved: radius 1.5, 512 + 256
+    samples per ray, dynamic ray sampling up to 8192 rays, analytic-gradient
+    VolumeSDF with 65 outputs and two hidden layers on a 16-level 2^19-row
+    progressive hash grid, VolumeDualColor texture (71 + 16 inputs), the
+    learned background (VolumeDensity + VolumeRadiance), the sparse-point,
+    colour, eikonal and curvature losses, AdamW.  Returns (model, system)."""
+    band = {"otype": "ProgressiveBandHashGrid", "n_levels": 16, "n_features_per_level": 2,
+            "log2_hashmap_size": log2_hashmap_size, "base_resolution": 32,
+            "per_level_scale": NEURALANGELO_SCALE, "include_xyz": True, "start_level": 4,
+            "start_step": 5000, "update_steps": 1000}
+    mlp = lambda nh, **kw: dict({"otype": "VanillaMLP", "activation": "ReLU",
+                                 "output_activation": "none", "n_neurons": 64,
+                                 "n_hidden_layers": nh}, **kw)
+    model = to_config({
+        "name": "neus", "radius": radius, "num_samples_per_ray": 512,
+        "train_num_rays": train_num_rays, "max_train_num_rays": 8192, "grid_prune": True,
+        "grid_prune_occ_thre": 0.001, "dynamic_ray_sampling": True,
+        "batch_image_sampling": True, "randomized": True, "ray_chunk": 2048,
+        "cos_anneal_end": 20000, "learned_background": True, "background_color": "random",
+        "variance": {"init_val": 0.3, "modulate": False},
+        "geometry": {
+            "name": "volume-sdf", "radius": radius, "feature_dim": 65, "grad_type": "analytic",
+            "finite_difference_eps": "progressive",
+            "isosurface": {"method": "mc", "resolution": 512, "chunk": 2097152,
+                           "threshold": 0.001},
+            "xyz_encoding_config": dict(band),
+            "mlp_network_config": mlp(2, sphere_init=True, sphere_init_radius=0.5,
+                                      weight_norm=True)},
+        "texture": {
+            "name": "volume-dual-color", "input_feature_dim": 65 + 6,
+            "diffuse_warmup_steps": 5000,
+            "dir_encoding_config": {"otype": "SphericalHarmonics", "degree": 4},
+            "mlp_network_config": mlp(2), "color_activation": "sigmoid"},
+        "num_samples_per_ray_bg": 256,
+        "geometry_bg": {
+            "name": "volume-density", "radius": radius, "feature_dim": 8,
+            "density_activation": "trunc_exp", "density_bias": -1, "isosurface": None,
+            "xyz_encoding_config": dict(band), "mlp_network_config": mlp(1)},
+        "texture_bg": {
+            "name": "volume-radiance", "input_feature_dim": 8,
+            "dir_encoding_config": {"otype": "SphericalHarmonics", "degree": 4},
+            "mlp_network_config": mlp(2), "color_activation": "sigmoid"},
+    })
+    system = to_config({
+        "loss": {"lambda_sdf_l1": [0, 1, 0, 20000], "lambda_normal": 0.0,
+                 "lambda_rgb_mse": 10.0, "lambda_rgb_l1": 0.0, "lambda_mask": 0.0,
+                 "lambda_eikonal": 0.1, "lambda_curvature": [0, 0, 0.5, 5000],
+                 "lambda_sparsity": 0.0, "lambda_distortion": 0.0, "lambda_distortion_bg": 0.0,
+                 "lambda_opaque": 0.0, "sparsity_scale": 1.0},
+        "optimizer": {"lr": 0.01, "betas": (0.9, 0.99), "eps": 1e-15, "variance_lr": 0.001},
+    })
+    return model, system
+
+
 def system_config():
     """Loss weights of configs/neuralangelo-colmap_sparse.yaml:121-133 (the
     zero-weighted terms omitted) and the AdamW settings of :134-150."""
@@ -135,6 +191,16 @@ WORKLOADS["neuralangelo-bg"] = dict(WORKLOADS["neuralangelo-fp32"], learned_back
 # finite-difference step of that band.
 WORKLOADS["neuralangelo-fp32-l4"] = dict(WORKLOADS["neuralangelo-fp32"], start_level=4,
                                          global_step=0)
+# The reference's own configuration (configs/neuralangelo-colmap_sparse.yaml,
+# neuralangelo_configs): radius 1.5, render step 1.732 * 3 / 512 with no
+# per-ray cap, analytic gradients, 65 SDF outputs and two hidden layers,
+# 2^19 rows per level, VolumeDualColor, the learned background; 8192 rays a
+# step (its max_train_num_rays, held fixed instead of the dynamic count).
+WORKLOADS["neuralangelo-config"] = dict(levels=16, log2T=19, base=32, scale=NEURALANGELO_SCALE,
+                                        rays=8192, samples_per_ray=512, step=None, cap=0,
+                                        grid_prune=True, grad_type="analytic",
+                                        learned_background=True, feature_dim=65,
+                                        geo_hidden=2, radius=1.5, reference_config=True)
 
 
 class Workload:
@@ -149,14 +215,23 @@ class Workload:
             spec["rays"] = rays
         self.name, self.spec, self.device = name, spec, device
         self.uniform = not spec["grid_prune"]
-        mcfg = model_config(spec["levels"], spec["log2T"], spec["base"], spec["scale"], spec["rays"],
-                            spec["samples_per_ray"], spec["step"], spec["cap"], spec["grid_prune"],
-                            seed=seed, precision=spec.get("precision", "fp32"),
-                            grad_type=spec.get("grad_type", "finite_difference"),
-                            learned_background=spec.get("learned_background", False),
-                            start_level=spec.get("start_level"))
+        if spec.get("reference_config"):
+            mcfg, scfg = neuralangelo_configs(train_num_rays=spec["rays"],
+                                              log2_hashmap_size=spec["log2T"])
+            mcfg.dynamic_ray_sampling = False
+            mcfg.max_train_num_rays = spec["rays"]
+            mcfg.precision = spec.get("precision", "fp32")
+        else:
+            mcfg = model_config(spec["levels"], spec["log2T"], spec["base"], spec["scale"],
+                                spec["rays"], spec["samples_per_ray"], spec["step"], spec["cap"],
+                                spec["grid_prune"], seed=seed,
+                                precision=spec.get("precision", "fp32"),
+                                grad_type=spec.get("grad_type", "finite_difference"),
+                                learned_background=spec.get("learned_background", False),
+                                start_level=spec.get("start_level"))
+            scfg = system_config()
         torch.manual_seed(seed)
-        self.mcfg, self.scfg = mcfg, system_config()
+        self.mcfg, self.scfg = mcfg, scfg
         self.model = make("neus", mcfg).to(device)
         self.model.train()
         self.trainer = NeuSTrainer(self.model, self.scfg, mcfg)
@@ -184,7 +259,8 @@ class Workload:
                                            background_color=mcfg.background_color)
             self.sample_gen = torch.Generator(device=device).manual_seed(seed + 17)
             og = self.model.occupancy_grid
-            binary = scenes.prewarm_occupancy(scenes.sdf_union, og.resolution, 1.0, band_cells, device)
+            binary = scenes.prewarm_occupancy(scenes.sdf_union, og.resolution,
+                                              float(mcfg.radius), band_cells, device)
             og.binary.copy_(binary)
             og.occs.copy_(binary.flatten().float())
         self.model.update_step(0, global_step - 1)

@@ -10,7 +10,11 @@ static inline int field_blocks(int64_t n) {
     return persistent_blocks((n + 15) / 16, 4, kFieldBlocksMax);
 }
 
-static inline int check_mlp_common(const am_mlp_desc* m) {
+// Widest output of the fused SDF MLP (five 16-row output tiles: the SDF and
+// up to 79 features, e.g. the 65 outputs of the Neuralangelo configurations).
+constexpr int kGeoMaxOut = 80;
+
+static inline int check_mlp_common(const am_mlp_desc* m, int max_out = 16) {
     AM_CHECK_ARG(m != nullptr, "mlp descriptor is NULL");
     AM_CHECK_ARG(m->n_neurons == 64, "n_neurons %d: only 64 is implemented", m->n_neurons);
     AM_CHECK_ARG(m->n_hidden_layers == 1 || m->n_hidden_layers == 2,
@@ -19,7 +23,8 @@ static inline int check_mlp_common(const am_mlp_desc* m) {
                      m->hidden_act == AM_ACT_NONE,
                  "hidden_act %d", m->hidden_act);
     AM_CHECK_ARG(m->output_act == AM_ACT_NONE, "fused field kernels need output_act none");
-    AM_CHECK_ARG(m->n_out >= 1 && m->n_out <= 16, "n_out %d out of [1,16]", m->n_out);
+    AM_CHECK_ARG(m->n_out >= 1 && m->n_out <= max_out, "n_out %d out of [1,%d]", m->n_out,
+                 max_out);
     AM_CHECK_ARG(m->precision == AM_PREC_FP32 || m->precision == AM_PREC_MIXED, "precision %d",
                  m->precision);
     return AM_OK;

@@ -49,6 +49,9 @@ struct GeoArgs {
     int n_in, n_out, act;
     float beta;
     int w1, b1, w2, b2, wo, bo, ld1, lds_floats;  // LDS offsets (floats)
+    // output tiles of 16 rows (n_out <= 16 * out_tiles); the rows of tiles
+    // >= 1 are features only, read by the base evaluation alone
+    int out_tiles;
     int64_t pw1, pb1, pw2, pb2, pwo, pbo, n_params;
     int n_evals;  // 1, 7 or 13
     int prec;     // AM_PREC_FP32 / AM_PREC_MIXED
@@ -107,11 +110,12 @@ __device__ __forceinline__ void geo_load_lds(const GeoArgs& a, const float* __re
         }
         for (int q = threadIdx.x; q < 64; q += blockDim.x) sm[a.b2 + q] = params[a.pb2 + q];
     }
-    for (int q = threadIdx.x; q < 16 * 64; q += blockDim.x) {
+    for (int q = threadIdx.x; q < a.out_tiles * 16 * 64; q += blockDim.x) {
         const int o = q >> 6, c = q & 63;
         sm[a.wo + o * 65 + c] = o < a.n_out ? params[a.pwo + (int64_t)o * 64 + c] : 0.0f;
     }
-    for (int q = threadIdx.x; q < 16; q += blockDim.x) sm[a.bo + q] = q < a.n_out ? params[a.pbo + q] : 0.0f;
+    for (int q = threadIdx.x; q < a.out_tiles * 16; q += blockDim.x)
+        sm[a.bo + q] = q < a.n_out ? params[a.pbo + q] : 0.0f;
     // unrolled: every kernel-argument access has a constant offset (a
     // per-thread index into the by-value struct would copy it to scratch)
 #pragma unroll
@@ -368,12 +372,33 @@ using TableT = typename std::conditional<PR == mlp::kF32, float2, __half2>::type
 template <int PR>
 using PairV = typename std::conditional<PR == mlp::kF32, Vals<2>, ValsH<2>>::type;
 
-// sdf and feature outputs of the base evaluation.
+// sdf and feature outputs of the base evaluation: output tile 0 from e.out,
+// the feature rows of tiles >= 1 (n_out > 16) from the last hidden layer.
 template <int LJ, int NH, bool DX>
-__device__ __forceinline__ void write_base(const GeoArgs& a, const GeoEval<LJ, NH, DX>& e,
-                                           const GeoSample& s, int64_t n, bool valid, int k,
-                                           int fdim, float* __restrict__ sdf,
-                                           float* __restrict__ feature) {
+__device__ __forceinline__ void write_base(const GeoArgs& a, const float* sm,
+                                           const GeoEval<LJ, NH, DX>& e, const GeoSample& s,
+                                           int64_t n, bool valid, int k, int fdim,
+                                           float* __restrict__ sdf, float* __restrict__ feature) {
+    // (wide outputs come with two hidden layers: make_geo_args rejects them
+    // with one, whose kernels keep their registers for the narrow form)
+    if (NH == 2 && feature && a.out_tiles > 1) {   // every lane takes part in the MFMA
+        v4f h[4];
+        const v4f* zl = NH == 2 ? e.z2 : e.z1;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) h[t] = mlp::act4(a.act, zl[t], a.beta);
+#pragma unroll 1
+        for (int t = 1; t < a.out_tiles; ++t) {
+            v4f o = mlp::bias_tile(sm + a.bo, t, a.n_out);
+            mlp::mm_fwd<16, 1, mlp::FragV, mlp::kF32>(sm + a.wo + 16 * t * 65, 65, mlp::FragV{h},
+                                                      &o);
+            if (valid)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int row = 16 * t + 4 * k + r;
+                    if (row < a.n_out) feature[n * fdim + row] = o[r];
+                }
+        }
+    }
     if (!valid) return;
     if (k == 0) sdf[n] = e.out[0];
     if (feature) {
@@ -386,27 +411,48 @@ __device__ __forceinline__ void write_base(const GeoArgs& a, const GeoEval<LJ, N
     }
 }
 
-// d sdf / d x01 of the base evaluation (analytic mode, one hidden layer):
-// u = W1^T (sigma'(z1) * Wo[0,:]) is the gradient of the SDF row with respect
+// Row 0 of the output layer (the SDF row), Wo[0, hidden], in D-tile layout.
+__device__ __forceinline__ void sdf_row_tiles(const GeoArgs& a, const float* sm, v4f w0[4]) {
+    const int k = lane_id() >> 4;
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) w0[t][r] = sm[a.wo + 16 * t + 4 * k + r];
+}
+
+// d sdf / d x01 of the base evaluation (analytic mode): u = W1^T c1, with
+// c1 = sigma'(z1) * Wo[0,:] (one hidden layer) or c1 = sigma'(z1) * W2^T
+// (sigma'(z2) * Wo[0,:]) (two), is the gradient of the SDF row with respect
 // to the MLP input (encoding slots, then xyz*2-1), contracted with the
 // encoding's position derivatives e.dx and with 2 for the xyz input; summed
 // over the four lane groups (every lane of the sample gets the sum).  u is
 // returned in B-fragment layout.
-template <int LJ>
+template <int LJ, int NH>
 __device__ __forceinline__ void geo_input_grad(const GeoArgs& a, const float* sm,
-                                               const GeoEval<LJ, 1, true>& e, float gpos[3],
+                                               const GeoEval<LJ, NH, true>& e, float gpos[3],
                                                float u[GeoDims<LJ>::KS1]) {
     constexpr int IT1 = GeoDims<LJ>::IT1;
     const int k = lane_id() >> 4;
     v4f c1[4];
+    sdf_row_tiles(a, sm, c1);
+    if constexpr (NH == 2) {
+        v4f c2[4], e1[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            v4f h, dh;
+            mlp::act_fb4(a.act, e.z2[t], a.beta, h, dh);
+            c2[t] = dh * c1[t];
+            e1[t] = v4f{0, 0, 0, 0};
+        }
+        mlp::mm_bwd_in<16, 4, mlp::FragV, mlp::kF32>(sm + a.w2, 65, mlp::FragV{c2}, e1);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) c1[t] = e1[t];
+    }
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
         v4f h, dh;
         mlp::act_fb4(a.act, e.z1[t], a.beta, h, dh);
-        v4f w;
-#pragma unroll
-        for (int r = 0; r < 4; ++r) w[r] = sm[a.wo + 16 * t + 4 * k + r];   // Wo[0, row]
-        c1[t] = dh * w;
+        c1[t] = dh * c1[t];
     }
     v4f gin[IT1];
 #pragma unroll
@@ -444,7 +490,6 @@ __global__ void __launch_bounds__(256) k_geo_fwd(GeoArgs a, const float* __restr
                                                  float* __restrict__ laplace,
                                                  float* __restrict__ fdv,
                                                  float* __restrict__ store) {
-    static_assert(!AN || NH == 1, "analytic gradients take one hidden layer");
     extern __shared__ float sm[];
     __shared__ LevelParams lv[AM_MAX_LEVELS];
     geo_load_lds<LJ, NH>(a, params, sm, lv);
@@ -469,10 +514,10 @@ __global__ void __launch_bounds__(256) k_geo_fwd(GeoArgs a, const float* __restr
             GeoEval<LJ, NH, true> e;   // with the position derivatives
             geo_eval<LJ, NH, true, mlp::kF32>(a, sm, lv, table, s.x0, e);
             if constexpr (ST) geo_store<LJ, NH, true>(store, a, tile, n_tiles, 0, 0, e);
-            write_base(a, e, s, m, valid, k, fdim, sdf, feature);
+            write_base(a, sm, e, s, m, valid, k, fdim, sdf, feature);
             if (!a.with_grad) continue;
             float gp[3], u[GeoDims<LJ>::KS1];
-            geo_input_grad<LJ>(a, sm, e, gp, u);
+            geo_input_grad<LJ, NH>(a, sm, e, gp, u);
 #pragma unroll
             for (int d = 0; d < 3; ++d) s.grad[d] = gp[d] * a.inv2r;   // d x01 / d x = 1 / 2r
             if (fdv && valid && k == 0)
@@ -483,7 +528,7 @@ __global__ void __launch_bounds__(256) k_geo_fwd(GeoArgs a, const float* __restr
                 GeoEval<LJ, NH, false> e;
                 geo_eval<LJ, NH, false, mlp::kF32>(a, sm, lv, table, s.x0, e);
                 if constexpr (ST) geo_store<LJ, NH, false>(store, a, tile, n_tiles, 0, 0, e);
-                write_base(a, e, s, m, valid, k, fdim, sdf, feature);
+                write_base(a, sm, e, s, m, valid, k, fdim, sdf, feature);
             }
             if (!a.with_grad) continue;
             float g[3] = {0.0f, 0.0f, 0.0f};
@@ -550,6 +595,54 @@ struct GeoGrad {
     v4f dbo;
 };
 
+// Feature rows of output tiles >= 1 (n_out > 16) in the base evaluation's
+// backward: their upstream gradients come from g_feature, their weight and
+// bias gradients accumulate in a wave-private LDS area (acc) instead of
+// registers: per tile t >= 1, [4 column tiles][64 lanes] v4f of dWo (the
+// mm_dw layout) then 16 row sums of dbo.
+struct OutExt {
+    const float* g_feature;
+    int64_t m;   // the sample's caller row
+    int fdim;
+    bool valid;
+    float* acc;
+};
+constexpr int kOutExtFloats = 4 * 64 * 4 + 16;   // per extra tile
+
+__device__ __forceinline__ void out_ext_bwd(const GeoArgs& a, const float* sm, const v4f h[4],
+                                            const OutExt& x, float* scratch, v4f gh[4]) {
+    const int l = lane_id(), i = l & 15, k = l >> 4;
+#pragma unroll 1
+    for (int t = 1; t < a.out_tiles; ++t) {
+        v4f got;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = 16 * t + 4 * k + r;
+            got[r] = (x.valid && x.g_feature && row < a.n_out) ? x.g_feature[x.m * x.fdim + row]
+                                                                 : 0.0f;
+        }
+        float* acc = x.acc + (t - 1) * kOutExtFloats;
+        v4f dw[4];
+#pragma unroll
+        for (int it = 0; it < 4; ++it) dw[it] = *reinterpret_cast<const v4f*>(acc + (it * 64 + l) * 4);
+        mlp::mm_dw<1, 4, 16, mlp::FragV, mlp::kF32>(&got, mlp::FragV{h}, scratch, dw);
+#pragma unroll
+        for (int it = 0; it < 4; ++it) *reinterpret_cast<v4f*>(acc + (it * 64 + l) * 4) = dw[it];
+        const v4f bs = mlp::rowsum_tile(got);
+        if (i == 0)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) acc[1024 + 4 * k + r] += bs[r];
+        mlp::mm_bwd_in<4, 4, mlp::FragV, mlp::kF32>(sm + a.wo + 16 * t * 65, 65, mlp::FragV{&got},
+                                                    gh);
+    }
+}
+
+// LDS float offset of the backward's per-wave OutExt areas (after the
+// weights and the four waves' transpose scratch; 16-byte aligned).
+__host__ __device__ inline int geo_ext_base(const GeoArgs& a) {
+    return (a.lds_floats + 4 * 128 * mlp::kScratchLd + 3) & ~3;
+}
+
 // Output-gradient tile with only row 0 (the SDF) set.
 __device__ __forceinline__ v4f go_row0(float g) {
     const int k = lane_id() >> 4;
@@ -564,7 +657,7 @@ template <int LJ, int NH, bool DX, int PR>
 __device__ __forceinline__ void geo_eval_bwd(const GeoArgs& a, const float* sm,
                                              const GeoEval<LJ, NH, DX>& e, v4f go,
                                              float* scratch, GeoGrad<LJ, NH>& G, float* genc,
-                                             float gpos[3]) {
+                                             float gpos[3], const OutExt* ext = nullptr) {
     constexpr int KS1 = GeoDims<LJ>::KS1, IT1 = GeoDims<LJ>::IT1;
     const int l = lane_id(), k = l >> 4;
     v4f h[4], dh[4];   // last hidden activation and its derivative
@@ -577,6 +670,8 @@ __device__ __forceinline__ void geo_eval_bwd(const GeoArgs& a, const float* sm,
 #pragma unroll
     for (int t = 0; t < 4; ++t) gh[t] = v4f{0, 0, 0, 0};
     mlp::mm_bwd_in<4, 4, mlp::FragV, PR>(sm + a.wo, 65, mlp::FragV{&go}, gh);
+    if constexpr (NH == 2)
+        if (ext) out_ext_bwd(a, sm, h, *ext, scratch, gh);
     if constexpr (NH == 2) {
         v4f gz2[4], h1[4];
 #pragma unroll
@@ -624,33 +719,33 @@ __device__ __forceinline__ void geo_eval_bwd(const GeoArgs& a, const float* sm,
     }
 }
 
-// Backward of the analytic base evaluation (one hidden layer).  With
-// v = d input / d x01 . g_x (the encoding's position derivatives and 2 for
-// xyz, contracted with the upstream gradient of d sdf / d x01) and
-// s = W1 v, the gradient term of the loss is s . (sigma'(z1) * Wo[0,:]):
-//   dWo[0,:] += s * sigma',  dW1 += (sigma' * Wo[0,:]) (x) v,
-//   d/dz1 += s * Wo[0,:] * sigma'',  d/dtable via u = W1^T (sigma' * Wo[0,:]):
-//   each corner c of level l gets u[l,f] * scale_l * grad(w_c) . g_x,
-// stored as u[l,f] * scale_l * g_x[d] in three extra slots (gext) for the
-// pair pass; the ordinary output gradient go flows as in geo_eval_bwd.
-template <int LJ>
+// Backward of the analytic base evaluation.  With v = d input / d x01 . g_x
+// (the encoding's position derivatives and 2 for xyz, contracted with the
+// upstream gradient of d sdf / d x01), the gradient term of the loss is the
+// directional derivative of the SDF row along v: a tangent pass
+// s1 = W1 v, a1' = sigma'(z1) s1 (, s2 = W2 a1', a2' = sigma'(z2) s2) and
+// sdf' = Wo[0,:] . a_last'.  Its reverse pass, with the ordinary output
+// gradient go alongside (adjoints zb of the pre-activations and sb of the
+// tangents):
+//   dWo += go (x) a_last + e0 (x) a_last'      (e0: row 0 = 1)
+//   layer l from the top: sb_l = sigma'(z_l) * ab'_l,
+//     zb_l = sigma'(z_l) * ab_l + sigma''(z_l) * s_l * ab'_l,
+//     dW_l += zb_l (x) a_{l-1} + sb_l (x) a'_{l-1},  db_l += zb_l,
+//     ab_{l-1} = W_l^T zb_l,  ab'_{l-1} = W_l^T sb_l
+//   (ab_last = Wo^T go, ab'_last = Wo[0,:]; a_0 = the input, a'_0 = v).
+// The input adjoint W1^T zb_1 is the ordinary encode gradient (genc); the
+// tangent adjoint u = W1^T sb_1 reaches the table through v: each corner c
+// of level l gets u[l,f] * scale_l * grad(w_c) . g_x, stored as
+// u[l,f] * scale_l * g_x[d] in three extra slots (gext) for the pair pass.
+template <int LJ, int NH>
 __device__ __forceinline__ void geo_eval_bwd_an(const GeoArgs& a, const float* sm,
                                                 const LevelParams* lv,
-                                                const GeoEval<LJ, 1, true>& e, v4f go,
+                                                const GeoEval<LJ, NH, true>& e, v4f go,
                                                 const float gx[3], float* scratch,
-                                                GeoGrad<LJ, 1>& G, float* genc, float* gext) {
+                                                GeoGrad<LJ, NH>& G, float* genc, float* gext,
+                                                const OutExt* ext) {
     constexpr int KS1 = GeoDims<LJ>::KS1, IT1 = GeoDims<LJ>::IT1;
     const int l = lane_id(), k = l >> 4;
-    v4f h[4], dh[4], d2[4], w0[4];
-#pragma unroll
-    for (int t = 0; t < 4; ++t) {
-        mlp::act_fb4(a.act, e.z1[t], a.beta, h[t], dh[t]);
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            d2[t][r] = mlp::act_d2(a.act, e.z1[t][r], a.beta, dh[t][r]);
-            w0[t][r] = sm[a.wo + 16 * t + 4 * k + r];   // Wo[0, row]
-        }
-    }
     float v[KS1];
 #pragma unroll
     for (int j = 0; j < LJ; ++j)
@@ -658,30 +753,96 @@ __device__ __forceinline__ void geo_eval_bwd_an(const GeoArgs& a, const float* s
         for (int f = 0; f < 2; ++f)
             v[2 * j + f] = (e.dx[j][f][0] * gx[0] + e.dx[j][f][1] * gx[1]) + e.dx[j][f][2] * gx[2];
     v[2 * LJ] = k < 3 ? 2.0f * pick3(gx, k) : 0.0f;
-    v4f sv[4];
-#pragma unroll
-    for (int t = 0; t < 4; ++t) sv[t] = v4f{0, 0, 0, 0};
-    mlp::mm_fwd<KS1, 4, mlp::FragF, mlp::kF32>(sm + a.w1, a.ld1, mlp::FragF{v}, sv);
-    // output layer: the ordinary term and the gradient term's row 0
-    mlp::mm_dw<1, 4, 16, mlp::FragV, mlp::kF32>(&go, mlp::FragV{h}, scratch, G.dWo);
-    G.dbo += go;
-    {
-        v4f sdh[4];
-#pragma unroll
-        for (int t = 0; t < 4; ++t) sdh[t] = sv[t] * dh[t];
-        const v4f e0 = go_row0(1.0f);
-        mlp::mm_dw<1, 4, 16, mlp::FragV, mlp::kF32>(&e0, mlp::FragV{sdh}, scratch, G.dWo);
-    }
-    v4f gh[4];
-#pragma unroll
-    for (int t = 0; t < 4; ++t) gh[t] = v4f{0, 0, 0, 0};
-    mlp::mm_bwd_in<4, 4, mlp::FragV, mlp::kF32>(sm + a.wo, 65, mlp::FragV{&go}, gh);
-    v4f gz1[4], c1[4];
+    v4f h1[4], dh1[4], sv1[4], w0[4];
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
-        gz1[t] = gh[t] * dh[t] + sv[t] * w0[t] * d2[t];
-        c1[t] = dh[t] * w0[t];
-        G.db1[t] += gz1[t];
+        mlp::act_fb4(a.act, e.z1[t], a.beta, h1[t], dh1[t]);
+        sv1[t] = v4f{0, 0, 0, 0};
+    }
+    sdf_row_tiles(a, sm, w0);
+    mlp::mm_fwd<KS1, 4, mlp::FragF, mlp::kF32>(sm + a.w1, a.ld1, mlp::FragF{v}, sv1);
+    v4f gz1[4], c1[4];   // zb_1 and sb_1
+    if constexpr (NH == 1) {
+        // output layer: the ordinary term and the gradient term's row 0
+        mlp::mm_dw<1, 4, 16, mlp::FragV, mlp::kF32>(&go, mlp::FragV{h1}, scratch, G.dWo);
+        G.dbo += go;
+        {
+            v4f sdh[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) sdh[t] = sv1[t] * dh1[t];
+            const v4f e0 = go_row0(1.0f);
+            mlp::mm_dw<1, 4, 16, mlp::FragV, mlp::kF32>(&e0, mlp::FragV{sdh}, scratch, G.dWo);
+        }
+        v4f gh[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) gh[t] = v4f{0, 0, 0, 0};
+        mlp::mm_bwd_in<4, 4, mlp::FragV, mlp::kF32>(sm + a.wo, 65, mlp::FragV{&go}, gh);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            v4f d2;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) d2[r] = mlp::act_d2(a.act, e.z1[t][r], a.beta, dh1[t][r]);
+            gz1[t] = gh[t] * dh1[t] + sv1[t] * w0[t] * d2;
+            c1[t] = dh1[t] * w0[t];
+            G.db1[t] += gz1[t];
+        }
+    } else {
+        // tangent of the second layer
+        v4f ad1[4], sv2[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            ad1[t] = dh1[t] * sv1[t];
+            sv2[t] = v4f{0, 0, 0, 0};
+        }
+        mlp::mm_fwd<16, 4, mlp::FragV, mlp::kF32>(sm + a.w2, 65, mlp::FragV{ad1}, sv2);
+        v4f gz2[4], c2[4];
+        {
+            v4f h2[4], dh2[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) mlp::act_fb4(a.act, e.z2[t], a.beta, h2[t], dh2[t]);
+            mlp::mm_dw<1, 4, 16, mlp::FragV, mlp::kF32>(&go, mlp::FragV{h2}, scratch, G.dWo);
+            G.dbo += go;
+            {
+                v4f ad2[4];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) ad2[t] = dh2[t] * sv2[t];
+                const v4f e0 = go_row0(1.0f);
+                mlp::mm_dw<1, 4, 16, mlp::FragV, mlp::kF32>(&e0, mlp::FragV{ad2}, scratch, G.dWo);
+            }
+            v4f gh[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) gh[t] = v4f{0, 0, 0, 0};
+            mlp::mm_bwd_in<4, 4, mlp::FragV, mlp::kF32>(sm + a.wo, 65, mlp::FragV{&go}, gh);
+            if (ext) out_ext_bwd(a, sm, h2, *ext, scratch, gh);
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                v4f d2;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) d2[r] = mlp::act_d2(a.act, e.z2[t][r], a.beta, dh2[t][r]);
+                gz2[t] = gh[t] * dh2[t] + sv2[t] * w0[t] * d2;
+                c2[t] = dh2[t] * w0[t];
+                G.db2[t] += gz2[t];
+            }
+        }
+        mlp::mm_dw<4, 4, 16, mlp::FragV, mlp::kF32>(gz2, mlp::FragV{h1}, scratch, G.dW2);
+        mlp::mm_dw<4, 4, 16, mlp::FragV, mlp::kF32>(c2, mlp::FragV{ad1}, scratch, G.dW2);
+        v4f gh1[4], e1[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            gh1[t] = v4f{0, 0, 0, 0};
+            e1[t] = v4f{0, 0, 0, 0};
+        }
+        mlp::mm_bwd_in<16, 4, mlp::FragV, mlp::kF32>(sm + a.w2, 65, mlp::FragV{gz2}, gh1);
+        mlp::mm_bwd_in<16, 4, mlp::FragV, mlp::kF32>(sm + a.w2, 65, mlp::FragV{c2}, e1);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            v4f d2;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) d2[r] = mlp::act_d2(a.act, e.z1[t][r], a.beta, dh1[t][r]);
+            gz1[t] = gh1[t] * dh1[t] + sv1[t] * e1[t] * d2;
+            c1[t] = dh1[t] * e1[t];
+            G.db1[t] += gz1[t];
+        }
     }
     mlp::mm_dw<4, IT1, KS1, mlp::FragF, mlp::kF32>(gz1, mlp::FragF{e.xb}, scratch, G.dW1);
     mlp::mm_dw<4, IT1, KS1, mlp::FragF, mlp::kF32>(c1, mlp::FragF{v}, scratch, G.dW1);
@@ -1089,6 +1250,12 @@ __device__ __forceinline__ void geo_bwd_body(
     const int l = lane_id(), p = l & 15, k = l >> 4;
     const int nw = blockDim.x >> 6, wave = threadIdx.x >> 6;
     float* scratch = sm + a.lds_floats + wave * (128 * mlp::kScratchLd);
+    // accumulators of the feature rows of output tiles >= 1 (n_out > 16, two
+    // hidden layers)
+    const int ext_floats = NH == 2 ? (a.out_tiles - 1) * kOutExtFloats : 0;
+    float* ext_acc = sm + geo_ext_base(a) + wave * ext_floats;
+    for (int q = l; q < ext_floats; q += 64) ext_acc[q] = 0.0f;
+    mlp::wave_sync();
     const int64_t n_tiles = (N + 15) / 16;
     const int fdim = a.n_out + 3;
     GeoGrad<LJ, NH> G;
@@ -1139,7 +1306,9 @@ __device__ __forceinline__ void geo_bwd_body(
             GeoEval<LJ, NH, false> e;
             geo_eval_b<LJ, NH, false, ST, mlp::kF32>(a, sm, lv, table, store, tile, n_tiles, 0, 0,
                                                      s.x0, e);
-            geo_eval_bwd<LJ, NH, false, mlp::kF32>(a, sm, e, go, scratch, G, genc, nullptr);
+            const OutExt ext{g_feature, m, fdim, valid, ext_acc};
+            geo_eval_bwd<LJ, NH, false, mlp::kF32>(a, sm, e, go, scratch, G, genc, nullptr,
+                                                   a.out_tiles > 1 ? &ext : nullptr);
         }
         if (a.with_laplace) {
             const float gl = (valid && g_laplace) ? g_laplace[m] : 0.0f;
@@ -1191,8 +1360,9 @@ __device__ __forceinline__ void geo_bwd_body(
             GeoEval<LJ, NH, true> e;
             geo_eval_b<LJ, NH, true, ST, mlp::kF32>(a, sm, lv, table, store, tile, n_tiles, 0, 0,
                                                     s.x0, e);
-            geo_eval_bwd_an<LJ>(a, sm, lv, e, go, gx, scratch, G, genc,
-                                genc + a.n_evals * LJ * 128);
+            const OutExt ext{g_feature, m, fdim, valid, ext_acc};
+            geo_eval_bwd_an<LJ, NH>(a, sm, lv, e, go, gx, scratch, G, genc,
+                                    genc + a.n_evals * LJ * 128, a.out_tiles > 1 ? &ext : nullptr);
         } else if (a.with_grad) {
             float gn2g[3];
             normalize3_bwd(s.grad, gnrm, gn2g);
@@ -1260,6 +1430,27 @@ __device__ __forceinline__ void geo_bwd_body(
 #pragma unroll
         for (int r = 0; r < 4; ++r)
             if (4 * k + r < a.n_out) out[a.pbo + 4 * k + r] = so[r];
+    // feature rows of output tiles >= 1
+    mlp::wave_sync();
+#pragma unroll 1
+    for (int t = 1; t < (NH == 2 ? a.out_tiles : 1); ++t) {
+        const float* acc = ext_acc + (t - 1) * kOutExtFloats;
+#pragma unroll
+        for (int it = 0; it < 4; ++it) {
+            const v4f dw = *reinterpret_cast<const v4f*>(acc + (it * 64 + l) * 4);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int o = 16 * t + 4 * k + r;
+                if (o < a.n_out) out[a.pwo + o * 64 + 16 * it + i] = dw[r];
+            }
+        }
+        if (i == 0)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int o = 16 * t + 4 * k + r;
+                if (o < a.n_out) out[a.pbo + o] = acc[1024 + 4 * k + r];
+            }
+    }
 }
 
 #define AM_GEO_BWD_PARAMS                                                                  \
@@ -1304,7 +1495,12 @@ __global__ void k_pair_total(const int64_t* __restrict__ offsets, const int64_t*
 static int make_geo_args(const am_grid_desc* g, const am_mlp_desc* m, const am_sdf_desc* s,
                          GeoArgs* out, int* LJ) {
     AM_CHECK_ARG(g && m && s, "NULL descriptor");
-    if (int rc = check_mlp_common(m)) return rc;
+    if (int rc = check_mlp_common(m, kGeoMaxOut)) return rc;
+    if (m->n_out > 16 && m->n_hidden_layers != 2) {
+        set_error("an SDF MLP with more than 16 outputs (got %d) takes two hidden layers",
+                  m->n_out);
+        return AM_ERR_UNSUPPORTED;
+    }
     AM_CHECK_ARG(g->n_features == 2, "fused geometry needs n_features = 2 (got %d)", g->n_features);
     AM_CHECK_ARG(g->n_levels >= 1 && g->n_levels <= 16, "fused geometry needs <= 16 levels");
     AM_CHECK_ARG(m->n_in == 3 + 2 * g->n_levels, "SDF MLP n_in %d != 3 + L*F", m->n_in);
@@ -1317,21 +1513,11 @@ static int make_geo_args(const am_grid_desc* g, const am_mlp_desc* m, const am_s
     AM_CHECK_ARG(!(s->with_laplace || (s->with_grad && !analytic)) || s->fd_eps > 0.0f,
                  "fd_eps <= 0");
     AM_CHECK_ARG(!s->with_laplace || s->with_grad, "laplace needs with_grad");
-    if (analytic && m->n_hidden_layers != 1) {
-        set_error("analytic gradients take a one-hidden-layer SDF MLP (got %d)",
-                  m->n_hidden_layers);
-        return AM_ERR_UNSUPPORTED;
-    }
     AM_CHECK_ARG(s->precision == AM_PREC_FP32 || s->precision == AM_PREC_MIXED,
                  "precision %d", s->precision);
     AM_CHECK_ARG(m->precision == AM_PREC_FP32,
                  "the SDF MLP runs in fp32 in every form (got precision %d): central "
                  "differences amplify per-evaluation rounding by 1/eps", m->precision);
-    if (s->precision == AM_PREC_MIXED && m->n_hidden_layers != 1) {
-        set_error("the mixed-precision SDF field supports 1 hidden layer (got %d)",
-                  m->n_hidden_layers);
-        return AM_ERR_UNSUPPORTED;
-    }
     GeoArgs a{};
     a.g = make_grid_args(g);
     a.r = s->radius;
@@ -1357,8 +1543,9 @@ static int make_geo_args(const am_grid_desc* g, const am_mlp_desc* m, const am_s
     a.b1 = o; o += 64;
     a.w2 = o; a.b2 = o;
     if (m->n_hidden_layers == 2) { a.w2 = o; o += 64 * 65; a.b2 = o; o += 64; }
-    a.wo = o; o += 16 * 65;
-    a.bo = o; o += 16;
+    a.out_tiles = (m->n_out + 15) / 16;
+    a.wo = o; o += a.out_tiles * 16 * 65;
+    a.bo = o; o += a.out_tiles * 16;
     a.lds_floats = o;
     layout_params(m->n_in, m->n_out, m->n_hidden_layers, &a.pw1, &a.pb1, &a.pw2, &a.pb2, &a.pwo,
                   &a.pbo, &a.n_params);
@@ -1380,15 +1567,7 @@ static int geo_fwd_launch_t(const GeoArgs& a, const float* pts, const float* ran
                             float* normal, float* feature, float* laplace, float* fdv,
                             float* store, hipStream_t s) {
     const size_t lds = (size_t)a.lds_floats * sizeof(float);
-    auto kern = k_geo_fwd<LJ, NH, ST, PR, false>;
-    if (a.analytic) {
-        if constexpr (NH == 1) {
-            kern = k_geo_fwd<LJ, NH, ST, PR, true>;
-        } else {
-            set_error("analytic gradients take a one-hidden-layer SDF MLP");
-            return AM_ERR_UNSUPPORTED;
-        }
-    }
+    auto kern = a.analytic ? k_geo_fwd<LJ, NH, ST, PR, true> : k_geo_fwd<LJ, NH, ST, PR, false>;
     AM_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)lds));
     {
@@ -1405,16 +1584,14 @@ static int geo_fwd_launch(const GeoArgs& a, const float* pts, const float* randn
                           const void* table, const float* params, float* sdf, float* grad,
                           float* normal, float* feature, float* laplace, float* fdv, float* store,
                           hipStream_t s) {
-    if constexpr (NH == 1) {
-        if (a.prec == AM_PREC_MIXED) {
-            if (store)
-                return geo_fwd_launch_t<LJ, NH, true, mlp::kBF16>(
-                    a, pts, randn, n, table, params, sdf, grad, normal, feature, laplace, fdv,
-                    store, s);
-            return geo_fwd_launch_t<LJ, NH, false, mlp::kBF16>(
-                a, pts, randn, n, table, params, sdf, grad, normal, feature, laplace, fdv, nullptr,
-                s);
-        }
+    if (a.prec == AM_PREC_MIXED) {
+        if (store)
+            return geo_fwd_launch_t<LJ, NH, true, mlp::kBF16>(a, pts, randn, n, table, params, sdf,
+                                                              grad, normal, feature, laplace, fdv,
+                                                              store, s);
+        return geo_fwd_launch_t<LJ, NH, false, mlp::kBF16>(a, pts, randn, n, table, params, sdf,
+                                                           grad, normal, feature, laplace, fdv,
+                                                           nullptr, s);
     }
     if (store)
         return geo_fwd_launch_t<LJ, NH, true, mlp::kF32>(a, pts, randn, n, table, params, sdf,
@@ -1585,15 +1762,18 @@ static int geo_bwd_launch_t(const GeoArgs& a, const am_grid_desc* g, const float
     TableReduceWs tw;
     if (int rc = table_reduce_carve(c, 2, st[0], g->n_entries, &tw, PR == mlp::kBF16)) return rc;
     if (!c.ok()) return workspace_short(c.used, ws_bytes);
-    const size_t lds = (size_t)(a.lds_floats + 4 * 128 * mlp::kScratchLd) * sizeof(float);
+    const size_t lds =
+        (size_t)(geo_ext_base(a) + 4 * (a.out_tiles - 1) * kOutExtFloats) * sizeof(float);
     AM_CHECK_ARG(lds <= 160 * 1024, "geometry backward needs %zu bytes of LDS", lds);
-    auto kern = store ? k_geo_bwd<LJ, NH, true, PR, false> : k_geo_bwd<LJ, NH, false, PR, false>;
+    auto kern = k_geo_bwd_o2<LJ, true, PR, false>;   // the type of every form's kernel
     if constexpr (NH == 1) {
         kern = store ? k_geo_bwd_o2<LJ, true, PR, false> : k_geo_bwd_o2<LJ, false, PR, false>;
         if (a.analytic)
             kern = store ? k_geo_bwd_o2<LJ, true, PR, true> : k_geo_bwd_o2<LJ, false, PR, true>;
     } else {
-        AM_CHECK_ARG(!a.analytic, "analytic gradients take a one-hidden-layer SDF MLP");
+        kern = store ? k_geo_bwd<LJ, NH, true, PR, false> : k_geo_bwd<LJ, NH, false, PR, false>;
+        if (a.analytic)
+            kern = store ? k_geo_bwd<LJ, NH, true, PR, true> : k_geo_bwd<LJ, NH, false, PR, true>;
     }
     AM_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)lds));
@@ -1625,13 +1805,10 @@ static int geo_bwd_launch(const GeoArgs& a, const am_grid_desc* g, const am_mlp_
                           const float* store, const float* g_sdf, const float* g_grad,
                           const float* g_normal, const float* g_feature, const float* g_laplace,
                           float* dtable, float* dparams, void* ws, size_t ws_bytes, hipStream_t s) {
-    if constexpr (NH == 1) {
-        if (a.prec == AM_PREC_MIXED)
-            return geo_bwd_launch_t<LJ, NH, mlp::kBF16>(a, g, pts, randn, n, table, params, fdv_in,
-                                                        store, g_sdf, g_grad, g_normal, g_feature,
-                                                        g_laplace, dtable, dparams, ws, ws_bytes,
-                                                        s);
-    }
+    if (a.prec == AM_PREC_MIXED)
+        return geo_bwd_launch_t<LJ, NH, mlp::kBF16>(a, g, pts, randn, n, table, params, fdv_in,
+                                                    store, g_sdf, g_grad, g_normal, g_feature,
+                                                    g_laplace, dtable, dparams, ws, ws_bytes, s);
     return geo_bwd_launch_t<LJ, NH, mlp::kF32>(a, g, pts, randn, n, table, params, fdv_in, store,
                                                g_sdf, g_grad, g_normal, g_feature, g_laplace,
                                                dtable, dparams, ws, ws_bytes, s);

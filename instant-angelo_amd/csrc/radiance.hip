@@ -16,6 +16,12 @@ struct RadArgs {
     int64_t pw1, pb1, pw2, pb2, pwo, pbo, n_params;
 };
 
+// Rows of a wave's transpose scratch for mm_dw: the 4 output tiles of a
+// hidden layer plus the first layer's IT1 = ceil(KS/4) input tiles.
+__host__ __device__ constexpr int rad_scratch_rows(int KS) {
+    return 16 * (4 + (KS + 3) / 4) > 128 ? 16 * (4 + (KS + 3) / 4) : 128;
+}
+
 // Radiance first-layer input column c sits at K-step c/4, lane group c%4,
 // i.e. row krow(c/4, c%4) of the row space.
 __device__ __forceinline__ int rad_col_of_row(int rs) {
@@ -201,7 +207,7 @@ __global__ void __launch_bounds__(256) k_rad_bwd(RadArgs a, const float* __restr
     __syncthreads();
     const int l = lane_id(), p = l & 15, k = l >> 4;
     const int nw = blockDim.x >> 6, wave = threadIdx.x >> 6;
-    float* scratch = sm + a.lds_floats + wave * (128 * mlp::kScratchLd);
+    float* scratch = sm + a.lds_floats + wave * (rad_scratch_rows(KS) * mlp::kScratchLd);
     const int64_t n_tiles = (N + 15) / 16;
     v4f dW1[4 * IT1], dW2[NH == 2 ? 16 : 1], dWo[4], db1[4], db2[NH == 2 ? 4 : 1], dbo;
 #pragma unroll
@@ -357,7 +363,7 @@ static int make_rad_args(const am_mlp_desc* m, int sh_degree, int base_color, in
     AM_CHECK_ARG(sh_degree >= 1 && sh_degree <= 4, "sh_degree %d out of [1,4]", sh_degree);
     AM_CHECK_ARG(fdim >= 0 && m->n_in == fdim + sh_degree * sh_degree + 3,
                  "radiance n_in %d != feature_dim + degree^2 + 3", m->n_in);
-    AM_CHECK_ARG(m->n_in <= 48, "radiance n_in %d > 48", m->n_in);
+    AM_CHECK_ARG(m->n_in <= 96, "radiance n_in %d > 96", m->n_in);
     RadArgs a{};
     a.n_in = m->n_in;
     a.n_out = m->n_out;
@@ -368,7 +374,9 @@ static int make_rad_args(const am_mlp_desc* m, int sh_degree, int base_color, in
     a.sh_deg = sh_degree;
     a.n_sh = sh_degree * sh_degree;
     const int ks = (m->n_in + 3) / 4;
-    const int ks_t = ks <= 8 ? 8 : (ks <= 10 ? 10 : 12);
+    // K-step counts compiled: 8, 10, 12 (inputs <= 48) and 24 (<= 96, the
+    // Neuralangelo texture's 65 + 3 features, 16 SH and the normal)
+    const int ks_t = ks <= 8 ? 8 : (ks <= 10 ? 10 : (ks <= 12 ? 12 : 24));
     const int rs1 = 16 * ((ks_t + 3) / 4);
     a.ld1 = rs1 + 1;
     int o = 0;
@@ -414,7 +422,9 @@ static int rad_bwd_launch(const RadArgs& a, const float* feature, const float* r
     float* partial = c.take<float>((size_t)waves * a.n_params);
     float* ctmp = c.take<float>((size_t)colsum_tmp_floats(waves, a.n_params) + 1);
     if (!c.ok()) return workspace_short(c.used, ws_bytes);
-    const size_t lds = (size_t)(a.lds_floats + 4 * 128 * mlp::kScratchLd) * sizeof(float);
+    const size_t lds =
+        (size_t)(a.lds_floats + 4 * rad_scratch_rows(KS) * mlp::kScratchLd) * sizeof(float);
+    AM_CHECK_ARG(lds <= 160 * 1024, "radiance backward needs %zu bytes of LDS", lds);
     AM_HIP(hipFuncSetAttribute((const void*)k_rad_bwd<KS, NH, PR>,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     {
@@ -432,7 +442,8 @@ static int rad_bwd_launch(const RadArgs& a, const float* feature, const float* r
         switch (KS) {                                                                     \
             case 8: return two ? FN<8, 2, PR>(__VA_ARGS__) : FN<8, 1, PR>(__VA_ARGS__);    \
             case 10: return two ? FN<10, 2, PR>(__VA_ARGS__) : FN<10, 1, PR>(__VA_ARGS__); \
-            default: return two ? FN<12, 2, PR>(__VA_ARGS__) : FN<12, 1, PR>(__VA_ARGS__); \
+            case 12: return two ? FN<12, 2, PR>(__VA_ARGS__) : FN<12, 1, PR>(__VA_ARGS__); \
+            default: return two ? FN<24, 2, PR>(__VA_ARGS__) : FN<24, 1, PR>(__VA_ARGS__); \
         }                                                                                 \
     } while (0)
 

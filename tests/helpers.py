@@ -101,10 +101,15 @@ def oracle_fns(model, dtype=torch.float64, mode="f64", active=None):
 
     field = ofield.field_from(encode, mlp_geo)
 
+    base = getattr(tex, "base_color", 0)
+
     def radiance(feature, dirs, normal):
         sh = onets.sh_encode((dirs + 1.0) / 2.0, 4)
         h = torch.cat([feature, sh, normal], -1)
-        return torch.sigmoid(onets.vanilla_mlp(h, onets.unflatten(rflat, rdims), "relu", "none"))
+        rgb = torch.sigmoid(onets.vanilla_mlp(h, onets.unflatten(rflat, rdims), "relu", "none"))
+        if base:   # VolumeDualColor (models/texture.py:51-55)
+            rgb = rgb + torch.sigmoid(feature[:, 1:4])
+        return rgb
 
     return field, radiance, dict(table=table, geo=gflat, rad=rflat)
 

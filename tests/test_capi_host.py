@@ -122,6 +122,25 @@ def test_product_has_no_cpu_fallback():
         model.geometry.forward_level(torch.zeros(4, 3))
 
 
+def test_host_tensors_never_reach_a_kernel():
+    """The occupancy update of a model still on the host (update_step at a
+    multiple of 16 with grid_prune on, in training mode) and every raw
+    pointer hand-off refuse host tensors before any launch: a host address
+    in a kernel argument faults the GPU."""
+    import angelo_amd
+    from angelo_amd import _native as nat
+    from angelo_amd.workloads import model_config
+    model = angelo_amd.make("neus", model_config(8, 15, 16, 1.5, rays=8, samples_per_ray=8,
+                                                 grid_prune=True, step=2.0 / 256))
+    model.train()
+    model.update_step(0, 30001)        # no occupancy update at this step
+    with pytest.raises(nat.NativeError, match="device tensors"):
+        model.update_step(0, 30000)
+    with pytest.raises(nat.NativeError, match="device tensors"):
+        nat.ptr(torch.zeros(3))
+    assert nat.ptr(None) is None
+
+
 def test_state_dict_names_mirror_reference():
     import angelo_amd
     from angelo_amd.workloads import model_config
@@ -305,3 +324,78 @@ def test_oracle_losses_closed_form():
     assert abs(t["eikonal"].item() - 2.0) < 1e-6          # ((3-1)^2 + 0) / 2
     assert abs(t["curvature"].item() - 0.2) < 1e-7
     assert abs(total.item() - (10 * mse + 0.1 * 2.0 + 0.5 * 0.2)) < 1e-5
+
+
+REF_CONFIG = "/root/reference/configs/neuralangelo-colmap_sparse.yaml"
+
+
+def _subset_equal(mine, ref, path=""):
+    """Every key of `mine` is in `ref` with the same value (lists / tuples
+    compared as lists, numbers as floats)."""
+    if isinstance(mine, dict):
+        for k, v in mine.items():
+            assert k in ref, f"{path}.{k} missing from the reference config"
+            _subset_equal(v, ref[k], f"{path}.{k}")
+        return
+    if isinstance(mine, (list, tuple)):
+        assert list(mine) == list(ref), (path, mine, ref)
+        return
+    if isinstance(mine, (int, float)) and not isinstance(mine, bool):
+        assert abs(float(mine) - float(ref)) <= 1e-12 * max(1.0, abs(float(ref))), (path, mine, ref)
+        return
+    assert mine == ref, (path, mine, ref)
+
+
+def test_neuralangelo_config_restates_the_reference_yaml():
+    """The restated Neuralangelo model and loss configuration equals the
+    reference's configs/neuralangelo-colmap_sparse.yaml as load_config
+    resolves it (interpolations ${model.radius} and ${add:65, 6}); skipped
+    where the reference checkout is absent."""
+    import os
+    import pytest
+    from angelo_amd.utils import load_config
+    from angelo_amd.workloads import neuralangelo_configs
+    if not os.path.exists(REF_CONFIG):
+        pytest.skip("reference checkout not present")
+    ref = load_config(REF_CONFIG, cli_args=["dataset.root_dir=/data/scene"])
+    assert ref.name == "neuralangelo-colmap_sparse-scene"
+    model, system = neuralangelo_configs()
+    _subset_equal(model, ref.model)
+    _subset_equal(system.loss, ref.system.loss)
+    o = ref.system.optimizer
+    assert system.optimizer.lr == o.args.lr == o.params.geometry.lr
+    assert list(system.optimizer.betas) == list(o.args.betas) and system.optimizer.eps == o.args.eps
+    assert system.optimizer.variance_lr == o.params.variance.lr
+
+
+def test_load_config_interpolations(tmp_path):
+    """utils/misc.py semantics: merge order, key.path=value overrides, key
+    interpolation, nested resolver calls, typed single interpolations."""
+    from angelo_amd.utils import load_config
+    a = tmp_path / "a.yaml"
+    b = tmp_path / "b.yaml"
+    a.write_text("x: {r: 1.5, n: 3}\ny: ${add:${x.n},6}\nz: ${mul:${x.r},2}\n"
+                 "w: s-${basename:${p}}\np: /a/b/c.txt\nq: ${calc_exp_lr_decay_rate:0.1,${sub:10,8}}\n"
+                 "m: ???\n")
+    b.write_text("x: {n: 65}\n")
+    c = load_config(str(a), str(b), cli_args=["x.r=2.0"])
+    assert c.y == 71 and isinstance(c.y, int)
+    assert c.z == 4.0 and c.w == "s-c.txt" and abs(c.q - 0.1 ** 0.5) < 1e-15
+    assert c.m == "???" and c.x.n == 65
+
+
+def test_neuralangelo_model_builds_with_the_reference_shapes():
+    """make('neus', <the Neuralangelo configuration>) builds the shapes the
+    fused kernels take: SDF MLP 35 -> 64 -> 64 -> 65 (analytic), texture 87
+    inputs with the base colour, the learned background."""
+    import angelo_amd
+    from angelo_amd.workloads import neuralangelo_configs
+    torch.manual_seed(0)
+    model = angelo_amd.make("neus", neuralangelo_configs()[0])
+    geo, tex = model.geometry, model.texture
+    assert [l.weight.shape[1] for l in geo.network.linears()] == [35, 64, 64]
+    assert geo.network.linears()[-1].weight.shape[0] == 65 and geo.grad_type == "analytic"
+    assert geo._fused_ok()
+    assert tex.n_input_dims == 87 and tex.base_color == 1
+    assert model.learned_background and model.geometry_bg.n_output_dims == 8
+    assert geo.grid_module.log2_hashmap_size == 19

@@ -203,7 +203,10 @@ def _bg_oracle(model, dtype=torch.float64):
         sh = onets.sh_encode((dirs + 1.0) / 2.0, 4)
         c = onets.vanilla_mlp(torch.cat([feature, sh], -1), onets.unflatten(rflat, rdims), "relu",
                               "none")
-        return torch.sigmoid(c) + torch.sigmoid(feature[:, 1:4])
+        rgb = torch.sigmoid(c)
+        if getattr(tex, "base_color", 0):
+            rgb = rgb + torch.sigmoid(feature[:, 1:4])
+        return rgb
 
     return field, radiance, dict(table=table, geo=gflat, rad=rflat), gm, ewn.network
 
@@ -336,8 +339,8 @@ def test_neus_step_with_background_matches_oracle(dev):
                 lin.weight.copy_(torch.randn(lin.weight.shape, generator=g) / lin.weight.shape[1] ** 0.5)
                 lin.bias.copy_(torch.randn(lin.bias.shape, generator=g) * 0.1)
     model.train()
-    model.update_step(0, 30000)
     model = model.to(dev)
+    model.update_step(0, 30000)
     bgc = torch.tensor([0.3, 0.6, 0.1])
     model.background_color = bgc.to(dev)
     n_rays = 48

@@ -4,6 +4,7 @@ import numpy as np
 import pytest
 import torch
 
+from oracle import march as omarch
 from oracle import render as orender
 
 import helpers
@@ -42,8 +43,8 @@ def test_neus_step_matches_oracle(dev, n_rays, geo_hidden, L, grad_type):
     cfg.train_num_rays = n_rays
     model = helpers.randomize_(angelo_amd.make("neus", cfg), seed=1)
     model.train()
-    model.update_step(0, 30000)
     model = model.to(dev)
+    model.update_step(0, 30000)
     rays, gt = scenes.config0_batch(n_rays=n_rays, n_views=4, seed=2)
     ridx, t0, t1, packed = scenes.uniform_samples(rays, 64)
     n = ridx.shape[0]
@@ -156,3 +157,108 @@ def test_occupancy_update_matches_oracle(dev):
     near = (occ - thre).abs() < 1e-6
     gb = og.binary.cpu().flatten().bool()
     assert torch.equal(gb[~near], ref_bin[~near])
+
+
+def test_neuralangelo_configuration_step_matches_oracle(dev):
+    """One NeuS training step of the reference's own Neuralangelo
+    configuration (configs/neuralangelo-colmap_sparse.yaml:16-133, restated
+    by workloads.neuralangelo_configs): radius 1.5, analytic-gradient SDF
+    field with two hidden layers and 65 outputs on a 16-level 2^19-row
+    grid, the VolumeDualColor texture (87 inputs), the learned background,
+    the colour, eikonal and curvature losses.  comp_rgb_full, the loss and
+    the gradients of both tables and every MLP against the FP64 oracle."""
+    import angelo_amd
+    import test_gpu_background as tb
+    from angelo_amd import scenes
+    from angelo_amd.system import NeuSTrainer
+    from angelo_amd.workloads import neuralangelo_configs
+    mcfg, scfg = neuralangelo_configs()
+    n_rays = 40
+    mcfg.train_num_rays = n_rays
+    torch.manual_seed(21)
+    model = helpers.randomize_(angelo_amd.make("neus", mcfg), seed=22)
+    g = torch.Generator().manual_seed(23)
+    with torch.no_grad():
+        last = model.geometry.network.linears()[-1]   # distinct feature rows
+        last.weight_v[1:] = torch.randn(last.weight_v[1:].shape, generator=g) * 0.2
+        last.weight_g[1:] = last.weight_v[1:].norm(dim=1, keepdim=True)
+        gm = model.geometry_bg.encoding_with_network.encoding.encoding.encoding
+        gm.params.copy_((torch.rand(gm.params.shape, generator=g) * 2 - 1) * 0.5)
+        for net in (model.geometry_bg.encoding_with_network.network, model.texture_bg.network):
+            for lin in net.linears():
+                lin.weight.copy_(torch.randn(lin.weight.shape, generator=g) / lin.weight.shape[1] ** 0.5)
+                lin.bias.copy_(torch.randn(lin.bias.shape, generator=g) * 0.1)
+    model.train()
+    model = model.to(dev)
+    model.update_step(0, 30000)
+    bgc = torch.tensor([0.2, 0.5, 0.7])
+    model.background_color = bgc.to(dev)
+    r = float(mcfg.radius)
+    rays, gt = scenes.config0_batch(n_rays=n_rays, n_views=4, seed=24)
+    ridx, t0, t1, packed = scenes.uniform_samples(rays, 32)
+    near = torch.full((n_rays,), 2.0)
+    far = torch.full((n_rays,), 6.0)
+    roi = [-r, -r, -r, r, r, r]
+    b_r, b_0, b_1, b_c = omarch.march_bg(rays.numpy(), near.numpy(), far.numpy(), roi, None,
+                                        0.01, tb._cone(24))
+    bcnt = torch.from_numpy(b_c).long()
+    bpacked = torch.stack([torch.cumsum(bcnt, 0) - bcnt, bcnt], -1).to(torch.int32)
+    bridx, bt0, bt1 = torch.from_numpy(b_r), torch.from_numpy(b_0), torch.from_numpy(b_1)
+    randn = torch.randn(ridx.shape[0], 3, generator=g)
+
+    # oracle, FP64 (and the fp32 restatement as the error scale)
+    def oracle(dtype, mode):
+        field, radiance, ol = helpers.oracle_fns(model, dtype, mode)
+        bfield, bradiance, bol, _, _ = tb._bg_oracle(model, dtype)
+        inv_s = torch.exp(model.variance.variance.detach().cpu().to(dtype) * 10.0).clip(1e-6, 1e6)
+        eps = float(model.geometry._finite_difference_eps)
+        o = orender.neus_forward(rays.to(dtype), ridx, t0.to(dtype), t1.to(dtype), n_rays, field,
+                                 radiance, inv_s, r, eps, model.cos_anneal_ratio, randn.to(dtype),
+                                 mode=mode, grad_type="analytic")
+        ob = orender.forward_bg(rays.to(dtype), bridx, bt0.to(dtype), bt1.to(dtype), n_rays,
+                                bfield, bradiance, r, -1.0, bgc.to(dtype), mode=mode)
+        full = o["comp_rgb"] + ob["comp_rgb"] * (1.0 - o["opacity"])
+        valid = (o["opacity"] > 0) | (ob["opacity"] > 0)
+        diff = (full - gt.to(dtype))[valid[:, 0]]
+        loss = 10.0 * (diff ** 2).mean()
+        loss = loss + 0.1 * ((torch.linalg.norm(o["grad"], ord=2, dim=-1) - 1.0) ** 2).mean()
+        loss = loss + 0.5 * o["laplace"].abs().mean()
+        loss.backward()
+        return full, loss, ol, bol
+
+    full64, loss64, ol, bol = oracle(torch.float64, "f64")
+    full32, loss32, ol32, bol32 = oracle(torch.float32, "f32")
+
+    # HIP path
+    leaves = {}
+    for name, net in (("geo", model.geometry.network), ("rad", model.texture.network),
+                      ("bgeo", model.geometry_bg.encoding_with_network.network),
+                      ("brad", model.texture_bg.network)):
+        leaf = net.flat_params().detach().clone().requires_grad_(True)
+        net.flat_params = (lambda l=leaf: l)
+        leaves[name] = leaf
+    trainer = NeuSTrainer(model, scfg, mcfg)
+    trainer.global_step = 30000     # lambda_sdf_l1 has reached 0, curvature 0.5
+    out = model(rays.to(dev), samples=(ridx.to(dev), t0.to(dev), t1.to(dev), packed.to(dev)),
+                randn=randn.to(dev),
+                samples_bg=(bridx.to(dev), bt0.to(dev), bt1.to(dev), bpacked.to(dev)))
+    loss, terms = trainer.losses(out, {"rgb": gt.to(dev)})
+    loss.backward()
+
+    def chk(a, b, b32, floor, name):
+        e = helpers.rel_err(a, b)
+        e32 = helpers.rel_err(b32, b)
+        assert e <= 8 * e32 + floor, (name, e, e32)
+
+    chk(out["comp_rgb_full"], full64, full32, 1e-4, "comp_rgb_full")
+    e_l32 = abs(loss32.item() - loss64.item()) / abs(loss64.item())
+    assert abs(loss.item() - loss64.item()) / abs(loss64.item()) <= 8 * e_l32 + 1e-4
+    helpers.assert_nontrivial(ol["table"].grad, "oracle table gradient")
+    gtab = model.geometry.grid_module.params.grad.view(-1, 2)
+    chk(gtab, ol["table"].grad, ol32["table"].grad, 2e-3, "d table")
+    chk(leaves["geo"].grad, ol["geo"].grad, ol32["geo"].grad, 2e-3, "d sdf mlp")
+    chk(leaves["rad"].grad, ol["rad"].grad, ol32["rad"].grad, 1e-3, "d radiance mlp")
+    gm = model.geometry_bg.encoding_with_network.encoding.encoding.encoding
+    chk(gm.params.grad.view(-1, 2), bol["table"].grad, bol32["table"].grad, 1e-3, "d bg table")
+    chk(leaves["bgeo"].grad, bol["geo"].grad, bol32["geo"].grad, 1e-3, "d bg mlp")
+    chk(leaves["brad"].grad, bol["rad"].grad, bol32["rad"].grad, 1e-3, "d bg radiance mlp")

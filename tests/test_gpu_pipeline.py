@@ -170,14 +170,23 @@ def _field_model(dev, L=8, geo_hidden=1, seed=0, **kw):
     cfg = helpers.model_config(L=L, geo_hidden=geo_hidden, seed=seed, **kw)
     model = angelo_amd.make("neus", cfg)
     helpers.randomize_(model, seed=seed)
+    model = model.to(dev)
     model.update_step(0, 30000)
-    return model.to(dev)
+    return model
 
 
-@pytest.mark.parametrize("geo_hidden,L,log2T,base,scale", [(1, 8, 15, 16, 1.5),
-                                                          (2, 16, 16, 32, 1.3195079107728942)])
-def test_sdf_fused_fwd_bwd(dev, geo_hidden, L, log2T, base, scale):
-    model = _field_model(dev, L=L, geo_hidden=geo_hidden, log2T=log2T, base=base, scale=scale)
+@pytest.mark.parametrize("geo_hidden,L,log2T,base,scale,feature_dim",
+                         [(1, 8, 15, 16, 1.5, 16), (2, 16, 16, 32, 1.3195079107728942, 16),
+                          (2, 16, 19, 32, 1.3195079107728942, 65)])
+def test_sdf_fused_fwd_bwd(dev, geo_hidden, L, log2T, base, scale, feature_dim):
+    """FD normal + curvature field: outputs and the table and SDF-MLP
+    gradients against the FP64 oracle; the last row has the Neuralangelo
+    configurations' 65 outputs and two hidden layers."""
+    from angelo_amd import functions as fn
+    model = _field_model(dev, L=L, geo_hidden=geo_hidden, log2T=log2T, base=base, scale=scale,
+                         feature_dim=feature_dim)
+    if feature_dim > 16:
+        _spread_outputs_(model, 15)
     geo = model.geometry
     g = torch.Generator().manual_seed(3)
     n = 1000 + 5
@@ -200,11 +209,15 @@ def test_sdf_fused_fwd_bwd(dev, geo_hidden, L, log2T, base, scale):
 
     o64, l64 = oracle(torch.float64, "f64")
     o32, l32 = oracle(torch.float32, "f32")
-    sdf, grad, normal, feature, lap = geo.fused(pts.to(dev), True, True, randn.to(dev))
+    grid = geo.grid_module
+    table = grid.params
+    flat = geo.network.flat_params().detach().clone().requires_grad_(True)
+    sdf, grad, normal, feature, lap = fn.SDFFieldFn.apply(
+        pts.to(dev), randn.to(dev), table, flat, grid.grid, geo.network.desc,
+        geo._sdf_desc(True, True), grid.kernel_table())
     loss = ((sdf * gw[0].to(dev)).sum() + (grad * gw[1].to(dev)).sum()
             + (normal * gw[2].to(dev)).sum() + (feature * gw[3].to(dev)).sum()
             + (lap * gw[4].to(dev)).sum())
-    table = geo.grid_module.params
     loss.backward()
     outs = dict(sdf=sdf, grad=grad, normal=normal, feature=feature, laplace=lap)
     for k, v in outs.items():
@@ -212,13 +225,15 @@ def test_sdf_fused_fwd_bwd(dev, geo_hidden, L, log2T, base, scale):
         e_hip = helpers.rel_err(v, ref)
         e_t32 = helpers.rel_err(o32[k].reshape(v.shape), ref)
         assert e_hip <= 8 * e_t32 + 1e-5, (k, e_hip, e_t32)
-    # gradients: table, SDF MLP (weight-norm folded: compare d/dW flat)
+    # gradients: table and the SDF MLP's flat [W, b] (weight norm folded)
     gt = table.grad.view(-1, 2).cpu()
     helpers.assert_nontrivial(l64["table"].grad, "oracle table gradient")
     helpers.assert_nontrivial(gt, "table gradient")
-    e_hip = helpers.rel_err(gt, l64["table"].grad)
-    e_t32 = helpers.rel_err(l32["table"].grad, l64["table"].grad)
-    assert e_hip <= 8 * e_t32 + 1e-5, ("table", e_hip, e_t32)
+    for name, got, ref64, ref32 in (("table", gt, l64["table"].grad, l32["table"].grad),
+                                    ("sdf mlp", flat.grad.cpu(), l64["geo"].grad, l32["geo"].grad)):
+        e_hip = helpers.rel_err(got, ref64)
+        e_t32 = helpers.rel_err(ref32, ref64)
+        assert e_hip <= 8 * e_t32 + 1e-5, (name, e_hip, e_t32)
 
 
 def test_sdf_bwd_deterministic(dev):
@@ -390,10 +405,14 @@ def test_spatial_order_same_results(dev, monkeypatch, grad_type):
         assert helpers.rel_err(b, a) < 1e-5
 
 
-@pytest.mark.parametrize("rad_hidden,dual", [(1, False), (2, False), (2, True)])
-def test_radiance_fused_fwd_bwd(dev, rad_hidden, dual):
-    """dual: VolumeDualColor (models/texture.py:38-64), + sigmoid(feature[:, 1:4])."""
-    model = _field_model(dev, rad_hidden=rad_hidden)
+@pytest.mark.parametrize("rad_hidden,dual,feature_dim", [(1, False, 16), (2, False, 16),
+                                                         (2, True, 16), (2, True, 65)])
+def test_radiance_fused_fwd_bwd(dev, rad_hidden, dual, feature_dim):
+    """dual: VolumeDualColor (models/texture.py:38-64), + sigmoid(feature[:, 1:4]).
+    feature_dim 65: the Neuralangelo texture (neuralangelo-colmap_sparse.yaml:
+    37,66-67), 68 features + 16 SH + the normal = 87 inputs."""
+    model = _field_model(dev, rad_hidden=rad_hidden, feature_dim=feature_dim,
+                         geo_hidden=2 if feature_dim > 16 else 1)
     tex = model.texture
     if dual:
         from angelo_amd.texture import VolumeDualColor
@@ -402,7 +421,7 @@ def test_radiance_fused_fwd_bwd(dev, rad_hidden, dual):
         tex = dual_tex
     g = torch.Generator().manual_seed(5)
     n = 777
-    feature = torch.randn(n, 19, generator=g)
+    feature = torch.randn(n, feature_dim + 3, generator=g)
     normal = torch.nn.functional.normalize(torch.randn(n, 3, generator=g), dim=-1)
     rays = _rays(100, 2)
     ridx = torch.randint(0, 100, (n,), generator=g).to(torch.int32)
@@ -428,15 +447,36 @@ def test_radiance_fused_fwd_bwd(dev, rad_hidden, dual):
     assert helpers.rel_err(gflat, leaves["rad"].grad) < 1e-4
 
 
+def _spread_outputs_(model, seed):
+    """Gives every output row of the SDF MLP its own weights (sphere init
+    makes the rows nearly equal), so each feature row's gradient is checked."""
+    g = torch.Generator().manual_seed(seed)
+    last = model.geometry.network.linears()[-1]
+    with torch.no_grad():
+        if hasattr(last, "weight_v"):
+            v = last.weight_v
+            v[1:] = torch.randn(v[1:].shape, generator=g) * 0.2
+            last.weight_g[1:] = v[1:].norm(dim=1, keepdim=True)
+        last.bias[1:] = torch.randn(last.bias[1:].shape, generator=g) * 0.1
+
+
 @pytest.mark.parametrize("laplace", [False, True])
-@pytest.mark.parametrize("L,log2T,base,scale", [(8, 15, 16, 1.5), (16, 16, 32, 1.3195079107728942)])
-def test_sdf_analytic_fwd_bwd(dev, laplace, L, log2T, base, scale):
+@pytest.mark.parametrize("L,log2T,base,scale,geo_hidden,feature_dim",
+                         [(8, 15, 16, 1.5, 1, 16), (16, 16, 32, 1.3195079107728942, 1, 16),
+                          (16, 19, 32, 1.3195079107728942, 2, 65)])
+def test_sdf_analytic_fwd_bwd(dev, laplace, L, log2T, base, scale, geo_hidden, feature_dim):
     """grad_type 'analytic': sdf, the analytic grad (d sdf / d x through the
     encoding and the MLP), normal, feature and laplace, and the gradients of
     a loss linear in all of them -- the grad term needs second derivatives --
-    against the oracle's autograd (create_graph), FP64 and FP32."""
+    against the oracle's autograd (create_graph), FP64 and FP32.  The last
+    row is the SDF field of the Neuralangelo configurations
+    (neuralangelo-colmap_sparse.yaml:37-38,48-61): 2^19 rows per level, two
+    hidden layers, 65 outputs."""
     from angelo_amd import functions as fn
-    model = _field_model(dev, L=L, log2T=log2T, base=base, scale=scale, grad_type="analytic")
+    model = _field_model(dev, L=L, log2T=log2T, base=base, scale=scale, grad_type="analytic",
+                         geo_hidden=geo_hidden, feature_dim=feature_dim)
+    if feature_dim > 16:
+        _spread_outputs_(model, 14)
     geo = model.geometry
     g = torch.Generator().manual_seed(13)
     n = 1000 + 3
