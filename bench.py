@@ -59,6 +59,9 @@ def algorithmic_bytes(kernel, spec, n_samples, n_rays, n_pairs, mixed=False):
     A workload held at its first band (start_level) gathers that many levels.
     """
     L, F = spec["levels"], 2
+    # per-sample outputs: sdf, grad, normal, feature (feature_dim + 3), laplace
+    # and the 12 FD / curvature values
+    n_outf = spec.get("feature_dim", 16) + 3 + 20
     La = spec.get("start_level") or L   # levels gathered (the band's active levels)
     bt = 2 if mixed else 4
     pair = 8 if mixed else 12
@@ -69,9 +72,9 @@ def algorithmic_bytes(kernel, spec, n_samples, n_rays, n_pairs, mixed=False):
     store = (n_ev * L * F + n_dx * 3 * L * F) * 4
     genc = n_genc * L * F * 4
     if kernel == "geo_fwd":
-        return n_samples * (n_ev * per_eval + 39 * 4 + store)
+        return n_samples * (n_ev * per_eval + n_outf * 4 + store)
     if kernel == "geo_bwd":
-        return n_samples * (store + 12 * 4 + 24 + 27 * 4 + genc)
+        return n_samples * (store + 12 * 4 + 24 + (n_outf - 12) * 4 + genc)
     if kernel == "pair_count":
         return n_samples * (24 + 12 * 4)
     if kernel == "pair_emit":
@@ -106,9 +109,26 @@ def profiled_traffic(workload):
     return None
 
 
-def cpu_baseline(spec_name, workload, seconds=15.0):
-    """The oracle (torch CPU, fp32, CUDA-semantics restatement) on a bounded
-    sample of the same workload: fwd + losses + bwd, samples/s."""
+def cpu_model_name():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(spec_name, workload, train_rays=64, render_rays=2048, warmup=3, steps=10):
+    """The oracle (torch CPU, fp32, CUDA-semantics restatement) on the host
+    cores, by the stated protocol on a bounded sample of the same workload:
+    training (NeuSModel.forward_ + losses + backward on train_rays rays of a
+    step's batch) with `warmup` untimed and the median of `steps` timed
+    steps, samples/s; rendering (the eval forward, chunk_batch semantics: one
+    chunk of render_rays rays of the same batch, no gradients), rays/s.
+    samples/s and rays/s are per sample and per ray, so the bounded sample
+    scales linearly to the workload's batch."""
     import numpy as np
     import torch
     from oracle import march as omarch, render as orender
@@ -117,21 +137,29 @@ def cpu_baseline(spec_name, workload, seconds=15.0):
     torch.set_num_threads(cores)
     model = workload.model
     spec = workload.spec
+    if spec.get("learned_background"):
+        return {"skipped": "the CPU baseline covers the foreground step (no background oracle "
+                           "pass in this measurement)"}
+    grad_type = spec.get("grad_type", "finite_difference")
+    r = float(model.geometry.radius)
+    n_all = max(train_rays, render_rays)
     b = workload.batches[0] if workload.uniform else \
         workload.sampler.sample(spec["rays"], torch.Generator(device=workload.device).manual_seed(1))
-    R = 128 if not workload.uniform else 64
-    rays = b["rays"][:R].detach().cpu()
-    gt = b["rgb"][:R].detach().cpu()
-    if workload.uniform:
-        from angelo_amd import scenes
-        ridx, t0, t1, _ = scenes.uniform_samples(rays, spec["samples_per_ray"])
-    else:
+    rays_all = b["rays"][:n_all].detach().cpu()
+    gt_all = b["rgb"][:n_all].detach().cpu()
+    box = [-r, -r, -r, r, r, r]
+
+    def samples_of(rays):
+        if workload.uniform:
+            from angelo_amd import scenes
+            ridx, t0, t1, _ = scenes.uniform_samples(rays, spec["samples_per_ray"])
+            return ridx, t0, t1
         og = model.occupancy_grid
-        tn, tf = omarch.ray_aabb(rays.numpy(), [-1, -1, -1, 1, 1, 1])
-        ri, a0, a1, _ = omarch.ray_marching(rays.numpy(), tn, tf, [-1, -1, -1, 1, 1, 1],
-                                            og.binary.cpu().numpy(), spec["step"], spec["cap"])
-        ridx, t0, t1 = torch.from_numpy(ri), torch.from_numpy(a0), torch.from_numpy(a1)
-    n = int(ridx.shape[0])
+        tn, tf = omarch.ray_aabb(rays.numpy(), box)
+        ri, a0, a1, _ = omarch.ray_marching(rays.numpy(), tn, tf, box, og.binary.cpu().numpy(),
+                                            model.render_step_size, model.max_samples_per_ray)
+        return torch.from_numpy(ri), torch.from_numpy(a0), torch.from_numpy(a1)
+
     geo, tex = model.geometry, model.texture
     g = geo.grid_module
     lt = ohg.level_table(g.n_levels, g.base_resolution, g.per_level_scale, g.log2_hashmap_size)
@@ -142,37 +170,64 @@ def cpu_baseline(spec_name, workload, seconds=15.0):
     rd = onets.mlp_dims(tex.network.dim_in, tex.network.dim_out, tex.network.n_hidden_layers)
     field = ofield.field_from(lambda x: ohg.encode_torch(x, table, lt, g.active_levels, "f32"),
                               lambda h: onets.vanilla_mlp(h, onets.unflatten(gflat, gd), "softplus"))
+    base = getattr(tex, "base_color", 0)
 
     def radiance(feat, dirs, nrm):
         sh = onets.sh_encode((dirs + 1.0) / 2.0, 4)
-        return torch.sigmoid(onets.vanilla_mlp(torch.cat([feat, sh, nrm], -1), onets.unflatten(rflat, rd)))
+        c = torch.sigmoid(onets.vanilla_mlp(torch.cat([feat, sh, nrm], -1),
+                                            onets.unflatten(rflat, rd)))
+        return c + torch.sigmoid(feat[:, 1:4]) if base else c
 
     var = model.variance.variance.detach().cpu().clone().requires_grad_(True)
-    randn = torch.randn(n, 3)
     eps = float(geo._finite_difference_eps)
-    times = []
-    t_start = time.time()
-    while len(times) < 3 or (time.time() - t_start < seconds and len(times) < 10):
-        t = time.perf_counter()
+
+    def forward(rays, ridx, t0, t1, randn):
         inv_s = torch.exp(var * 10.0).clip(1e-6, 1e6)
-        o = orender.neus_forward(rays, ridx, t0, t1, R, field, radiance, inv_s, 1.0, eps,
-                                 model.cos_anneal_ratio, randn)
-        loss, _ = orender.losses(o, gt)
+        return orender.neus_forward(rays, ridx, t0, t1, rays.shape[0], field, radiance, inv_s, r,
+                                    eps, model.cos_anneal_ratio, randn, grad_type=grad_type)
+
+    # training
+    rays, gt = rays_all[:train_rays], gt_all[:train_rays]
+    ridx, t0, t1 = samples_of(rays)
+    n = int(ridx.shape[0])
+    randn = torch.randn(n, 3)
+    times = []
+    for i in range(warmup + steps):
+        t = time.perf_counter()
+        loss, _ = orender.losses(forward(rays, ridx, t0, t1, randn), gt)
         loss.backward()
-        times.append(time.perf_counter() - t)
-        if time.time() - t_start > 4 * seconds:
-            break
+        if i >= warmup:
+            times.append(time.perf_counter() - t)
     med = sorted(times)[len(times) // 2]
+    # rendering: one chunk, no gradients
+    rays = rays_all[:render_rays]
+    ridx_r, t0_r, t1_r = samples_of(rays)
+    n_r = int(ridx_r.shape[0])
+    with torch.no_grad():
+        t = time.perf_counter()
+        forward(rays, ridx_r, t0_r, t1_r, torch.randn(n_r, 3))
+        t_render = time.perf_counter() - t
     return {"value": n / med, "unit": "samples/s", "cores": cores, "kind": "port",
-            "sample": f"{R} rays of the same workload, {n} samples, oracle fwd+loss+bwd in "
-                      f"torch CPU fp32, median of {len(times)} steps ({med:.2f} s/step)"}
+            "cpu_model": cpu_model_name(),
+            "sample": f"training: {train_rays} rays of a step's batch ({n} samples), oracle "
+                      f"fwd + loss + bwd in torch CPU fp32, {warmup} warm-up steps then the "
+                      f"median of {steps} ({med:.2f} s/step); samples/s is per sample, so it "
+                      f"scales linearly to the workload's {spec['rays']} rays",
+            "render_rays_per_s": round(rays.shape[0] / t_render, 1),
+            "render_sample": f"eval forward of one chunk of {rays.shape[0]} rays "
+                             f"({n_r} samples, chunk_batch chunk size 2048) in "
+                             f"{t_render:.2f} s, no gradients"}
 
 
-def serving_bench(wl, n_requests, rate_hz, seed=5):
+def serving_bench(wl, n_requests, load=0.6, seed=5):
     """Continuous-batching serving of a Poisson mix of renders (256^2, 512^2,
     1024^2 from seeded poses) and SDF lattices (128^3, 256^3, 512^3 in 64^3
-    chunks) on the workload's field, resident on the device, in the fp32 and
-    the mixed-precision form.  Returns p50/p99 latency and throughput."""
+    chunks) on the workload's field, resident on the device, with graph
+    replay, in the fp32 and the mixed-precision form.  Per form: a
+    saturation run (every request offered at once, offered load above
+    capacity: capacity rays/s and points/s, and the latency of queued
+    requests), then Poisson arrivals at `load` times that capacity (in
+    requests/s) for the latency at a stated load below capacity."""
     import torch
     from angelo_amd import serving
     model = wl.model
@@ -180,21 +235,102 @@ def serving_bench(wl, n_requests, rate_hz, seed=5):
     out = {}
     for prec in ("fp32", "mixed"):
         model.set_precision(prec)
-        srv = serving.FieldServer(model, rays_per_step=1 << 17, sdf_chunks_per_step=8)
-        warm = serving.poisson_requests(4, 1000.0, seed=seed + 100)
-        srv.run(warm, realtime=False)       # warm-up: workspaces, ray-direction tables
-        srv.rays_rendered = srv.points_queried = 0
-        reqs = serving.poisson_requests(n_requests, rate_hz, seed=seed)
-        wall = srv.run(reqs, realtime=True)
-        st = serving.latency_stats(reqs)
-        out[prec] = {"p50_ms": round(st["p50_s"] * 1e3, 2), "p99_ms": round(st["p99_s"] * 1e3, 2),
-                     "mean_ms": round(st["mean_s"] * 1e3, 2), "requests": st["n"],
-                     "rays_per_s": round(srv.rays_rendered / wall, 1),
-                     "sdf_points_per_s": round(srv.points_queried / wall, 1),
-                     "wall_s": round(wall, 3), "arrival_rate_hz": rate_hz,
-                     "graph_replay": False}
+        srv = serving.FieldServer(model, rays_per_step=1 << 17, sdf_chunks_per_step=8,
+                                  use_graphs=True)
+        warm = serving.poisson_requests(8, 1000.0, seed=seed + 100)
+        srv.run(warm, realtime=False)       # warm-up: graphs, workspaces, direction tables
+        res = {}
+        rate = None
+        for mode in ("saturation", "offered"):
+            srv.rays_rendered = srv.points_queried = 0
+            if mode == "saturation":
+                reqs = serving.poisson_requests(n_requests, 1000.0, seed=seed)
+                wall = srv.run(reqs, realtime=False)
+                rate = load * len(reqs) / wall
+            else:
+                reqs = serving.poisson_requests(max(n_requests // 2, 1), rate, seed=seed + 1)
+                wall = srv.run(reqs, realtime=True)
+            st = serving.latency_stats(reqs)
+            res[mode] = {"p50_ms": round(st["p50_s"] * 1e3, 2),
+                         "p99_ms": round(st["p99_s"] * 1e3, 2),
+                         "mean_ms": round(st["mean_s"] * 1e3, 2), "requests": st["n"],
+                         "rays_per_s": round(srv.rays_rendered / wall, 1),
+                         "sdf_points_per_s": round(srv.points_queried / wall, 1),
+                         "wall_s": round(wall, 3)}
+            if mode == "offered":
+                res[mode]["arrival_rate_hz"] = round(rate, 3)
+            log(f"[bench] serving {prec} {mode}: {json.dumps(res[mode])}")
         torch.cuda.synchronize()
+        sat = res["saturation"]
+        out[prec] = {"capacity_rays_per_s": sat["rays_per_s"],
+                     "capacity_sdf_points_per_s": sat["sdf_points_per_s"],
+                     "p50_ms": sat["p50_ms"], "p99_ms": sat["p99_ms"], "requests": sat["requests"],
+                     "saturation": sat, "offered": res["offered"], "graph_replay": srv.use_graphs,
+                     "captured_graphs": len(srv._render_graphs) + (srv._sdf_graph is not None)}
     model.set_precision("fp32")
+    return out
+
+
+def time_steps(wl, steps, warmup):
+    """W untimed steps, then K timed steps bracketed by device syncs and HIP
+    events on the current stream, with the library's kernel timers on."""
+    import torch
+    from angelo_amd import _native as nat
+    for i in range(warmup):
+        wl.step(i)
+    torch.cuda.synchronize()
+    log(f"[bench] warmup done ({warmup} steps); timing {steps} steps")
+    nat.timer_enable(True)
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    total_samples = 0
+    bg_before = wl.bg_samples
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    start.record()
+    step_wall = []
+    for i in range(steps):
+        ts = time.perf_counter()
+        total_samples += wl.step(warmup + i)
+        step_wall.append(time.perf_counter() - ts)
+    stop.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    nat.sdf_bwd_check()   # the pair passes of every timed backward agreed
+    kern = {}
+    for k in TIMED_KERNELS:
+        ms, cnt = nat.timer_read(k)
+        if cnt:
+            kern[k] = {"total_ms": ms, "launches": cnt, "avg_ms": ms / cnt}
+    pairs = nat.timer_units("table_pairs") / steps
+    nat.timer_enable(False)
+    return {"gpu_ms": start.elapsed_time(stop), "wall": wall, "step_wall": step_wall,
+            "kern": kern, "samples": total_samples, "pairs_per_step": pairs,
+            "bg_samples": wl.bg_samples - bg_before}
+
+
+def mixed_bench(dev, name, steps, warmup, fp32_value, rays=None):
+    """The mixed-precision form of the same step (fp16 table rows, bf16 MFMA
+    radiance, fp16 table-gradient pairs with dynamic loss scaling, Kahan
+    compositing, fp64 loss reductions): samples/s, its ratio to the fp32
+    form, per-kernel times and the loss scaler's state."""
+    from angelo_amd.workloads import Workload, WORKLOADS
+    if name not in WORKLOADS:
+        return None
+    log(f"[bench] building workload {name}")
+    wl = Workload(name, dev, rays=rays)
+    t = time_steps(wl, steps, warmup)
+    value = t["samples"] / (t["gpu_ms"] / 1e3)
+    out = {"workload": name, "value": round(value, 1), "unit": "samples/s",
+           "ms_per_step": round(t["gpu_ms"] / steps, 4),
+           "samples_per_step": round(t["samples"] / steps, 1),
+           "ratio_to_fp32": round(value / fp32_value, 4),
+           "kernels_ms": {k: round(v["avg_ms"], 4) for k, v in t["kern"].items()},
+           "table_pairs_per_step": round(t["pairs_per_step"], 1),
+           "final_loss_scale": wl.scaler.scale if wl.scaler else None,
+           "skipped_steps": wl.skipped_steps}
+    log(f"[bench] mixed: {out['ms_per_step']:.3f} ms/step, {value / 1e6:.2f} M samples/s, "
+        f"{out['ratio_to_fp32']:.3f}x fp32")
     return out
 
 
@@ -205,10 +341,16 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="neuralangelo-fp32")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-rays", type=int, default=64,
+                    help="rays of the CPU baseline's training steps (1024: the full protocol)")
+    ap.add_argument("--no-mixed", action="store_true",
+                    help="skip the mixed-precision form of the workload")
     ap.add_argument("--rays", type=int, default=None)
-    ap.add_argument("--serve-requests", type=int, default=16,
+    ap.add_argument("--serve-requests", type=int, default=200,
                     help="requests in the serving measurement (0: skip it)")
-    ap.add_argument("--serve-rate", type=float, default=4.0, help="Poisson arrival rate (1/s)")
+    ap.add_argument("--serve-load", type=float, default=0.6,
+                    help="Poisson arrival rate of the offered-load run, as a fraction of the "
+                         "saturation run's requests/s")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run a plain 1 GiB device-to-device copy three times (the "
                          "reference kernel for scaling HBM counters of a profile)")
@@ -237,38 +379,12 @@ def main():
     log(f"[bench] building workload {args.workload}")
     wl = Workload(args.workload, dev, rays=args.rays)
     spec = wl.spec
-    for i in range(args.warmup):
-        wl.step(i)
-    torch.cuda.synchronize()
-    log(f"[bench] warmup done ({args.warmup} steps); timing {args.steps} steps")
-    nat.timer_enable(True)
-    start = torch.cuda.Event(enable_timing=True)
-    stop = torch.cuda.Event(enable_timing=True)
-    total_samples = 0
-    bg_before = wl.bg_samples
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    start.record()
-    step_wall = []
-    for i in range(args.steps):
-        ts = time.perf_counter()
-        total_samples += wl.step(args.warmup + i)
-        step_wall.append(time.perf_counter() - ts)
-    stop.record()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    nat.sdf_bwd_check()   # the pair passes of every timed backward agreed
-    gpu_ms = start.elapsed_time(stop)
-    kern = {}
-    for k in TIMED_KERNELS:
-        ms, cnt = nat.timer_read(k)
-        if cnt:
-            kern[k] = {"total_ms": ms, "launches": cnt, "avg_ms": ms / cnt}
-    pairs_per_step = nat.timer_units("table_pairs") / args.steps
-    nat.timer_enable(False)
+    t = time_steps(wl, args.steps, args.warmup)
+    gpu_ms, wall, step_wall, kern = t["gpu_ms"], t["wall"], t["step_wall"], t["kern"]
+    total_samples, pairs_per_step = t["samples"], t["pairs_per_step"]
     ms_per_step = gpu_ms / args.steps
     samples_per_step = total_samples / args.steps
-    bg_per_step = (wl.bg_samples - bg_before) / args.steps
+    bg_per_step = t["bg_samples"] / args.steps
     fg_per_step = samples_per_step - bg_per_step   # the byte models count foreground samples
     rays = spec["rays"]
     value = total_samples / (gpu_ms / 1e3)
@@ -313,16 +429,22 @@ def main():
             if pk.get("traffic"):
                 roof["traffic_source"] = f"profiles/hbm_traffic_{args.workload}.md"
                 roof["traffic_gbs"] = round(pk["traffic"] / (pk["avg_ms"] / 1e3) / 1e9, 1)
+    mixed = None
+    mixed_name = args.workload.replace("-fp32", "-mixed") if "-fp32" in args.workload else None
+    if not args.no_mixed and not wl.mixed and mixed_name:
+        del t
+        mixed = mixed_bench(dev, mixed_name, args.steps, args.warmup, value, rays=args.rays)
     serve = None
     if args.serve_requests > 0:
-        log(f"[bench] serving: {args.serve_requests} requests at {args.serve_rate}/s ...")
-        serve = serving_bench(wl, args.serve_requests, args.serve_rate)
+        log(f"[bench] serving: {args.serve_requests} requests at saturation, then at "
+            f"{args.serve_load} of capacity ...")
+        serve = serving_bench(wl, args.serve_requests, args.serve_load)
         log(f"[bench] serving: {json.dumps(serve)}")
     cpu = None
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline (oracle) ...")
         try:
-            cpu = cpu_baseline(args.workload, wl)
+            cpu = cpu_baseline(args.workload, wl, train_rays=args.cpu_rays)
         except Exception as e:  # noqa: BLE001 - reported, not fatal for the GPU number
             cpu = {"error": repr(e)}
     line = {
@@ -347,6 +469,7 @@ def main():
         "kernels_roofline": per_kernel,
         "table_pairs_per_step": round(pairs_per_step, 1),
         "roofline": roof,
+        "mixed": mixed,
         "serving": serve,
         "cpu_baseline": cpu,
     }

@@ -13,7 +13,8 @@ its result buffers fit the HBM budget next to those already admitted
 Graph replay (use_graphs=True): the SDF batch is one captured graph over a
 fixed point buffer; the render batch marches eagerly (the sample count is
 data-dependent), then replays the graph captured for the sample-count bucket
-(`sample_bucket` times the smallest power of two that holds the count) over
+(`sample_bucket`: base * 2^k * {1, 5/4, 3/2, 7/4}, the smallest that holds
+the count) over
 sample positions, fused SDF field (base + FD normal), fused radiance and
 compositing, with padded samples outside every ray's range.  The captured
 regions hold only kernel launches on buffers that live as long as the
@@ -89,6 +90,19 @@ def _lattice_axes(resolution, radius):
 def _lattice_size(resolution, radius):
     a, b, c = _lattice_axes(resolution, radius)
     return len(a) * len(b) * len(c)
+
+
+def sample_bucket(n, base):
+    """Sample-count bucket of a render batch: the smallest of base * 2^k *
+    {1, 5/4, 3/2, 7/4} that holds n (a captured graph per bucket; padding
+    stays under a quarter of the batch)."""
+    nb = int(base)
+    while True:
+        for q in (4, 5, 6, 7):
+            cand = nb * q // 4
+            if cand >= n:
+                return cand
+        nb *= 2
 
 
 class FieldServer:
@@ -201,9 +215,7 @@ class FieldServer:
         ridx, t0, t1, packed = fn.ray_marching(rays, t_near, t_far, self.aabb, grid,
                                                m.render_step_size, None, m.max_samples_per_ray)
         n = int(ridx.shape[0])
-        nb = self.bucket
-        while nb < n:   # power-of-two buckets: few captured graphs
-            nb *= 2
+        nb = sample_bucket(n, self.bucket)
         if not self.use_graphs or getattr(m, "learned_background", False):
             return self._render_stage(rays, ridx, t0, t1, packed)
         g = self._render_graphs.get(nb)

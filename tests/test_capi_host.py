@@ -399,3 +399,19 @@ def test_neuralangelo_model_builds_with_the_reference_shapes():
     assert tex.n_input_dims == 87 and tex.base_color == 1
     assert model.learned_background and model.geometry_bg.n_output_dims == 8
     assert geo.grid_module.log2_hashmap_size == 19
+
+
+def test_serving_sample_buckets():
+    """Render-stage graph buckets: base * 2^k * {1, 5/4, 3/2, 7/4}, the
+    smallest holding the batch; padding stays under a quarter."""
+    from angelo_amd.serving import sample_bucket
+    base = 1 << 10
+    assert sample_bucket(0, base) == base and sample_bucket(base, base) == base
+    assert sample_bucket(base + 1, base) == base * 5 // 4
+    prev = 0
+    for n in range(1, 40 * base, 97):
+        nb = sample_bucket(n, base)
+        assert nb >= n and nb % 16 == 0
+        assert n <= base or (nb - n) < nb / 4 + 1
+        assert nb >= prev
+        prev = nb
