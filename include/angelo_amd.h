@@ -56,6 +56,15 @@ int am_timer_read(const char* name, double* total_ms, int64_t* count);
  * "table_pairs": the (row, value) pairs the table-gradient sorts received). */
 int am_timer_units(const char* name, int64_t* units);
 
+/* Implementation variant switches, for measurement and tests (no
+ * counterpart in the reference).  "table_prepartition" (default 0): 1 makes
+ * the SDF backward's pair pass write its (row, value) pairs partitioned by
+ * bits 9-16 of their row, so the table-gradient sort makes one pass over the
+ * remaining bits instead of two (it measured slower: scattered stores); 0:
+ * compact per-tile pairs and two sort passes.  Both give bit-identical gradients.  Unknown names return
+ * AM_ERR_INVALID. */
+int am_set_flag(const char* name, int32_t value);
+
 /* ------------------------------------------------------------------------ */
 /* Multiresolution hash grid (tcnn HashGrid through ProgressiveBandHashGrid,  */
 /* models/network_utils.py:40-66, and get_encoding, :83-93).                 */

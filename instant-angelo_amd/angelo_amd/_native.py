@@ -93,6 +93,7 @@ _SIGS = {
     "am_timer_enable": (ctypes.c_int, [_I32]),
     "am_timer_read": (ctypes.c_int, [ctypes.c_char_p, _P, _P]),
     "am_timer_units": (ctypes.c_int, [ctypes.c_char_p, _P]),
+    "am_set_flag": (ctypes.c_int, [ctypes.c_char_p, _I32]),
     "am_grid_init": (ctypes.c_int, [_P, _I32, _I32, _I32, _I32, _F, _P]),
     "am_hashgrid_fwd": (ctypes.c_int, [_P, _P, _I64, _P, _I32, _P, _P]),
     "am_hashgrid_bwd_workspace_size": (_SZ, [_P, _I64, _I32]),
@@ -251,6 +252,12 @@ def timer_read(name):
     cnt = ctypes.c_int64(0)
     check(load().am_timer_read(name.encode(), ctypes.byref(ms), ctypes.byref(cnt)), "am_timer_read")
     return ms.value, cnt.value
+
+
+def set_flag(name, value):
+    """Selects an implementation variant (am_set_flag), e.g.
+    set_flag("table_prepartition", 0)."""
+    check(load().am_set_flag(name.encode(), int(value)), "am_set_flag")
 
 
 def timer_units(name):

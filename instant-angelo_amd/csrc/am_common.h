@@ -51,6 +51,9 @@ bool timer_on();
 void timer_push(const char* name, hipEvent_t a, hipEvent_t b);
 // Adds work units (e.g. table-gradient pairs) under `name` while timers are on.
 void timer_units(const char* name, int64_t units);
+// Implementation variant switches set by am_set_flag (name -> value), read
+// on the host at launch time; `dflt` when the flag was never set.
+int flag_value(const char* name, int dflt);
 hipEvent_t timer_event();
 // While set, timed launches are booked under this name instead of their own
 // (an entry point reusing another's kernels, e.g. the occupancy update).

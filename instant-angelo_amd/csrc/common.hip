@@ -5,6 +5,10 @@
 #include <string>
 #include <vector>
 #include "am_common.h"
+#include <mutex>
+#include <string>
+#include <utility>
+#include <vector>
 
 namespace am {
 
@@ -148,6 +152,34 @@ extern "C" int am_timer_read(const char* name, double* total_ms, int64_t* count)
     }
     *total_ms = tot;
     *count = c;
+    return AM_OK;
+}
+
+namespace am {
+static std::mutex g_flag_mu;
+static std::vector<std::pair<std::string, int>> g_flags;
+
+int flag_value(const char* name, int dflt) {
+    std::lock_guard<std::mutex> lock(g_flag_mu);
+    for (const auto& f : g_flags)
+        if (f.first == name) return f.second;
+    return dflt;
+}
+}  // namespace am
+
+extern "C" int am_set_flag(const char* name, int32_t value) {
+    AM_CHECK_ARG(name, "NULL flag name");
+    static const char* known[] = {"table_prepartition"};
+    bool ok = false;
+    for (const char* k : known) ok = ok || std::string(k) == name;
+    AM_CHECK_ARG(ok, "unknown flag %s", name);
+    std::lock_guard<std::mutex> lock(am::g_flag_mu);
+    for (auto& f : am::g_flags)
+        if (f.first == name) {
+            f.second = value;
+            return AM_OK;
+        }
+    am::g_flags.emplace_back(name, value);
     return AM_OK;
 }
 

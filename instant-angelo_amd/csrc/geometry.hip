@@ -875,11 +875,34 @@ __device__ __forceinline__ void geo_eval_bwd_an(const GeoArgs& a, const float* s
 // writes past limit.
 template <class VT>
 struct EmitTile {
-    uint32_t* keys;   // tile's first pair (keys)
-    VT* vals;         // tile's first pair (values)
+    uint32_t* keys;   // tile's first pair (keys); partitioned: the arrays' start
+    VT* vals;         // tile's first pair (values); partitioned: the arrays' start
     int cursor;       // wave-uniform write position
     int limit;        // pairs reserved for the tile
+    // Partitioned form (part != nullptr): the pairs of the tile with radix
+    // digit d = (row >> 9) & 255 go to base[d] + [0, lim[d]); part[d] counts
+    // them as they are written (in the count pass: the counts themselves).
+    // Wave-private LDS arrays of kPartDigits entries.
+    int32_t* part;
+    const int32_t* lim;
+    const int64_t* base;
 };
+
+// Partitioned pair emission (flag "table_prepartition", off by default): the
+// first 8-bit digit of the table-gradient radix sort (key bits [9, 17)) is
+// applied as pairs are written, and the sort makes one pass over [17, 25)
+// instead of two.  Measured on the Neuralangelo step it loses: the sort
+// drops from 4.6 to 2.4 ms, but the emission's stores scatter over 256
+// partitions and write partial lines (emit 3.0 -> 7.7 ms, count 0.4 ->
+// 0.9 ms); 32 partitions with one 11-bit pass measured no better (emit 5.5,
+// sort 4.7 ms).
+constexpr int kPartShift = 9;
+constexpr int kPartBits = 8;
+constexpr int kPartDigits = 1 << kPartBits;
+
+__device__ __forceinline__ uint32_t pair_digit(uint32_t row) {
+    return (row >> kPartShift) & (uint32_t)(kPartDigits - 1);
+}
 
 __device__ __forceinline__ uint64_t lanemask_lt() {
     const int l = lane_id();
@@ -906,6 +929,41 @@ __device__ __forceinline__ void emit_pairs(EmitTile<VT>& et, bool on, const uint
                                            const float vals[2 * K]) {
     const uint64_t mask = __ballot(on);
     const int n_on = __popcll(mask);
+    if (et.part) {   // partitioned form (rows are computed in both passes)
+        if (mask == 0) return;
+        const uint64_t lt = lanemask_lt();
+#pragma unroll
+        for (int c = 0; c < K; ++c) {
+            const uint32_t d = on ? pair_digit(rows[c]) : 0u;
+            if constexpr (COUNT) {
+                if (on) atomicAdd(&et.part[d], 1);   // counts: order-free
+            } else {
+                // lanes with the same digit (one ballot a bit), ranked by lane: the
+                // order inside a partition is corner-major, then lane, as in
+                // the compact form, so every sum keeps its order
+                uint64_t peers = mask;
+#pragma unroll
+                for (int b = 0; b < kPartBits; ++b) {
+                    const uint64_t ones = __ballot(on && ((d >> b) & 1u));
+                    peers &= ((d >> b) & 1u) ? ones : ~ones;
+                }
+                const int before = __popcll(peers & lt);
+                // one wave's LDS operations complete in issue order: every
+                // lane reads part[d] before its group's first lane updates it
+                const int have = on ? et.part[d] : 0;
+                if (on && before == 0) et.part[d] = have + __popcll(peers);
+                if (on && have + before < et.lim[d]) {
+                    const int64_t at = et.base[d] + have + before;
+                    et.keys[at] = rows[c];
+                    VT v;
+                    set_pair(v, vals[2 * c], vals[2 * c + 1]);
+                    et.vals[at] = v;
+                }
+            }
+        }
+        et.cursor += K * n_on;
+        return;
+    }
     if (!COUNT && on && et.cursor + K * n_on <= et.limit) {
         const int off = et.cursor + __popcll(mask & lanemask_lt());
 #pragma unroll
@@ -1057,8 +1115,8 @@ __device__ __forceinline__ void flush_cluster(const GeoArgs& a, const LevelParam
                 const bool em = on && !same && !adj;
                 uint32_t rows[8];
                 float v[16];
+                if ((!COUNT || et.part) && em) corner_rows(lp, pp.c, rows);
                 if (!COUNT && em) {
-                    corner_rows(lp, pp.c, rows);
 #pragma unroll
                     for (int c = 0; c < 8; ++c) {
                         const float w = corner_weight(pp, c);
@@ -1073,13 +1131,19 @@ __device__ __forceinline__ void flush_cluster(const GeoArgs& a, const LevelParam
                 const bool em = on && adj;
                 uint32_t rows[4];
                 float v[8];
-                if (!COUNT && em) {
+                if ((!COUNT || et.part) && em) {
 #pragma unroll
                     for (int i = 0; i < 4; ++i) {
                         const int c = face_corner(i, axm, far_bit);
                         rows[i] = lp.offset + grid_row(lp, pp.c[0] + (c & 1),
                                                        pp.c[1] + ((c >> 1) & 1),
                                                        pp.c[2] + ((c >> 2) & 1));
+                    }
+                }
+                if (!COUNT && em) {
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const int c = face_corner(i, axm, far_bit);
                         const float w = corner_weight(pp, c);
                         v[2 * i] = w * g0;
                         v[2 * i + 1] = w * g1;
@@ -1139,7 +1203,7 @@ __device__ __forceinline__ void flush_cluster(const GeoArgs& a, const LevelParam
         const uint32_t u0 = row_up1(key0), u1 = row_up1(key1), u2 = row_up1(key2);
         const bool head = live && (p == 0 || !(u0 == key0 && u1 == key1 && u2 == key2));
         uint32_t rows[8];
-        if (!COUNT && head) corner_rows(lp, B, rows);
+        if ((!COUNT || et.part) && head) corner_rows(lp, B, rows);
         emit_pairs<8, COUNT, VT>(et, head, rows, acc);
     }
 }
@@ -1194,13 +1258,21 @@ __host__ __device__ inline int64_t geo_genc_floats(int64_t tiles, int n_evals, i
 // samples emits, from positions alone.  Otherwise: the same decisions with
 // the encode gradients, each tile's pairs written from offsets[tile] (never
 // past counts[tile]; a different count sets status[1]).
-template <int LJ, bool COUNT, class VT>
+// PART: the pairs are partitioned by their digit d = (row >> 9) & 255 as
+// they are written: counts and offsets then hold kPartDigits entries per
+// tile, digit-major ([d * n_tiles + tile]), and a tile's pairs of digit d go
+// to [offsets[d n_tiles + tile], + counts[...]) in emission order.
+template <int LJ, bool COUNT, class VT, bool PART>
 __global__ void __launch_bounds__(256) k_geo_pairs(
     GeoArgs a, const float* __restrict__ pts, const float* __restrict__ randn, int64_t N,
     const float* __restrict__ fdv, const float* __restrict__ genc, uint32_t* __restrict__ keys,
     VT* __restrict__ vals, const int64_t* __restrict__ offsets, int64_t* __restrict__ counts,
     int64_t* __restrict__ status) {
     __shared__ LevelParams lv[AM_MAX_LEVELS];
+    // per wave: running count, limit and first position of each digit
+    __shared__ int32_t part_s[PART ? 4 : 1][PART ? kPartDigits : 1];
+    __shared__ int32_t lim_s[PART && !COUNT ? 4 : 1][PART && !COUNT ? kPartDigits : 1];
+    __shared__ int64_t base_s[PART && !COUNT ? 4 : 1][PART && !COUNT ? kPartDigits : 1];
 #pragma unroll
     for (int q = 0; q < AM_MAX_LEVELS; ++q)
         if ((int)threadIdx.x == q && q < a.g.L) lv[q] = a.g.lv[q];
@@ -1214,17 +1286,46 @@ __global__ void __launch_bounds__(256) k_geo_pairs(
         const bool valid = n < N;
         GeoSample s;
         geo_rebuild(a, pts, randn, fdv, n, caller_row(a, n, valid), valid, s);
-        EmitTile<VT> et{nullptr, nullptr, 0, 0};
+        EmitTile<VT> et{nullptr, nullptr, 0, 0, nullptr, nullptr, nullptr};
         const float* gt = nullptr;
-        if constexpr (!COUNT) {
+        if constexpr (PART) {
+            int32_t* part = part_s[wave];
+            for (int d = l; d < kPartDigits; d += 64) {
+                part[d] = 0;
+                if constexpr (!COUNT) {
+                    const int64_t at = (int64_t)d * n_tiles + tile;
+                    base_s[wave][d] = offsets[at];
+                    lim_s[wave][d] = (int32_t)counts[at];
+                }
+            }
+            mlp::wave_sync();
+            if constexpr (COUNT) {
+                et.part = part;
+            } else {
+                et = EmitTile<VT>{keys, vals, 0, 0, part, lim_s[wave], base_s[wave]};
+                gt = genc + tile * a.genc_slots * LJ * 128;
+            }
+        } else if constexpr (!COUNT) {
             const int64_t off = offsets[tile];
-            et = EmitTile<VT>{keys + off, vals + off, 0, (int)counts[tile]};
+            et = EmitTile<VT>{keys + off, vals + off, 0, (int)counts[tile], nullptr, nullptr,
+                              nullptr};
             gt = genc + tile * a.genc_slots * LJ * 128;
         }
         if (a.with_laplace) flush_cluster<LJ, 1, COUNT>(a, lv, s, valid, gt, 6, a.curv0, et);
         flush_cluster<LJ, 0, COUNT>(a, lv, s, valid, gt, a.with_grad && !a.analytic ? 7 : 1, 0,
                                     et);
-        if constexpr (COUNT) {
+        if constexpr (PART) {
+            mlp::wave_sync();
+            bool bad = false;
+            for (int d = l; d < kPartDigits; d += 64) {
+                const int32_t got = et.part[d];
+                if constexpr (COUNT) counts[(int64_t)d * n_tiles + tile] = got;
+                else bad = bad || got != et.lim[d];
+            }
+            if constexpr (!COUNT)
+                if (__ballot(bad) != 0 && l == 0) status[1] = 1;
+            mlp::wave_sync();
+        } else if constexpr (COUNT) {
             if (l == 0) counts[tile] = et.cursor;
         } else {
             if (l == 0 && et.cursor != et.limit) status[1] = 1;
@@ -1479,8 +1580,9 @@ k_geo_bwd_o2(AM_GEO_BWD_PARAMS) {
     geo_bwd_body<LJ, 1, ST, PR, AN>(AM_GEO_BWD_ARGS);
 }
 
-// status[0] = number of pairs (offset + count of the last tile); status[1]
-// = 0, set by the backward on a count mismatch.
+// status[0] = number of pairs (offset + count of the last entry: the last
+// tile, or the last digit of the last tile); status[1] = 0, set by the
+// backward on a count mismatch.
 __global__ void k_pair_total(const int64_t* __restrict__ offsets, const int64_t* __restrict__ counts,
                              int64_t tiles, int64_t* __restrict__ status) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
@@ -1603,12 +1705,26 @@ static int geo_fwd_launch(const GeoArgs& a, const float* pts, const float* randn
 
 template <int LJ>
 static int geo_count_launch(const GeoArgs& a, const float* pts, const float* randn, int64_t n,
-                            const float* fdv, int64_t* counts, hipStream_t s) {
+                            const float* fdv, int64_t* counts, bool part, hipStream_t s) {
     KernelTimer kt_("pair_count", s);
-    hipLaunchKernelGGL((k_geo_pairs<LJ, true, Vals<2>>), dim3(field_blocks(n)), dim3(256), 0, s, a,
-                       pts, randn, n, fdv, nullptr, nullptr, nullptr, nullptr, counts, nullptr);
+    if (part)
+        hipLaunchKernelGGL((k_geo_pairs<LJ, true, Vals<2>, true>), dim3(field_blocks(n)), dim3(256),
+                           0, s, a, pts, randn, n, fdv, nullptr, nullptr, nullptr, nullptr, counts,
+                           nullptr);
+    else
+        hipLaunchKernelGGL((k_geo_pairs<LJ, true, Vals<2>, false>), dim3(field_blocks(n)),
+                           dim3(256), 0, s, a, pts, randn, n, fdv, nullptr, nullptr, nullptr,
+                           nullptr, counts, nullptr);
     AM_LAUNCH_CHECK();
     return AM_OK;
+}
+
+// Partitioned pair emission applies when the table reduce sorts key bits
+// [9, end) (9 < end <= 25) and the "table_prepartition" flag is on (off by
+// default: it measured slower, see kPartDigits).
+static bool geo_prepartition(const am_grid_desc* g) {
+    return table_bucket_bits(table_key_bits(g->n_entries)) == kPartShift &&
+           flag_value("table_prepartition", 0) != 0;
 }
 
 // The backward's workspace before the pair arrays: the pair count is only
@@ -1622,26 +1738,30 @@ struct GeoBwdWs {
     float* ctmp;
     int64_t* counts;
     int64_t* offsets;
+    int64_t entries;   // count / offset entries (tiles, or tiles x digits)
     int64_t* status;
     void* scan_tmp;
     size_t scan_bytes;
 };
 
 static void carve_geo_bwd_head(WsCarver& c, const GeoArgs& a, int LJ, int64_t n, bool need_fdv,
-                               GeoBwdWs* w) {
+                               bool part, GeoBwdWs* w) {
     const int waves = field_blocks(n) * 4;
     const int64_t tiles = (n + 15) / 16;
+    const int64_t entries = tiles * (part ? kPartDigits : 1);   // count / offset entries
     w->fdv = need_fdv ? c.take<float>((size_t)n * 12 + 1) : nullptr;
     w->sdf_tmp = need_fdv ? c.take<float>((size_t)n + 1) : nullptr;
     w->genc = c.take<float>((size_t)geo_genc_floats(tiles, a.genc_slots, LJ) + 1);
     w->partial = c.take<float>((size_t)waves * a.n_params);
     w->ctmp = c.take<float>((size_t)colsum_tmp_floats(waves, a.n_params) + 1);
-    w->counts = c.take<int64_t>((size_t)tiles + 1);
-    w->offsets = c.take<int64_t>((size_t)tiles + 1);
+    w->counts = c.take<int64_t>((size_t)entries + 1);
+    w->offsets = c.take<int64_t>((size_t)entries + 1);
+    w->entries = entries;
     w->status = c.take<int64_t>(2);
     size_t sb = 0;
     rocprim::exclusive_scan(nullptr, sb, (const int64_t*)nullptr, (int64_t*)nullptr, (int64_t)0,
-                            (size_t)(tiles > 0 ? tiles : 1), rocprim::plus<int64_t>(), 0, false);
+                            (size_t)(entries > 0 ? entries : 1), rocprim::plus<int64_t>(), 0,
+                            false);
     w->scan_bytes = sb;
     w->scan_tmp = c.take<char>(sb + 256);
 }
@@ -1731,7 +1851,8 @@ static int geo_bwd_launch_t(const GeoArgs& a, const am_grid_desc* g, const float
     AM_CHECK_ARG(!store || !need_fdv, "a stored encoding needs the stored FD values too");
     WsCarver c(ws, ws_bytes);
     GeoBwdWs w;
-    carve_geo_bwd_head(c, a, LJ, n, need_fdv, &w);
+    const bool part = geo_prepartition(g);
+    carve_geo_bwd_head(c, a, LJ, n, need_fdv, part, &w);
     if (!c.ok()) return workspace_short(c.used, ws_bytes);
     const float* fdv = fdv_in;
     if (need_fdv) {  // recompute the stored SDF values with a forward pass
@@ -1741,15 +1862,16 @@ static int geo_bwd_launch_t(const GeoArgs& a, const am_grid_desc* g, const float
         fdv = w.fdv;
     }
     // 1. pairs per tile, 2. exclusive prefix sum -> each tile's first pair
-    if (int rc = geo_count_launch<LJ>(a, pts, randn, n, fdv, w.counts, s)) return rc;
+    if (int rc = geo_count_launch<LJ>(a, pts, randn, n, fdv, w.counts, part, s)) return rc;
     size_t sb = w.scan_bytes;
     hipError_t e = rocprim::exclusive_scan(w.scan_tmp, sb, w.counts, w.offsets, (int64_t)0,
-                                           (size_t)tiles, rocprim::plus<int64_t>(), s, false);
+                                           (size_t)w.entries, rocprim::plus<int64_t>(), s, false);
     if (e != hipSuccess) {
         set_error("tile-count scan failed: %s", hipGetErrorString(e));
         return AM_ERR_HIP;
     }
-    hipLaunchKernelGGL(k_pair_total, dim3(1), dim3(64), 0, s, w.offsets, w.counts, tiles, w.status);
+    hipLaunchKernelGGL(k_pair_total, dim3(1), dim3(64), 0, s, w.offsets, w.counts, w.entries,
+                       w.status);
     AM_LAUNCH_CHECK();
     // 3. one host read of the pair count; the pair arrays are carved for
     // exactly that many pairs (a workspace of the size query's worst-case
@@ -1760,7 +1882,8 @@ static int geo_bwd_launch_t(const GeoArgs& a, const am_grid_desc* g, const float
     AM_HIP(hipStreamSynchronize(s));
     if (int rc = collect_flags(false)) return rc;
     TableReduceWs tw;
-    if (int rc = table_reduce_carve(c, 2, st[0], g->n_entries, &tw, PR == mlp::kBF16)) return rc;
+    if (int rc = table_reduce_carve(c, 2, st[0], g->n_entries, &tw, PR == mlp::kBF16, part))
+        return rc;
     if (!c.ok()) return workspace_short(c.used, ws_bytes);
     const size_t lds =
         (size_t)(geo_ext_base(a) + 4 * (a.out_tiles - 1) * kOutExtFloats) * sizeof(float);
@@ -1786,9 +1909,16 @@ static int geo_bwd_launch_t(const GeoArgs& a, const am_grid_desc* g, const float
     AM_LAUNCH_CHECK();
     {
         KernelTimer kt_("pair_emit", s);
-        hipLaunchKernelGGL((k_geo_pairs<LJ, false, PairV<PR>>), dim3(blocks), dim3(256), 0, s, a, pts,
-                           randn, n, fdv, w.genc, tw.keys[0],
-                           reinterpret_cast<PairV<PR>*>(tw.vals[0]), w.offsets, w.counts, w.status);
+        if (part)
+            hipLaunchKernelGGL((k_geo_pairs<LJ, false, PairV<PR>, true>), dim3(blocks), dim3(256),
+                               0, s, a, pts, randn, n, fdv, w.genc, tw.keys[0],
+                               reinterpret_cast<PairV<PR>*>(tw.vals[0]), w.offsets, w.counts,
+                               w.status);
+        else
+            hipLaunchKernelGGL((k_geo_pairs<LJ, false, PairV<PR>, false>), dim3(blocks),
+                               dim3(256), 0, s, a, pts, randn, n, fdv, w.genc, tw.keys[0],
+                               reinterpret_cast<PairV<PR>*>(tw.vals[0]), w.offsets, w.counts,
+                               w.status);
     }
     AM_LAUNCH_CHECK();
     if (int rc = colsum_f32(w.partial, waves, a.n_params, dparams, w.ctmp, s)) return rc;
@@ -1881,9 +2011,11 @@ extern "C" size_t am_sdf_bwd_workspace_size(const am_grid_desc* g, const am_mlp_
     if (make_geo_args(g, m, sd, &a, &LJ) || n < 0) return 0;
     WsCarver c(nullptr, 0);
     GeoBwdWs w;
-    carve_geo_bwd_head(c, a, LJ, n, true, &w);
+    const bool part = geo_prepartition(g);
+    carve_geo_bwd_head(c, a, LJ, n, true, part, &w);
     TableReduceWs tw;
-    if (table_reduce_carve(c, 2, geo_max_pairs(a, LJ, n), g->n_entries, &tw)) return 0;
+    if (table_reduce_carve(c, 2, geo_max_pairs(a, LJ, n), g->n_entries, &tw, false, part))
+        return 0;
     return c.used + 256;
 }
 

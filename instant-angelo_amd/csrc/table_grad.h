@@ -50,7 +50,15 @@ struct TableReduceWs {
     int begin_bit;          // key bits sorted: [begin_bit, end_bit)
     int end_bit;
     bool half_vals;  // values are ValsH<F> instead of Vals<F>
+    // The emitter wrote the pairs partitioned by key bits [9, 17) already
+    // (in emission order inside each partition): the sort makes one pass over
+    // bits [17, end_bit), or none for end_bit <= 17.
+    bool prepart;
 };
+
+// First key bit the sort of prepartitioned pairs orders (bits [9, 17) are
+// the emitter's partition digit).
+constexpr int kPrepartBits = 17;
 
 // Bucket bits of the bucketed form: the low key bits left unsorted (at most
 // 9, so a wave's accumulators span at most 512 rows).
@@ -65,7 +73,7 @@ static inline uint32_t table_sentinel(int end_bit) { return (uint32_t)((1ull << 
 // Returns the sort's temp byte count through TableReduceWs (0 when c has no
 // base pointer: size query; the sort size query needs a device).
 int table_reduce_carve(WsCarver& c, int F, int64_t m, uint32_t n_entries, TableReduceWs* out,
-                       bool half_vals = false);
+                       bool half_vals = false, bool prepart = false);
 
 // Writes v to p[0..n) on `stream` (a fill kernel, safe under graph capture).
 int fill_f32(float* p, int64_t n, float v, hipStream_t stream);

@@ -52,7 +52,15 @@ static hipError_t sort_pairs(void* tmp, size_t& bytes, TableReduceWs& w, uint32_
     rocprim::double_buffer<uint32_t> kb(w.keys[0], w.keys[1]);
     rocprim::double_buffer<VT> vb(reinterpret_cast<VT*>(w.vals[0]), reinterpret_cast<VT*>(w.vals[1]));
     hipError_t e;
-    if (w.n_buckets > 0 && digit_bits(w) == 8)
+    if (w.prepart) {   // one 8-bit pass over [17, end_bit), or none
+        e = hipSuccess;
+        if (w.end_bit > kPrepartBits)
+            e = rocprim::radix_sort_pairs<Sort8Config>(tmp, bytes, kb, vb, (size_t)w.m,
+                                                       (unsigned)kPrepartBits,
+                                                       (unsigned)w.end_bit, s, false);
+        else if (tmp == nullptr)
+            bytes = 0;
+    } else if (w.n_buckets > 0 && digit_bits(w) == 8)
         e = rocprim::radix_sort_pairs<Sort8Config>(tmp, bytes, kb, vb, (size_t)w.m,
                                                    (unsigned)w.begin_bit, (unsigned)w.end_bit, s,
                                                    false);
@@ -69,11 +77,17 @@ static hipError_t sort_pairs(void* tmp, size_t& bytes, TableReduceWs& w, uint32_
 }
 
 int table_reduce_carve(WsCarver& c, int F, int64_t m, uint32_t n_entries, TableReduceWs* out,
-                       bool half_vals) {
+                       bool half_vals, bool prepart) {
     TableReduceWs w{};
     w.m = m;
     w.end_bit = table_key_bits(n_entries);
     w.half_vals = half_vals;
+    if (prepart && (F != 2 || table_bucket_bits(w.end_bit) != 9 || w.end_bit > 25)) {
+        set_error("prepartitioned pairs need two features and 9 < key bits <= 25 (got %d, %d)",
+                  F, w.end_bit);
+        return AM_ERR_INTERNAL;
+    }
+    w.prepart = prepart;
     const size_t vbytes = (size_t)F * (half_vals ? sizeof(__half) : sizeof(float));
     w.keys[0] = c.take<uint32_t>((size_t)m);
     w.keys[1] = c.take<uint32_t>((size_t)m);

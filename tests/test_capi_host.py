@@ -415,3 +415,13 @@ def test_serving_sample_buckets():
         assert n <= base or (nb - n) < nb / 4 + 1
         assert nb >= prev
         prev = nb
+
+
+def test_set_flag_validates_names():
+    """am_set_flag knows its variant switches and refuses other names on the
+    host (no device needed)."""
+    from angelo_amd import _native as nat
+    lib = nat.load()
+    assert lib.am_set_flag(b"no_such_flag", 1) == -1 and b"unknown flag" in lib.am_last_error()
+    nat.set_flag("table_prepartition", 1)
+    nat.set_flag("table_prepartition", 0)
