@@ -303,9 +303,11 @@ def time_steps(wl, steps, warmup):
         if cnt:
             kern[k] = {"total_ms": ms, "launches": cnt, "avg_ms": ms / cnt}
     pairs = nat.timer_units("table_pairs") / steps
+    host_reads = nat.timer_units("host_syncs") / steps
     nat.timer_enable(False)
     return {"gpu_ms": start.elapsed_time(stop), "wall": wall, "step_wall": step_wall,
             "kern": kern, "samples": total_samples, "pairs_per_step": pairs,
+            "sdf_bwd_host_reads": host_reads,
             "bg_samples": wl.bg_samples - bg_before}
 
 
@@ -346,6 +348,9 @@ def main():
     ap.add_argument("--no-mixed", action="store_true",
                     help="skip the mixed-precision form of the workload")
     ap.add_argument("--rays", type=int, default=None)
+    ap.add_argument("--sync-free", action="store_true",
+                    help="SDF backward without its pair-count host read (pair arrays and "
+                         "sort sized for the worst-case pair count)")
     ap.add_argument("--serve-requests", type=int, default=200,
                     help="requests in the serving measurement (0: skip it)")
     ap.add_argument("--serve-load", type=float, default=0.6,
@@ -376,12 +381,16 @@ def main():
         torch.cuda.synchronize()
         log("[bench] calibration copy: 3 x 1 GiB device-to-device")
         del src, dst
+    if args.sync_free:
+        from angelo_amd import functions as fn
+        fn.set_sdf_backward(sync_free=True)
     log(f"[bench] building workload {args.workload}")
     wl = Workload(args.workload, dev, rays=args.rays)
     spec = wl.spec
     t = time_steps(wl, args.steps, args.warmup)
     gpu_ms, wall, step_wall, kern = t["gpu_ms"], t["wall"], t["step_wall"], t["kern"]
     total_samples, pairs_per_step = t["samples"], t["pairs_per_step"]
+    host_reads = t["sdf_bwd_host_reads"]
     ms_per_step = gpu_ms / args.steps
     samples_per_step = total_samples / args.steps
     bg_per_step = t["bg_samples"] / args.steps
@@ -468,6 +477,8 @@ def main():
         "kernels_ms": {k: round(v["avg_ms"], 4) for k, v in kern.items()},
         "kernels_roofline": per_kernel,
         "table_pairs_per_step": round(pairs_per_step, 1),
+        "sdf_bwd_sync_free": bool(args.sync_free),
+        "sdf_bwd_host_reads_per_step": host_reads,
         "roofline": roof,
         "mixed": mixed,
         "serving": serve,

@@ -61,7 +61,12 @@ int am_timer_units(const char* name, int64_t* units);
  * the SDF backward's pair pass write its (row, value) pairs partitioned by
  * bits 9-16 of their row, so the table-gradient sort makes one pass over the
  * remaining bits instead of two (it measured slower: scattered stores); 0:
- * compact per-tile pairs and two sort passes.  Both give bit-identical gradients.  Unknown names return
+ * compact per-tile pairs and two sort passes.  Both give bit-identical gradients.
+ * "sdf_bwd_sync_free" (default 0): 1 makes am_sdf_bwd size its pair arrays
+ * for the worst-case pair count instead of reading the exact count back to
+ * the host (no host wait; the sort covers the bound, so it is slower; the
+ * workspace must then hold am_sdf_bwd_workspace_size(), else AM_ERR_WORKSPACE
+ * with that need); 0: one host read of the pair count.  Unknown names return
  * AM_ERR_INVALID. */
 int am_set_flag(const char* name, int32_t value);
 
@@ -346,6 +351,10 @@ size_t am_sdf_bwd_workspace_size(const am_grid_desc* g, const am_mlp_desc* m,
  * equal the count pass's) is copied to pinned host memory without a wait and
  * read by later calls, after their own count sync, and by am_sdf_bwd_check;
  * a set flag returns AM_ERR_INTERNAL there.
+ * A workspace of at least am_sdf_bwd_workspace_size makes the call free of
+ * host reads: the pair arrays take the worst-case bound, their unused tail
+ * sentinel keys, and the sort covers the bound (more sort work, identical
+ * gradients).
  * Replaces the autograd backward of VolumeSDF.forward
  * (models/geometry.py:195-287) through tcnn's encoding backward. */
 /* Waits for the consistency flags of every am_sdf_bwd call so far:

@@ -216,12 +216,15 @@ class GrowingWorkspace:
     def held_bytes(self):
         return sum(b.numel() for b in self._bufs.values())
 
-    def run(self, device, call, what):
-        """call(ws_ptr, ws_bytes) -> status."""
+    def run(self, device, call, what, min_bytes=0):
+        """call(ws_ptr, ws_bytes) -> status.  min_bytes: hold at least this
+        many bytes before the call (e.g. a worst-case bound that spares the
+        call its host read)."""
         key = (device.index, torch.cuda.current_stream(device).cuda_stream)
         buf = self._bufs.get(key)
-        if buf is None:
-            buf = self._bufs[key] = workspace(self.initial, device)
+        if buf is None or buf.numel() < min_bytes:
+            self._bufs[key] = None
+            buf = self._bufs[key] = workspace(max(self.initial, int(min_bytes)), device)
         for _ in range(4):   # head short, then pairs short, then fits
             rc = call(ptr(buf), buf.numel())
             if rc != AM_ERR_WORKSPACE:

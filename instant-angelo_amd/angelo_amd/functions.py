@@ -119,19 +119,26 @@ def sh_encode(dirs01, degree):
 # derivatives), so the backward reads them back in coalesced rows instead of
 # gathering the table again; "recompute": the backward re-gathers (no extra
 # memory); "auto": store while the store fits store_limit_bytes.
-SDF_BACKWARD = {"mode": "auto", "store_limit_bytes": 24 << 30}
+# sync_free: the backward sizes its pair arrays for the worst-case pair
+# count (library flag "sdf_bwd_sync_free") in a workspace held at that
+# bound, so it makes no host read of the count (its sort then covers the
+# bound; same gradients).
+SDF_BACKWARD = {"mode": "auto", "store_limit_bytes": 24 << 30, "sync_free": False}
 
 # The SDF backward's workspace is sized by its actual table-gradient pairs
 # (known only after its count pass), held and grown across steps.
 SDF_BWD_WORKSPACE = nat.GrowingWorkspace()
 
 
-def set_sdf_backward(mode="auto", store_limit_bytes=None):
+def set_sdf_backward(mode="auto", store_limit_bytes=None, sync_free=None):
     if mode not in ("auto", "store", "recompute"):
         raise ValueError(f"unknown SDF backward form {mode!r}")
     SDF_BACKWARD["mode"] = mode
     if store_limit_bytes is not None:
         SDF_BACKWARD["store_limit_bytes"] = int(store_limit_bytes)
+    if sync_free is not None:
+        nat.set_flag("sdf_bwd_sync_free", 1 if sync_free else 0)
+        SDF_BACKWARD["sync_free"] = bool(sync_free)
 
 
 def _sdf_store_floats(lib, grid, mdesc, sdesc, n):
@@ -249,13 +256,14 @@ class SDFFieldFn(torch.autograd.Function):
                 nat.ptr(gs), nat.ptr(gg), nat.ptr(gn), nat.ptr(gf), nat.ptr(gl), nat.ptr(dtable),
                 nat.ptr(dparams))
         ds = (nat.sptr(grid), nat.sptr(mdesc), nat.sptr(sdesc))
+        need = int(lib.am_sdf_bwd_workspace_size(*ds, n)) if SDF_BACKWARD["sync_free"] else 0
         if order.numel():
             SDF_BWD_WORKSPACE.run(pts.device, lambda wp, wb: lib.am_sdf_bwd_ordered(
                 *ds, nat.ptr(order), nat.ptr(pts), *rest, wp, wb, nat.stream()),
-                "am_sdf_bwd_ordered")
+                "am_sdf_bwd_ordered", need)
         else:
             SDF_BWD_WORKSPACE.run(pts.device, lambda wp, wb: lib.am_sdf_bwd(
-                *ds, nat.ptr(pts), *rest, wp, wb, nat.stream()), "am_sdf_bwd")
+                *ds, nat.ptr(pts), *rest, wp, wb, nat.stream()), "am_sdf_bwd", need)
         return None, None, dtable, dparams, None, None, None, None, None, None
 
 

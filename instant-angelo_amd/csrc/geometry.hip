@@ -1267,7 +1267,7 @@ __global__ void __launch_bounds__(256) k_geo_pairs(
     GeoArgs a, const float* __restrict__ pts, const float* __restrict__ randn, int64_t N,
     const float* __restrict__ fdv, const float* __restrict__ genc, uint32_t* __restrict__ keys,
     VT* __restrict__ vals, const int64_t* __restrict__ offsets, int64_t* __restrict__ counts,
-    int64_t* __restrict__ status) {
+    int64_t* __restrict__ status, int64_t capacity) {
     __shared__ LevelParams lv[AM_MAX_LEVELS];
     // per wave: running count, limit and first position of each digit
     __shared__ int32_t part_s[PART ? 4 : 1][PART ? kPartDigits : 1];
@@ -1293,9 +1293,15 @@ __global__ void __launch_bounds__(256) k_geo_pairs(
             for (int d = l; d < kPartDigits; d += 64) {
                 part[d] = 0;
                 if constexpr (!COUNT) {
+                    // a run past the arrays' capacity is cut there (and the
+                    // consistency flag set): never a write out of range
                     const int64_t at = (int64_t)d * n_tiles + tile;
-                    base_s[wave][d] = offsets[at];
-                    lim_s[wave][d] = (int32_t)counts[at];
+                    const int64_t b0 = offsets[at];
+                    const int64_t room = capacity - b0;
+                    const int64_t want = counts[at];
+                    base_s[wave][d] = b0;
+                    lim_s[wave][d] = (int32_t)(want <= room ? want : (room > 0 ? room : 0));
+                    if (want > room) status[1] = 1;
                 }
             }
             mlp::wave_sync();
@@ -1307,8 +1313,13 @@ __global__ void __launch_bounds__(256) k_geo_pairs(
             }
         } else if constexpr (!COUNT) {
             const int64_t off = offsets[tile];
-            et = EmitTile<VT>{keys + off, vals + off, 0, (int)counts[tile], nullptr, nullptr,
-                              nullptr};
+            const int64_t room = capacity - off;
+            int64_t lim = counts[tile];
+            if (lim > room) {   // cut at the arrays' capacity, flagged
+                lim = room > 0 ? room : 0;
+                if (l == 0) status[1] = 1;
+            }
+            et = EmitTile<VT>{keys + off, vals + off, 0, (int)lim, nullptr, nullptr, nullptr};
             gt = genc + tile * a.genc_slots * LJ * 128;
         }
         if (a.with_laplace) flush_cluster<LJ, 1, COUNT>(a, lv, s, valid, gt, 6, a.curv0, et);
@@ -1580,6 +1591,22 @@ k_geo_bwd_o2(AM_GEO_BWD_PARAMS) {
     geo_bwd_body<LJ, 1, ST, PR, AN>(AM_GEO_BWD_ARGS);
 }
 
+// Sentinel keys and zero values in [status[0], bound): the unused tail of
+// pair arrays carved for the worst-case bound.
+template <class VT>
+__global__ void k_pair_tail(uint32_t* __restrict__ keys, VT* __restrict__ vals,
+                            const int64_t* __restrict__ status, int64_t bound, uint32_t sentinel) {
+    const int64_t first = status[0];
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = first + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < bound;
+         i += stride) {
+        keys[i] = sentinel;
+        VT v;
+        set_pair(v, 0.0f, 0.0f);
+        vals[i] = v;
+    }
+}
+
 // status[0] = number of pairs (offset + count of the last entry: the last
 // tile, or the last digit of the last tile); status[1] = 0, set by the
 // backward on a count mismatch.
@@ -1710,11 +1737,11 @@ static int geo_count_launch(const GeoArgs& a, const float* pts, const float* ran
     if (part)
         hipLaunchKernelGGL((k_geo_pairs<LJ, true, Vals<2>, true>), dim3(field_blocks(n)), dim3(256),
                            0, s, a, pts, randn, n, fdv, nullptr, nullptr, nullptr, nullptr, counts,
-                           nullptr);
+                           nullptr, (int64_t)0);
     else
         hipLaunchKernelGGL((k_geo_pairs<LJ, true, Vals<2>, false>), dim3(field_blocks(n)),
                            dim3(256), 0, s, a, pts, randn, n, fdv, nullptr, nullptr, nullptr,
-                           nullptr, counts, nullptr);
+                           nullptr, counts, nullptr, (int64_t)0);
     AM_LAUNCH_CHECK();
     return AM_OK;
 }
@@ -1766,11 +1793,15 @@ static void carve_geo_bwd_head(WsCarver& c, const GeoArgs& a, int LJ, int64_t n,
     w->scan_tmp = c.take<char>(sb + 256);
 }
 
-// Most pairs a tile can emit, per lane and level slot: FD cluster 6 x 8
-// (the base point is always in B) + 8 for B; curvature cluster 6 x 8, or
-// 5 x 8 + 8 when B is touched; base only: 8.  All within n_evals x 8.
+// Most pairs a tile can emit, per lane and level slot.  FD cluster: the
+// base point is always in B, so 6 x 8 + 8 for B = 56; analytic or without
+// gradient: the base alone, 8.  Curvature cluster (its centre is not
+// evaluated): 6 x 8 = 48 with B untouched; with one point in a face
+// neighbour of B, 5 x 8 + 4 far corners + 8 for B = 52 (the most); more
+// points in or beside B emit less.
 static int64_t geo_max_pairs(const GeoArgs& a, int LJ, int64_t n) {
-    return (n + 15) / 16 * ((int64_t)a.n_evals * LJ * 8 * 64);
+    const int64_t per_slot = (a.with_grad && !a.analytic ? 56 : 8) + (a.with_laplace ? 52 : 0);
+    return (n + 15) / 16 * (per_slot * LJ * 64);
 }
 
 // Consistency flags of recent backward calls.  The emit pass flags a tile
@@ -1873,18 +1904,35 @@ static int geo_bwd_launch_t(const GeoArgs& a, const am_grid_desc* g, const float
     hipLaunchKernelGGL(k_pair_total, dim3(1), dim3(64), 0, s, w.offsets, w.counts, w.entries,
                        w.status);
     AM_LAUNCH_CHECK();
-    // 3. one host read of the pair count; the pair arrays are carved for
-    // exactly that many pairs (a workspace of the size query's worst-case
-    // bound always suffices; a smaller one is rejected here, before any
-    // output is written, with am_last_workspace_need() = what this call needs)
+    // 3. the pair arrays.  Sync-free form (flag "sdf_bwd_sync_free"): arrays
+    // of the worst-case pair bound, no host read; the tail past the
+    // device-side count is filled with sentinel keys (they sort last and are
+    // skipped) and the sort covers the bound.  Otherwise one host read of the
+    // pair count carves the arrays for exactly that many pairs.  A workspace
+    // too small is rejected here, before any output is written, with
+    // am_last_workspace_need() = what this call needs.
+    const int64_t max_pairs = geo_max_pairs(a, LJ, n);
+    const bool bounded = flag_value("sdf_bwd_sync_free", 0) != 0;
     int64_t st[2] = {0, 0};
-    AM_HIP(hipMemcpyAsync(st, w.status, sizeof(st), hipMemcpyDeviceToHost, s));
-    AM_HIP(hipStreamSynchronize(s));
+    if (!bounded) {
+        AM_HIP(hipMemcpyAsync(st, w.status, sizeof(st), hipMemcpyDeviceToHost, s));
+        AM_HIP(hipStreamSynchronize(s));
+        timer_units("host_syncs", 1);
+    }
     if (int rc = collect_flags(false)) return rc;
     TableReduceWs tw;
-    if (int rc = table_reduce_carve(c, 2, st[0], g->n_entries, &tw, PR == mlp::kBF16, part))
+    if (int rc = table_reduce_carve(c, 2, bounded ? max_pairs : st[0], g->n_entries, &tw,
+                                    PR == mlp::kBF16, part))
         return rc;
     if (!c.ok()) return workspace_short(c.used, ws_bytes);
+    tw.m = bounded ? max_pairs : st[0];   // the arrays' capacity (and the sort's size)
+    if (bounded && max_pairs > 0) {
+        const int64_t want = ceil_div(max_pairs, 256);
+        hipLaunchKernelGGL((k_pair_tail<PairV<PR>>), dim3((unsigned)(want < 16384 ? want : 16384)),
+                           dim3(256), 0, s, tw.keys[0], reinterpret_cast<PairV<PR>*>(tw.vals[0]),
+                           w.status, max_pairs, table_sentinel(tw.end_bit));
+        AM_LAUNCH_CHECK();
+    }
     const size_t lds =
         (size_t)(geo_ext_base(a) + 4 * (a.out_tiles - 1) * kOutExtFloats) * sizeof(float);
     AM_CHECK_ARG(lds <= 160 * 1024, "geometry backward needs %zu bytes of LDS", lds);
@@ -1913,17 +1961,16 @@ static int geo_bwd_launch_t(const GeoArgs& a, const am_grid_desc* g, const float
             hipLaunchKernelGGL((k_geo_pairs<LJ, false, PairV<PR>, true>), dim3(blocks), dim3(256),
                                0, s, a, pts, randn, n, fdv, w.genc, tw.keys[0],
                                reinterpret_cast<PairV<PR>*>(tw.vals[0]), w.offsets, w.counts,
-                               w.status);
+                               w.status, tw.m);
         else
             hipLaunchKernelGGL((k_geo_pairs<LJ, false, PairV<PR>, false>), dim3(blocks),
                                dim3(256), 0, s, a, pts, randn, n, fdv, w.genc, tw.keys[0],
                                reinterpret_cast<PairV<PR>*>(tw.vals[0]), w.offsets, w.counts,
-                               w.status);
+                               w.status, tw.m);
     }
     AM_LAUNCH_CHECK();
     if (int rc = colsum_f32(w.partial, waves, a.n_params, dparams, w.ctmp, s)) return rc;
-    tw.m = st[0];
-    timer_units("table_pairs", tw.m);
+    if (!bounded) timer_units("table_pairs", tw.m);
     if (int rc = table_reduce_run(tw, 2, g->n_entries, dtable, s)) return rc;
     return arm_flag(w.status + 1, s);
 }

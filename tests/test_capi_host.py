@@ -425,3 +425,5 @@ def test_set_flag_validates_names():
     assert lib.am_set_flag(b"no_such_flag", 1) == -1 and b"unknown flag" in lib.am_last_error()
     nat.set_flag("table_prepartition", 1)
     nat.set_flag("table_prepartition", 0)
+    nat.set_flag("sdf_bwd_sync_free", 1)
+    nat.set_flag("sdf_bwd_sync_free", 0)
