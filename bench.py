@@ -26,7 +26,8 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 FP32_MFMA_PEAK_TFLOPS = 157.3
-TIMED_KERNELS = ["geo_fwd", "geo_bwd", "pair_count", "pair_emit", "radix_sort", "seg_reduce",
+TIMED_KERNELS = ["geo_fwd", "geo_bwd", "pair_count", "pair_emit", "pair_atomic", "radix_sort",
+                 "seg_reduce",
                  "rad_fwd", "rad_bwd", "composite_fwd", "composite_bwd", "march_count",
                  "march_fill", "occ_sdf", "sample_rays", "hg_fwd", "hg_emit", "march_bg_count",
                  "march_bg_fill", "march_bg_visible", "composite_bg_fwd", "composite_bg_bwd"]
@@ -348,6 +349,9 @@ def main():
     ap.add_argument("--no-mixed", action="store_true",
                     help="skip the mixed-precision form of the workload")
     ap.add_argument("--rays", type=int, default=None)
+    ap.add_argument("--table-atomic", action="store_true",
+                    help="SDF table gradient by float atomics (run-to-run summation order "
+                         "varies) instead of the deterministic sort + reduce")
     ap.add_argument("--sync-free", action="store_true",
                     help="SDF backward without its pair-count host read (pair arrays and "
                          "sort sized for the worst-case pair count)")
@@ -384,6 +388,8 @@ def main():
     if args.sync_free:
         from angelo_amd import functions as fn
         fn.set_sdf_backward(sync_free=True)
+    if args.table_atomic:
+        nat.set_flag("table_atomic", 1)
     log(f"[bench] building workload {args.workload}")
     wl = Workload(args.workload, dev, rays=args.rays)
     spec = wl.spec
@@ -478,6 +484,7 @@ def main():
         "kernels_roofline": per_kernel,
         "table_pairs_per_step": round(pairs_per_step, 1),
         "sdf_bwd_sync_free": bool(args.sync_free),
+        "table_gradient": "atomic" if args.table_atomic else "deterministic (sort + reduce)",
         "sdf_bwd_host_reads_per_step": host_reads,
         "roofline": roof,
         "mixed": mixed,

@@ -66,7 +66,11 @@ int am_timer_units(const char* name, int64_t* units);
  * for the worst-case pair count instead of reading the exact count back to
  * the host (no host wait; the sort covers the bound, so it is slower; the
  * workspace must then hold am_sdf_bwd_workspace_size(), else AM_ERR_WORKSPACE
- * with that need); 0: one host read of the pair count.  Unknown names return
+ * with that need); 0: one host read of the pair count.  "table_atomic"
+ * (default 0): 1 makes am_sdf_bwd add its merged pairs into the table
+ * gradient with float atomics (no sort, no host read; the summation order
+ * varies run to run; measured 3x slower than the sort on a 0.87 M-sample
+ * step).  Unknown names return
  * AM_ERR_INVALID. */
 int am_set_flag(const char* name, int32_t value);
 

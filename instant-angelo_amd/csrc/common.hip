@@ -169,7 +169,7 @@ int flag_value(const char* name, int dflt) {
 
 extern "C" int am_set_flag(const char* name, int32_t value) {
     AM_CHECK_ARG(name, "NULL flag name");
-    static const char* known[] = {"table_prepartition", "sdf_bwd_sync_free"};
+    static const char* known[] = {"table_prepartition", "sdf_bwd_sync_free", "table_atomic"};
     bool ok = false;
     for (const char* k : known) ok = ok || std::string(k) == name;
     AM_CHECK_ARG(ok, "unknown flag %s", name);

@@ -909,6 +909,17 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
     return l == 0 ? 0ull : (~0ull >> (64 - l));
 }
 
+// Atomic form (flag "table_atomic"): the "pair array" is the table gradient
+// itself, [n_entries] rows of 2 floats, and each pair is added into its row
+// with hardware float atomics: no count pass, no sort, no host read, but the
+// summation order varies from run to run (the tcnn hash-grid backward's
+// semantics).
+struct AtomicRow {
+    float v[2];
+};
+template <class VT>
+constexpr bool kAtomicPairs = std::is_same<VT, AtomicRow>::value;
+
 // Lanes with `on` write K pairs (rows, values); COUNT only advances the
 // cursor.
 __device__ __forceinline__ void set_pair(Vals<2>& v, float a, float b) {
@@ -919,6 +930,10 @@ __device__ __forceinline__ void set_pair(ValsH<2>& v, float a, float b) {
     v.v[0] = __float2half_rn(a);
     v.v[1] = __float2half_rn(b);
 }
+__device__ __forceinline__ void set_pair(AtomicRow& v, float a, float b) {   // type-checks only
+    v.v[0] = a;
+    v.v[1] = b;
+}
 
 // The group's K * n pairs (n lanes on) fill [cursor, cursor + K n) corner-
 // major: pair c of the lane of rank r goes to cursor + c n + r, so each store
@@ -927,6 +942,18 @@ __device__ __forceinline__ void set_pair(ValsH<2>& v, float a, float b) {
 template <int K, bool COUNT, class VT>
 __device__ __forceinline__ void emit_pairs(EmitTile<VT>& et, bool on, const uint32_t rows[K],
                                            const float vals[2 * K]) {
+    if constexpr (kAtomicPairs<VT>) {
+        static_assert(!COUNT, "the atomic form has no count pass");
+        if (on) {
+#pragma unroll
+            for (int c = 0; c < K; ++c) {
+                float* row = et.vals[rows[c]].v;
+                unsafeAtomicAdd(row, vals[2 * c]);
+                unsafeAtomicAdd(row + 1, vals[2 * c + 1]);
+            }
+        }
+        return;
+    }
     const uint64_t mask = __ballot(on);
     const int n_on = __popcll(mask);
     if (et.part) {   // partitioned form (rows are computed in both passes)
@@ -1311,6 +1338,9 @@ __global__ void __launch_bounds__(256) k_geo_pairs(
                 et = EmitTile<VT>{keys, vals, 0, 0, part, lim_s[wave], base_s[wave]};
                 gt = genc + tile * a.genc_slots * LJ * 128;
             }
+        } else if constexpr (kAtomicPairs<VT>) {
+            et = EmitTile<VT>{nullptr, vals, 0, 0, nullptr, nullptr, nullptr};
+            gt = genc + tile * a.genc_slots * LJ * 128;
         } else if constexpr (!COUNT) {
             const int64_t off = offsets[tile];
             const int64_t room = capacity - off;
@@ -1338,7 +1368,7 @@ __global__ void __launch_bounds__(256) k_geo_pairs(
             mlp::wave_sync();
         } else if constexpr (COUNT) {
             if (l == 0) counts[tile] = et.cursor;
-        } else {
+        } else if constexpr (!kAtomicPairs<VT>) {
             if (l == 0 && et.cursor != et.limit) status[1] = 1;
         }
     }
@@ -1892,6 +1922,46 @@ static int geo_bwd_launch_t(const GeoArgs& a, const am_grid_desc* g, const float
             return rc;
         fdv = w.fdv;
     }
+    // the backward proper: MLP parameter partials and the encode gradients
+    auto launch_bwd = [&]() -> int {
+        const size_t lds =
+            (size_t)(geo_ext_base(a) + 4 * (a.out_tiles - 1) * kOutExtFloats) * sizeof(float);
+        AM_CHECK_ARG(lds <= 160 * 1024, "geometry backward needs %zu bytes of LDS", lds);
+        auto kern = k_geo_bwd_o2<LJ, true, PR, false>;   // the type of every form's kernel
+        if constexpr (NH == 1) {
+            kern = store ? k_geo_bwd_o2<LJ, true, PR, false> : k_geo_bwd_o2<LJ, false, PR, false>;
+            if (a.analytic)
+                kern = store ? k_geo_bwd_o2<LJ, true, PR, true> : k_geo_bwd_o2<LJ, false, PR, true>;
+        } else {
+            kern = store ? k_geo_bwd<LJ, NH, true, PR, false> : k_geo_bwd<LJ, NH, false, PR, false>;
+            if (a.analytic)
+                kern = store ? k_geo_bwd<LJ, NH, true, PR, true> : k_geo_bwd<LJ, NH, false, PR, true>;
+        }
+        AM_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)lds));
+        {
+            KernelTimer kt_("geo_bwd", s);
+            hipLaunchKernelGGL(kern, dim3(blocks), dim3(256), lds, s, a, pts, randn, n, table,
+                               params, fdv, store, g_sdf, g_grad, g_normal, g_feature, g_laplace,
+                               w.genc, w.partial);
+        }
+        AM_LAUNCH_CHECK();
+        return AM_OK;
+    };
+    if (flag_value("table_atomic", 0) != 0) {
+        // atomic form: the pairs are added into the zeroed table gradient
+        if (int rc = fill_f32(dtable, (int64_t)g->n_entries * 2, 0.0f, s)) return rc;
+        if (int rc = launch_bwd()) return rc;
+        {
+            KernelTimer kt_("pair_atomic", s);
+            hipLaunchKernelGGL((k_geo_pairs<LJ, false, AtomicRow, false>), dim3(blocks),
+                               dim3(256), 0, s, a, pts, randn, n, fdv, w.genc, nullptr,
+                               reinterpret_cast<AtomicRow*>(dtable), nullptr, nullptr, nullptr,
+                               (int64_t)0);
+        }
+        AM_LAUNCH_CHECK();
+        return colsum_f32(w.partial, waves, a.n_params, dparams, w.ctmp, s);
+    }
     // 1. pairs per tile, 2. exclusive prefix sum -> each tile's first pair
     if (int rc = geo_count_launch<LJ>(a, pts, randn, n, fdv, w.counts, part, s)) return rc;
     size_t sb = w.scan_bytes;
@@ -1933,28 +2003,7 @@ static int geo_bwd_launch_t(const GeoArgs& a, const am_grid_desc* g, const float
                            w.status, max_pairs, table_sentinel(tw.end_bit));
         AM_LAUNCH_CHECK();
     }
-    const size_t lds =
-        (size_t)(geo_ext_base(a) + 4 * (a.out_tiles - 1) * kOutExtFloats) * sizeof(float);
-    AM_CHECK_ARG(lds <= 160 * 1024, "geometry backward needs %zu bytes of LDS", lds);
-    auto kern = k_geo_bwd_o2<LJ, true, PR, false>;   // the type of every form's kernel
-    if constexpr (NH == 1) {
-        kern = store ? k_geo_bwd_o2<LJ, true, PR, false> : k_geo_bwd_o2<LJ, false, PR, false>;
-        if (a.analytic)
-            kern = store ? k_geo_bwd_o2<LJ, true, PR, true> : k_geo_bwd_o2<LJ, false, PR, true>;
-    } else {
-        kern = store ? k_geo_bwd<LJ, NH, true, PR, false> : k_geo_bwd<LJ, NH, false, PR, false>;
-        if (a.analytic)
-            kern = store ? k_geo_bwd<LJ, NH, true, PR, true> : k_geo_bwd<LJ, NH, false, PR, true>;
-    }
-    AM_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)lds));
-    {
-        KernelTimer kt_("geo_bwd", s);
-        hipLaunchKernelGGL(kern, dim3(blocks), dim3(256), lds, s, a, pts, randn, n, table, params,
-                           fdv, store, g_sdf, g_grad, g_normal, g_feature, g_laplace, w.genc,
-                           w.partial);
-    }
-    AM_LAUNCH_CHECK();
+    if (int rc = launch_bwd()) return rc;
     {
         KernelTimer kt_("pair_emit", s);
         if (part)

@@ -107,3 +107,42 @@ def test_sync_free_field_backward_is_identical(dev, grad_type, laplace, part):
     helpers.assert_nontrivial(grads[True][0], "table gradient")
     for a, b in zip(grads[True], grads[False]):
         assert torch.equal(a, b)
+
+
+@pytest.mark.parametrize("grad_type", ["finite_difference", "analytic"])
+@pytest.mark.parametrize("laplace", [False, True])
+def test_atomic_table_gradient_matches_sorted(dev, grad_type, laplace):
+    """The atomic table-gradient form (each merged pair added into its row
+    with float atomics) sums the same terms as the sorted reduce in another
+    order: equal within fp32 reassociation error, MLP gradients identical
+    (they do not pass through the table reduce)."""
+    from angelo_amd import _native as nat
+    from test_gpu_pipeline import _field_model
+    import helpers
+    model = _field_model(dev, L=16, log2T=19, base=32, scale=1.3195079107728942,
+                         grad_type=grad_type)
+    geo = model.geometry
+    g = torch.Generator().manual_seed(71)
+    n = 20000
+    pts = ((torch.rand(n, 3, generator=g) * 2 - 1) * 0.9).to(dev)
+    randn = torch.randn(n, 3, generator=g).to(dev)
+    grads = {}
+    try:
+        for atomic in (1, 0):
+            nat.set_flag("table_atomic", atomic)
+            model.zero_grad()
+            sdf, grad, normal, feature, lap = geo.fused(pts, True, laplace, randn)
+            loss = sdf.sum() + grad.square().sum() + feature.sum()
+            if laplace:
+                loss = loss + lap.sum()
+            loss.backward()
+            grads[atomic] = [p.grad.clone() for p in geo.parameters()]
+    finally:
+        nat.set_flag("table_atomic", 0)
+    table_a, table_s = grads[1][0], grads[0][0]
+    helpers.assert_nontrivial(table_s, "table gradient")
+    # fp32 sums of up to a few hundred terms per row in another order
+    scale = table_s.abs().max()
+    torch.testing.assert_close(table_a, table_s, rtol=1e-4, atol=1e-6 * float(scale))
+    for a, b in zip(grads[1][1:], grads[0][1:]):
+        assert torch.equal(a, b)
