@@ -312,6 +312,57 @@ def time_steps(wl, steps, warmup):
             "bg_samples": wl.bg_samples - bg_before}
 
 
+def level_sweep(wl, levels=(1, 2, 4, 8, 12, 16), reps=3):
+    """The fused SDF forward (training form: FD + curvature evaluations,
+    stored encodings) over one step's marched samples with 1..L active
+    levels, `reps` launches per setting, then `reps` launches of the same
+    forward without the saved buffers (for HBM counters per setting under a
+    profiler: the training-form launches are the first `reps` of each
+    setting): per setting the average time from HIP events on the current
+    stream."""
+    import torch
+    from angelo_amd import functions as fn
+    dev = wl.device
+    model, geo = wl.model, wl.model.geometry
+    b = wl.sampler.sample(wl.spec["rays"], wl.sample_gen, model)
+    rays = b["rays"].float().contiguous()
+    with torch.no_grad():
+        ridx, t0, t1, _ = model.march(rays)
+    pts = fn.sample_points(rays, ridx, t0, t1)[0].detach()
+    randn = torch.randn(pts.shape[0], 3, device=dev,
+                        generator=torch.Generator(device=dev).manual_seed(5))
+    keep = geo.grid_module.active_levels
+    out = {"samples": int(pts.shape[0]), "launches_per_setting": reps, "ms": {}}
+    try:
+        for k in levels:
+            if k > geo.grid_module.n_levels:
+                continue
+            geo.grid_module.active_levels = k
+            ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+            torch.cuda.synchronize()
+            ev[0].record()
+            for _ in range(reps):
+                geo.fused(pts, True, True, randn)
+            ev[1].record()
+            torch.cuda.synchronize()
+            out["ms"][k] = round(ev[0].elapsed_time(ev[1]) / reps, 4)
+            # the same forward without the backward's saved buffers (no grad
+            # mode: no stored encodings, no FD values)
+            torch.cuda.synchronize()
+            with torch.no_grad():
+                ev[0].record()
+                for _ in range(reps):
+                    geo.fused(pts, True, True, randn)
+                ev[1].record()
+            torch.cuda.synchronize()
+            out.setdefault("ms_no_store", {})[k] = round(ev[0].elapsed_time(ev[1]) / reps, 4)
+            log(f"[bench] level sweep: {k} active levels, {out['ms'][k]:.3f} ms per forward, "
+                f"{out['ms_no_store'][k]:.3f} ms without the saved buffers")
+    finally:
+        geo.grid_module.active_levels = keep
+    return out
+
+
 def mixed_bench(dev, name, steps, warmup, fp32_value, rays=None):
     """The mixed-precision form of the same step (fp16 table rows, bf16 MFMA
     radiance, fp16 table-gradient pairs with dynamic loss scaling, Kahan
@@ -352,6 +403,10 @@ def main():
     ap.add_argument("--table-atomic", action="store_true",
                     help="SDF table gradient by float atomics (run-to-run summation order "
                          "varies) instead of the deterministic sort + reduce")
+    ap.add_argument("--level-sweep", action="store_true",
+                    help="only time the fused SDF forward over one step's samples with 1, 2, "
+                         "4, 8, 12, 16 active levels (3 launches each; for per-level HBM "
+                         "counters under rocprofv3) and print that as the JSON line")
     ap.add_argument("--sync-free", action="store_true",
                     help="SDF backward without its pair-count host read (pair arrays and "
                          "sort sized for the worst-case pair count)")
@@ -393,6 +448,15 @@ def main():
     log(f"[bench] building workload {args.workload}")
     wl = Workload(args.workload, dev, rays=args.rays)
     spec = wl.spec
+    if args.level_sweep:
+        if wl.uniform:
+            print("--level-sweep needs a marched workload (neuralangelo-*)", file=sys.stderr)
+            sys.exit(2)
+        for i in range(args.warmup):
+            wl.step(i)
+        sweep = level_sweep(wl)
+        print(json.dumps({"level_sweep": sweep, "workload": args.workload}), flush=True)
+        return
     t = time_steps(wl, args.steps, args.warmup)
     gpu_ms, wall, step_wall, kern = t["gpu_ms"], t["wall"], t["step_wall"], t["kern"]
     total_samples, pairs_per_step = t["samples"], t["pairs_per_step"]
