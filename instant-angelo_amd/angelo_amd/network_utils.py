@@ -80,8 +80,13 @@ class HashGrid(nn.Module):
         self.params = nn.Parameter(init)
         self.active_levels = self.n_levels
         # "mixed": the fused kernels read an fp16 copy of the fp32 master
-        # table, refreshed whenever the master changes (the optimizer's
-        # in-place update bumps its version).
+        # table.  torch's fused AdamW updates parameters without bumping
+        # their version counter, so the version alone cannot say when the
+        # master changed: the copy is rebuilt on every call that can lead to
+        # an optimizer step (grad mode on), at every update_step (the start
+        # of a training step, before its no-grad occupancy update) and on
+        # train / eval switches, and otherwise whenever the version or the
+        # storage changes (load_state_dict, plain optimizers).
         self.precision = "fp32"
         self._half = None
         self._half_key = None
@@ -96,11 +101,24 @@ class HashGrid(nn.Module):
         if self.precision != "mixed":
             return self.table
         key = (self.params.data_ptr(), self.params._version)
-        if self._half is None or self._half_key != key:
+        fresh = torch.is_grad_enabled() and self.params.requires_grad
+        if self._half is None or self._half_key != key or fresh:
             with torch.no_grad():
                 self._half = self.table.detach().to(torch.float16).contiguous()
             self._half_key = key
         return self._half
+
+    def invalidate_half(self):
+        """Drops the fp16 copy (the master may have changed unseen)."""
+        self._half = None
+        self._half_key = None
+
+    def update_step(self, epoch, global_step):
+        self.invalidate_half()
+
+    def train(self, mode=True):
+        self.invalidate_half()
+        return super().train(mode)
 
     def level_table(self):
         return descs.grid_arrays(self.grid)
@@ -141,6 +159,7 @@ class ProgressiveBandHashGrid(nn.Module):
         return self.encoding(x)
 
     def update_step(self, epoch, global_step):
+        self.encoding.update_step(epoch, global_step)
         current_level = min(self.start_level + max(global_step - self.start_step, 0) // self.update_steps,
                             self.n_level)
         self.current_level = current_level

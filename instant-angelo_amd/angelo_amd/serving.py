@@ -126,6 +126,11 @@ class FieldServer:
         self._dirs = {}
         self._render_graphs = {}
         self._sdf_graph = None
+        # the tables the captured kernels read (the mixed form's fp16 copy
+        # belongs to the grid, which drops it when the master may have
+        # changed): held here for as long as the graphs live.  Graphs serve
+        # the weights of their capture; a server is rebuilt after training.
+        self._pinned = []
         # static buffers of the captured steps
         f32 = dict(device=self.dev, dtype=torch.float32)
         self.s_rays = torch.zeros(self.R, 6, **f32)
@@ -247,6 +252,7 @@ class FieldServer:
         with torch.cuda.stream(side):
             self._render_stage(st["rays"], st["ridx"], st["t0"], st["t1"], st["packed"])
         torch.cuda.current_stream().wait_stream(side)
+        self._pinned.append(self.model.geometry.grid_module.kernel_table())
         graph = torch.cuda.CUDAGraph()
         with torch.cuda.graph(graph):
             out = self._render_stage(st["rays"], st["ridx"], st["t0"], st["t1"], st["packed"])
@@ -269,6 +275,7 @@ class FieldServer:
             with torch.cuda.stream(side):
                 self._sdf_stage()
             torch.cuda.current_stream().wait_stream(side)
+            self._pinned.append(self.model.geometry.grid_module.kernel_table())
             graph = torch.cuda.CUDAGraph()
             with torch.cuda.graph(graph):
                 out = self._sdf_stage()

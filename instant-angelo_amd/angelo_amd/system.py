@@ -161,6 +161,19 @@ class DynamicLossScaler:
             self.scale *= self.backoff_factor
             self._good = 0
 
+    def state_dict(self):
+        """What a checkpoint keeps (GradScaler's state_dict keys)."""
+        return {"scale": self.scale, "growth_factor": self.growth_factor,
+                "backoff_factor": self.backoff_factor, "growth_interval": self.growth_interval,
+                "_growth_tracker": self._good}
+
+    def load_state_dict(self, state):
+        self.scale = float(state["scale"])
+        self.growth_factor = float(state["growth_factor"])
+        self.backoff_factor = float(state["backoff_factor"])
+        self.growth_interval = int(state["growth_interval"])
+        self._good = int(state["_growth_tracker"])
+
     def step(self, optimizer, params):
         ok = self.unscale_and_check(params)
         if ok:

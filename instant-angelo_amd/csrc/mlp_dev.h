@@ -66,7 +66,9 @@ __device__ __forceinline__ void wave_sync() {
 
 // ---- activations (torch semantics) ----
 // log1p(t) for t in [0, 1]: t * ln(u) / (u - 1) with u = fl(1 + t), which
-// cancels the rounding of u (t itself when u == 1); hardware log2/rcp.
+// cancels the rounding of u (t itself when u == 1); hardware log2/rcp.  The
+// method: D. Goldberg, "What every computer scientist should know about
+// floating-point arithmetic", ACM Computing Surveys 23(1), 1991, Theorem 4.
 __device__ __forceinline__ float log1p_unit(float t) {
     const float u = 1.0f + t;
     const float d = u - 1.0f;

@@ -204,6 +204,15 @@ def test_dynamic_loss_scaler():
     p.grad = torch.tensor([0.0, 0.0, 0.0])
     s.step(opt, [p])
     assert s.scale == 1024.0
+    # checkpoint state: a restored scaler continues the growth count
+    p.grad = torch.tensor([0.0, 0.0, 0.0])
+    s.step(opt, [p])
+    r = DynamicLossScaler()
+    r.load_state_dict(s.state_dict())
+    assert r.state_dict() == s.state_dict() and r.scale == 1024.0 and r._good == 1
+    p.grad = torch.tensor([0.0, 0.0, 0.0])
+    r.step(opt, [p])
+    assert r.scale == 2048.0
 
 
 def test_losses_on_cpu_tensors():
@@ -427,3 +436,40 @@ def test_set_flag_validates_names():
     nat.set_flag("table_prepartition", 0)
     nat.set_flag("sdf_bwd_sync_free", 1)
     nat.set_flag("sdf_bwd_sync_free", 0)
+
+
+def test_mixed_table_copy_follows_fused_optimizer_steps():
+    """torch's fused AdamW updates parameters without bumping their version
+    counter; the fp16 table copy the mixed-form kernels read must still
+    follow the master: rebuilt on a grad-mode call, after update_step and on
+    a train / eval switch (host-only: the copy is a torch conversion)."""
+    from angelo_amd.network_utils import HashGrid
+    g = HashGrid(3, {"n_levels": 2, "n_features_per_level": 2, "log2_hashmap_size": 8,
+                     "base_resolution": 4, "per_level_scale": 1.5}, seed=3)
+    g.precision = "mixed"
+    opt = torch.optim.AdamW(g.parameters(), lr=0.1, fused=True)
+
+    def fused_step():
+        g.params.grad = torch.ones_like(g.params)
+        v = g.params._version
+        opt.step()
+        assert g.params._version == v   # the premise: no version bump
+
+    def matches():
+        return torch.equal(g.kernel_table(), g.table.detach().to(torch.float16))
+
+    with torch.no_grad():
+        g.kernel_table()
+    fused_step()
+    assert matches()                     # grad mode on: rebuilt
+    fused_step()
+    g.update_step(0, 1)
+    with torch.no_grad():
+        assert matches()                 # update_step dropped the copy
+    fused_step()
+    g.eval()
+    with torch.no_grad():
+        assert matches()                 # mode switch dropped it
+    g.load_state_dict({"params": torch.zeros_like(g.params)})
+    with torch.no_grad():
+        assert matches()                 # version bump of load_state_dict

@@ -258,3 +258,19 @@ def test_forward_bg_gradcheck():
                                  mode="f64")
         return out["comp_rgb"], out["depth"], out["opacity"]
     assert torch.autograd.gradcheck(f, (W,), eps=1e-6, atol=1e-6)
+
+
+def test_losses_closed_forms():
+    """The oracle's loss glue (systems/neus.py:130-159) on a hand case whose
+    terms have closed forms: the colour MSE over valid rays only (the invalid
+    ray's colour would change it), the eikonal mean of (|g| - 1)^2 over
+    samples and the mean |laplace|, weighted 10 / 0.1 / 0.5."""
+    out = {"rays_valid": torch.tensor([[True], [False], [True]]),
+           "comp_rgb_full": torch.tensor([[0.5] * 3, [9.0] * 3, [0.5] * 3], dtype=D),
+           "grad": torch.tensor([[0.0, 2.0, 0.0], [3.0, 4.0, 0.0]], dtype=D),
+           "laplace": torch.tensor([0.2, -0.4], dtype=D)}
+    total, terms = orender.losses(out, torch.zeros(3, 3, dtype=D))
+    assert math.isclose(terms["rgb_mse"].item(), 0.25, rel_tol=1e-15)
+    assert math.isclose(terms["eikonal"].item(), (1.0 + 16.0) / 2, rel_tol=1e-15)
+    assert math.isclose(terms["curvature"].item(), 0.3, rel_tol=1e-15)
+    assert math.isclose(total.item(), 0.25 * 10 + 8.5 * 0.1 + 0.3 * 0.5, rel_tol=1e-15)
