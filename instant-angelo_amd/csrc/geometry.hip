@@ -1108,8 +1108,25 @@ __device__ __forceinline__ void flush_cluster(const GeoArgs& a, const LevelParam
         float acc[16];
 #pragma unroll
         for (int q = 0; q < 16; ++q) acc[q] = 0.0f;
+        // encode gradients one evaluation ahead: each load pair is issued
+        // before the previous evaluation's pair stores, so its latency
+        // overlaps that evaluation's work instead of stalling every one
+        // (carrying it across level slots too measured no better: more
+        // registers)
+        float g0n = 0.0f, g1n = 0.0f;
+        if constexpr (!COUNT) {
+            g0n = genc_base[(slot0 * LJ + j) * 128 + l];
+            g1n = genc_base[(slot0 * LJ + j) * 128 + 64 + l];
+        }
 #pragma unroll 1
         for (int q = 0; q < n_q; ++q) {
+            const float g0 = g0n, g1 = g1n;
+            if constexpr (!COUNT) {
+                if (q + 1 < n_q) {
+                    g0n = genc_base[((slot0 + q + 1) * LJ + j) * 128 + l];
+                    g1n = genc_base[((slot0 + q + 1) * LJ + j) * 128 + 64 + l];
+                }
+            }
             float x[3];
             cluster_point<CL>(a, s, q, x);
             const LevelPos pp = level_pos(lp, x);
@@ -1131,12 +1148,6 @@ __device__ __forceinline__ void flush_cluster(const GeoArgs& a, const LevelParam
             const int axm = ax < 0 ? 0 : ax;
             const int far_bit = sgn > 0 ? 1 : 0;  // C's corners off B's face
             touched = touched || same || adj;
-            float g0 = 0.0f, g1 = 0.0f;
-            if constexpr (!COUNT) {
-                const int slot = slot0 + q;
-                g0 = genc_base[(slot * LJ + j) * 128 + l];
-                g1 = genc_base[(slot * LJ + j) * 128 + 64 + l];
-            }
             // all 8 corners of C on their own
             {
                 const bool em = on && !same && !adj;
