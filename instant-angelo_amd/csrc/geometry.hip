@@ -39,6 +39,15 @@
 #include <rocprim/device/device_scan.hpp>
 
 namespace am {
+// Row stride (floats) of the 64-column weight matrices in LDS (W2 and the
+// output layer): with 67, the row-major reads of the forward (lane i row i,
+// lane group k column 4k) and the transposed reads of the backward (lane
+// group k row 4k, lane i column i) both hit at most two lanes per bank; 65
+// put four on one bank both ways.
+constexpr int kWLd = 67;
+}  // namespace am
+
+namespace am {
 
 using mlp::krow;
 
@@ -106,13 +115,13 @@ __device__ __forceinline__ void geo_load_lds(const GeoArgs& a, const float* __re
     if (NH == 2) {
         for (int q = threadIdx.x; q < 64 * 64; q += blockDim.x) {
             const int o = q >> 6, c = q & 63;
-            sm[a.w2 + o * 65 + c] = params[a.pw2 + q];
+            sm[a.w2 + o * kWLd + c] = params[a.pw2 + q];
         }
         for (int q = threadIdx.x; q < 64; q += blockDim.x) sm[a.b2 + q] = params[a.pb2 + q];
     }
     for (int q = threadIdx.x; q < a.out_tiles * 16 * 64; q += blockDim.x) {
         const int o = q >> 6, c = q & 63;
-        sm[a.wo + o * 65 + c] = o < a.n_out ? params[a.pwo + (int64_t)o * 64 + c] : 0.0f;
+        sm[a.wo + o * kWLd + c] = o < a.n_out ? params[a.pwo + (int64_t)o * 64 + c] : 0.0f;
     }
     for (int q = threadIdx.x; q < a.out_tiles * 16; q += blockDim.x)
         sm[a.bo + q] = q < a.n_out ? params[a.pbo + q] : 0.0f;
@@ -180,7 +189,7 @@ __device__ __forceinline__ void geo_mlp_hidden(const GeoArgs& a, const float* sm
         for (int t = 0; t < 4; ++t) h[t] = mlp::act4(a.act, e.z1[t], a.beta);
 #pragma unroll
         for (int t = 0; t < 4; ++t) e.z2[t] = mlp::bias_tile(sm + a.b2, t, 64);
-        mlp::mm_fwd<16, 4, mlp::FragV, PR>(sm + a.w2, 65, mlp::FragV{h}, e.z2);
+        mlp::mm_fwd<16, 4, mlp::FragV, PR>(sm + a.w2, kWLd, mlp::FragV{h}, e.z2);
     }
 }
 
@@ -194,7 +203,7 @@ __device__ __forceinline__ void geo_mlp(const GeoArgs& a, const float* sm, const
 #pragma unroll
     for (int t = 0; t < 4; ++t) h[t] = mlp::act4(a.act, zl[t], a.beta);
     e.out = mlp::bias_tile(sm + a.bo, 0, a.n_out);
-    mlp::mm_fwd<16, 1, mlp::FragV, PR>(sm + a.wo, 65, mlp::FragV{h}, &e.out);
+    mlp::mm_fwd<16, 1, mlp::FragV, PR>(sm + a.wo, kWLd, mlp::FragV{h}, &e.out);
 }
 
 // Encode + SDF MLP of this lane's sample at contracted point x.
@@ -389,7 +398,7 @@ __device__ __forceinline__ void write_base(const GeoArgs& a, const float* sm,
 #pragma unroll 1
         for (int t = 1; t < a.out_tiles; ++t) {
             v4f o = mlp::bias_tile(sm + a.bo, t, a.n_out);
-            mlp::mm_fwd<16, 1, mlp::FragV, mlp::kF32>(sm + a.wo + 16 * t * 65, 65, mlp::FragV{h},
+            mlp::mm_fwd<16, 1, mlp::FragV, mlp::kF32>(sm + a.wo + 16 * t * kWLd, kWLd, mlp::FragV{h},
                                                       &o);
             if (valid)
 #pragma unroll
@@ -444,7 +453,7 @@ __device__ __forceinline__ void geo_input_grad(const GeoArgs& a, const float* sm
             c2[t] = dh * c1[t];
             e1[t] = v4f{0, 0, 0, 0};
         }
-        mlp::mm_bwd_in<16, 4, mlp::FragV, mlp::kF32>(sm + a.w2, 65, mlp::FragV{c2}, e1);
+        mlp::mm_bwd_in<16, 4, mlp::FragV, mlp::kF32>(sm + a.w2, kWLd, mlp::FragV{c2}, e1);
 #pragma unroll
         for (int t = 0; t < 4; ++t) c1[t] = e1[t];
     }
@@ -632,7 +641,7 @@ __device__ __forceinline__ void out_ext_bwd(const GeoArgs& a, const float* sm, c
         if (i == 0)
 #pragma unroll
             for (int r = 0; r < 4; ++r) acc[1024 + 4 * k + r] += bs[r];
-        mlp::mm_bwd_in<4, 4, mlp::FragV, mlp::kF32>(sm + a.wo + 16 * t * 65, 65, mlp::FragV{&got},
+        mlp::mm_bwd_in<4, 4, mlp::FragV, mlp::kF32>(sm + a.wo + 16 * t * kWLd, kWLd, mlp::FragV{&got},
                                                     gh);
     }
 }
@@ -669,7 +678,7 @@ __device__ __forceinline__ void geo_eval_bwd(const GeoArgs& a, const float* sm,
     v4f gh[4];
 #pragma unroll
     for (int t = 0; t < 4; ++t) gh[t] = v4f{0, 0, 0, 0};
-    mlp::mm_bwd_in<4, 4, mlp::FragV, PR>(sm + a.wo, 65, mlp::FragV{&go}, gh);
+    mlp::mm_bwd_in<4, 4, mlp::FragV, PR>(sm + a.wo, kWLd, mlp::FragV{&go}, gh);
     if constexpr (NH == 2)
         if (ext) out_ext_bwd(a, sm, h, *ext, scratch, gh);
     if constexpr (NH == 2) {
@@ -683,7 +692,7 @@ __device__ __forceinline__ void geo_eval_bwd(const GeoArgs& a, const float* sm,
         mlp::mm_dw<4, 4, 16, mlp::FragV, PR>(gz2, mlp::FragV{h1}, scratch, G.dW2);
 #pragma unroll
         for (int t = 0; t < 4; ++t) gh[t] = v4f{0, 0, 0, 0};
-        mlp::mm_bwd_in<16, 4, mlp::FragV, PR>(sm + a.w2, 65, mlp::FragV{gz2}, gh);
+        mlp::mm_bwd_in<16, 4, mlp::FragV, PR>(sm + a.w2, kWLd, mlp::FragV{gz2}, gh);
     }
     v4f gz1[4];   // dh now holds the first hidden layer's derivative
 #pragma unroll
@@ -776,7 +785,7 @@ __device__ __forceinline__ void geo_eval_bwd_an(const GeoArgs& a, const float* s
         v4f gh[4];
 #pragma unroll
         for (int t = 0; t < 4; ++t) gh[t] = v4f{0, 0, 0, 0};
-        mlp::mm_bwd_in<4, 4, mlp::FragV, mlp::kF32>(sm + a.wo, 65, mlp::FragV{&go}, gh);
+        mlp::mm_bwd_in<4, 4, mlp::FragV, mlp::kF32>(sm + a.wo, kWLd, mlp::FragV{&go}, gh);
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
             v4f d2;
@@ -794,7 +803,7 @@ __device__ __forceinline__ void geo_eval_bwd_an(const GeoArgs& a, const float* s
             ad1[t] = dh1[t] * sv1[t];
             sv2[t] = v4f{0, 0, 0, 0};
         }
-        mlp::mm_fwd<16, 4, mlp::FragV, mlp::kF32>(sm + a.w2, 65, mlp::FragV{ad1}, sv2);
+        mlp::mm_fwd<16, 4, mlp::FragV, mlp::kF32>(sm + a.w2, kWLd, mlp::FragV{ad1}, sv2);
         v4f gz2[4], c2[4];
         {
             v4f h2[4], dh2[4];
@@ -812,7 +821,7 @@ __device__ __forceinline__ void geo_eval_bwd_an(const GeoArgs& a, const float* s
             v4f gh[4];
 #pragma unroll
             for (int t = 0; t < 4; ++t) gh[t] = v4f{0, 0, 0, 0};
-            mlp::mm_bwd_in<4, 4, mlp::FragV, mlp::kF32>(sm + a.wo, 65, mlp::FragV{&go}, gh);
+            mlp::mm_bwd_in<4, 4, mlp::FragV, mlp::kF32>(sm + a.wo, kWLd, mlp::FragV{&go}, gh);
             if (ext) out_ext_bwd(a, sm, h2, *ext, scratch, gh);
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
@@ -832,8 +841,8 @@ __device__ __forceinline__ void geo_eval_bwd_an(const GeoArgs& a, const float* s
             gh1[t] = v4f{0, 0, 0, 0};
             e1[t] = v4f{0, 0, 0, 0};
         }
-        mlp::mm_bwd_in<16, 4, mlp::FragV, mlp::kF32>(sm + a.w2, 65, mlp::FragV{gz2}, gh1);
-        mlp::mm_bwd_in<16, 4, mlp::FragV, mlp::kF32>(sm + a.w2, 65, mlp::FragV{c2}, e1);
+        mlp::mm_bwd_in<16, 4, mlp::FragV, mlp::kF32>(sm + a.w2, kWLd, mlp::FragV{gz2}, gh1);
+        mlp::mm_bwd_in<16, 4, mlp::FragV, mlp::kF32>(sm + a.w2, kWLd, mlp::FragV{c2}, e1);
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
             v4f d2;
@@ -1712,9 +1721,9 @@ static int make_geo_args(const am_grid_desc* g, const am_mlp_desc* m, const am_s
     a.w1 = o; o += 64 * a.ld1;
     a.b1 = o; o += 64;
     a.w2 = o; a.b2 = o;
-    if (m->n_hidden_layers == 2) { a.w2 = o; o += 64 * 65; a.b2 = o; o += 64; }
+    if (m->n_hidden_layers == 2) { a.w2 = o; o += 64 * kWLd; a.b2 = o; o += 64; }
     a.out_tiles = (m->n_out + 15) / 16;
-    a.wo = o; o += a.out_tiles * 16 * 65;
+    a.wo = o; o += a.out_tiles * 16 * kWLd;
     a.bo = o; o += a.out_tiles * 16;
     a.lds_floats = o;
     layout_params(m->n_in, m->n_out, m->n_hidden_layers, &a.pw1, &a.pb1, &a.pw2, &a.pb2, &a.pwo,

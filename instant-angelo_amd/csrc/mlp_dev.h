@@ -239,11 +239,14 @@ __device__ __forceinline__ void mm_bwd_in(const float* __restrict__ W, int ld, F
     }
 }
 
-constexpr int kScratchLd = 17;
+// Row stride of the transpose scratch: 20 keeps both its row writes (lane
+// group k row 4k + r) and its column reads (lane i row i, column 4q + k)
+// free of bank conflicts (17 put four lanes on a bank when writing).
+constexpr int kScratchLd = 20;
 
 // dW[ot*IT + it] += G (OT D tiles, rows = out) x X^T (KS B fragments, rows =
 // input row space) summed over the 16 points of the tile.  Transposes go
-// through the wave-private LDS scratch (16*(OT+IT) rows x 17).
+// through the wave-private LDS scratch (16*(OT+IT) rows x kScratchLd).
 // Result layout: lane (i,k), register r -> dW[16ot + 4k + r][16it + i].
 template <int OT, int IT, int KS, class Frag, int PR = kF32>
 __device__ __forceinline__ void mm_dw(const v4f* G, Frag X, float* scratch, v4f* dW) {
