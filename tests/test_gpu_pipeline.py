@@ -71,6 +71,41 @@ def test_march_bitexact(dev, use_grid, jit, cap, staged, monkeypatch):
         assert oc.max() <= cap
 
 
+@pytest.mark.parametrize("cap,full", [(40, False), (512, False), (40, True)])
+def test_march_bitexact_where_steps_stop_advancing(dev, cap, full):
+    """Rays from far away (|origin| 3e4 to 1e7) reach the box at t where
+    t + step rounds back to t or advances by whole ulps: the march ends
+    where a step no longer advances (occupied, all-occupied grid: after one
+    sample; empty: in the voxel skip) exactly as the oracle's rule does,
+    mixed with ordinary rays in the same launch."""
+    from angelo_amd import functions as fn, scenes
+    far = []
+    for dist in (3e4, 5e5, 1e7):
+        for ax in range(3):
+            for sg in (1.0, -1.0):
+                o = [0.05, -0.1, 0.02]
+                o[ax] = -sg * dist
+                dvec = [0.0, 0.0, 0.0]
+                dvec[ax] = sg
+                far.append(o + dvec)
+    rays = torch.cat([_rays(300, seed=5), torch.tensor(far)]).contiguous()
+    aabb = [-1.0, -1.0, -1.0, 1.0, 1.0, 1.0]
+    step = 2.0 / 256
+    grid = scenes.prewarm_occupancy(scenes.sdf_union, res=32, radius=1.0, band_cells=1.0)
+    if full:
+        grid = torch.ones_like(grid)
+    tn, tf = fn.ray_aabb(rays.to(dev), aabb)
+    ridx, t0, t1, packed = fn.ray_marching(rays.to(dev), tn, tf, aabb, grid.to(dev), step, None,
+                                           cap)
+    on, of = omarch.ray_aabb(rays.numpy(), aabb)
+    oi, o0, o1, oc = omarch.ray_marching(rays.numpy(), on, of, aabb, grid.numpy(), step, cap, None)
+    assert oc[-len(far):].max() >= 1   # some far rays do sample
+    assert np.array_equal(ridx.cpu().numpy(), oi)
+    assert np.array_equal(t0.cpu().numpy(), o0)
+    assert np.array_equal(t1.cpu().numpy(), o1)
+    assert np.array_equal(packed[:, 1].cpu().numpy(), oc)
+
+
 def test_march_all_rays_miss(dev):
     """A batch whose rays all miss the box yields no samples and zero counts."""
     from angelo_amd import functions as fn
