@@ -90,50 +90,97 @@ __device__ __forceinline__ float dist_next_voxel(const MarchArgs& a, const float
     return fmaxf(t, 0.0f);
 }
 
-// Marches one ray; writes samples when out pointers are given.  Returns count.
-__device__ int march_one(const MarchArgs& a, const float* o, const float* d, float near,
-                         float far, const uint8_t* __restrict__ grid, float* ts, float* te,
-                         int32_t* ri, int32_t ray) {
+// ---- marching, G lanes to a ray ----
+// The rule for one ray (nerfacc 0.3.3): from t0 = near, t1 = t0 + step,
+// t_mid = (t0 + t1) / 2, while t_mid < far and fewer than cap samples (cap 0:
+// no cap): the midpoint's cell occupied -> a sample [t0, t1], then t0 = t1,
+// t1 = t0 + step, t_mid = (t0 + t1) / 2, and the ray ends when t_mid no
+// longer grows; unoccupied -> t_mid advances by whole steps to the next voxel
+// boundary (the ray ends when a step no longer advances t), t0 / t1 = t_mid
+// -/+ step / 2.
+// G lanes run it together.  From the group's state (t0, t1, t_mid, j), lane
+// g takes g occupied steps (the same IEEE sequence) and tests its midpoint,
+// so the group's grid loads are in flight together instead of one after
+// another.  The first lane whose state is out of range or unoccupied ends the
+// round: the lanes before it are the next samples; if it is in range, the
+// voxel skip runs from its midpoint.  Samples and counts are those of the
+// one-lane rule, bit for bit.  Outputs ts / te / ri may be null (count
+// only); every lane returns the count.
+template <int G>
+__device__ int march_group(const MarchArgs& a, const float* o, const float* d, float near,
+                           float far, const uint8_t* __restrict__ grid, float* __restrict__ ts,
+                           float* __restrict__ te, int32_t* __restrict__ ri, int32_t ray, int g,
+                           int gbase) {
+    static_assert(G >= 2 && G <= 16 && (64 % G) == 0, "a group is a power of two within a wave");
     const float inv[3] = {1.0f / d[0], 1.0f / d[1], 1.0f / d[2]};
     const float step = a.step;
-    int j = 0;
+    const uint32_t mask = (1u << G) - 1u;
     float t0 = near;
     float t1 = t0 + step;
-    float t_mid = (t0 + t1) * 0.5f;
-    while (t_mid < far) {
-        if (a.cap > 0 && j >= a.cap) break;
+    float tm = (t0 + t1) * 0.5f;
+    int j = 0;
+    while (true) {
+        // this lane's state: g occupied steps ahead
+        float c0 = t0, c1 = t1, cm = tm;
+        bool alive = true;
+        for (int q = 0; q < g; ++q) {
+            const float prev = cm;
+            c0 = c1;
+            c1 = c0 + step;
+            cm = (c0 + c1) * 0.5f;
+            alive = alive && (cm > prev);
+        }
+        const bool cond = alive && cm < far && !(a.cap > 0 && j + g >= a.cap);
         float xyz[3];
 #pragma unroll
-        for (int k = 0; k < 3; ++k) xyz[k] = __fmaf_rn(t_mid, d[k], o[k]);
-        if (occupied(a, xyz, grid)) {
-            if (ts) {
-                ts[j] = t0;
-                te[j] = t1;
-                if (ri) ri[j] = ray;
-            }
-            ++j;
-            const float prev = t_mid;
-            t0 = t1;
-            t1 = t0 + step;
-            t_mid = (t0 + t1) * 0.5f;
-            if (!(t_mid > prev)) break;
-        } else {
-            const float target = t_mid + dist_next_voxel(a, xyz, d, inv);
-            float tt = t_mid;
-            bool stuck = false;
-            do {
-                const float nt = tt + step;
-                if (!(nt > tt)) { stuck = true; break; }
-                tt = nt;
-            } while (tt < target);
-            if (stuck) break;
-            t_mid = tt;
-            t0 = t_mid - step * 0.5f;
-            t1 = t_mid + step * 0.5f;
+        for (int k = 0; k < 3; ++k) xyz[k] = __fmaf_rn(cm, d[k], o[k]);
+        const bool occ = cond && occupied(a, xyz, grid);
+        const uint32_t bo = (uint32_t)(__ballot(occ) >> gbase) & mask;
+        const uint32_t bc = (uint32_t)(__ballot(cond) >> gbase) & mask;
+        const int m = __builtin_ctz(~bo);   // first lane not an occupied in-range state (G: none)
+        if (g < m && ts) {
+            ts[j + g] = c0;
+            te[j + g] = c1;
+            if (ri) ri[j + g] = ray;
         }
+        if (m == G) {   // all G were samples: continue from the last lane's successor
+            const float n0 = c1, n1 = n0 + step, nm = (n0 + n1) * 0.5f;
+            const int ok = nm > cm ? 1 : 0;
+            t0 = __shfl(n0, gbase + G - 1, 64);
+            t1 = __shfl(n1, gbase + G - 1, 64);
+            tm = __shfl(nm, gbase + G - 1, 64);
+            const int okg = __shfl(ok, gbase + G - 1, 64);
+            j += G;
+            if (!okg) break;
+            continue;
+        }
+        j += m;
+        if (!((bc >> m) & 1u)) break;   // state m ends the march
+        // state m is in range and unoccupied: the voxel skip from its midpoint
+        const float tmm = __shfl(cm, gbase + m, 64);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) xyz[k] = __fmaf_rn(tmm, d[k], o[k]);
+        const float target = tmm + dist_next_voxel(a, xyz, d, inv);
+        float tt = tmm;
+        bool stuck = false;
+        do {
+            const float nt = tt + step;
+            if (!(nt > tt)) {
+                stuck = true;
+                break;
+            }
+            tt = nt;
+        } while (tt < target);
+        if (stuck) break;
+        tm = tt;
+        t0 = tm - step * 0.5f;
+        t1 = tm + step * 0.5f;
     }
     return j;
 }
+
+// Lanes per ray of every marching kernel (8 measured no faster).
+constexpr int kMarchGroup = 4;
 
 // ---- background marching (nerfacc 0.3.3 ray_marching with
 // ContractionType.UN_BOUNDED_SPHERE and a cone angle, as called by
@@ -175,52 +222,89 @@ __device__ __forceinline__ float cone_dt(const MarchBgArgs& b, float t) {
     return fminf(fmaxf(t * b.cone, b.m.step), 1e10f);
 }
 
-__device__ int march_one_bg(const MarchBgArgs& b, const float* o, const float* d, float near,
-                            float far, const uint8_t* __restrict__ grid, float* ts, float* te,
-                            int32_t* ri, int32_t ray) {
-    int j = 0;
+// The background rule: from t0 = near, t1 = t0 + dt(t0), while t_mid < far
+// and fewer than cap samples: the midpoint's contracted cell occupied -> a
+// sample [t0, t1]; then t0 = t1, t1 = t0 + dt(t0), and the ray ends when
+// t_mid no longer grows.  The steps do not depend on occupancy, so G lanes
+// take G steps a round: lane g's state is g steps ahead, its sample index j
+// plus the occupied lanes before it; the first state out of range (or at
+// the cap) ends the ray.  Bit-exact with the one-lane rule.
+template <int G>
+__device__ int march_group_bg(const MarchBgArgs& b, const float* o, const float* d, float near,
+                              float far, const uint8_t* __restrict__ grid, float* __restrict__ ts,
+                              float* __restrict__ te, int32_t* __restrict__ ri, int32_t ray,
+                              int g, int gbase) {
+    const uint32_t mask = (1u << G) - 1u;
     float t0 = near;
     float t1 = t0 + cone_dt(b, t0);
-    float t_mid = (t0 + t1) * 0.5f;
-    while (t_mid < far) {
-        if (b.m.cap > 0 && j >= b.m.cap) break;
+    float tm = (t0 + t1) * 0.5f;
+    int j = 0;
+    while (true) {
+        float c0 = t0, c1 = t1, cm = tm;
+        bool alive = true;
+        for (int q = 0; q < g; ++q) {
+            const float prev = cm;
+            c0 = c1;
+            c1 = c0 + cone_dt(b, c0);
+            cm = (c0 + c1) * 0.5f;
+            alive = alive && (cm > prev);
+        }
+        const bool valid = alive && cm < far;
         float xyz[3];
 #pragma unroll
-        for (int k = 0; k < 3; ++k) xyz[k] = __fmaf_rn(t_mid, d[k], o[k]);
-        if (occupied_unbounded(b.m, xyz, grid)) {
-            if (ts) {
-                ts[j] = t0;
-                te[j] = t1;
-                ri[j] = ray;
-            }
-            ++j;
+        for (int k = 0; k < 3; ++k) xyz[k] = __fmaf_rn(cm, d[k], o[k]);
+        const bool occ = valid && occupied_unbounded(b.m, xyz, grid);
+        const uint32_t bo = (uint32_t)(__ballot(occ) >> gbase) & mask;
+        const int jg = j + __builtin_popcount(bo & ((1u << g) - 1u));   // this state's index
+        const bool ends = !valid || (b.m.cap > 0 && jg >= b.m.cap);
+        const uint32_t be = (uint32_t)(__ballot(ends) >> gbase) & mask;
+        const int e = be ? __builtin_ctz(be) : G;   // first state that ends the ray
+        if (g < e && occ && ts) {
+            ts[jg] = c0;
+            te[jg] = c1;
+            ri[jg] = ray;
         }
-        const float prev = t_mid;
-        t0 = t1;
-        t1 = t0 + cone_dt(b, t0);
-        t_mid = (t0 + t1) * 0.5f;
-        if (!(t_mid > prev)) break;
+        j += __builtin_popcount(bo & (e == G ? mask : ((1u << e) - 1u)));
+        if (e < G) break;
+        const float n0 = c1, n1 = n0 + cone_dt(b, n0), nm = (n0 + n1) * 0.5f;
+        const int ok = nm > cm ? 1 : 0;
+        t0 = __shfl(n0, gbase + G - 1, 64);
+        t1 = __shfl(n1, gbase + G - 1, 64);
+        tm = __shfl(nm, gbase + G - 1, 64);
+        if (!__shfl(ok, gbase + G - 1, 64)) break;
     }
     return j;
 }
 
-__global__ void k_march_bg_count(MarchBgArgs b, const float* __restrict__ rays, int64_t n,
-                                 const float* __restrict__ t_near, const float* __restrict__ t_far,
-                                 const uint8_t* __restrict__ grid, int32_t* __restrict__ counts) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// G lanes to a ray (whole groups share a ray and exit together).
+__global__ void __launch_bounds__(64) k_march_bg_count(MarchBgArgs b,
+                                                      const float* __restrict__ rays, int64_t n,
+                                                      const float* __restrict__ t_near,
+                                                      const float* __restrict__ t_far,
+                                                      const uint8_t* __restrict__ grid,
+                                                      int32_t* __restrict__ counts) {
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t i = tid / kMarchGroup;
+    const int g = (int)(tid % kMarchGroup);
     if (i >= n) return;
     const float o[3] = {rays[6 * i], rays[6 * i + 1], rays[6 * i + 2]};
     const float d[3] = {rays[6 * i + 3], rays[6 * i + 4], rays[6 * i + 5]};
-    counts[i] = march_one_bg(b, o, d, t_near[i], t_far[i], grid, nullptr, nullptr, nullptr,
-                             (int32_t)i);
+    const int j = march_group_bg<kMarchGroup>(b, o, d, t_near[i], t_far[i], grid, nullptr, nullptr,
+                                              nullptr, (int32_t)i, g, lane_id() - g);
+    if (g == 0) counts[i] = j;
 }
 
-__global__ void k_march_bg_fill(MarchBgArgs b, const float* __restrict__ rays, int64_t n,
-                                const float* __restrict__ t_near, const float* __restrict__ t_far,
-                                const uint8_t* __restrict__ grid,
-                                const int32_t* __restrict__ packed, int32_t* __restrict__ ridx,
-                                float* __restrict__ ts, float* __restrict__ te) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(64) k_march_bg_fill(MarchBgArgs b, const float* __restrict__ rays,
+                                                     int64_t n, const float* __restrict__ t_near,
+                                                     const float* __restrict__ t_far,
+                                                     const uint8_t* __restrict__ grid,
+                                                     const int32_t* __restrict__ packed,
+                                                     int32_t* __restrict__ ridx,
+                                                     float* __restrict__ ts,
+                                                     float* __restrict__ te) {
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t i = tid / kMarchGroup;
+    const int g = (int)(tid % kMarchGroup);
     if (i >= n) return;
     const int32_t base = packed[2 * i];
     const float o[3] = {rays[6 * i], rays[6 * i + 1], rays[6 * i + 2]};
@@ -228,8 +312,8 @@ __global__ void k_march_bg_fill(MarchBgArgs b, const float* __restrict__ rays, i
     MarchBgArgs c = b;
     c.m.cap = packed[2 * i + 1];  // never write past the counted samples
     if (c.m.cap == 0) return;
-    march_one_bg(c, o, d, t_near[i], t_far[i], grid, ts + base, te + base, ridx + base,
-                 (int32_t)i);
+    march_group_bg<kMarchGroup>(c, o, d, t_near[i], t_far[i], grid, ts + base, te + base,
+                                ridx + base, (int32_t)i, g, lane_id() - g);
 }
 
 // Early-stop visibility of the background samples (nerfacc 0.3.3
@@ -272,16 +356,22 @@ __global__ void k_ray_aabb(const float* __restrict__ rays, int64_t n, float aabb
     aabb_one(o, d, aabb, t_near + i, t_far + i);
 }
 
-__global__ void k_march_count(MarchArgs a, const float* __restrict__ rays, int64_t n,
-                              const float* __restrict__ t_near, const float* __restrict__ t_far,
-                              const float* __restrict__ jitter, const uint8_t* __restrict__ grid,
-                              int32_t* __restrict__ counts) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(64) k_march_count(MarchArgs a, const float* __restrict__ rays,
+                                                   int64_t n, const float* __restrict__ t_near,
+                                                   const float* __restrict__ t_far,
+                                                   const float* __restrict__ jitter,
+                                                   const uint8_t* __restrict__ grid,
+                                                   int32_t* __restrict__ counts) {
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t i = tid / kMarchGroup;
+    const int g = (int)(tid % kMarchGroup);
     if (i >= n) return;
     const float o[3] = {rays[6 * i], rays[6 * i + 1], rays[6 * i + 2]};
     const float d[3] = {rays[6 * i + 3], rays[6 * i + 4], rays[6 * i + 5]};
-    counts[i] = march_one(a, o, d, ray_near(t_near, jitter, i, a.step), t_far[i], grid, nullptr,
-                          nullptr, nullptr, (int32_t)i);
+    const int j = march_group<kMarchGroup>(a, o, d, ray_near(t_near, jitter, i, a.step), t_far[i],
+                                           grid, nullptr, nullptr, nullptr, (int32_t)i, g,
+                                           lane_id() - g);
+    if (g == 0) counts[i] = j;
 }
 
 // packed[i] = (start, count) from counts and their exclusive scan; total.
@@ -295,12 +385,17 @@ __global__ void k_march_pack(const int32_t* __restrict__ counts, const int32_t* 
     if (i == n - 1) total[0] = starts[i] + counts[i];
 }
 
-__global__ void k_march_fill(MarchArgs a, const float* __restrict__ rays, int64_t n,
-                             const float* __restrict__ t_near, const float* __restrict__ t_far,
-                             const float* __restrict__ jitter, const uint8_t* __restrict__ grid,
-                             const int32_t* __restrict__ packed, int32_t* __restrict__ ridx,
-                             float* __restrict__ ts, float* __restrict__ te) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(64) k_march_fill(MarchArgs a, const float* __restrict__ rays,
+                                                  int64_t n, const float* __restrict__ t_near,
+                                                  const float* __restrict__ t_far,
+                                                  const float* __restrict__ jitter,
+                                                  const uint8_t* __restrict__ grid,
+                                                  const int32_t* __restrict__ packed,
+                                                  int32_t* __restrict__ ridx,
+                                                  float* __restrict__ ts, float* __restrict__ te) {
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t i = tid / kMarchGroup;
+    const int g = (int)(tid % kMarchGroup);
     if (i >= n) return;
     const int32_t base = packed[2 * i];
     const float o[3] = {rays[6 * i], rays[6 * i + 1], rays[6 * i + 2]};
@@ -308,106 +403,34 @@ __global__ void k_march_fill(MarchArgs a, const float* __restrict__ rays, int64_
     MarchArgs b = a;
     b.cap = packed[2 * i + 1];  // never write past the counted samples
     if (b.cap == 0) return;
-    march_one(b, o, d, ray_near(t_near, jitter, i, a.step), t_far[i], grid, ts + base, te + base,
-              ridx + base, (int32_t)i);
+    march_group<kMarchGroup>(b, o, d, ray_near(t_near, jitter, i, a.step), t_far[i], grid,
+                             ts + base, te + base, ridx + base, (int32_t)i, g, lane_id() - g);
 }
 
 // Staged marching (a per-ray sample cap is set): the count pass also writes
 // each ray's samples into a cap-strided staging area, and the fill pass
-// copies them to their packed positions instead of marching again.  The
-// count pass runs march_one's rule with G lanes to a ray: from the group's state (t0, t1,
-// t_mid, j), lane g takes g occupied steps (the same IEEE sequence as
-// march_one) and tests its midpoint, so the group's grid loads are in flight
-// together instead of one after another.  The first lane whose state is out
-// of range or unoccupied ends the round: lanes before it are the next
-// samples; if it is in range, the voxel skip runs from its midpoint, as in
-// march_one.  Sample intervals and counts are bit-exact with march_one.
-template <int G>
-__global__ void __launch_bounds__(64) k_march_count_group(MarchArgs a, const float* __restrict__ rays,
-                                                         int64_t n, const float* __restrict__ t_near,
-                                                         const float* __restrict__ t_far,
-                                                         const float* __restrict__ jitter,
-                                                         const uint8_t* __restrict__ grid,
-                                                         int32_t* __restrict__ counts,
-                                                         float* __restrict__ st0,
-                                                         float* __restrict__ st1) {
-    static_assert(G >= 2 && G <= 16 && (64 % G) == 0, "a group is a power of two within a wave");
+// copies them to their packed positions instead of marching again.
+__global__ void __launch_bounds__(64) k_march_count_staged(MarchArgs a,
+                                                          const float* __restrict__ rays, int64_t n,
+                                                          const float* __restrict__ t_near,
+                                                          const float* __restrict__ t_far,
+                                                          const float* __restrict__ jitter,
+                                                          const uint8_t* __restrict__ grid,
+                                                          int32_t* __restrict__ counts,
+                                                          float* __restrict__ st0,
+                                                          float* __restrict__ st1) {
     const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t i = tid / G;   // the ray: the same for the whole group
-    const int g = (int)(tid % G);
-    const int gbase = lane_id() - g;
-    if (i >= n) return;          // whole groups (n G threads, G divides the wave)
+    const int64_t i = tid / kMarchGroup;
+    const int g = (int)(tid % kMarchGroup);
+    if (i >= n) return;
     const float o[3] = {rays[6 * i], rays[6 * i + 1], rays[6 * i + 2]};
     const float d[3] = {rays[6 * i + 3], rays[6 * i + 4], rays[6 * i + 5]};
-    const float inv[3] = {1.0f / d[0], 1.0f / d[1], 1.0f / d[2]};
-    const float far = t_far[i];
-    const float step = a.step;
     const int64_t base = i * (int64_t)a.cap;   // a.cap > 0: samples j < cap
-    float t0 = ray_near(t_near, jitter, i, step);
-    float t1 = t0 + step;
-    float tm = (t0 + t1) * 0.5f;
-    int j = 0;
-    while (true) {
-        // this lane's state: g occupied steps ahead
-        float c0 = t0, c1 = t1, cm = tm;
-        bool alive = true;
-        for (int q = 0; q < g; ++q) {
-            const float prev = cm;
-            c0 = c1;
-            c1 = c0 + step;
-            cm = (c0 + c1) * 0.5f;
-            alive = alive && (cm > prev);
-        }
-        const bool cond = alive && cm < far && !(a.cap > 0 && j + g >= a.cap);
-        float xyz[3];
-#pragma unroll
-        for (int k = 0; k < 3; ++k) xyz[k] = __fmaf_rn(cm, d[k], o[k]);
-        const bool occ = cond && occupied(a, xyz, grid);
-        const uint32_t mask = (1u << G) - 1u;
-        const uint32_t bo = (uint32_t)(__ballot(occ) >> gbase) & mask;
-        const uint32_t bc = (uint32_t)(__ballot(cond) >> gbase) & mask;
-        const int m = __builtin_ctz(~bo);   // first lane not an occupied in-range state (G: none)
-        if (g < m) {
-            st0[base + j + g] = c0;
-            st1[base + j + g] = c1;
-        }
-        if (m == G) {   // all G were samples: continue from the last lane's successor
-            const float n0 = c1, n1 = n0 + step, nm = (n0 + n1) * 0.5f;
-            const int ok = nm > cm ? 1 : 0;
-            t0 = __shfl(n0, gbase + G - 1, 64);
-            t1 = __shfl(n1, gbase + G - 1, 64);
-            tm = __shfl(nm, gbase + G - 1, 64);
-            const int okg = __shfl(ok, gbase + G - 1, 64);
-            j += G;
-            if (!okg) break;
-            continue;
-        }
-        j += m;
-        if (!((bc >> m) & 1u)) break;   // state m ends the march
-        // state m is in range and unoccupied: the voxel skip from its midpoint
-        const float tmm = __shfl(cm, gbase + m, 64);
-#pragma unroll
-        for (int k = 0; k < 3; ++k) xyz[k] = __fmaf_rn(tmm, d[k], o[k]);
-        const float target = tmm + dist_next_voxel(a, xyz, d, inv);
-        float tt = tmm;
-        bool stuck = false;
-        do {
-            const float nt = tt + step;
-            if (!(nt > tt)) {
-                stuck = true;
-                break;
-            }
-            tt = nt;
-        } while (tt < target);
-        if (stuck) break;
-        tm = tt;
-        t0 = tm - step * 0.5f;
-        t1 = tm + step * 0.5f;
-    }
+    const int j = march_group<kMarchGroup>(a, o, d, ray_near(t_near, jitter, i, a.step), t_far[i],
+                                           grid, st0 + base, st1 + base, nullptr, (int32_t)i, g,
+                                           lane_id() - g);
     if (g == 0) counts[i] = j;
 }
-
-constexpr int kMarchGroup = 4;   // 8 measured no faster
 
 // One wave per ray: its staged samples to the packed outputs, coalesced.
 __global__ void __launch_bounds__(256) k_march_fill_staged(const int32_t* __restrict__ packed,
@@ -518,10 +541,10 @@ extern "C" int am_march_count(const float* rays, int64_t n_rays, const float* t_
     char* stmp = c.take<char>(sb + 256);
     if (!c.ok()) return workspace_short(c.used, workspace_bytes);
     const MarchArgs a = march_args(roi6_host, grid, grid_res, step, cap);
-    const unsigned nb = (unsigned)ceil_div(n_rays, 128);
+    const unsigned nb = (unsigned)ceil_div(n_rays * kMarchGroup, 64);
     {
         KernelTimer kt_("march_count", s);
-        hipLaunchKernelGGL(k_march_count, dim3(nb), dim3(128), 0, s, a, rays, n_rays, t_near, t_far,
+        hipLaunchKernelGGL(k_march_count, dim3(nb), dim3(64), 0, s, a, rays, n_rays, t_near, t_far,
                            jitter, grid, counts);
     }
     AM_LAUNCH_CHECK();
@@ -549,7 +572,7 @@ extern "C" int am_march_fill(const float* rays, int64_t n_rays, const float* t_n
     const MarchArgs a = march_args(roi6_host, grid, grid_res, step, cap);
     {
         KernelTimer kt_("march_fill", to_stream(stream));
-        hipLaunchKernelGGL(k_march_fill, dim3((unsigned)ceil_div(n_rays, 128)), dim3(128), 0,
+        hipLaunchKernelGGL(k_march_fill, dim3((unsigned)ceil_div(n_rays * kMarchGroup, 64)), dim3(64), 0,
                            to_stream(stream), a, rays, n_rays, t_near, t_far, jitter, grid, packed,
                            ray_indices, t_starts, t_ends);
     }
@@ -595,7 +618,7 @@ extern "C" int am_march_count_staged(const float* rays, int64_t n_rays, const fl
     const MarchArgs a = march_args(roi6_host, grid, grid_res, step, cap);
     {
         KernelTimer kt_("march_count", s);
-        hipLaunchKernelGGL(k_march_count_group<kMarchGroup>,
+        hipLaunchKernelGGL(k_march_count_staged,
                            dim3((unsigned)ceil_div(n_rays * kMarchGroup, 64)), dim3(64), 0, s, a,
                            rays, n_rays, t_near, t_far, jitter, grid, counts, st0, st1);
     }
@@ -665,7 +688,8 @@ extern "C" int am_march_bg_count(const float* rays, int64_t n_rays, const float*
     b.cone = cone_angle;
     {
         KernelTimer kt_("march_bg_count", s);
-        hipLaunchKernelGGL(k_march_bg_count, dim3((unsigned)ceil_div(n_rays, 64)), dim3(64), 0, s,
+        hipLaunchKernelGGL(k_march_bg_count, dim3((unsigned)ceil_div(n_rays * kMarchGroup, 64)),
+                           dim3(64), 0, s,
                            b, rays, n_rays, t_near, t_far, grid, counts);
     }
     AM_LAUNCH_CHECK();
@@ -696,7 +720,8 @@ extern "C" int am_march_bg_fill(const float* rays, int64_t n_rays, const float* 
     b.cone = cone_angle;
     {
         KernelTimer kt_("march_bg_fill", to_stream(stream));
-        hipLaunchKernelGGL(k_march_bg_fill, dim3((unsigned)ceil_div(n_rays, 64)), dim3(64), 0,
+        hipLaunchKernelGGL(k_march_bg_fill, dim3((unsigned)ceil_div(n_rays * kMarchGroup, 64)),
+                           dim3(64), 0,
                            to_stream(stream), b, rays, n_rays, t_near, t_far, grid, packed,
                            ray_indices, t_starts, t_ends);
     }
