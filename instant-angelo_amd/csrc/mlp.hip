@@ -85,42 +85,27 @@ __device__ __forceinline__ void out_layer(const MlpArgs& a, const float* sm, Fra
     mlp::mm_fwd<16, OT>(sm + a.w_lds[L], a.ld[L], h, o);
 }
 
-// Forward of one 16-point tile; keeps pre-activations for backward.
-template <int KS, int OT>
+// Forward of one 16-point tile (NH hidden layers); keeps the hidden
+// pre-activations for the backward.
+template <int KS, int OT, int NH>
 __device__ __forceinline__ void tile_fwd(const MlpArgs& a, const float* sm, const float* xb,
-                                         v4f (&z)[4][4], v4f* zo) {
-    {
-        v4f acc[4];
+                                         v4f (&z)[NH][4], v4f* zo) {
 #pragma unroll
-        for (int t = 0; t < 4; ++t) acc[t] = mlp::bias_tile(sm + a.b_lds[0], t, 64);
-        mlp::mm_fwd<KS, 4>(sm + a.w_lds[0], a.ld[0], mlp::FragF{xb}, acc);
+    for (int t = 0; t < 4; ++t) z[0][t] = mlp::bias_tile(sm + a.b_lds[0], t, 64);
+    mlp::mm_fwd<KS, 4>(sm + a.w_lds[0], a.ld[0], mlp::FragF{xb}, z[0]);
 #pragma unroll
-        for (int t = 0; t < 4; ++t) z[0][t] = acc[t];
-    }
+    for (int L = 1; L < NH; ++L) {
+        v4f h[4];
 #pragma unroll
-    for (int L = 1; L < 4; ++L) {
-        if (L < a.n_layers - 1) {
-            v4f h[4], acc[4];
-#pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                h[t] = mlp::act4(a.act, z[L - 1][t], a.beta);
-                acc[t] = mlp::bias_tile(sm + a.b_lds[L], t, 64);
-            }
-            mlp::mm_fwd<16, 4>(sm + a.w_lds[L], a.ld[L], mlp::FragV{h}, acc);
-#pragma unroll
-            for (int t = 0; t < 4; ++t) z[L][t] = acc[t];
+        for (int t = 0; t < 4; ++t) {
+            h[t] = mlp::act4(a.act, z[L - 1][t], a.beta);
+            z[L][t] = mlp::bias_tile(sm + a.b_lds[L], t, 64);
         }
+        mlp::mm_fwd<16, 4>(sm + a.w_lds[L], a.ld[L], mlp::FragV{h}, z[L]);
     }
     v4f h[4];
-    const int Lh = a.n_layers - 2;
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-        v4f zz = z[0][t];
-#pragma unroll
-        for (int L = 1; L < 4; ++L)
-            if (L == Lh) zz = z[L][t];
-        h[t] = mlp::act4(a.act, zz, a.beta);
-    }
+    for (int t = 0; t < 4; ++t) h[t] = mlp::act4(a.act, z[NH - 1][t], a.beta);
     out_layer<OT>(a, sm, mlp::FragV{h}, zo);
 }
 
@@ -135,7 +120,7 @@ __device__ __forceinline__ void load_x(const MlpArgs& a, const float* __restrict
     }
 }
 
-template <int KS, int OT>
+template <int KS, int OT, int NH>
 __global__ void __launch_bounds__(256) k_mlp_fwd(MlpArgs a, const float* __restrict__ x, int64_t n,
                                                  const float* __restrict__ params,
                                                  float* __restrict__ y) {
@@ -151,8 +136,8 @@ __global__ void __launch_bounds__(256) k_mlp_fwd(MlpArgs a, const float* __restr
         const bool valid = p < n;
         float xb[KS];
         load_x<KS>(a, x, p, valid, xb);
-        v4f z[4][4], zo[OT];
-        tile_fwd<KS, OT>(a, sm, xb, z, zo);
+        v4f z[NH][4], zo[OT];
+        tile_fwd<KS, OT, NH>(a, sm, xb, z, zo);
         if (valid) {
 #pragma unroll
             for (int t = 0; t < OT; ++t)
@@ -165,53 +150,77 @@ __global__ void __launch_bounds__(256) k_mlp_fwd(MlpArgs a, const float* __restr
     }
 }
 
-// Adds a register dW tile set (OT x IT tiles) into the LDS accumulator.
+// Writes a register dW tile set (OT x IT tiles; lane (i, k), register r ->
+// row 16 ot + 4k + r, column 16 it + i) into a parameter-layout row.
 template <int OT, int IT>
-__device__ __forceinline__ void add_dw_lds(float* acc, int ld, const v4f* dW) {
+__device__ __forceinline__ void write_dw(float* out, int64_t w_par, int rows, int cols,
+                                         const v4f* dW) {
     const int l = lane_id(), i = l & 15, k = l >> 4;
 #pragma unroll
     for (int ot = 0; ot < OT; ++ot)
 #pragma unroll
         for (int it = 0; it < IT; ++it)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) acc[(16 * ot + 4 * k + r) * ld + 16 * it + i] += dW[ot * IT + it][r];
+            for (int r = 0; r < 4; ++r) {
+                const int o = 16 * ot + 4 * k + r, c = 16 * it + i;
+                if (o < rows && c < cols) out[w_par + (int64_t)o * cols + c] = dW[ot * IT + it][r];
+            }
 }
 
+// Bias gradients: per-lane sums over the wave's samples, summed over the
+// tile's 16 lanes at the end.
 template <int T>
-__device__ __forceinline__ void add_db_lds(float* acc, const v4f* g) {
+__device__ __forceinline__ void write_db(float* out, int64_t b_par, int rows, const v4f* g) {
     const int l = lane_id(), i = l & 15, k = l >> 4;
 #pragma unroll
     for (int t = 0; t < T; ++t) {
-        const v4f s = mlp::rowsum_tile(g[t]);
+        const v4f sum = mlp::rowsum_tile(g[t]);
         if (i == 0)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) acc[16 * t + 4 * k + r] += s[r];
+            for (int r = 0; r < 4; ++r)
+                if (16 * t + 4 * k + r < rows) out[b_par + 16 * t + 4 * k + r] = sum[r];
     }
 }
 
-// One wave per block; LDS = weights | dW accumulator (same layout) | scratch.
-template <int KS, int OT>
-__global__ void __launch_bounds__(64) k_mlp_bwd(MlpArgs a, const float* __restrict__ x, int64_t n,
-                                                const float* __restrict__ params,
-                                                const float* __restrict__ dy,
-                                                float* __restrict__ dx,
-                                                float* __restrict__ partial) {
+// Backward: waves of a block share the weights in LDS; each wave keeps its
+// weight and bias gradients in registers over its tiles and writes them as
+// one partial row (summed over the rows afterwards, a fixed order).
+template <int KS, int OT, int NH>
+__global__ void __launch_bounds__(256) k_mlp_bwd(MlpArgs a, const float* __restrict__ x, int64_t n,
+                                                 const float* __restrict__ params,
+                                                 const float* __restrict__ dy,
+                                                 float* __restrict__ dx,
+                                                 float* __restrict__ partial) {
+    constexpr int IT0 = KS / 4;
     extern __shared__ float sm[];
-    float* gacc = sm + a.lds_floats;
-    float* scratch = gacc + a.lds_floats;
     load_mlp_lds(a, params, sm);
-    for (int q = threadIdx.x; q < a.lds_floats; q += blockDim.x) gacc[q] = 0.0f;
     __syncthreads();
     const int l = lane_id(), i = l & 15, k = l >> 4;
+    const int nw = blockDim.x >> 6, wave = threadIdx.x >> 6;
+    float* scratch = sm + a.lds_floats + wave * (128 * mlp::kScratchLd);
     const int64_t n_tiles = (n + 15) / 16;
-    const int Lo = a.n_layers - 1;
-    for (int64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+    const int Lo = NH;   // the output layer's index
+    v4f dW0[4 * IT0], dWh[NH > 1 ? 16 * (NH - 1) : 1], dWo[4 * OT], db[NH][4], dbo[OT];
+#pragma unroll
+    for (int q = 0; q < 4 * IT0; ++q) dW0[q] = v4f{0, 0, 0, 0};
+#pragma unroll
+    for (int q = 0; q < (NH > 1 ? 16 * (NH - 1) : 1); ++q) dWh[q] = v4f{0, 0, 0, 0};
+#pragma unroll
+    for (int q = 0; q < 4 * OT; ++q) dWo[q] = v4f{0, 0, 0, 0};
+#pragma unroll
+    for (int L = 0; L < NH; ++L)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) db[L][t] = v4f{0, 0, 0, 0};
+#pragma unroll
+    for (int t = 0; t < OT; ++t) dbo[t] = v4f{0, 0, 0, 0};
+    for (int64_t tile = (int64_t)blockIdx.x * nw + wave; tile < n_tiles;
+         tile += (int64_t)gridDim.x * nw) {
         const int64_t p = tile * 16 + i;
         const bool valid = p < n;
         float xb[KS];
         load_x<KS>(a, x, p, valid, xb);
-        v4f z[4][4], zo[OT];
-        tile_fwd<KS, OT>(a, sm, xb, z, zo);
+        v4f z[NH][4], zo[OT];
+        tile_fwd<KS, OT, NH>(a, sm, xb, z, zo);
         v4f go[OT];
 #pragma unroll
         for (int t = 0; t < OT; ++t)
@@ -227,47 +236,34 @@ __global__ void __launch_bounds__(64) k_mlp_bwd(MlpArgs a, const float* __restri
         {
             v4f hin[4];
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                v4f zz = z[0][t];
+            for (int t = 0; t < 4; ++t) hin[t] = mlp::act4(a.act, z[NH - 1][t], a.beta);
+            mlp::mm_dw<OT, 4, 16>(go, mlp::FragV{hin}, scratch, dWo);
 #pragma unroll
-                for (int L = 1; L < 4; ++L)
-                    if (L == Lo - 1) zz = z[L][t];
-                hin[t] = mlp::act4(a.act, zz, a.beta);
-            }
-            v4f dW[OT * 4];
-#pragma unroll
-            for (int q = 0; q < OT * 4; ++q) dW[q] = v4f{0, 0, 0, 0};
-            mlp::mm_dw<OT, 4, 16>(go, mlp::FragV{hin}, scratch, dW);
-            add_dw_lds<OT, 4>(gacc + a.w_lds[Lo], a.ld[Lo], dW);
-            add_db_lds<OT>(gacc + a.b_lds[Lo], go);
+            for (int t = 0; t < OT; ++t) dbo[t] += go[t];
 #pragma unroll
             for (int t = 0; t < 4; ++t) gh[t] = v4f{0, 0, 0, 0};
             mlp::mm_bwd_in<4 * OT, 4>(sm + a.w_lds[Lo], a.ld[Lo], mlp::FragV{go}, gh);
         }
         // hidden layers, last to first
 #pragma unroll
-        for (int L = 3; L >= 0; --L) {
-            if (L >= Lo) continue;
+        for (int L = NH - 1; L >= 0; --L) {
             v4f gz[4];
 #pragma unroll
             for (int t = 0; t < 4; ++t)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) gz[t][r] = gh[t][r] * mlp::act_bwd(a.act, z[L][t][r], a.beta);
-            add_db_lds<4>(gacc + a.b_lds[L], gz);
+#pragma unroll
+            for (int t = 0; t < 4; ++t) db[L][t] += gz[t];
             if (L == 0) {
-                v4f dW[4 * (KS / 4)];
-#pragma unroll
-                for (int q = 0; q < 4 * (KS / 4); ++q) dW[q] = v4f{0, 0, 0, 0};
-                mlp::mm_dw<4, KS / 4, KS>(gz, mlp::FragF{xb}, scratch, dW);
-                add_dw_lds<4, KS / 4>(gacc + a.w_lds[0], a.ld[0], dW);
+                mlp::mm_dw<4, IT0, KS>(gz, mlp::FragF{xb}, scratch, dW0);
                 if (dx) {
-                    v4f gx[KS / 4];
+                    v4f gx[IT0];
 #pragma unroll
-                    for (int t = 0; t < KS / 4; ++t) gx[t] = v4f{0, 0, 0, 0};
-                    mlp::mm_bwd_in<16, KS / 4>(sm + a.w_lds[0], a.ld[0], mlp::FragV{gz}, gx);
+                    for (int t = 0; t < IT0; ++t) gx[t] = v4f{0, 0, 0, 0};
+                    mlp::mm_bwd_in<16, IT0>(sm + a.w_lds[0], a.ld[0], mlp::FragV{gz}, gx);
                     if (valid)
 #pragma unroll
-                        for (int t = 0; t < KS / 4; ++t)
+                        for (int t = 0; t < IT0; ++t)
 #pragma unroll
                             for (int r = 0; r < 4; ++r) {
                                 const int row = 16 * t + 4 * k + r;
@@ -278,78 +274,98 @@ __global__ void __launch_bounds__(64) k_mlp_bwd(MlpArgs a, const float* __restri
                 v4f hin[4];
 #pragma unroll
                 for (int t = 0; t < 4; ++t) hin[t] = mlp::act4(a.act, z[L - 1][t], a.beta);
-                v4f dW[16];
-#pragma unroll
-                for (int q = 0; q < 16; ++q) dW[q] = v4f{0, 0, 0, 0};
-                mlp::mm_dw<4, 4, 16>(gz, mlp::FragV{hin}, scratch, dW);
-                add_dw_lds<4, 4>(gacc + a.w_lds[L], a.ld[L], dW);
+                mlp::mm_dw<4, 4, 16>(gz, mlp::FragV{hin}, scratch, dWh + 16 * (L - 1));
 #pragma unroll
                 for (int t = 0; t < 4; ++t) gh[t] = v4f{0, 0, 0, 0};
                 mlp::mm_bwd_in<16, 4>(sm + a.w_lds[L], a.ld[L], mlp::FragV{gz}, gh);
             }
         }
     }
-    __syncthreads();
-    // LDS accumulator -> this block's partial row in parameter layout.
-    float* out = partial + (int64_t)blockIdx.x * a.n_params;
-    for (int L = 0; L < a.n_layers; ++L) {
-        const int in = a.cols_in[L], ld = a.ld[L];
-        for (int q = threadIdx.x; q < a.rows_out[L] * in; q += blockDim.x) {
-            const int o = q / in, c = q - o * in;
-            out[a.w_par[L] + q] = gacc[a.w_lds[L] + o * ld + c];
-        }
-        for (int q = threadIdx.x; q < a.rows_out[L]; q += blockDim.x)
-            out[a.b_par[L] + q] = gacc[a.b_lds[L] + q];
+    // this wave's partial row in parameter layout (every wave writes all of it)
+    float* out = partial + ((int64_t)blockIdx.x * nw + wave) * a.n_params;
+    write_dw<4, IT0>(out, a.w_par[0], a.rows_out[0], a.cols_in[0], dW0);
+    write_db<4>(out, a.b_par[0], a.rows_out[0], db[0]);
+#pragma unroll
+    for (int L = 1; L < NH; ++L) {
+        write_dw<4, 4>(out, a.w_par[L], a.rows_out[L], a.cols_in[L], dWh + 16 * (L - 1));
+        write_db<4>(out, a.b_par[L], a.rows_out[L], db[L]);
     }
+    write_dw<OT, 4>(out, a.w_par[Lo], a.rows_out[Lo], a.cols_in[Lo], dWo);
+    write_db<OT>(out, a.b_par[Lo], a.rows_out[Lo], dbo);
 }
 
-static constexpr int kMlpBwdMaxBlocks = 1024;
+// Blocks of four waves; at most this many (the partial rows the column sum
+// reads: waves x n_params floats).
+static constexpr int kMlpBwdMaxBlocks = 512;
 
 static size_t mlp_bwd_lds_bytes(const MlpArgs& a) {
-    return (size_t)(2 * a.lds_floats + (16 * 4 + 16 * 4) * mlp::kScratchLd) * sizeof(float);
+    return (size_t)(a.lds_floats + 4 * 128 * mlp::kScratchLd) * sizeof(float);
 }
 
-template <int KS, int OT>
+static int mlp_bwd_waves(int64_t n) {
+    return 4 * persistent_blocks((n + 15) / 16, 4, kMlpBwdMaxBlocks);
+}
+
+template <int KS, int OT, int NH>
 static int launch_fwd(const MlpArgs& a, const float* x, int64_t n, const float* params, float* y,
                       hipStream_t s) {
     const size_t lds = (size_t)a.lds_floats * sizeof(float);
-    AM_HIP(hipFuncSetAttribute((const void*)k_mlp_fwd<KS, OT>,
+    AM_HIP(hipFuncSetAttribute((const void*)k_mlp_fwd<KS, OT, NH>,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     const int blocks = persistent_blocks((n + 15) / 16, 4, 2048);
-    hipLaunchKernelGGL((k_mlp_fwd<KS, OT>), dim3(blocks), dim3(256), lds, s, a, x, n, params, y);
+    hipLaunchKernelGGL((k_mlp_fwd<KS, OT, NH>), dim3(blocks), dim3(256), lds, s, a, x, n, params,
+                       y);
     AM_LAUNCH_CHECK();
     return AM_OK;
 }
 
-template <int KS, int OT>
+template <int KS, int OT, int NH>
 static int launch_bwd(const MlpArgs& a, const float* x, int64_t n, const float* params,
                       const float* dy, float* dx, float* dparams, void* ws, size_t ws_bytes,
                       hipStream_t s) {
     const size_t lds = mlp_bwd_lds_bytes(a);
     AM_CHECK_ARG(lds <= 160 * 1024, "MLP too large for the LDS-resident backward (%zu bytes)", lds);
-    const int blocks = persistent_blocks((n + 15) / 16, 1, kMlpBwdMaxBlocks);
+    const int waves = mlp_bwd_waves(n);
     WsCarver c(ws, ws_bytes);
-    float* partial = c.take<float>((size_t)blocks * a.n_params);
-    float* tmp = c.take<float>((size_t)colsum_tmp_floats(blocks, a.n_params) + 1);
+    float* partial = c.take<float>((size_t)waves * a.n_params);
+    float* tmp = c.take<float>((size_t)colsum_tmp_floats(waves, a.n_params) + 1);
     if (!c.ok()) return workspace_short(c.used, ws_bytes);
-    AM_HIP(hipFuncSetAttribute((const void*)k_mlp_bwd<KS, OT>,
+    AM_HIP(hipFuncSetAttribute((const void*)k_mlp_bwd<KS, OT, NH>,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL((k_mlp_bwd<KS, OT>), dim3(blocks), dim3(64), lds, s, a, x, n, params, dy, dx,
-                       partial);
+    hipLaunchKernelGGL((k_mlp_bwd<KS, OT, NH>), dim3(waves / 4), dim3(256), lds, s, a, x, n, params,
+                       dy, dx, partial);
     AM_LAUNCH_CHECK();
-    return colsum_f32(partial, blocks, a.n_params, dparams, tmp, s);
+    return colsum_f32(partial, waves, a.n_params, dparams, tmp, s);
 }
 
-#define AM_MLP_DISPATCH(FN, ...)                                              \
-    do {                                                                      \
-        const int ks = 4 * (int)ceil_div(a.n_in, 16);                         \
-        const bool big = a.n_out > 16;                                        \
-        switch (ks) {                                                         \
-            case 4: return big ? FN<4, 4>(__VA_ARGS__) : FN<4, 1>(__VA_ARGS__);     \
-            case 8: return big ? FN<8, 4>(__VA_ARGS__) : FN<8, 1>(__VA_ARGS__);     \
-            case 12: return big ? FN<12, 4>(__VA_ARGS__) : FN<12, 1>(__VA_ARGS__);  \
-            default: return big ? FN<16, 4>(__VA_ARGS__) : FN<16, 1>(__VA_ARGS__);  \
-        }                                                                     \
+#define AM_MLP_DISPATCH_NH(FN, KS, OT, ...)                                  \
+    do {                                                                     \
+        switch (a.n_layers - 1) {                                            \
+            case 1: return FN<KS, OT, 1>(__VA_ARGS__);                       \
+            case 2: return FN<KS, OT, 2>(__VA_ARGS__);                       \
+            case 3: return FN<KS, OT, 3>(__VA_ARGS__);                       \
+            default: return FN<KS, OT, 4>(__VA_ARGS__);                      \
+        }                                                                    \
+    } while (0)
+
+#define AM_MLP_DISPATCH(FN, ...)                                                          \
+    do {                                                                                  \
+        const int ks = 4 * (int)ceil_div(a.n_in, 16);                                     \
+        const bool big = a.n_out > 16;                                                    \
+        switch (ks) {                                                                     \
+            case 4:                                                                       \
+                if (big) AM_MLP_DISPATCH_NH(FN, 4, 4, __VA_ARGS__);                       \
+                AM_MLP_DISPATCH_NH(FN, 4, 1, __VA_ARGS__);                                \
+            case 8:                                                                       \
+                if (big) AM_MLP_DISPATCH_NH(FN, 8, 4, __VA_ARGS__);                       \
+                AM_MLP_DISPATCH_NH(FN, 8, 1, __VA_ARGS__);                                \
+            case 12:                                                                      \
+                if (big) AM_MLP_DISPATCH_NH(FN, 12, 4, __VA_ARGS__);                      \
+                AM_MLP_DISPATCH_NH(FN, 12, 1, __VA_ARGS__);                               \
+            default:                                                                      \
+                if (big) AM_MLP_DISPATCH_NH(FN, 16, 4, __VA_ARGS__);                      \
+                AM_MLP_DISPATCH_NH(FN, 16, 1, __VA_ARGS__);                               \
+        }                                                                                 \
     } while (0)
 
 }  // namespace am
@@ -376,10 +392,10 @@ extern "C" int am_mlp_fwd(const am_mlp_desc* m, const float* x, int64_t n, const
 extern "C" size_t am_mlp_bwd_workspace_size(const am_mlp_desc* m, int64_t n) {
     MlpArgs a;
     if (make_mlp_args(m, &a) || n < 0) return 0;
-    const int blocks = persistent_blocks((n + 15) / 16, 1, kMlpBwdMaxBlocks);
+    const int waves = mlp_bwd_waves(n);
     WsCarver c(nullptr, 0);
-    c.take<float>((size_t)blocks * a.n_params);
-    c.take<float>((size_t)colsum_tmp_floats(blocks, a.n_params) + 1);
+    c.take<float>((size_t)waves * a.n_params);
+    c.take<float>((size_t)colsum_tmp_floats(waves, a.n_params) + 1);
     return c.used + 256;
 }
 
