@@ -2,6 +2,8 @@
 include/angelo_amd.h).  Each forward and backward is one or a few native
 launches on the current stream; there is no CPU or eager-torch fallback.
 """
+import math
+
 import torch
 
 from . import _native as nat
@@ -386,9 +388,17 @@ def ray_aabb(rays, aabb):
     return t_near, t_far
 
 
-# Marching with a per-ray cap stages the count pass's samples (one march per
-# ray); off: count and fill both march (the form without a cap).
+# Marching stages the count pass's samples (one march per ray); off: count
+# and fill both march.  Without a per-ray cap the staging area is sized by
+# the most samples a unit-direction ray can take in the box (its diagonal
+# over the step); a batch in which some ray reaches that bound marches again
+# in the unstaged form, so the samples never depend on it.
 MARCH_STAGED = {"on": True}
+
+
+def _implicit_stage_cap(roi, step):
+    diag = math.sqrt(sum((float(roi[3 + d]) - float(roi[d])) ** 2 for d in range(3)))
+    return int(math.ceil(diag / float(step) * 1.001)) + 4
 
 
 def ray_marching(rays, t_near, t_far, roi, grid_binary, step, jitter=None, cap=0):
@@ -410,22 +420,32 @@ def ray_marching(rays, t_near, t_far, roi, grid_binary, step, jitter=None, cap=0
     packed = torch.empty(R, 2, device=dev, dtype=torch.int32)
     total = torch.empty(1, device=dev, dtype=torch.int32)
     jit = _f32c(jitter) if jitter is not None else None
-    if cap > 0 and MARCH_STAGED["on"]:
+    stage_cap = int(cap) if cap > 0 else (_implicit_stage_cap(roi, step) if R > 0 else 0)
+    if stage_cap > 0 and MARCH_STAGED["on"]:
         # one march per ray: the count pass stages the samples, the fill copies
-        ws = nat.workspace(lib.am_march_staged_workspace_size(R, int(cap)), dev)
+        ws = nat.workspace(lib.am_march_staged_workspace_size(R, stage_cap), dev)
         nat.check(lib.am_march_count_staged(nat.ptr(rays), R, nat.ptr(t_near), nat.ptr(t_far),
-                                            nat.ptr(jit), roi_p, gptr, res, float(step), int(cap),
-                                            nat.ptr(packed), nat.ptr(total), nat.ptr(ws),
-                                            ws.numel(), nat.stream()), "am_march_count_staged")
-        N = int(total.item())
-        ridx = torch.empty(N, device=dev, dtype=torch.int32)
-        t0 = torch.empty(N, device=dev, dtype=torch.float32)
-        t1 = torch.empty(N, device=dev, dtype=torch.float32)
-        if N > 0:
-            nat.check(lib.am_march_fill_staged(R, int(cap), nat.ptr(packed), nat.ptr(ws),
-                                               ws.numel(), nat.ptr(ridx), nat.ptr(t0),
-                                               nat.ptr(t1), nat.stream()), "am_march_fill_staged")
-        return ridx, t0, t1, packed
+                                            nat.ptr(jit), roi_p, gptr, res, float(step),
+                                            stage_cap, nat.ptr(packed), nat.ptr(total),
+                                            nat.ptr(ws), ws.numel(), nat.stream()),
+                  "am_march_count_staged")
+        if cap > 0:
+            N = int(total.item())
+        else:   # the total and the largest count in one host read
+            N, most = torch.stack([total[0], packed[:, 1].max()]).tolist()
+            N = int(N)
+            if most >= stage_cap:
+                N = -1   # some ray may be longer than the bound: march unstaged
+        if N >= 0:
+            ridx = torch.empty(N, device=dev, dtype=torch.int32)
+            t0 = torch.empty(N, device=dev, dtype=torch.float32)
+            t1 = torch.empty(N, device=dev, dtype=torch.float32)
+            if N > 0:
+                nat.check(lib.am_march_fill_staged(R, stage_cap, nat.ptr(packed), nat.ptr(ws),
+                                                   ws.numel(), nat.ptr(ridx), nat.ptr(t0),
+                                                   nat.ptr(t1), nat.stream()),
+                          "am_march_fill_staged")
+            return ridx, t0, t1, packed
     ws = nat.workspace(lib.am_march_workspace_size(R), dev)
     nat.check(lib.am_march_count(nat.ptr(rays), R, nat.ptr(t_near), nat.ptr(t_far), nat.ptr(jit),
                                  roi_p, gptr, res, float(step), int(cap), nat.ptr(packed),

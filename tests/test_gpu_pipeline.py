@@ -106,6 +106,34 @@ def test_march_bitexact_where_steps_stop_advancing(dev, cap, full):
     assert np.array_equal(packed[:, 1].cpu().numpy(), oc)
 
 
+@pytest.mark.parametrize("dir_scale", [1.0, 0.05])
+def test_march_without_cap_is_staged_and_exact(dev, dir_scale):
+    """Without a per-ray cap the staged form sizes its staging area by the
+    box diagonal over the step (the most samples a unit-direction ray
+    takes); direction vectors of length 0.05 make rays longer than that, and
+    the batch then marches again unstaged.  Both are bit-exact with the
+    oracle."""
+    from angelo_amd import functions as fn, scenes
+    rays = _rays(600, seed=9).clone()
+    rays[:, 3:] *= dir_scale
+    aabb = [-1.0, -1.0, -1.0, 1.0, 1.0, 1.0]
+    step = 2.0 / 256
+    grid = scenes.prewarm_occupancy(scenes.sdf_union, res=32, radius=1.0, band_cells=1.0)
+    tn, tf = fn.ray_aabb(rays.to(dev), aabb)
+    ridx, t0, t1, packed = fn.ray_marching(rays.to(dev), tn, tf, aabb, grid.to(dev), step)
+    on, of = omarch.ray_aabb(rays.numpy(), aabb)
+    oi, o0, o1, oc = omarch.ray_marching(rays.numpy(), on, of, aabb, grid.numpy(), step, 0, None)
+    bound = fn._implicit_stage_cap(aabb, step)
+    if dir_scale < 1.0:
+        assert oc.max() > bound   # the fallback is exercised
+    else:
+        assert 0 < oc.max() < bound
+    assert np.array_equal(ridx.cpu().numpy(), oi)
+    assert np.array_equal(t0.cpu().numpy(), o0)
+    assert np.array_equal(t1.cpu().numpy(), o1)
+    assert np.array_equal(packed[:, 1].cpu().numpy(), oc)
+
+
 def test_march_all_rays_miss(dev):
     """A batch whose rays all miss the box yields no samples and zero counts."""
     from angelo_amd import functions as fn
