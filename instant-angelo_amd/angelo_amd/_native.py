@@ -197,6 +197,30 @@ def workspace(nbytes, device):
     return torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device=device)
 
 
+_HELD = {}
+
+
+def held_workspace(kind, nbytes, device, headroom=1.25):
+    """A grow-only device buffer of at least nbytes for one kind of call,
+    kept per (kind, device, stream): the steps of a training loop, whose
+    workspace sizes follow their sample counts, then allocate nothing once
+    the largest has been seen (a fresh allocation per call costs a
+    hipMalloc whenever the size outgrows the allocator's cached blocks).
+    Calls of one kind on one stream run in stream order, so reusing the
+    buffer between them is safe.  A graph-captured call must not use one
+    (a later growth would free the address the graph holds)."""
+    if torch.device(device).type != "cuda":
+        raise NativeError("angelo_amd kernels take device tensors (a workspace was asked for "
+                          f"on {device}); the HIP path has no CPU fallback")
+    key = (kind, device.index, torch.cuda.current_stream(device).cuda_stream)
+    buf = _HELD.get(key)
+    n = max(int(nbytes), 1)
+    if buf is None or buf.numel() < n:
+        _HELD[key] = None
+        buf = _HELD[key] = workspace(int(n * headroom), device)
+    return buf
+
+
 AM_ERR_WORKSPACE = -3
 
 

@@ -52,7 +52,7 @@ class HashGridFn(torch.autograd.Function):
         dtable = torch.empty_like(table) if ctx.needs_input_grad[1] else None
         ws = None
         if dtable is not None:
-            ws = nat.workspace(lib.am_hashgrid_bwd_workspace_size(nat.sptr(grid), n, ctx.active),
+            ws = nat.held_workspace("hashgrid_bwd", lib.am_hashgrid_bwd_workspace_size(nat.sptr(grid), n, ctx.active),
                                x.device)
         rest = (nat.ptr(x), n, nat.ptr(table), nat.ptr(g), ctx.active, nat.ptr(dtable),
                 nat.ptr(dx), nat.ptr(ws), ws.numel() if ws is not None else 0, nat.stream())
@@ -90,7 +90,7 @@ class MLPFn(torch.autograd.Function):
         n = x.shape[0]
         dx = torch.empty_like(x) if ctx.needs_input_grad[0] else None
         dparams = torch.empty_like(params)
-        ws = nat.workspace(lib.am_mlp_bwd_workspace_size(nat.sptr(ctx.desc), n), x.device)
+        ws = nat.held_workspace("mlp_bwd", lib.am_mlp_bwd_workspace_size(nat.sptr(ctx.desc), n), x.device)
         nat.check(lib.am_mlp_bwd(nat.sptr(ctx.desc), nat.ptr(x), n, nat.ptr(params), nat.ptr(g),
                                  nat.ptr(dx), nat.ptr(dparams), nat.ptr(ws), ws.numel(),
                                  nat.stream()), "am_mlp_bwd")
@@ -175,7 +175,7 @@ def spatial_order(pts, radius, bits=None):
     bits = int(SPATIAL_ORDER["bits"] if bits is None else bits)
     perm = torch.empty(n, device=pts.device, dtype=torch.int32)
     inv = torch.empty_like(perm)
-    ws = nat.workspace(lib.am_morton_order_workspace_size(n, bits), pts.device)
+    ws = nat.held_workspace("morton_order", lib.am_morton_order_workspace_size(n, bits), pts.device)
     nat.check(lib.am_morton_order(nat.ptr(pts), n, float(radius), bits, nat.ptr(perm),
                                   nat.ptr(inv), nat.ptr(ws), ws.numel(), nat.stream()),
               "am_morton_order")
@@ -300,7 +300,7 @@ class RadianceFn(torch.autograd.Function):
         gf = torch.zeros_like(feature) if ctx.needs_input_grad[0] else None
         gn = torch.zeros_like(normal) if ctx.needs_input_grad[1] else None
         dparams = torch.empty_like(params)
-        ws = nat.workspace(lib.am_radiance_bwd_workspace_size(nat.sptr(ctx.mdesc), n),
+        ws = nat.held_workspace("radiance_bwd", lib.am_radiance_bwd_workspace_size(nat.sptr(ctx.mdesc), n),
                            feature.device)
         nat.check(lib.am_radiance_bwd(nat.sptr(ctx.mdesc), ctx.deg, ctx.base, nat.ptr(feature),
                                       feature.shape[1], nat.ptr(rays), nat.ptr(ridx),
@@ -357,7 +357,7 @@ class CompositeFn(torch.autograd.Function):
         gn = torch.empty_like(normal)
         gr = torch.empty_like(rgb)
         gi = torch.empty(1, device=sdf.device, dtype=torch.float32)
-        ws = nat.workspace(lib.am_composite_bwd_workspace_size(R, n), sdf.device)
+        ws = nat.held_workspace("composite_bwd", lib.am_composite_bwd_workspace_size(R, n), sdf.device)
         nat.check(lib.am_composite_bwd(nat.sptr(ctx.cdesc), nat.ptr(sdf), nat.ptr(normal),
                                        nat.ptr(rays), nat.ptr(ridx), nat.ptr(t0), nat.ptr(t1),
                                        nat.ptr(rgb), nat.ptr(inv_s), nat.ptr(packed), R, n,
@@ -423,7 +423,7 @@ def ray_marching(rays, t_near, t_far, roi, grid_binary, step, jitter=None, cap=0
     stage_cap = int(cap) if cap > 0 else (_implicit_stage_cap(roi, step) if R > 0 else 0)
     if stage_cap > 0 and MARCH_STAGED["on"]:
         # one march per ray: the count pass stages the samples, the fill copies
-        ws = nat.workspace(lib.am_march_staged_workspace_size(R, stage_cap), dev)
+        ws = nat.held_workspace("march_staged", lib.am_march_staged_workspace_size(R, stage_cap), dev)
         nat.check(lib.am_march_count_staged(nat.ptr(rays), R, nat.ptr(t_near), nat.ptr(t_far),
                                             nat.ptr(jit), roi_p, gptr, res, float(step),
                                             stage_cap, nat.ptr(packed), nat.ptr(total),
@@ -446,7 +446,7 @@ def ray_marching(rays, t_near, t_far, roi, grid_binary, step, jitter=None, cap=0
                                                    nat.ptr(t1), nat.stream()),
                           "am_march_fill_staged")
             return ridx, t0, t1, packed
-    ws = nat.workspace(lib.am_march_workspace_size(R), dev)
+    ws = nat.held_workspace("march", lib.am_march_workspace_size(R), dev)
     nat.check(lib.am_march_count(nat.ptr(rays), R, nat.ptr(t_near), nat.ptr(t_far), nat.ptr(jit),
                                  roi_p, gptr, res, float(step), int(cap), nat.ptr(packed),
                                  nat.ptr(total), nat.ptr(ws), ws.numel(), nat.stream()),
@@ -485,7 +485,7 @@ def ray_marching_bg(rays, t_near, t_far, roi, grid_binary, step, cone_angle, cap
     t_near, t_far = _f32c(t_near), _f32c(t_far)
     packed = torch.empty(R, 2, device=dev, dtype=torch.int32)
     total = torch.empty(1, device=dev, dtype=torch.int32)
-    ws = nat.workspace(lib.am_march_workspace_size(R), dev)
+    ws = nat.held_workspace("march_bg", lib.am_march_workspace_size(R), dev)
     nat.check(lib.am_march_bg_count(nat.ptr(rays), R, nat.ptr(t_near), nat.ptr(t_far), roi_p, gptr,
                                     res, float(step), float(cone_angle), int(cap), nat.ptr(packed),
                                     nat.ptr(total), nat.ptr(ws), ws.numel(), nat.stream()),

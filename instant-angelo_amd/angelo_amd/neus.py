@@ -188,7 +188,8 @@ class NeuSModel(nn.Module):
         sd = geo._sdf_desc(False, False)
         lib = nat.load()
         inv_s = self.inv_s().reshape(1).float().contiguous()
-        ws = nat.workspace(lib.am_occ_update_workspace_size(og.resolution, m), og.occs.device)
+        ws = nat.held_workspace("occ_update", lib.am_occ_update_workspace_size(og.resolution, m),
+                                 og.occs.device)
         params = geo.network.flat_params().detach().contiguous()
         nat.check(lib.am_occ_update(nat.sptr(g.grid), nat.sptr(geo.network.desc), nat.sptr(sd),
                                     nat.ptr(g.kernel_table()), nat.ptr(params), nat.ptr(inv_s),
