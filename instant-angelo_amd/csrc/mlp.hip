@@ -14,6 +14,12 @@ struct MlpArgs {
     int64_t n_params;
 };
 
+// Output tiles of 16 rows the kernels run for n_out outputs (AM_MLP_DISPATCH
+// picks the same): 1 up to 16 outputs, else 4.  The output layer's LDS rows
+// are padded (with zeros) to all of them, since the backward's W^T * G reads
+// every row of every tile.
+static inline int mlp_out_tiles(int n_out) { return n_out > 16 ? 4 : 1; }
+
 static int make_mlp_args(const am_mlp_desc* m, MlpArgs* a) {
     AM_CHECK_ARG(m != nullptr, "mlp descriptor is NULL");
     AM_CHECK_ARG(m->n_neurons == 64, "n_neurons %d: only 64 is implemented", m->n_neurons);
@@ -45,7 +51,7 @@ static int make_mlp_args(const am_mlp_desc* m, MlpArgs* a) {
         const int in_rows = 16 * (int)ceil_div(in, 16);
         r.cols_in[L] = in;
         r.rows_out[L] = out;
-        r.n_rows_lds[L] = 16 * (int)ceil_div(out, 16);
+        r.n_rows_lds[L] = L == r.n_layers - 1 ? 16 * mlp_out_tiles(out) : 64;
         r.ld[L] = in_rows + 1;
         r.w_lds[L] = lds;
         lds += r.n_rows_lds[L] * r.ld[L];
@@ -351,7 +357,7 @@ static int launch_bwd(const MlpArgs& a, const float* x, int64_t n, const float* 
 #define AM_MLP_DISPATCH(FN, ...)                                                          \
     do {                                                                                  \
         const int ks = 4 * (int)ceil_div(a.n_in, 16);                                     \
-        const bool big = a.n_out > 16;                                                    \
+        const bool big = mlp_out_tiles(a.n_out) == 4;                                     \
         switch (ks) {                                                                     \
             case 4:                                                                       \
                 if (big) AM_MLP_DISPATCH_NH(FN, 4, 4, __VA_ARGS__);                       \
