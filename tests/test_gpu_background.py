@@ -395,3 +395,38 @@ def test_neus_step_with_background_matches_oracle(dev):
     helpers.assert_nontrivial(ol["table"].grad, "oracle fg table gradient")
     assert helpers.rel_err(model.geometry.grid_module.params.grad.view(-1, 2),
                            ol["table"].grad) < 2e-3
+
+
+def test_bg_occupancy_update_duplicate_cells_take_the_max(dev):
+    """The background grid's update with a cell drawn three times (different
+    jitter): that cell becomes max(old * decay, max of its three estimates),
+    the same on every run; a cell drawn once follows the plain EMA rule."""
+    import angelo_amd
+    from angelo_amd.neus import contract_inv_unbounded
+    cfg = _bg_model_config(grid_prune=True)
+    model = helpers.randomize_(angelo_amd.make("neus", cfg), seed=5).to(dev)
+    og = model.occupancy_grid_bg
+    R = og.resolution
+    c0 = (R // 2) * R * R + (R // 2) * R + R // 4        # inside the contracted ball
+    c1 = (R // 2) * R * R + (R // 3) * R + R // 2
+    cells = torch.tensor([c0, c1, c0, c0], device=dev, dtype=torch.int32)
+    jitter = torch.rand(4, 3, device=dev, generator=torch.Generator(device=dev).manual_seed(6))
+    decay = 0.95
+    with torch.no_grad():
+        idx = cells.long()
+        coords = torch.stack([idx // (R * R), (idx // R) % R, idx % R], -1).float()
+        pts = contract_inv_unbounded((coords + jitter) / R, og.roi_aabb)
+        est = model.geometry_bg(pts)[0] * model.render_step_size_bg
+    results = []
+    for _ in range(2):
+        og.occs.fill_(0.0)
+        og.occs[c0] = 1e-6
+        og.occs[c1] = 1e-6
+        model._occ_update_bg(cells, jitter, 4, 0.01, decay)
+        results.append(og.occs.clone())
+    assert torch.equal(results[0], results[1])
+    e0 = torch.maximum(torch.tensor(1e-6, device=dev) * decay, est[[0, 2, 3]].max())
+    e1 = torch.maximum(torch.tensor(1e-6, device=dev) * decay, est[1])
+    assert torch.equal(results[0][c0], e0)
+    assert torch.equal(results[0][c1], e1)
+    assert int((results[0] != 0).sum()) == 2
