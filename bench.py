@@ -287,6 +287,8 @@ def time_steps(wl, steps, warmup):
     total_samples = 0
     bg_before = wl.bg_samples
     torch.cuda.synchronize()
+    keys = ("num_device_alloc", "num_device_free", "num_alloc_retries", "num_sync_all_streams")
+    mem0 = torch.cuda.memory_stats()
     t0 = time.perf_counter()
     start.record()
     step_wall = []
@@ -294,9 +296,16 @@ def time_steps(wl, steps, warmup):
         ts = time.perf_counter()
         total_samples += wl.step(warmup + i)
         step_wall.append(time.perf_counter() - ts)
+    wl.drain()    # a march started for the step after the window is timed too
     stop.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    mem1 = torch.cuda.memory_stats()
+    # device allocations and frees by torch's caching allocator inside the
+    # timed steps (0 in steady state), its out-of-memory retries, and the
+    # peak of reserved device memory
+    alloc = {k: int(mem1.get(k, 0) - mem0.get(k, 0)) for k in keys}
+    alloc["peak_reserved_gb"] = round(mem1.get("reserved_bytes.all.peak", 0) / 1e9, 2)
     nat.sdf_bwd_check()   # the pair passes of every timed backward agreed
     kern = {}
     for k in TIMED_KERNELS:
@@ -308,7 +317,7 @@ def time_steps(wl, steps, warmup):
     nat.timer_enable(False)
     return {"gpu_ms": start.elapsed_time(stop), "wall": wall, "step_wall": step_wall,
             "kern": kern, "samples": total_samples, "pairs_per_step": pairs,
-            "sdf_bwd_host_reads": host_reads,
+            "sdf_bwd_host_reads": host_reads, "allocator": alloc,
             "bg_samples": wl.bg_samples - bg_before}
 
 
@@ -363,7 +372,7 @@ def level_sweep(wl, levels=(1, 2, 4, 8, 12, 16), reps=3):
     return out
 
 
-def mixed_bench(dev, name, steps, warmup, fp32_value, rays=None):
+def mixed_bench(dev, name, steps, warmup, fp32_value, rays=None, pipeline=False):
     """The mixed-precision form of the same step (fp16 table rows, bf16 MFMA
     radiance, fp16 table-gradient pairs with dynamic loss scaling, Kahan
     compositing, fp64 loss reductions): samples/s, its ratio to the fp32
@@ -372,7 +381,7 @@ def mixed_bench(dev, name, steps, warmup, fp32_value, rays=None):
     if name not in WORKLOADS:
         return None
     log(f"[bench] building workload {name}")
-    wl = Workload(name, dev, rays=rays)
+    wl = Workload(name, dev, rays=rays, pipeline=pipeline)
     t = time_steps(wl, steps, warmup)
     value = t["samples"] / (t["gpu_ms"] / 1e3)
     out = {"workload": name, "value": round(value, 1), "unit": "samples/s",
@@ -410,6 +419,12 @@ def main():
     ap.add_argument("--sync-free", action="store_true",
                     help="SDF backward without its pair-count host read (pair arrays and "
                          "sort sized for the worst-case pair count)")
+    ap.add_argument("--no-pipeline", action="store_true",
+                    help="march each step's rays at the start of the step (by default the "
+                         "next step's rays are drawn and marched on a second stream while "
+                         "this step's backward runs, with the same step sequence; off with "
+                         "--sync-free, whose worst-case pair arrays already fill most of "
+                         "the device memory)")
     ap.add_argument("--serve-requests", type=int, default=200,
                     help="requests in the serving measurement (0: skip it)")
     ap.add_argument("--serve-load", type=float, default=0.6,
@@ -446,7 +461,8 @@ def main():
     if args.table_atomic:
         nat.set_flag("table_atomic", 1)
     log(f"[bench] building workload {args.workload}")
-    wl = Workload(args.workload, dev, rays=args.rays)
+    pipeline = not (args.no_pipeline or args.sync_free)
+    wl = Workload(args.workload, dev, rays=args.rays, pipeline=pipeline)
     spec = wl.spec
     if args.level_sweep:
         if wl.uniform:
@@ -461,6 +477,7 @@ def main():
     gpu_ms, wall, step_wall, kern = t["gpu_ms"], t["wall"], t["step_wall"], t["kern"]
     total_samples, pairs_per_step = t["samples"], t["pairs_per_step"]
     host_reads = t["sdf_bwd_host_reads"]
+    alloc = t["allocator"]
     ms_per_step = gpu_ms / args.steps
     samples_per_step = total_samples / args.steps
     bg_per_step = t["bg_samples"] / args.steps
@@ -469,6 +486,7 @@ def main():
     value = total_samples / (gpu_ms / 1e3)
     log(f"[bench] {ms_per_step:.3f} ms/step, {samples_per_step:.0f} samples/step, wall {wall:.2f}s")
     sw = sorted(step_wall)
+    log(f"[bench] caching allocator in the timed steps: {alloc}")
     log(f"[bench] host time per step: min {1e3 * sw[0]:.2f} ms, median "
         f"{1e3 * sw[len(sw) // 2]:.2f} ms, max {1e3 * sw[-1]:.2f} ms")
     for k, v in sorted(kern.items(), key=lambda kv: -kv[1]["total_ms"]):
@@ -512,7 +530,8 @@ def main():
     mixed_name = args.workload.replace("-fp32", "-mixed") if "-fp32" in args.workload else None
     if not args.no_mixed and not wl.mixed and mixed_name:
         del t
-        mixed = mixed_bench(dev, mixed_name, args.steps, args.warmup, value, rays=args.rays)
+        mixed = mixed_bench(dev, mixed_name, args.steps, args.warmup, value, rays=args.rays,
+                            pipeline=pipeline)
     serve = None
     if args.serve_requests > 0:
         log(f"[bench] serving: {args.serve_requests} requests at saturation, then at "
@@ -548,8 +567,10 @@ def main():
         "kernels_roofline": per_kernel,
         "table_pairs_per_step": round(pairs_per_step, 1),
         "sdf_bwd_sync_free": bool(args.sync_free),
+        "march_pipelined": bool(wl.pipeline),
         "table_gradient": "atomic" if args.table_atomic else "deterministic (sort + reduce)",
         "sdf_bwd_host_reads_per_step": host_reads,
+        "allocator_in_timed_steps": alloc,
         "roofline": roof,
         "mixed": mixed,
         "serving": serve,

@@ -84,9 +84,14 @@ class OccupancyGrid(nn.Module):
         self.register_buffer("binary", torch.ones(self.resolution, self.resolution, self.resolution,
                                                   dtype=torch.uint8))
 
+    @staticmethod
+    def due(step, n=16):
+        """Whether every_n_step updates the grid at this step."""
+        return step % n == 0
+
     @torch.no_grad()
     def every_n_step(self, step, update_fn, occ_thre=1e-2, ema_decay=0.95, warmup_steps=256, n=16):
-        if step % n == 0:
+        if self.due(step, n):
             self.update(step, update_fn, occ_thre, ema_decay, warmup_steps)
 
     @torch.no_grad()
@@ -176,6 +181,12 @@ class NeuSModel(nn.Module):
                     step=global_step, update_fn=self._occ_update_bg,
                     occ_thre=self.config.get("grid_prune_occ_thre_bg", 0.01))
 
+    def grid_update_due(self, global_step):
+        """Whether update_step(global_step) rewrites the occupancy grids
+        (which marching reads)."""
+        return (self.training and bool(self.config.get("grid_prune", False))
+                and self.occupancy_grid.due(global_step))
+
     def inv_s(self):
         return self.variance(torch.zeros([1, 3]))[:, :1].clip(1e-6, 1e6)
 
@@ -205,7 +216,11 @@ class NeuSModel(nn.Module):
         unbounded-sphere grid, with nerfacc 0.3.3's update rule: jittered
         cell points inside the contracted ball, mapped back to the world by
         `contract_inv_unbounded`; occs = max(occs * decay, occ) at the
-        sampled cells; binary = occs > min(mean(occs), occ_thre)."""
+        sampled cells; binary = occs > min(mean(occs), occ_thre).  A cell
+        drawn more than once takes the max of its estimates (nerfacc's
+        indexed assignment keeps whichever write lands last; the scatter-max
+        is one of those outcomes, the same on every run, and the rule of the
+        foreground grid's update)."""
         og = self.occupancy_grid_bg
         R = og.resolution
         dev = og.occs.device
@@ -219,7 +234,8 @@ class NeuSModel(nn.Module):
         for a in range(0, idx.shape[0], chunk):
             density, _ = self.geometry_bg(pts[a:a + chunk])
             occ[a:a + chunk] = density * self.render_step_size_bg
-        og.occs[idx] = torch.maximum(og.occs[idx] * ema_decay, occ)
+        cand = torch.maximum(og.occs[idx] * ema_decay, occ)
+        og.occs.scatter_reduce_(0, idx, cand, reduce="amax", include_self=False)
         thre = torch.clamp(og.occs.mean(), max=occ_thre)
         og.binary.copy_((og.occs > thre).view(R, R, R).to(torch.uint8))
 

@@ -207,9 +207,12 @@ class Workload:
     """Model, trainer, optimizer and the step's data: a pool of
     device-resident batches with uniform samples (sphere workload), or rays
     drawn on the device each step from a resident image set (Neuralangelo
-    workloads)."""
+    workloads).  pipeline=True draws and marches the next step's rays on a
+    second stream while the current step's backward runs (marched workloads
+    only; bit-identical to the sequential loop)."""
 
-    def __init__(self, name, device, pool=8, seed=0, rays=None, global_step=20000, band_cells=1.5):
+    def __init__(self, name, device, pool=8, seed=0, rays=None, global_step=20000, band_cells=1.5,
+                 pipeline=False):
         spec = dict(WORKLOADS[name])
         if rays is not None:
             spec["rays"] = rays
@@ -264,26 +267,76 @@ class Workload:
             og.binary.copy_(binary)
             og.occs.copy_(binary.flatten().float())
         self.model.update_step(0, global_step - 1)
+        # pipeline: the next step's rays are drawn and marched on a side
+        # stream while this step's backward runs (marching reads the
+        # occupancy grid only), so its sample-count read does not drain the
+        # main stream.  Not before a step that updates the grid.  The global
+        # generator's draws (march jitter, grid updates) keep their order, so
+        # the step sequence is the sequential loop's.
+        self.pipeline = bool(pipeline) and not self.uniform
+        self._side = torch.cuda.Stream(device) if self.pipeline else None
+        self._ahead = None      # (step, batch, event) marched on the side stream
+        self._grid_ev = None    # the main stream after this step's grid update
+
+    def _draw(self, set_model=True):
+        b = dict(self.sampler.sample(self.spec["rays"], self.sample_gen,
+                                     self.model if set_model else None), samples=None)
+        b["bg"] = self.sampler.last_draw[3]
+        return b
+
+    def _prefetch(self, i):
+        main = torch.cuda.current_stream(self.device)
+        side = self._side
+        with torch.cuda.stream(side):
+            if self._grid_ev is not None:
+                side.wait_event(self._grid_ev)
+            b = self._draw(set_model=False)
+            b["samples"] = self.model.march(b["rays"])
+            ev = torch.cuda.Event()
+            ev.record(side)
+        for v in b.values():
+            for t in (v if isinstance(v, (tuple, list)) else (v,)):
+                if isinstance(t, torch.Tensor) and t.is_cuda:
+                    t.record_stream(main)   # freed memory waits for the main stream's use
+        self._ahead = (i, b, ev)
+
+    def drain(self):
+        """Make the current stream wait for a march already started for the
+        next step (so a timed region that ends here includes it)."""
+        if self._ahead is not None:
+            torch.cuda.current_stream(self.device).wait_event(self._ahead[2])
 
     def step(self, i):
         """One training step; returns the number of samples processed."""
         if self.uniform:
             b = self.batches[i % len(self.batches)]
+        elif self._ahead is not None and self._ahead[0] == i:
+            _, b, ev = self._ahead
+            self._ahead = None
+            torch.cuda.current_stream(self.device).wait_event(ev)
+            self.model.background_color = b["bg"]
         else:
-            b = dict(self.sampler.sample(self.spec["rays"], self.sample_gen, self.model),
-                     samples=None)
+            self._ahead = None
+            b = self._draw()
         gs = self.global_step + i
         self.trainer.global_step = gs
         self.model.update_step(0, gs)
+        if self.pipeline:
+            self._grid_ev = torch.cuda.Event()
+            self._grid_ev.record()
         out = self.model(b["rays"], samples=b["samples"])
         loss, _ = self.trainer.losses(out, b)
         self.optimizer.zero_grad(set_to_none=True)
         if self.scaler is not None:
             self.scaler.scale_loss(loss).backward()
+        else:
+            loss.backward()
+        if self.pipeline and not self.model.grid_update_due(gs + 1):
+            self._prefetch(i + 1)
+        if self.scaler is not None:
             if not self.scaler.step(self.optimizer, list(self.model.parameters())):
                 self.skipped_steps += 1
         else:
-            loss.backward()
             self.optimizer.step()
         n = out["ray_indices"].shape[0]
         if "ray_indices_bg" in out:
