@@ -1,6 +1,7 @@
 // Shared device/host helpers for the MI355X (gfx950) NeuS hot-path library.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <mutex>
 #include <stdint.h>
 #include <stddef.h>
 #include "../../include/angelo_amd.h"
@@ -130,6 +131,28 @@ static inline int persistent_blocks(int64_t tiles, int per_block, int max_blocks
     if (b > max_blocks) b = max_blocks;
     if (b < 1) b = 1;
     return (int)b;
+}
+
+// Blocks of `block` threads with `lds` bytes of dynamic LDS that the device
+// holds at once (occupancy per CU x CUs), for kernels whose register count
+// leaves wave slots a fixed-size grid would not fill in one round.  Cached
+// per (kernel, LDS size); at least 1.
+static inline int resident_blocks(const void* kern, int block, size_t lds) {
+    struct Entry { const void* k; size_t lds; int blocks; };
+    static Entry cache[16];
+    static int n_cached = 0;
+    static std::mutex mu;
+    std::lock_guard<std::mutex> lock(mu);
+    for (int i = 0; i < n_cached; ++i)
+        if (cache[i].k == kern && cache[i].lds == lds) return cache[i].blocks;
+    int per_cu = 0, dev = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, block, lds) != hipSuccess) per_cu = 1;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        cus = 1;
+    const int blocks = (per_cu < 1 ? 1 : per_cu) * (cus < 1 ? 1 : cus);
+    if (n_cached < 16) cache[n_cached++] = Entry{kern, lds, blocks};
+    return blocks;
 }
 
 // ---- wave helpers (wave64) ----

@@ -10,6 +10,17 @@ static inline int field_blocks(int64_t n) {
     return persistent_blocks((n + 15) / 16, 4, kFieldBlocksMax);
 }
 
+// Grids of the SDF forward and of the pair passes (count, emit); their
+// tiles' results do not depend on the grid.
+static constexpr int kFwdBlocksMax = 1024;
+static constexpr int kPairBlocksMax = 1536;
+static constexpr int kCountBlocksMax = 2048;
+static inline int fwd_blocks(int64_t n) { return persistent_blocks((n + 15) / 16, 4, kFwdBlocksMax); }
+static inline int pair_blocks(int64_t n) { return persistent_blocks((n + 15) / 16, 4, kPairBlocksMax); }
+static inline int count_blocks(int64_t n) {
+    return persistent_blocks((n + 15) / 16, 4, kCountBlocksMax);
+}
+
 // Widest output of the fused SDF MLP (five 16-row output tiles: the SDF and
 // up to 79 features, e.g. the 65 outputs of the Neuralangelo configurations).
 constexpr int kGeoMaxOut = 80;

@@ -1751,7 +1751,7 @@ static int geo_fwd_launch_t(const GeoArgs& a, const float* pts, const float* ran
                                (int)lds));
     {
         KernelTimer kt_("geo_fwd", s);
-        hipLaunchKernelGGL(kern, dim3(field_blocks(n)), dim3(256), lds, s, a, pts, randn, n, table,
+        hipLaunchKernelGGL(kern, dim3(fwd_blocks(n)), dim3(256), lds, s, a, pts, randn, n, table,
                            params, sdf, grad, normal, feature, laplace, fdv, store);
     }
     AM_LAUNCH_CHECK();
@@ -1785,11 +1785,11 @@ static int geo_count_launch(const GeoArgs& a, const float* pts, const float* ran
                             const float* fdv, int64_t* counts, bool part, hipStream_t s) {
     KernelTimer kt_("pair_count", s);
     if (part)
-        hipLaunchKernelGGL((k_geo_pairs<LJ, true, Vals<2>, true>), dim3(field_blocks(n)), dim3(256),
+        hipLaunchKernelGGL((k_geo_pairs<LJ, true, Vals<2>, true>), dim3(count_blocks(n)), dim3(256),
                            0, s, a, pts, randn, n, fdv, nullptr, nullptr, nullptr, nullptr, counts,
                            nullptr, (int64_t)0);
     else
-        hipLaunchKernelGGL((k_geo_pairs<LJ, true, Vals<2>, false>), dim3(field_blocks(n)),
+        hipLaunchKernelGGL((k_geo_pairs<LJ, true, Vals<2>, false>), dim3(count_blocks(n)),
                            dim3(256), 0, s, a, pts, randn, n, fdv, nullptr, nullptr, nullptr,
                            nullptr, counts, nullptr, (int64_t)0);
     AM_LAUNCH_CHECK();
@@ -1974,7 +1974,7 @@ static int geo_bwd_launch_t(const GeoArgs& a, const am_grid_desc* g, const float
         if (int rc = launch_bwd()) return rc;
         {
             KernelTimer kt_("pair_atomic", s);
-            hipLaunchKernelGGL((k_geo_pairs<LJ, false, AtomicRow, false>), dim3(blocks),
+            hipLaunchKernelGGL((k_geo_pairs<LJ, false, AtomicRow, false>), dim3(pair_blocks(n)),
                                dim3(256), 0, s, a, pts, randn, n, fdv, w.genc, nullptr,
                                reinterpret_cast<AtomicRow*>(dtable), nullptr, nullptr, nullptr,
                                (int64_t)0);
@@ -2027,12 +2027,12 @@ static int geo_bwd_launch_t(const GeoArgs& a, const am_grid_desc* g, const float
     {
         KernelTimer kt_("pair_emit", s);
         if (part)
-            hipLaunchKernelGGL((k_geo_pairs<LJ, false, PairV<PR>, true>), dim3(blocks), dim3(256),
+            hipLaunchKernelGGL((k_geo_pairs<LJ, false, PairV<PR>, true>), dim3(pair_blocks(n)), dim3(256),
                                0, s, a, pts, randn, n, fdv, w.genc, tw.keys[0],
                                reinterpret_cast<PairV<PR>*>(tw.vals[0]), w.offsets, w.counts,
                                w.status, tw.m);
         else
-            hipLaunchKernelGGL((k_geo_pairs<LJ, false, PairV<PR>, false>), dim3(blocks),
+            hipLaunchKernelGGL((k_geo_pairs<LJ, false, PairV<PR>, false>), dim3(pair_blocks(n)),
                                dim3(256), 0, s, a, pts, randn, n, fdv, w.genc, tw.keys[0],
                                reinterpret_cast<PairV<PR>*>(tw.vals[0]), w.offsets, w.counts,
                                w.status, tw.m);

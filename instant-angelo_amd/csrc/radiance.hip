@@ -403,7 +403,10 @@ static int rad_fwd_launch(const RadArgs& a, const float* feature, const float* r
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     {
         KernelTimer kt_("rad_fwd", s);
-        hipLaunchKernelGGL((k_rad_fwd<KS, NH, PR>), dim3(field_blocks(n)), dim3(256), lds, s, a, feature,
+        // one round of resident blocks (its registers allow 3 waves a SIMD)
+        const int blocks = persistent_blocks(
+            (n + 15) / 16, 4, resident_blocks((const void*)k_rad_fwd<KS, NH, PR>, 256, lds));
+        hipLaunchKernelGGL((k_rad_fwd<KS, NH, PR>), dim3(blocks), dim3(256), lds, s, a, feature,
                            rays, ridx, normal, n, params, rgb);
     }
     AM_LAUNCH_CHECK();
