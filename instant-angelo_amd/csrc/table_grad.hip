@@ -272,6 +272,18 @@ __device__ __forceinline__ int popc_below(uint64_t mask) {
                                           __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
+// One round of a stable split of the wave's 64 keys on bit 6 + `bit`: lanes
+// with the bit clear keep their order at the front (destination = clear
+// lanes below), set ones follow (all clear lanes + set lanes below).
+// Branch-free: both destinations come from one count of set lanes below.
+__device__ __forceinline__ uint32_t split_round(uint32_t rk, int bit, int lane) {
+    const bool one = (rk >> (6 + bit)) & 1u;
+    const uint64_t ones = __ballot(one);
+    const int below = popc_below(ones);
+    const int dst = one ? (64 - (int)__popcll(ones)) + below : lane - below;
+    return (uint32_t)__builtin_amdgcn_ds_permute(dst * 4, (int)rk);
+}
+
 __device__ __forceinline__ float permute_f(int dst_byte, float v) {
     return __int_as_float(__builtin_amdgcn_ds_permute(dst_byte, __float_as_int(v)));
 }
@@ -369,15 +381,16 @@ __device__ __forceinline__ void reduce_range(const uint32_t* __restrict__ keys,
             load_pair(vals, c + 64 + lane, na, nb);
         }
         // stable split sort of (row << 6 | source lane): one permute per
-        // round; the values follow once at the end
+        // round; the values follow once at the end.  A partial chunk sorts
+        // one more bit (its idle lanes carry row `rows`).
         uint32_t rk = (r0 << 6) | (uint32_t)lane;
-        const int nbits = (e - c < 64) ? B + 1 : B;
-        for (int bit = 0; bit < nbits; ++bit) {
-            const bool one = (rk >> (6 + bit)) & 1u;
-            const uint64_t ones = __ballot(one);
-            const uint64_t zeros = ~ones;
-            const int dst = one ? __popcll(zeros) + popc_below(ones) : popc_below(zeros);
-            rk = (uint32_t)__builtin_amdgcn_ds_permute(dst * 4, (int)rk);
+        if (B == 9) {   // the bucketed sort's digit: unrolled
+#pragma unroll
+            for (int bit = 0; bit < 9; ++bit) rk = split_round(rk, bit, lane);
+            if (e - c < 64) rk = split_round(rk, 9, lane);
+        } else {
+            const int nbits = (e - c < 64) ? B + 1 : B;
+            for (int bit = 0; bit < nbits; ++bit) rk = split_round(rk, bit, lane);
         }
         const uint32_t r = rk >> 6;
         const int from = (int)(rk & 63u) * 4;
