@@ -300,6 +300,15 @@ int am_sample_points(const float* rays, const int32_t* ray_indices, const float*
                      const float* t_ends, int64_t n, float* pts, float* mid, float* dists,
                      void* stream);
 
+/* Unbounded-sphere contraction of VolumeDensity: contract_to_unisphere
+ * (models/geometry.py:19-31) with ContractionType.UN_BOUNDED_SPHERE, as
+ * VolumeDensity.forward calls it (:160): x [n,3] world points -> out [n,3]
+ * in [0,1]^3 (cube [-r,r]^3 -> [-1,1]^3; norm m > 1 -> (2 - 1/m) x/m; then
+ * x/4 + 1/2).  _bwd: g_x = J^T g_out at the same points. */
+int am_contract_unbounded(const float* x, int64_t n, float radius, float* out, void* stream);
+int am_contract_unbounded_bwd(const float* x, const float* g_out, int64_t n, float radius,
+                              float* g_x, void* stream);
+
 /* ------------------------------------------------------------------------ */
 /* Fused VolumeSDF.forward (models/geometry.py:195-287): contraction,         */
 /* encode + SDF MLP of the base point, 6 central-difference points (grad)    */

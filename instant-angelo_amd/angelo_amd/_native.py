@@ -110,6 +110,8 @@ _SIGS = {
     "am_march_count": (ctypes.c_int, [_P, _I64, _P, _P, _P, _P, _P, _I32, _F, _I32, _P, _P, _P, _SZ, _P]),
     "am_march_fill": (ctypes.c_int, [_P, _I64, _P, _P, _P, _P, _P, _I32, _F, _I32, _P, _P, _P, _P, _P]),
     "am_sample_points": (ctypes.c_int, [_P, _P, _P, _P, _I64, _P, _P, _P, _P]),
+    "am_contract_unbounded": (ctypes.c_int, [_P, _I64, _F, _P, _P]),
+    "am_contract_unbounded_bwd": (ctypes.c_int, [_P, _P, _I64, _F, _P, _P]),
     "am_sdf_fwd": (ctypes.c_int, [_P] * 5 + [_I64] + [_P] * 9 + [_P]),
     "am_sdf_store_floats": (_I64, [_P, _P, _P, _I64]),
     "am_sdf_bwd_workspace_size": (_SZ, [_P, _P, _P, _I64]),

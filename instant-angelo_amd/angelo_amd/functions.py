@@ -567,6 +567,41 @@ def composite_density(density, rgb, t_starts, t_ends, packed):
     return DensityCompositeFn.apply(density, rgb, t_starts, t_ends, packed)
 
 
+class ContractUnboundedFn(torch.autograd.Function):
+    """contract_to_unisphere(x, r, UN_BOUNDED_SPHERE) (models/geometry.py:
+    19-31) as VolumeDensity.forward applies it: one HIP launch forward, one
+    backward (J^T g at the saved points)."""
+
+    @staticmethod
+    def forward(ctx, x, radius):
+        nat.require_cuda(x)
+        shape = x.shape
+        xf = _f32c(x).reshape(-1, 3)
+        out = torch.empty_like(xf)
+        nat.check(nat.load().am_contract_unbounded(nat.ptr(xf), xf.shape[0], float(radius),
+                                                   nat.ptr(out), nat.stream()),
+                  "am_contract_unbounded")
+        ctx.save_for_backward(xf)
+        ctx.radius, ctx.shape, ctx.dtype = float(radius), shape, x.dtype
+        return out.view(shape)
+
+    @staticmethod
+    def backward(ctx, g):
+        (xf,) = ctx.saved_tensors
+        gf = _f32c(g).reshape(-1, 3)
+        gx = torch.empty_like(xf)
+        nat.check(nat.load().am_contract_unbounded_bwd(nat.ptr(xf), nat.ptr(gf), xf.shape[0],
+                                                       ctx.radius, nat.ptr(gx), nat.stream()),
+                  "am_contract_unbounded_bwd")
+        return gx.view(ctx.shape).to(ctx.dtype), None
+
+
+def contract_unbounded(x, radius):
+    if x.shape[-1] != 3:
+        raise ValueError(f"points must be [..., 3] (got {tuple(x.shape)})")
+    return ContractUnboundedFn.apply(x, radius)
+
+
 def sample_points(rays, ray_indices, t_starts, t_ends):
     """models/neus.py:222-227: positions, midpoints, dists."""
     n = ray_indices.shape[0]

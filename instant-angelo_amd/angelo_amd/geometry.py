@@ -255,8 +255,15 @@ class VolumeDensity(BaseImplicitGeometry):
                 density + float(self.config.density_bias))
         return density
 
+    def _contract(self, points):
+        """contract_to_unisphere (models/geometry.py:19-31); the unbounded
+        sphere of the background runs as one HIP launch."""
+        if self.contraction_type == "un_bounded_sphere":
+            return fn.contract_unbounded(points, self.radius)
+        return contract_to_unisphere(points, self.radius, self.contraction_type)
+
     def forward(self, points):
-        points = contract_to_unisphere(points, self.radius, self.contraction_type)
+        points = self._contract(points)
         out = self.encoding_with_network(points.view(-1, self.n_input_dims)).view(
             *points.shape[:-1], self.n_output_dims).float()
         density, feature = self._density(out), out
@@ -265,7 +272,7 @@ class VolumeDensity(BaseImplicitGeometry):
         return density, feature
 
     def forward_level(self, points):
-        points = contract_to_unisphere(points, self.radius, self.contraction_type)
+        points = self._contract(points)
         out = self.encoding_with_network(points.reshape(-1, self.n_input_dims)).reshape(
             *points.shape[:-1], self.n_output_dims)
         return -self._density(out)

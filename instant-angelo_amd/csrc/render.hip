@@ -430,9 +430,88 @@ __global__ void __launch_bounds__(256) k_composite_density_bwd(
     }
 }
 
+// Unbounded-sphere contraction of VolumeDensity (contract_to_unisphere,
+// models/geometry.py:19-31, ContractionType.UN_BOUNDED_SPHERE), in the
+// order of the reference's fp32 operations: u = (x + r) * (1 / 2r)
+// (scale_anything; torch divides by a scalar as a multiply by its fp32
+// reciprocal), y = 2u - 1, m = |y|; for m > 1, y = (2 - 1/m) (y / m); then
+// y / 4 + 1/2.
+__device__ __forceinline__ void unbounded_y(const float* __restrict__ x, int64_t i, float inv2r,
+                                            float r, float y[3], float& m) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) y[d] = ((x[3 * i + d] + r) * inv2r) * 2.0f - 1.0f;
+    m = sqrtf((y[0] * y[0] + y[1] * y[1]) + y[2] * y[2]);
+}
+
+__global__ void k_contract_unbounded(const float* __restrict__ x, int64_t n, float r, float inv2r,
+                                     float* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float y[3], m;
+    unbounded_y(x, i, inv2r, r, y, m);
+    if (m > 1.0f) {
+        const float k = 2.0f - 1.0f / m;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) y[d] = k * (y[d] / m);
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) out[3 * i + d] = y[d] * 0.25f + 0.5f;
+}
+
+// Its backward: with f = phi(m) y outside the unit ball, phi = (2 - 1/m) / m,
+// J^T g = phi g + (phi'(m) / m) (y . g) y and phi'(m) / m = 2 (1 - m) / m^4;
+// the identity inside; then the chain factors 1/4 (output), 2 and 1/(2r).
+__global__ void k_contract_unbounded_bwd(const float* __restrict__ x, const float* __restrict__ g,
+                                         int64_t n, float r, float inv2r,
+                                         float* __restrict__ gx) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float y[3], m;
+    unbounded_y(x, i, inv2r, r, y, m);
+    float gy[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) gy[d] = g[3 * i + d] * 0.25f;
+    if (m > 1.0f) {
+        const float phi = (2.0f - 1.0f / m) / m;
+        const float m2 = m * m;
+        const float c = 2.0f * (1.0f - m) / (m2 * m2);
+        const float yg = (y[0] * gy[0] + y[1] * gy[1]) + y[2] * gy[2];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) gy[d] = phi * gy[d] + (c * yg) * y[d];
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) gx[3 * i + d] = (gy[d] * 2.0f) * inv2r;
+}
+
 }  // namespace am
 
 using namespace am;
+
+static float contract_inv2r(float radius) { return 1.0f / (float)(2.0 * (double)radius); }
+
+extern "C" int am_contract_unbounded(const float* x, int64_t n, float radius, float* out,
+                                     void* stream) {
+    AM_CHECK_ARG(n >= 0, "n < 0");
+    AM_CHECK_ARG(radius > 0.0f, "radius %g <= 0", (double)radius);
+    if (n == 0) return AM_OK;
+    AM_CHECK_ARG(x && out, "NULL buffer");
+    hipLaunchKernelGGL(k_contract_unbounded, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0,
+                       to_stream(stream), x, n, radius, contract_inv2r(radius), out);
+    AM_LAUNCH_CHECK();
+    return AM_OK;
+}
+
+extern "C" int am_contract_unbounded_bwd(const float* x, const float* g_out, int64_t n,
+                                         float radius, float* g_x, void* stream) {
+    AM_CHECK_ARG(n >= 0, "n < 0");
+    AM_CHECK_ARG(radius > 0.0f, "radius %g <= 0", (double)radius);
+    if (n == 0) return AM_OK;
+    AM_CHECK_ARG(x && g_out && g_x, "NULL buffer");
+    hipLaunchKernelGGL(k_contract_unbounded_bwd, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0,
+                       to_stream(stream), x, g_out, n, radius, contract_inv2r(radius), g_x);
+    AM_LAUNCH_CHECK();
+    return AM_OK;
+}
 
 extern "C" int am_sample_points(const float* rays, const int32_t* ray_indices,
                                 const float* t_starts, const float* t_ends, int64_t n,

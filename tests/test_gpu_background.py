@@ -430,3 +430,58 @@ def test_bg_occupancy_update_duplicate_cells_take_the_max(dev):
     assert torch.equal(results[0][c0], e0)
     assert torch.equal(results[0][c1], e1)
     assert int((results[0] != 0).sum()) == 2
+
+
+def _contract_points(n, seed):
+    g = torch.Generator().manual_seed(seed)
+    p = torch.randn(n, 3, generator=g, dtype=torch.float64) * torch.logspace(
+        -2, 3, n, dtype=torch.float64)[:, None]
+    edge = torch.tensor([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.0, -1.0, 0.0], [0.6, 0.8, 0.0],
+                         [1e6, -1e6, 1e6]], dtype=torch.float64)
+    return torch.cat([p, edge]).float()
+
+
+def test_contract_unbounded_matches_oracle(dev):
+    """The HIP unbounded-sphere contraction (VolumeDensity's
+    contract_to_unisphere, models/geometry.py:19-31) against the oracle's
+    float64 restatement on the same fp32 points: the unit cube, the unit
+    ball's boundary, points up to 1e6 out; outputs in [0, 1]."""
+    from angelo_amd import functions as fn
+    from oracle import field as ofield
+    p = _contract_points(20000, 31)
+    for r in (1.0, 0.5, 1.5):
+        got = fn.contract_unbounded(p.to(dev), r).cpu().double()
+        ref = ofield.contract_unbounded(p.double(), r, mode="f64")
+        assert torch.allclose(got, ref, rtol=0, atol=1e-6), float((got - ref).abs().max())
+        assert float(got.min()) >= 0.0 and float(got.max()) <= 1.0
+    e = torch.zeros(0, 3, device=dev)
+    assert fn.contract_unbounded(e, 1.0).shape == (0, 3)
+
+
+def test_contract_unbounded_backward(dev):
+    """J^T g against float64 autograd through the oracle's restatement, and
+    the on-axis closed form: at y = (m, 0, 0), m > 1 (y = x / r), the
+    contraction's Jacobian is diag(1/m^2, (2 - 1/m)/m, (2 - 1/m)/m) / (4r)."""
+    from angelo_amd import functions as fn
+    from oracle import field as ofield
+    p = _contract_points(20000, 32)
+    g = torch.randn(p.shape, generator=torch.Generator().manual_seed(33))
+    for r in (1.0, 0.5):
+        x = p.to(dev).requires_grad_(True)
+        fn.contract_unbounded(x, r).backward(g.to(dev))
+        x64 = p.double().requires_grad_(True)
+        ofield.contract_unbounded(x64, r, mode="f64").backward(g.double())
+        scale = x64.grad.abs().clamp_min(1e-3)
+        rel = ((x.grad.cpu().double() - x64.grad).abs() / scale).max()
+        assert float(rel) < 1e-4, float(rel)
+    r = 0.5
+    for m in (1.5, 4.0, 100.0):
+        x = torch.tensor([[m * r, 0.0, 0.0]], device=dev)
+        jac = []
+        for d in range(3):
+            xi = x.clone().requires_grad_(True)
+            fn.contract_unbounded(xi, r)[0, d].backward()
+            jac.append(float(xi.grad[0, d]))
+        want = [1.0 / m ** 2 / (4 * r), (2 - 1 / m) / m / (4 * r), (2 - 1 / m) / m / (4 * r)]
+        for a, b in zip(jac, want):
+            assert abs(a - b) <= 1e-5 * abs(b), (m, jac, want)
