@@ -123,5 +123,8 @@ class TrainRaySampler:
         return batch
 
     def out_of_range(self):
-        """Draws clamped so far (synchronises; 0 unless an index list is bad)."""
+        """Draws clamped so far (0 unless an index list is bad).  Waits for
+        the whole device, so draws made on another stream (a pipelined
+        training loop) are counted too."""
+        torch.cuda.synchronize(self.device)
         return int(self._bad.item())
