@@ -21,6 +21,12 @@ static constexpr int kPairBlocksMax = 1536;
 static constexpr int kCountBlocksMax = 2048;
 static inline int fwd_blocks(int64_t n) { return persistent_blocks((n + 15) / 16, 4, kFwdBlocksMax); }
 static inline int pair_blocks(int64_t n) { return persistent_blocks((n + 15) / 16, 4, kPairBlocksMax); }
+// The backward passes of the SDF and radiance MLPs (over 200 VGPRs: two
+// waves a SIMD, 512 resident blocks) in one round; the grid fixes the
+// per-wave parameter-gradient rows and so their summation order, the same
+// for every form of a kernel.
+static constexpr int kBwdBlocksMax = 512;
+static inline int bwd_blocks(int64_t n) { return persistent_blocks((n + 15) / 16, 4, kBwdBlocksMax); }
 static inline int count_blocks(int64_t n) {
     return persistent_blocks((n + 15) / 16, 4, kCountBlocksMax);
 }

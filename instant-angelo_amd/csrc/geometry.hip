@@ -1823,7 +1823,7 @@ struct GeoBwdWs {
 
 static void carve_geo_bwd_head(WsCarver& c, const GeoArgs& a, int LJ, int64_t n, bool need_fdv,
                                bool part, GeoBwdWs* w) {
-    const int waves = field_blocks(n) * 4;
+    const int waves = bwd_blocks(n) * 4;
     const int64_t tiles = (n + 15) / 16;
     const int64_t entries = tiles * (part ? kPartDigits : 1);   // count / offset entries
     w->fdv = need_fdv ? c.take<float>((size_t)n * 12 + 1) : nullptr;
@@ -1925,7 +1925,7 @@ static int geo_bwd_launch_t(const GeoArgs& a, const am_grid_desc* g, const float
                           const float* store, const float* g_sdf, const float* g_grad,
                           const float* g_normal, const float* g_feature, const float* g_laplace,
                           float* dtable, float* dparams, void* ws, size_t ws_bytes, hipStream_t s) {
-    const int blocks = field_blocks(n);
+    const int blocks = bwd_blocks(n);
     const int waves = blocks * 4;
     const int64_t tiles = (n + 15) / 16;
     const bool need_fdv = a.with_grad && fdv_in == nullptr;

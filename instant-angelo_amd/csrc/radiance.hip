@@ -419,7 +419,7 @@ static int rad_bwd_launch(const RadArgs& a, const float* feature, const float* r
                           const float* params, const float* g_rgb, float* g_feature,
                           float* g_normal, float* dparams, void* ws, size_t ws_bytes,
                           hipStream_t s) {
-    const int blocks = field_blocks(n);
+    const int blocks = bwd_blocks(n);
     const int waves = blocks * 4;
     WsCarver c(ws, ws_bytes);
     float* partial = c.take<float>((size_t)waves * a.n_params);
@@ -477,7 +477,7 @@ extern "C" int am_radiance_fwd(const am_mlp_desc* m, int32_t sh_degree, int32_t 
 
 extern "C" size_t am_radiance_bwd_workspace_size(const am_mlp_desc* m, int64_t n) {
     if (!m || n < 0) return 0;
-    const int waves = field_blocks(n) * 4;
+    const int waves = bwd_blocks(n) * 4;
     int64_t pw1, pb1, pw2, pb2, pwo, pbo, np;
     layout_params(m->n_in, m->n_out, m->n_hidden_layers, &pw1, &pb1, &pw2, &pb2, &pwo, &pbo, &np);
     WsCarver c(nullptr, 0);
