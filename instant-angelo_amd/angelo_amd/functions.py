@@ -297,8 +297,10 @@ class RadianceFn(torch.autograd.Function):
         lib = nat.load()
         n = feature.shape[0]
         g_rgb = _f32c(g_rgb)
-        gf = torch.zeros_like(feature) if ctx.needs_input_grad[0] else None
-        gn = torch.zeros_like(normal) if ctx.needs_input_grad[1] else None
+        # the kernel writes every column of both (the input always ends with
+        # the three normal columns)
+        gf = torch.empty_like(feature) if ctx.needs_input_grad[0] else None
+        gn = torch.empty_like(normal) if ctx.needs_input_grad[1] else None
         dparams = torch.empty_like(params)
         ws = nat.held_workspace("radiance_bwd", lib.am_radiance_bwd_workspace_size(nat.sptr(ctx.mdesc), n),
                            feature.device)

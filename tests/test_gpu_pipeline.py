@@ -533,7 +533,14 @@ def test_radiance_fused_fwd_bwd(dev, rad_hidden, dual, feature_dim):
     fd = feature.to(dev).requires_grad_(True)
     nd = normal.to(dev).requires_grad_(True)
     rgb = tex.fused(fd, nd, rays.to(dev), ridx.to(dev))
-    (rgb * g_rgb.to(dev)).sum().backward()
+    loss = (rgb * g_rgb.to(dev)).sum()
+    # NaN-filled blocks of the input gradients' sizes, freed just before the
+    # backward, so its unfilled gradient buffers are likely to reuse them:
+    # every element must be written
+    poison = [torch.full_like(fd, float("nan")), torch.full_like(nd, float("nan"))]
+    del poison
+    loss.backward()
+    assert bool(torch.isfinite(fd.grad).all()) and bool(torch.isfinite(nd.grad).all())
     torch.testing.assert_close(rgb.detach().cpu().double(), ref.detach(), rtol=1e-5, atol=2e-6)
     assert helpers.rel_err(fd.grad, fl.grad) < 1e-4
     assert helpers.rel_err(nd.grad, nl.grad) < 1e-4
