@@ -473,52 +473,83 @@ def ray_marching_bg(rays, t_near, t_far, roi, grid_binary, step, cone_angle, cap
     the samples behind transmittance < early_stop_eps are dropped (each ray
     keeps a prefix: am_march_bg_visible, then the fill pass again with the
     kept counts).  Returns (ray_indices int32, t_starts, t_ends, packed)."""
+    cand = ray_marching_bg_candidates(rays, t_near, t_far, roi, grid_binary, step, cone_angle,
+                                      cap)
+    return ray_marching_bg_filter(cand, sigma_fn, early_stop_eps)
+
+
+def _march_bg_ctx(rays, t_near, t_far, roi, grid_binary, step, cone_angle, cap):
     import ctypes
-    lib = nat.load()
     nat.require_cuda(rays, t_near, t_far)
-    R = rays.shape[0]
-    dev = rays.device
     roi_h = (ctypes.c_float * 6)(*[float(v) for v in roi])
-    roi_p = ctypes.cast(roi_h, ctypes.c_void_p)
     res, gptr = 0, None
     if grid_binary is not None:
         res = int(grid_binary.shape[-1])
         gptr = nat.ptr(grid_binary)
-    t_near, t_far = _f32c(t_near), _f32c(t_far)
+    return dict(rays=rays, t_near=_f32c(t_near), t_far=_f32c(t_far), roi_h=roi_h,
+                roi_p=ctypes.cast(roi_h, ctypes.c_void_p), grid=grid_binary, gptr=gptr, res=res,
+                step=float(step), cone=float(cone_angle), cap=int(cap))
+
+
+def _march_bg_fill(c, pk, n):
+    lib = nat.load()
+    dev = c["rays"].device
+    ridx = torch.empty(n, device=dev, dtype=torch.int32)
+    t0 = torch.empty(n, device=dev, dtype=torch.float32)
+    t1 = torch.empty(n, device=dev, dtype=torch.float32)
+    if n > 0:
+        nat.check(lib.am_march_bg_fill(nat.ptr(c["rays"]), c["rays"].shape[0], nat.ptr(c["t_near"]),
+                                       nat.ptr(c["t_far"]), c["roi_p"], c["gptr"], c["res"],
+                                       c["step"], c["cone"], c["cap"], nat.ptr(pk), nat.ptr(ridx),
+                                       nat.ptr(t0), nat.ptr(t1), nat.stream()), "am_march_bg_fill")
+    return ridx, t0, t1
+
+
+def ray_marching_bg_candidates(rays, t_near, t_far, roi, grid_binary, step, cone_angle, cap=0):
+    """The background marching before the early-stop filter (count, one
+    host read of the sample count, fill): it reads the occupancy grid only,
+    not the weights, so a pipelined training loop runs it ahead.  Returns the
+    state ray_marching_bg_filter takes."""
+    lib = nat.load()
+    c = _march_bg_ctx(rays, t_near, t_far, roi, grid_binary, step, cone_angle, cap)
+    R = rays.shape[0]
+    dev = rays.device
     packed = torch.empty(R, 2, device=dev, dtype=torch.int32)
     total = torch.empty(1, device=dev, dtype=torch.int32)
     ws = nat.held_workspace("march_bg", lib.am_march_workspace_size(R), dev)
-    nat.check(lib.am_march_bg_count(nat.ptr(rays), R, nat.ptr(t_near), nat.ptr(t_far), roi_p, gptr,
-                                    res, float(step), float(cone_angle), int(cap), nat.ptr(packed),
-                                    nat.ptr(total), nat.ptr(ws), ws.numel(), nat.stream()),
-              "am_march_bg_count")
-
-    def fill(pk, n):
-        ridx = torch.empty(n, device=dev, dtype=torch.int32)
-        t0 = torch.empty(n, device=dev, dtype=torch.float32)
-        t1 = torch.empty(n, device=dev, dtype=torch.float32)
-        if n > 0:
-            nat.check(lib.am_march_bg_fill(nat.ptr(rays), R, nat.ptr(t_near), nat.ptr(t_far), roi_p,
-                                           gptr, res, float(step), float(cone_angle), int(cap),
-                                           nat.ptr(pk), nat.ptr(ridx), nat.ptr(t0), nat.ptr(t1),
-                                           nat.stream()), "am_march_bg_fill")
-        return ridx, t0, t1
-
+    nat.check(lib.am_march_bg_count(nat.ptr(rays), R, nat.ptr(c["t_near"]), nat.ptr(c["t_far"]),
+                                    c["roi_p"], c["gptr"], c["res"], c["step"], c["cone"],
+                                    c["cap"], nat.ptr(packed), nat.ptr(total), nat.ptr(ws),
+                                    ws.numel(), nat.stream()), "am_march_bg_count")
     N = int(total.item())
-    ridx, t0, t1 = fill(packed, N)
+    ridx, t0, t1 = _march_bg_fill(c, packed, N)
+    c.update(ridx=ridx, t0=t0, t1=t1, packed=packed, n=N)
+    return c
+
+
+def ray_marching_bg_filter(c, sigma_fn=None, early_stop_eps=1e-4):
+    """The early-stop filter of ray_marching_bg on candidates from
+    ray_marching_bg_candidates (on the stream they were made on, or after
+    it): each ray keeps the prefix before transmittance < early_stop_eps."""
+    ridx, t0, t1, packed, N = c["ridx"], c["t0"], c["t1"], c["packed"], c["n"]
     if sigma_fn is None or early_stop_eps <= 0.0 or N == 0:
         return ridx, t0, t1, packed
+    lib = nat.load()
+    R = packed.shape[0]
+    dev = packed.device
     sigma = _f32c(sigma_fn(t0, t1, ridx).reshape(-1))
     if sigma.shape[0] != N:
         raise ValueError(f"sigma_fn returned {sigma.shape[0]} values for {N} samples")
     kept = torch.empty_like(packed)
+    total = torch.empty(1, device=dev, dtype=torch.int32)
+    ws = nat.held_workspace("march_bg", lib.am_march_workspace_size(R), dev)
     nat.check(lib.am_march_bg_visible(nat.ptr(sigma), nat.ptr(t0), nat.ptr(t1), nat.ptr(packed), R,
                                       float(early_stop_eps), nat.ptr(kept), nat.ptr(total),
                                       nat.ptr(ws), ws.numel(), nat.stream()), "am_march_bg_visible")
     K = int(total.item())
     if K == N:
         return ridx, t0, t1, packed
-    ridx, t0, t1 = fill(kept, K)
+    ridx, t0, t1 = _march_bg_fill(c, kept, K)
     return ridx, t0, t1, kept
 
 

@@ -263,12 +263,12 @@ class NeuSModel(nn.Module):
         return fn.ray_marching(rays, t_near, t_far, roi, grid, self.render_step_size, jitter,
                                self.max_samples_per_ray)
 
-    def march_bg(self, rays):
-        """Background sampling of forward_bg_ (models/neus.py:141-171): near =
-        the far AABB hit (0.1 on a miss), far 1e3, stratified jitter of one
-        step, cone-angle marching over the unbounded grid, then the samples
-        behind transmittance 1e-4 dropped (sigma_fn = the background
-        density)."""
+    def march_bg_candidates(self, rays):
+        """The background marching of forward_bg_ up to the early-stop filter
+        (models/neus.py:141-168): near = the far AABB hit (0.1 on a miss),
+        far 1e3, stratified jitter of one step, cone-angle marching over the
+        unbounded grid.  Reads the occupancy grid, not the weights."""
+        rays = rays.float().contiguous()
         _, t_max = fn.ray_aabb(rays, self._aabb_host)
         near = torch.where(t_max > 1e9, torch.full_like(t_max, self.near_plane_bg), t_max)
         near = torch.clamp(near, min=0.0)
@@ -276,24 +276,36 @@ class NeuSModel(nn.Module):
         if self.randomized:
             near = near + torch.rand_like(near) * self.render_step_size_bg
         grid = self.occupancy_grid_bg.binary if self.config.get("grid_prune", False) else None
+        return fn.ray_marching_bg_candidates(rays, near, far, self._aabb_host, grid,
+                                             self.render_step_size_bg, self.cone_angle_bg,
+                                             self.max_samples_per_ray_bg)
+
+    def march_bg(self, rays, candidates=None):
+        """Background sampling of forward_bg_ (models/neus.py:141-171): the
+        candidates (march_bg_candidates, or the ones given), then the samples
+        behind transmittance 1e-4 dropped (sigma_fn = the background
+        density, models/neus.py:144-150)."""
+        rays = rays.float().contiguous()
+        if candidates is None:
+            candidates = self.march_bg_candidates(rays)
 
         def sigma_fn(t0, t1, ridx):
             pts, _, _ = fn.sample_points(rays, ridx, t0, t1)
             density, _ = self.geometry_bg(pts)
             return density
 
-        return fn.ray_marching_bg(rays, near, far, self._aabb_host, grid,
-                                  self.render_step_size_bg, self.cone_angle_bg,
-                                  self.max_samples_per_ray_bg, sigma_fn, 1e-4)
+        return fn.ray_marching_bg_filter(candidates, sigma_fn, 1e-4)
 
-    def forward_bg_(self, rays, samples=None):
+    def forward_bg_(self, rays, samples=None, candidates=None):
         """models/neus.py:141-203.  samples: optional (ray_indices int32,
-        t_starts, t_ends, packed) replacing the sampling."""
+        t_starts, t_ends, packed) replacing the sampling; candidates: the
+        marching before the early-stop filter, made ahead
+        (march_bg_candidates)."""
         rays = rays.float().contiguous()
         n_rays = rays.shape[0]
         if samples is None:
             with torch.no_grad():
-                ray_indices, t_starts, t_ends, packed = self.march_bg(rays)
+                ray_indices, t_starts, t_ends, packed = self.march_bg(rays, candidates)
         else:
             ray_indices, t_starts, t_ends, packed = samples
         pts, mid, dists = fn.sample_points(rays, ray_indices, t_starts, t_ends)
@@ -326,7 +338,7 @@ class NeuSModel(nn.Module):
         return out
 
     # -- forward ------------------------------------------------------------
-    def forward_(self, rays, samples=None, randn=None, samples_bg=None):
+    def forward_(self, rays, samples=None, randn=None, samples_bg=None, bg_candidates=None):
         """models/neus.py:205-283.  samples / samples_bg: optional
         (ray_indices int32, t_starts, t_ends, packed) replacing the
         foreground / background sampling."""
@@ -369,7 +381,7 @@ class NeuSModel(nn.Module):
                 "sdf_laplace_samples": lap.view(-1, 1),
             })
         if self.learned_background:
-            out_bg = self.forward_bg_(rays, samples_bg)
+            out_bg = self.forward_bg_(rays, samples_bg, bg_candidates)
         else:
             bg = self.background_color
             if bg is None:
@@ -387,13 +399,16 @@ class NeuSModel(nn.Module):
         return {**out, **{k + "_bg": v for k, v in out_bg.items()},
                 **{k + "_full": v for k, v in out_full.items()}}
 
-    def forward(self, rays, samples=None, randn=None, samples_bg=None):
-        """models/neus.py:285-293."""
+    def forward(self, rays, samples=None, randn=None, samples_bg=None, bg_candidates=None):
+        """models/neus.py:285-293.  bg_candidates: the learned background's
+        marching before its early-stop filter, made ahead for these rays
+        (march_bg_candidates; a pipelined training loop)."""
         if self.training:
-            out = self.forward_(rays, samples, randn, samples_bg)
+            out = self.forward_(rays, samples, randn, samples_bg, bg_candidates)
         else:
             out = chunk_batch(lambda r: self.forward_(r), self.config.ray_chunk, False, rays) \
-                if samples is None else self.forward_(rays, samples, randn, samples_bg)
+                if samples is None else self.forward_(rays, samples, randn, samples_bg,
+                                                      bg_candidates)
         return {**out, "inv_s": self.variance.inv_s}
 
     def train(self, mode=True):

@@ -292,10 +292,17 @@ class Workload:
                 side.wait_event(self._grid_ev)
             b = self._draw(set_model=False)
             b["samples"] = self.model.march(b["rays"])
+            # the step's other global-generator draws, in the forward's
+            # order: the curvature directions, then the background jitter
+            # (with the background's marching up to its early-stop filter)
+            b["randn"] = torch.randn(b["samples"][0].shape[0], 3, device=self.device)
+            if self.model.learned_background:
+                b["bg_candidates"] = self.model.march_bg_candidates(b["rays"])
             ev = torch.cuda.Event()
             ev.record(side)
         for v in b.values():
-            for t in (v if isinstance(v, (tuple, list)) else (v,)):
+            vals = v.values() if isinstance(v, dict) else (v if isinstance(v, (tuple, list)) else (v,))
+            for t in vals:
                 if isinstance(t, torch.Tensor) and t.is_cuda:
                     t.record_stream(main)   # freed memory waits for the main stream's use
         self._ahead = (i, b, ev)
@@ -324,7 +331,8 @@ class Workload:
         if self.pipeline:
             self._grid_ev = torch.cuda.Event()
             self._grid_ev.record()
-        out = self.model(b["rays"], samples=b["samples"])
+        out = self.model(b["rays"], samples=b["samples"], randn=b.get("randn"),
+                         bg_candidates=b.get("bg_candidates"))
         loss, _ = self.trainer.losses(out, b)
         self.optimizer.zero_grad(set_to_none=True)
         if self.scaler is not None:
