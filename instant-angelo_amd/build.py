@@ -31,9 +31,17 @@ CFLAGS = [
 ]
 
 
+# MFMA accumulators in VGPRs rather than AGPRs (on the unified register file
+# the AGPRs add to a wave's registers): the radiance forward goes from one to
+# two waves a SIMD (0.36 -> 0.23 ms a step).  Not for geometry.hip (the SDF
+# forward at three waves a SIMD measured slower, 3.54 -> 3.90 ms) nor
+# mlp.hip (the option crashes the compiler there).
+FILE_FLAGS = {"radiance.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
+
+
 def _compile(src):
     obj = os.path.join(OBJ, os.path.basename(src) + ".o")
-    cmd = [HIPCC] + CFLAGS + ["-c", src, "-o", obj]
+    cmd = [HIPCC] + CFLAGS + FILE_FLAGS.get(os.path.basename(src), []) + ["-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stdout}\n{r.stderr}")
